@@ -1,0 +1,126 @@
+"""ctypes binding of ``libaerojax_hip.so`` (C ABI in ``include/aerojax.h``).
+
+The library is required: every compute entry point of this package goes
+through it, and there is no CPU fallback.  A missing or stale library raises
+``AeroJaxLibraryError`` at the first call.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libaerojax_hip.so")
+
+AJ_OK = 0
+AJ_ERR_INVALID = 1
+AJ_ERR_GRID = 2
+AJ_ERR_HIP = 3
+AJ_ERR_NONFINITE = 4
+AJ_ERR_UNSUPPORTED = 5
+
+AJ_F32 = 0
+AJ_F64 = 1
+AJ_MG_REFERENCE = 0
+AJ_MG_TEXTBOOK = 1
+AJ_MG_MAX_NORMS = 64
+
+# Every symbol the header declares; tests check the library exports them.
+EXPORTS = (
+    "aj_version", "aj_last_error", "aj_rk_step", "aj_les_smagorinsky", "aj_divergence_rhs",
+    "aj_project", "aj_mg_create", "aj_mg_destroy", "aj_mg_workspace_bytes",
+    "aj_mg_num_levels", "aj_poisson_mg", "aj_mg_get_stats",
+)
+
+
+class AeroJaxLibraryError(RuntimeError):
+    """The HIP library is missing or could not be loaded."""
+
+
+class AeroJaxError(RuntimeError):
+    """A C ABI call returned an error code."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[aj error {code}] {msg}")
+        self.code = code
+
+
+class RkParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("nu", ctypes.c_double), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("U_inf", ctypes.c_double),
+                ("brinkman_eta", ctypes.c_double), ("nu_hyper_ratio", ctypes.c_double),
+                ("slip_walls", ctypes.c_int), ("fast_mode", ctypes.c_int)]
+
+
+class MgStats(ctypes.Structure):
+    _fields_ = [("cycles", ctypes.c_int), ("outer", ctypes.c_int), ("converged", ctypes.c_int),
+                ("nonfinite", ctypes.c_int), ("norm", ctypes.c_double),
+                ("norms", ctypes.c_double * AJ_MG_MAX_NORMS)]
+
+    def as_dict(self, textbook=False):
+        # reference: one norm per cycle evaluation (outer = evaluations);
+        # textbook: initial residual + one norm per outer iteration
+        k = self.outer + 1 if textbook else self.outer
+        return {"cycles": self.cycles, "outer": self.outer, "converged": bool(self.converged),
+                "nonfinite": bool(self.nonfinite), "norm": self.norm,
+                "norms": [self.norms[i] for i in range(min(k, AJ_MG_MAX_NORMS))]}
+
+
+_lock = threading.Lock()
+_lib = None
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+_SIGS = {
+    "aj_version": (ctypes.c_char_p, []),
+    "aj_last_error": (ctypes.c_char_p, []),
+    "aj_rk_step": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
+                        ctypes.POINTER(RkParams), _vp]),
+    "aj_les_smagorinsky": (_i, [_i, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
+    "aj_divergence_rhs": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _vp]),
+    "aj_project": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
+    "aj_mg_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _d, _d, _i]),
+    "aj_mg_destroy": (_i, [_vp]),
+    "aj_mg_workspace_bytes": (ctypes.c_size_t, [_vp]),
+    "aj_mg_num_levels": (_i, [_vp]),
+    "aj_poisson_mg": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
+    "aj_mg_get_stats": (_i, [_vp, ctypes.POINTER(MgStats), _vp]),
+}
+
+
+def load(path: str | None = None):
+    """Load the library once and bind every exported function."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise AeroJaxLibraryError(
+                f"{p} not found: build it with `python -m aerojax_amd.build` "
+                "(there is no CPU fallback)")
+        try:
+            lib = ctypes.CDLL(p)
+        except OSError as e:  # pragma: no cover - depends on the machine
+            raise AeroJaxLibraryError(f"cannot load {p}: {e}") from e
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def check(rc: int, what: str):
+    if rc != AJ_OK:
+        msg = load().aj_last_error().decode(errors="replace")
+        if rc == AJ_ERR_GRID:
+            raise ValueError(msg)
+        raise AeroJaxError(rc, f"{what}: {msg}")
+
+
+def version() -> str:
+    return load().aj_version().decode()

@@ -1,0 +1,57 @@
+// Shared host/device helpers for the aerojax HIP library (gfx950 / CDNA4).
+//
+// Layout convention of every field: (nx, ny) row-major, y contiguous
+// (index = i*ny + j); row i = 0 is the inlet, i = nx-1 the outlet,
+// columns j = 0 and j = ny-1 the walls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/aerojax.h"
+
+namespace aj {
+
+// ---- error state --------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define AJ_HIP(call)                                        \
+  do {                                                      \
+    hipError_t aj_e_ = (call);                              \
+    if (aj_e_ != hipSuccess) return ::aj::hip_fail(aj_e_, #call); \
+  } while (0)
+
+#define AJ_LAUNCH_CHECK(what)                               \
+  do {                                                      \
+    hipError_t aj_e_ = hipGetLastError();                   \
+    if (aj_e_ != hipSuccess) return ::aj::hip_fail(aj_e_, what); \
+  } while (0)
+
+inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// ---- device helpers -----------------------------------------------------
+__device__ __forceinline__ int clampi(int x, int lo, int hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// Block-wide sum of a double (blockDim.x multiple of 64, <= 1024) with a
+// fixed reduction tree, so repeated runs give identical bits.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  double t = 0.0;
+  if (threadIdx.x < 64) {
+    t = (lane < nw) ? sh[lane] : 0.0;
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  }
+  __syncthreads();
+  return t;  // valid in threads 0..63
+}
+
+}  // namespace aj

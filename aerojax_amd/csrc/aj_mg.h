@@ -1,0 +1,44 @@
+// Multigrid plan shared by the reference-exact and the textbook solvers.
+#pragma once
+
+#include <vector>
+
+#include "aj_common.h"
+
+struct aj_mg_plan {
+  int dtype = AJ_F32;
+  int variant = AJ_MG_REFERENCE;
+  int nx = 0, ny = 0;
+  int levels = 4;     // reference: requested levels
+  int ml = 0;         // reference: recursion depth (coarsest level index)
+  double dx = 0, dy = 0;
+  std::vector<int> lnx, lny;  // level shapes, level 0 = fine grid
+  int device = 0;
+
+  // one allocation holds every buffer below
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+
+  // reference variant, per level (index = level); level 0 uses the
+  // caller's rhs as b and the caller's output as E
+  std::vector<void*> B, Y, E;
+
+  // textbook variant
+  std::vector<float*> tB, tY, tE;   // FP32 V-cycle buffers per level
+  double* p64[2] = {nullptr, nullptr};  // FP64 iterate (device-selected ping-pong)
+  double* coarse_inv = nullptr;     // dense inverse of the coarsest operator
+  int coarse_rows = 0;              // unknown rows of the coarsest level
+  int bottom_level = 0;             // first level handled by the bottom kernel
+
+  // reductions and device-side state
+  double* partial = nullptr;
+  int npartial = 0;
+  aj_mg_stats* stats = nullptr;     // device copy, also the cycle gate
+  int* state = nullptr;             // textbook: [0] = current p64 buffer
+};
+
+namespace aj {
+int tb_create(aj_mg_plan* p);
+int tb_solve(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out, int max_cycles,
+             double tol, hipStream_t st);
+}  // namespace aj

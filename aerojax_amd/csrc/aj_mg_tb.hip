@@ -1,0 +1,659 @@
+// Textbook geometric multigrid for the consistent outlet-Dirichlet operator,
+// FP32 V(2,2)-cycles inside an FP64 defect correction.
+//
+// Discrete problem (rows 0..nx-2 carry equations, p[nx-1, :] = 0):
+//   ax (p[i+1] + p[i-1] - 2p) + ay (p[j+1] + p[j-1] - 2p) = b
+// with edge (Neumann) ghosts at the inlet row and both walls.
+//
+// Hierarchy: cell-centred 2x coarsening down to <= 64 cells; coarse level l
+// has coefficients ax/4^l, ay/4^l and an outlet ghost gamma_l * last that
+// puts its Dirichlet point where the fine grid has it
+// (gamma_l = 1 - 1/(1/2 - 2^-(l+1))).  Smoother: red-black Gauss-Seidel
+// with the exact row solve.  Restriction: 2x2 average.  Prolongation:
+// cell-centred bilinear.  Coarsest level: dense FP64 inverse.
+//
+// Kernels per V-cycle on the large levels (temporal blocking in LDS):
+//   pre  : 2 RB-GS sweeps (4 half-sweeps, halo 5) + residual + restriction,
+//          writes the smoothed iterate Y and the coarse rhs
+//   post : Y + prolong(E_coarse), 2 RB-GS sweeps (halo 4), writes E
+// Levels with <= kBottomCells cells run in one workgroup entirely in LDS
+// (bottom kernel).  Defect correction per outer iteration:
+//   resid: p64' = p64 + e (on the fly), r = b - A p64' in FP64, writes
+//          r32 and p64', FP64 partial sums of r^2
+//   final: norm = sqrt(sum r^2 dx dy), convergence gate, buffer flip
+#include "aj_mg.h"
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+namespace aj {
+
+constexpr int kTbNPartial = 1024;
+constexpr int kBottomCells = 4096;
+constexpr int kBottomThreads = 1024;
+constexpr int kTbCoarseCells = 64;
+constexpr int kCyclesPerOuter = 2;
+
+__device__ __forceinline__ bool tb_gated(const aj_mg_stats* g) {
+  return *(volatile const int*)&g->converged != 0;
+}
+
+static double outlet_gamma(int level) {
+  return 1.0 - 1.0 / (0.5 - std::ldexp(1.0, -level - 1));
+}
+
+struct TbLevel {
+  int n, m;        // rows (x), cols (y)
+  int level;       // 0 = fine
+  float axl, ayl;  // coefficients
+  float gam;       // outlet ghost factor (level >= 1)
+  float dE_last;   // (1 - gamma) * axl
+};
+
+// Exact row solve of the level operator at (i, j) given a neighbour fetch.
+// Neighbour values outside the domain are never read.
+template <typename F>
+__device__ __forceinline__ float tb_relax(const TbLevel& L, int i, int j, float pc, float bv,
+                                          F&& nb) {
+  const bool hasW = i > 0, hasS = j > 0, hasN = j < L.m - 1;
+  const bool last = (i == L.n - 1);
+  const float E = (!last) ? nb(1, 0) : 0.0f;
+  const float W = hasW ? nb(-1, 0) : 0.0f;
+  const float S = hasS ? nb(0, -1) : 0.0f;
+  const float N = hasN ? nb(0, 1) : 0.0f;
+  const float dE = (L.level > 0 && last) ? L.dE_last : L.axl;
+  const float diag = ((dE + (hasW ? L.axl : 0.0f)) + (hasS ? L.ayl : 0.0f)) + (hasN ? L.ayl : 0.0f);
+  const float off = L.axl * (E + W) + L.ayl * (N + S);
+  (void)pc;
+  return (off - bv) / diag;
+}
+
+// r = b - A p at (i, j) (level operator; level 0 outlet row -> 0)
+template <typename F>
+__device__ __forceinline__ float tb_resid(const TbLevel& L, int i, int j, float pc, float bv,
+                                          F&& nb) {
+  if (L.level == 0 && i == L.n - 1) return 0.0f;
+  const float W = i > 0 ? nb(-1, 0) : pc;
+  float E;
+  if (i < L.n - 1) E = nb(1, 0);
+  else E = L.gam * pc;  // level >= 1 only (level 0 returned above)
+  const float S = j > 0 ? nb(0, -1) : pc;
+  const float N = j < L.m - 1 ? nb(0, 1) : pc;
+  const float Ap = L.axl * ((E - pc) + (W - pc)) + L.ayl * ((N - pc) + (S - pc));
+  return bv - Ap;
+}
+
+// cell-centred bilinear prolongation of coarse e (nc x mc) at fine (i, j);
+// gamc = outlet ghost factor of the coarse level
+__device__ __forceinline__ float tb_prolong(const float* __restrict__ e, int nc, int mc,
+                                            float gamc, int i, int j) {
+  const int I = i >> 1, J = j >> 1;
+  int In = (i & 1) ? I + 1 : I - 1;
+  int Jn = (j & 1) ? J + 1 : J - 1;
+  float sx = 1.0f;
+  if (In < 0) In = I;
+  else if (In >= nc) { In = I; sx = gamc; }
+  if (Jn < 0) Jn = J;
+  else if (Jn >= mc) Jn = J;
+  const float own = e[I * mc + J];
+  const float xn = e[In * mc + J] * sx;
+  const float yn = e[I * mc + Jn];
+  const float dn = e[In * mc + Jn] * sx;
+  return 0.5625f * own + 0.1875f * (xn + yn) + 0.0625f * dn;
+}
+
+// ---------------------------------------------------------------------------
+// pre: 2 RB-GS sweeps + residual + restriction (tile TX x TY, both even)
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+__global__ __launch_bounds__(256) void k_tb_pre(TbLevel L, const float* __restrict__ p_in,
+                                                const float* __restrict__ b,
+                                                float* __restrict__ Y, float* __restrict__ Bc,
+                                                const aj_mg_stats* gate) {
+  if (tb_gated(gate)) return;
+  constexpr int H = 5, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
+  __shared__ float s_p[RN];
+  __shared__ float s_b[RN];
+  __shared__ float s_r[TX * TY];
+  const int n = L.n, m = L.m;
+  const int gx0 = blockIdx.y * TX - H, gy0 = blockIdx.x * TY - H;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < RN; idx += 256) {
+    const int lx = idx / RW, ly = idx - lx * RW;
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    if (gx < 0 || gx >= n || gy < 0 || gy >= m) continue;
+    const long g = (long)gx * m + gy;
+    s_p[idx] = p_in ? p_in[g] : 0.0f;
+    if (lx >= 1 && lx < RH - 1 && ly >= 1 && ly < RW - 1) s_b[idx] = b[g];
+  }
+  __syncthreads();
+  const int active_rows = (L.level == 0) ? n - 1 : n;
+#pragma unroll
+  for (int k = 1; k <= 4; ++k) {
+    const int mg = H - k;
+    const int color = (k - 1) & 1;
+    const int W = TY + 2 * mg, Hh = TX + 2 * mg;
+    for (int idx = tid; idx < W * Hh; idx += 256) {
+      const int lx = H - mg + idx / W, ly = H - mg + idx % W;
+      const int gx = gx0 + lx, gy = gy0 + ly;
+      if (gx < 0 || gx >= active_rows || gy < 0 || gy >= m) continue;
+      if (((gx + gy) & 1) != color) continue;
+      const int l = lx * RW + ly;
+      s_p[l] = tb_relax(L, gx, gy, s_p[l], s_b[l],
+                        [&](int di, int dj) { return s_p[l + di * RW + dj]; });
+    }
+    __syncthreads();
+  }
+  // residual on the tile, write Y
+  for (int idx = tid; idx < TX * TY; idx += 256) {
+    const int tx = idx / TY, ty = idx % TY;
+    const int lx = H + tx, ly = H + ty;
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    if (gx >= n || gy >= m) continue;
+    const int l = lx * RW + ly;
+    const float pc = s_p[l];
+    Y[(long)gx * m + gy] = pc;
+    s_r[idx] = tb_resid(L, gx, gy, pc, s_b[l], [&](int di, int dj) { return s_p[l + di * RW + dj]; });
+  }
+  __syncthreads();
+  const int mc = m >> 1;
+  for (int idx = tid; idx < (TX / 2) * (TY / 2); idx += 256) {
+    const int cx = idx / (TY / 2), cy = idx % (TY / 2);
+    const int gx = gx0 + H + 2 * cx, gy = gy0 + H + 2 * cy;
+    if (gx >= n || gy >= m) continue;
+    const int t = (2 * cx) * TY + 2 * cy;
+    const float r00 = s_r[t], r10 = s_r[t + TY], r01 = s_r[t + 1], r11 = s_r[t + TY + 1];
+    Bc[(long)(gx >> 1) * mc + (gy >> 1)] = 0.25f * ((r00 + r10) + (r01 + r11));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// post: Y + prolong(E_coarse), 2 RB-GS sweeps
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+__global__ __launch_bounds__(256) void k_tb_post(TbLevel L, const float* __restrict__ Yin,
+                                                 const float* __restrict__ Ec, int nc, int mc,
+                                                 float gamc, const float* __restrict__ b,
+                                                 float* __restrict__ E,
+                                                 const aj_mg_stats* gate) {
+  if (tb_gated(gate)) return;
+  constexpr int H = 4, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
+  __shared__ float s_p[RN];
+  __shared__ float s_b[RN];
+  const int n = L.n, m = L.m;
+  const int gx0 = blockIdx.y * TX - H, gy0 = blockIdx.x * TY - H;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < RN; idx += 256) {
+    const int lx = idx / RW, ly = idx - lx * RW;
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    if (gx < 0 || gx >= n || gy < 0 || gy >= m) continue;
+    const long g = (long)gx * m + gy;
+    float pv = Yin[g];
+    if (!(L.level == 0 && gx == n - 1)) pv = pv + tb_prolong(Ec, nc, mc, gamc, gx, gy);
+    s_p[idx] = pv;
+    if (lx >= 1 && lx < RH - 1 && ly >= 1 && ly < RW - 1) s_b[idx] = b[g];
+  }
+  __syncthreads();
+  const int active_rows = (L.level == 0) ? n - 1 : n;
+#pragma unroll
+  for (int k = 1; k <= 4; ++k) {
+    const int mg = H - k;
+    const int color = (k - 1) & 1;
+    const int W = TY + 2 * mg, Hh = TX + 2 * mg;
+    for (int idx = tid; idx < W * Hh; idx += 256) {
+      const int lx = H - mg + idx / W, ly = H - mg + idx % W;
+      const int gx = gx0 + lx, gy = gy0 + ly;
+      if (gx < 0 || gx >= active_rows || gy < 0 || gy >= m) continue;
+      if (((gx + gy) & 1) != color) continue;
+      const int l = lx * RW + ly;
+      s_p[l] = tb_relax(L, gx, gy, s_p[l], s_b[l],
+                        [&](int di, int dj) { return s_p[l + di * RW + dj]; });
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < TX * TY; idx += 256) {
+    const int lx = H + idx / TY, ly = H + idx % TY;
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    if (gx >= n || gy >= m) continue;
+    E[(long)gx * m + gy] = s_p[lx * RW + ly];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bottom: levels bot..L of the V-cycle in one workgroup (LDS resident)
+// ---------------------------------------------------------------------------
+constexpr int kMaxLevels = 24;
+struct TbBottom {
+  int nlev;                 // number of levels handled (bot..L)
+  TbLevel lev[kMaxLevels];
+  int off[kMaxLevels];      // LDS offsets of each level's arrays
+  float gam_next[kMaxLevels];
+  int coarse_rows;          // unknown rows of the coarsest level
+  int total;                // total cells
+};
+
+__device__ void tb_bottom_rbgs(const TbLevel& L, float* p, const float* b, int sweeps) {
+  const int n = L.n, m = L.m;
+  const int active_rows = (L.level == 0) ? n - 1 : n;
+  const int cells = active_rows * m;
+  for (int s = 0; s < 2 * sweeps; ++s) {
+    const int color = s & 1;
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+      const int i = c / m, j = c - i * m;
+      if (((i + j) & 1) != color) continue;
+      p[c] = tb_relax(L, i, j, p[c], b[c], [&](int di, int dj) { return p[c + di * m + dj]; });
+    }
+    __syncthreads();
+  }
+}
+
+// p_in (may be null = zero guess, may alias e_out) is the top level's start
+__global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(TbBottom B,
+                                                              const float* b_in,
+                                                              const float* p_in, float* e_out,
+                                                              const double* __restrict__ Ainv,
+                                                              const aj_mg_stats* gate) {
+  if (tb_gated(gate)) return;
+  extern __shared__ float sm[];
+  float* sp = sm;               // iterates
+  float* sb = sm + B.total;     // right-hand sides
+  const int nl = B.nlev;
+  {
+    const int n0 = B.lev[0].n * B.lev[0].m;
+    for (int c = threadIdx.x; c < n0; c += blockDim.x) {
+      sb[c] = b_in[c];
+      sp[c] = p_in ? p_in[c] : 0.0f;
+    }
+  }
+  __syncthreads();
+  // down
+  for (int k = 0; k < nl - 1; ++k) {
+    const TbLevel& L = B.lev[k];
+    float* p = sp + B.off[k];
+    float* bb = sb + B.off[k];
+    tb_bottom_rbgs(L, p, bb, 2);
+    const TbLevel& C = B.lev[k + 1];
+    float* pc = sp + B.off[k + 1];
+    float* bc = sb + B.off[k + 1];
+    const int m = L.m;
+    for (int c = threadIdx.x; c < C.n * C.m; c += blockDim.x) {
+      const int I = c / C.m, J = c - I * C.m;
+      const int i = 2 * I, j = 2 * J;
+      auto R = [&](int ii, int jj) {
+        const int q = ii * m + jj;
+        return tb_resid(L, ii, jj, p[q], bb[q], [&](int di, int dj) { return p[q + di * m + dj]; });
+      };
+      const float r00 = R(i, j), r10 = R(i + 1, j), r01 = R(i, j + 1), r11 = R(i + 1, j + 1);
+      bc[c] = 0.25f * ((r00 + r10) + (r01 + r11));
+      pc[c] = 0.0f;
+    }
+    __syncthreads();
+  }
+  // coarsest: dense FP64 inverse
+  {
+    const TbLevel& L = B.lev[nl - 1];
+    float* p = sp + B.off[nl - 1];
+    const float* bb = sb + B.off[nl - 1];
+    const int N = B.coarse_rows * L.m;
+    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+      double acc = 0.0;
+      for (int k = 0; k < N; ++k) acc += Ainv[(long)r * N + k] * (double)bb[k];
+      p[r] = (float)acc;
+    }
+    for (int r = N + threadIdx.x; r < L.n * L.m; r += blockDim.x) p[r] = 0.0f;
+  }
+  __syncthreads();
+  // up
+  for (int k = nl - 2; k >= 0; --k) {
+    const TbLevel& L = B.lev[k];
+    const TbLevel& C = B.lev[k + 1];
+    float* p = sp + B.off[k];
+    const float* e = sp + B.off[k + 1];
+    for (int c = threadIdx.x; c < L.n * L.m; c += blockDim.x) {
+      const int i = c / L.m, j = c - i * L.m;
+      if (L.level == 0 && i == L.n - 1) continue;
+      p[c] = p[c] + tb_prolong(e, C.n, C.m, B.gam_next[k], i, j);
+    }
+    __syncthreads();
+    tb_bottom_rbgs(L, p, sb + B.off[k], 2);
+  }
+  {
+    const int n0 = B.lev[0].n * B.lev[0].m;
+    for (int c = threadIdx.x; c < n0; c += blockDim.x) e_out[c] = sp[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 defect correction
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, double* __restrict__ P,
+                                                 int nx, int ny) {
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= (long)nx * ny) return;
+  const int i = (int)(c / ny);
+  P[c] = (p0 && i < nx - 1) ? (double)p0[c] : 0.0;
+}
+
+// p' = P[cur] + e (e may be null), r = b - A p' in FP64 on rows < nx-1.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tb_resid(double* __restrict__ P0, double* __restrict__ P1,
+                                                  const int* __restrict__ state,
+                                                  const float* __restrict__ e,
+                                                  const T* __restrict__ b, float* __restrict__ r32,
+                                                  double* __restrict__ partial, int nx, int ny,
+                                                  double ax, double ay, const aj_mg_stats* gate) {
+  if (tb_gated(gate)) return;
+  __shared__ double sh[16];
+  const int cur = state[0];
+  const double* src = cur ? P1 : P0;
+  double* dst = cur ? P0 : P1;
+  const long n = (long)nx * ny;
+  double acc = 0.0;
+  for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long)gridDim.x * 256) {
+    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+    if (i == nx - 1) {
+      r32[c] = 0.0f;
+      if (e) dst[c] = 0.0;
+      continue;
+    }
+    auto P = [&](long q) { return e ? src[q] + (double)e[q] : src[q]; };
+    const double pc = P(c);
+    const double W = i > 0 ? P(c - ny) : pc;
+    const double E = (i + 1 < nx - 1) ? P(c + ny) : 0.0;
+    const double S = j > 0 ? P(c - 1) : pc;
+    const double N = j < ny - 1 ? P(c + 1) : pc;
+    const double Ap = ax * ((E - pc) + (W - pc)) + ay * ((N - pc) + (S - pc));
+    const double r = (double)b[c] - Ap;
+    r32[c] = (float)r;
+    if (e) dst[c] = pc;
+    acc += r * r;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void k_tb_norm_final(const double* __restrict__ partial, int np, double dxdy, double tol,
+                                int flip, int cycles_added, aj_mg_stats* st, int* state) {
+  if (st->converged) return;  // this outer iteration was skipped
+  __shared__ double sh[16];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < np; k += 256) acc += partial[k];
+  const double tot = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(tot * dxdy);
+    // norms[0] = initial residual, norms[k] = after outer iteration k
+    const int k = st->outer + (flip ? 1 : 0);
+    if (k < AJ_MG_MAX_NORMS) st->norms[k] = nrm;
+    if (flip) {
+      state[0] ^= 1;
+      st->outer = k;
+      st->cycles += cycles_added;
+    }
+    st->norm = nrm;
+    if (!isfinite(nrm)) st->nonfinite = 1;
+    st->converged = (nrm <= tol) ? 1 : 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tb_out(const double* __restrict__ P0,
+                                                const double* __restrict__ P1,
+                                                const int* __restrict__ state, T* __restrict__ out,
+                                                long n) {
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  out[c] = (T)(state[0] ? P1[c] : P0[c]);
+}
+
+__global__ void k_tb_begin(aj_mg_stats* st, int* state) {
+  if (threadIdx.x == 0) {
+    st->cycles = 0;
+    st->outer = 0;
+    st->converged = 0;
+    st->nonfinite = 0;
+    st->norm = 0.0;
+    state[0] = 0;
+  }
+  for (int k = threadIdx.x; k < AJ_MG_MAX_NORMS; k += blockDim.x) st->norms[k] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+constexpr int kPX = 32, kPY = 64;  // pre tile
+constexpr int kQX = 32, kQY = 64;  // post tile
+
+static TbLevel make_level(const aj_mg_plan* p, int l) {
+  TbLevel L;
+  L.n = p->lnx[l];
+  L.m = p->lny[l];
+  L.level = l;
+  const double s = std::ldexp(1.0, 2 * l);
+  const double ax = 1.0 / (p->dx * p->dx) / s, ay = 1.0 / (p->dy * p->dy) / s;
+  L.axl = (float)ax;
+  L.ayl = (float)ay;
+  const double g = l > 0 ? outlet_gamma(l) : 1.0;
+  L.gam = (float)g;
+  L.dE_last = (float)(1.0 - g) * L.axl;
+  return L;
+}
+
+// dense operator of level l on its unknown rows (row-major, cell index i*m+j)
+static std::vector<double> dense_op(const aj_mg_plan* p, int l, int* rows_out) {
+  const int n = p->lnx[l], m = p->lny[l];
+  const int rows = (l == 0) ? n - 1 : n;
+  const int N = rows * m;
+  const double s = std::ldexp(1.0, 2 * l);
+  const double ax = 1.0 / (p->dx * p->dx) / s, ay = 1.0 / (p->dy * p->dy) / s;
+  const double g = l > 0 ? outlet_gamma(l) : 0.0;
+  std::vector<double> A((size_t)N * N, 0.0);
+  auto at = [&](int r, int c) -> double& { return A[(size_t)r * N + c]; };
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < m; ++j) {
+      const int r = i * m + j;
+      // east
+      if (i + 1 < rows) { at(r, r + m) += ax; at(r, r) -= ax; }
+      else if (l == 0) { at(r, r) -= ax; }                 // Dirichlet row value 0
+      else { at(r, r) += ax * (g - 1.0); }                 // ghost g * p
+      if (i > 0) { at(r, r - m) += ax; at(r, r) -= ax; }
+      if (j > 0) { at(r, r - 1) += ay; at(r, r) -= ay; }
+      if (j < m - 1) { at(r, r + 1) += ay; at(r, r) -= ay; }
+    }
+  *rows_out = rows;
+  return A;
+}
+
+static bool invert(std::vector<double>& A, int N) {
+  std::vector<double> I((size_t)N * N, 0.0);
+  for (int k = 0; k < N; ++k) I[(size_t)k * N + k] = 1.0;
+  for (int c = 0; c < N; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < N; ++r)
+      if (std::fabs(A[(size_t)r * N + c]) > std::fabs(A[(size_t)piv * N + c])) piv = r;
+    if (A[(size_t)piv * N + c] == 0.0) return false;
+    if (piv != c)
+      for (int k = 0; k < N; ++k) {
+        std::swap(A[(size_t)c * N + k], A[(size_t)piv * N + k]);
+        std::swap(I[(size_t)c * N + k], I[(size_t)piv * N + k]);
+      }
+    const double d = A[(size_t)c * N + c];
+    for (int k = 0; k < N; ++k) { A[(size_t)c * N + k] /= d; I[(size_t)c * N + k] /= d; }
+    for (int r = 0; r < N; ++r) {
+      if (r == c) continue;
+      const double f = A[(size_t)r * N + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < N; ++k) {
+        A[(size_t)r * N + k] -= f * A[(size_t)c * N + k];
+        I[(size_t)r * N + k] -= f * I[(size_t)c * N + k];
+      }
+    }
+  }
+  A.swap(I);
+  return true;
+}
+
+struct Carve {
+  char* base;
+  size_t off = 0;
+  void* take(size_t bytes) {
+    void* r = base ? base + off : nullptr;
+    off += (bytes + 255) & ~size_t(255);
+    return r;
+  }
+};
+
+int tb_create(aj_mg_plan* p) {
+  // hierarchy
+  p->lnx.assign(1, p->nx);
+  p->lny.assign(1, p->ny);
+  {
+    int n = p->nx, m = p->ny;
+    while ((long)n * m > kTbCoarseCells && n % 2 == 0 && m % 2 == 0 && n >= 4 && m >= 2) {
+      n /= 2;
+      m /= 2;
+      p->lnx.push_back(n);
+      p->lny.push_back(m);
+    }
+  }
+  if (p->nx < 3 || p->ny < 2) return fail(AJ_ERR_GRID, "textbook multigrid needs nx >= 3, ny >= 2");
+  const int L = (int)p->lnx.size() - 1;
+  if (L + 1 > kMaxLevels) return fail(AJ_ERR_GRID, "textbook multigrid: too many levels");
+  int rows = 0;
+  std::vector<double> A = dense_op(p, L, &rows);
+  const int N = rows * p->lny[L];
+  if (N > 1024) {
+    char msg[200];
+    std::snprintf(msg, sizeof(msg),
+                  "Grid dimensions (%d, %d) not suitable for multigrid: coarsest level has %d "
+                  "unknowns",
+                  p->nx, p->ny, N);
+    return fail(AJ_ERR_GRID, msg);
+  }
+  if (!invert(A, N)) return fail(AJ_ERR_GRID, "textbook multigrid: singular coarse operator");
+  p->coarse_rows = rows;
+  // first level handled by the bottom kernel
+  int bot = L;
+  while (bot > 0 && (long)p->lnx[bot - 1] * p->lny[bot - 1] <= kBottomCells) --bot;
+  p->bottom_level = bot;
+  p->ml = L;
+
+  const long n0 = (long)p->nx * p->ny;
+  for (int pass = 0; pass < 2; ++pass) {
+    Carve cv{pass == 0 ? nullptr : (char*)p->ws};
+    p->tB.assign(L + 1, nullptr);
+    p->tY.assign(L + 1, nullptr);
+    p->tE.assign(L + 1, nullptr);
+    for (int l = 0; l <= L; ++l) {
+      const size_t n = (size_t)p->lnx[l] * p->lny[l];
+      p->tB[l] = (float*)cv.take(n * 4);
+      p->tY[l] = (float*)cv.take(n * 4);
+      p->tE[l] = (float*)cv.take(n * 4);
+    }
+    p->p64[0] = (double*)cv.take(n0 * 8);
+    p->p64[1] = (double*)cv.take(n0 * 8);
+    p->coarse_inv = (double*)cv.take((size_t)N * N * 8);
+    p->partial = (double*)cv.take(kTbNPartial * 8);
+    p->npartial = kTbNPartial;
+    p->stats = (aj_mg_stats*)cv.take(sizeof(aj_mg_stats));
+    p->state = (int*)cv.take(64);
+    if (pass == 0) {
+      p->ws_bytes = cv.off;
+      AJ_HIP(hipMalloc(&p->ws, p->ws_bytes));
+    }
+  }
+  AJ_HIP(hipMemcpy(p->coarse_inv, A.data(), (size_t)N * N * 8, hipMemcpyHostToDevice));
+  return AJ_OK;
+}
+
+// one FP32 V-cycle for A_0 e = tB[0]; result in tE[0]; warm = use tE[0] as start
+static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
+  const int bot = p->bottom_level;
+  if (lev == bot) {
+    TbBottom B;
+    const int L = p->ml;
+    B.nlev = L - bot + 1;
+    int off = 0;
+    for (int k = 0; k < B.nlev; ++k) {
+      B.lev[k] = make_level(p, bot + k);
+      B.off[k] = off;
+      off += B.lev[k].n * B.lev[k].m;
+      B.gam_next[k] = (bot + k + 1 <= L) ? (float)outlet_gamma(bot + k + 1) : 1.0f;
+    }
+    B.total = off;
+    B.coarse_rows = p->coarse_rows;
+    const size_t smem = (size_t)2 * off * sizeof(float);
+    hipLaunchKernelGGL(k_tb_bottom, dim3(1), dim3(kBottomThreads), smem, st, B,
+                       (const float*)p->tB[lev], warm ? (const float*)p->tE[lev] : (const float*)nullptr,
+                       p->tE[lev], (const double*)p->coarse_inv,
+                       (const aj_mg_stats*)p->stats);
+    AJ_LAUNCH_CHECK("k_tb_bottom");
+    return AJ_OK;
+  }
+  const TbLevel Lv = make_level(p, lev);
+  const int n = Lv.n, m = Lv.m;
+  dim3 gpre(cdiv(m, kPY), cdiv(n, kPX));
+  hipLaunchKernelGGL((k_tb_pre<kPX, kPY>), gpre, dim3(256), 0, st, Lv,
+                     warm ? (const float*)p->tE[lev] : (const float*)nullptr,
+                     (const float*)p->tB[lev], p->tY[lev], p->tB[lev + 1],
+                     (const aj_mg_stats*)p->stats);
+  AJ_LAUNCH_CHECK("k_tb_pre");
+  if (int rc = tb_vcycle(p, lev + 1, false, st)) return rc;
+  dim3 gpost(cdiv(m, kQY), cdiv(n, kQX));
+  hipLaunchKernelGGL((k_tb_post<kQX, kQY>), gpost, dim3(256), 0, st, Lv, (const float*)p->tY[lev],
+                     (const float*)p->tE[lev + 1], p->lnx[lev + 1], p->lny[lev + 1],
+                     (float)outlet_gamma(lev + 1), (const float*)p->tB[lev], p->tE[lev],
+                     (const aj_mg_stats*)p->stats);
+  AJ_LAUNCH_CHECK("k_tb_post");
+  return AJ_OK;
+}
+
+template <typename T>
+static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int max_cycles,
+                      double tol, hipStream_t st) {
+  const int nx = p->nx, ny = p->ny;
+  const long n = (long)nx * ny;
+  const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
+  const double dxdy = p->dx * p->dy;
+  const int nb = (int)std::min<long>(kTbNPartial, cdiv(n, 256));
+  hipLaunchKernelGGL(k_tb_begin, dim3(1), dim3(64), 0, st, p->stats, p->state);
+  AJ_LAUNCH_CHECK("k_tb_begin");
+  hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p0, p->p64[0], nx, ny);
+  AJ_LAUNCH_CHECK("k_tb_init");
+  // outer iteration 0: residual of the initial guess
+  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
+                     (const int*)p->state, (const float*)nullptr, rhs, p->tB[0], p->partial, nx,
+                     ny, ax, ay, (const aj_mg_stats*)p->stats);
+  AJ_LAUNCH_CHECK("k_tb_resid");
+  hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
+                     dxdy, tol, 0, 0, p->stats, p->state);
+  AJ_LAUNCH_CHECK("k_tb_norm_final");
+  const int max_outer = (max_cycles + kCyclesPerOuter - 1) / kCyclesPerOuter;
+  for (int k = 0; k < max_outer; ++k) {
+    for (int c = 0; c < kCyclesPerOuter; ++c)
+      if (int rc = tb_vcycle(p, 0, c > 0, st)) return rc;
+    hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
+                       (const int*)p->state, (const float*)p->tE[0], rhs, p->tB[0], p->partial, nx,
+                       ny, ax, ay, (const aj_mg_stats*)p->stats);
+    AJ_LAUNCH_CHECK("k_tb_resid");
+    hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
+                       dxdy, tol, 1, kCyclesPerOuter, p->stats, p->state);
+    AJ_LAUNCH_CHECK("k_tb_norm_final");
+  }
+  hipLaunchKernelGGL(k_tb_out<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, (const double*)p->p64[0],
+                     (const double*)p->p64[1], (const int*)p->state, p_out, n);
+  AJ_LAUNCH_CHECK("k_tb_out");
+  return AJ_OK;
+}
+
+int tb_solve(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out, int max_cycles,
+             double tol, hipStream_t st) {
+  if (p->dtype == AJ_F32)
+    return tb_solve_t<float>(p, (const float*)rhs, (const float*)p0, (float*)p_out, max_cycles,
+                             tol, st);
+  return tb_solve_t<double>(p, (const double*)rhs, (const double*)p0, (double*)p_out, max_cycles,
+                            tol, st);
+}
+
+}  // namespace aj

@@ -1,0 +1,470 @@
+// Step kernels: fused RK predictor, Smagorinsky eddy viscosity, pressure
+// right-hand side and projection (collocated grid, von Karman BCs).
+//
+// All kernels are HBM-bound stencils.  The RK predictor fuses its 2 or 3
+// stages into one pass: a workgroup loads u, v on its (TX x TY) tile plus a
+// halo of (stages x per-stage radius) cells into LDS, evaluates each stage on
+// a region that shrinks by the stencil radius, and writes only the tile, so
+// HBM sees u, v, chi-source, mask read once and u*, v* written once.
+// Neighbour reads clamp the *global* index at the domain edge, which is the
+// reference's jnp.pad(mode='edge') applied to every stage field.
+#include "aj_common.h"
+
+#include <algorithm>
+
+namespace aj {
+
+// --------------------------------------------------------------------------
+// RK predictor (advection_schemes/rk3_simple_new.py:175-358)
+// --------------------------------------------------------------------------
+
+template <typename T>
+struct RkArgs {
+  const T* u;
+  const T* v;
+  const T* mask;
+  const T* sdf;     // may be null -> chi = 1 - mask
+  const T* nu_sgs;  // may be null
+  T* u_out;
+  T* v_out;
+  int nx, ny;
+  T dt, nu, inv_dx, inv_dy, inv_dx2, inv_dy2, U_inf, inv_eta, sdf_scale, nu_h;
+  int slip;
+};
+
+// One stencil evaluation of rhs = -upwind(f; u, v) + nu_tot * lap(f)
+// [- nu_h * lap(lap f)] for f = u and f = v at LDS cell l.
+template <typename T, bool HYPER>
+__device__ __forceinline__ void rk_rhs(const T* fu, const T* fv, const T* lu, const T* lv,
+                                       int l, int lW, int lE, int lS, int lN, T nu_tot,
+                                       const RkArgs<T>& a, T& ru, T& rv) {
+  const T uc = fu[l], vc = fv[l];
+  const T uW = fu[lW], uE = fu[lE], uS = fu[lS], uN = fu[lN];
+  const T vW = fv[lW], vE = fv[lE], vS = fv[lS], vN = fv[lN];
+  // upwind_advection (rk3_simple_new.py:287-298): vel > 0 -> backward diff
+  T ax_u = (uc > T(0)) ? uc * (uc - uW) * a.inv_dx : uc * (uE - uc) * a.inv_dx;
+  T ay_u = (vc > T(0)) ? vc * (uc - uS) * a.inv_dy : vc * (uN - uc) * a.inv_dy;
+  T ax_v = (uc > T(0)) ? uc * (vc - vW) * a.inv_dx : uc * (vE - vc) * a.inv_dx;
+  T ay_v = (vc > T(0)) ? vc * (vc - vS) * a.inv_dy : vc * (vN - vc) * a.inv_dy;
+  // laplacian (rk3_simple_new.py:215-219)
+  T lap_u = (uE + uW - T(2) * uc) * a.inv_dx2 + (uN + uS - T(2) * uc) * a.inv_dy2;
+  T lap_v = (vE + vW - T(2) * vc) * a.inv_dx2 + (vN + vS - T(2) * vc) * a.inv_dy2;
+  ru = -(ax_u + ay_u) + nu_tot * lap_u;
+  rv = -(ax_v + ay_v) + nu_tot * lap_v;
+  if constexpr (HYPER) {
+    const T bu = (lu[lE] + lu[lW] - T(2) * lu[l]) * a.inv_dx2 +
+                 (lu[lN] + lu[lS] - T(2) * lu[l]) * a.inv_dy2;
+    const T bv = (lv[lE] + lv[lW] - T(2) * lv[l]) * a.inv_dx2 +
+                 (lv[lN] + lv[lS] - T(2) * lv[l]) * a.inv_dy2;
+    ru = ru - a.nu_h * bu;
+    rv = rv - a.nu_h * bv;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T masked(T val, T m) {
+  return (m > T(0.5)) ? val : T(0);
+}
+
+// Fused RK kernel.  FAST = RK2 (2 stages), else RK3.  HYPER adds the
+// biharmonic term (per-stage stencil radius 2).
+template <typename T, int TX, int TY, bool FAST, bool HYPER>
+__global__ __launch_bounds__(256) void k_rk_fused(RkArgs<T> a) {
+  constexpr int S = FAST ? 2 : 3;
+  constexpr int R1 = HYPER ? 2 : 1;  // per-stage stencil radius
+  constexpr int H = S * R1;          // total halo
+  constexpr int RH = TX + 2 * H;     // region rows (x)
+  constexpr int RW = TY + 2 * H;     // region cols (y, contiguous)
+  constexpr int RN = RH * RW;
+
+  extern __shared__ unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  T* s_u = sm;              // u (margin H)
+  T* s_v = s_u + RN;
+  T* s_iD = s_v + RN;       // 1/(1 + dt chi/eta) (margin H - R1)
+  T* s_nt = s_iD + RN;      // nu + nu_sgs        (margin H - R1)
+  T* s_au = s_nt + RN;      // stage buffers
+  T* s_av = s_au + RN;
+  T* s_bu = s_av + RN;
+  T* s_bv = s_bu + RN;      // RK3: stage-2 output; RK2: k1
+  T* s_lu = s_bv + RN;      // HYPER only: laplacians of the stage input
+  T* s_lv = s_lu + RN;
+
+  const int nx = a.nx, ny = a.ny;
+  const int y0 = blockIdx.x * TY;
+  const int x0 = blockIdx.y * TX;
+  const int gx0 = x0 - H, gy0 = y0 - H;  // global index of LDS (0, 0)
+  const int tid = threadIdx.x;
+  const bool has_nsgs = a.nu_sgs != nullptr;
+  const bool has_sdf = a.sdf != nullptr;
+
+  // ---- load u, v on margin H; chi factor and nu_tot on margin H - R1 ----
+  for (int idx = tid; idx < RN; idx += 256) {
+    const int lx = idx / RW, ly = idx - lx * RW;
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+    const long g = (long)gx * ny + gy;
+    s_u[idx] = a.u[g];
+    s_v[idx] = a.v[g];
+    const int m = H - R1;
+    if (lx >= H - m && lx < H + TX + m && ly >= H - m && ly < H + TY + m) {
+      T chi;
+      if (has_sdf) {
+        // sigmoid(-sdf / (0.1 dx)) (rk3_simple_new.py:198)
+        chi = T(1) / (T(1) + exp(a.sdf[g] * a.sdf_scale));
+      } else {
+        chi = T(1) - a.mask[g];
+      }
+      s_iD[idx] = T(1) / (T(1) + a.dt * chi * a.inv_eta);
+      s_nt[idx] = has_nsgs ? a.nu + a.nu_sgs[g] : a.nu;
+    }
+  }
+  __syncthreads();
+
+  const T* fu = s_u;
+  const T* fv = s_v;
+#pragma unroll
+  for (int s = 1; s <= S; ++s) {
+    const int m = (S - s) * R1;  // output margin of this stage
+    if constexpr (HYPER) {
+      // laplacian of the stage input on margin m + 1
+      const int mm = m + 1;
+      const int W = TY + 2 * mm, Hh = TX + 2 * mm;
+      for (int idx = tid; idx < W * Hh; idx += 256) {
+        const int lx = H - mm + idx / W, ly = H - mm + idx % W;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+        const int l = lx * RW + ly;
+        const int lW = gx > 0 ? l - RW : l, lE = gx < nx - 1 ? l + RW : l;
+        const int lS = gy > 0 ? l - 1 : l, lN = gy < ny - 1 ? l + 1 : l;
+        s_lu[l] = (fu[lE] + fu[lW] - T(2) * fu[l]) * a.inv_dx2 +
+                  (fu[lN] + fu[lS] - T(2) * fu[l]) * a.inv_dy2;
+        s_lv[l] = (fv[lE] + fv[lW] - T(2) * fv[l]) * a.inv_dx2 +
+                  (fv[lN] + fv[lS] - T(2) * fv[l]) * a.inv_dy2;
+      }
+      __syncthreads();
+    }
+    T* ou = (s == 1) ? s_au : s_bu;
+    T* ov = (s == 1) ? s_av : s_bv;
+    const int W = TY + 2 * m, Hh = TX + 2 * m;
+    for (int idx = tid; idx < W * Hh; idx += 256) {
+      const int lx = H - m + idx / W, ly = H - m + idx % W;
+      const int gx = gx0 + lx, gy = gy0 + ly;
+      if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+      const int l = lx * RW + ly;
+      const int lW = gx > 0 ? l - RW : l, lE = gx < nx - 1 ? l + RW : l;
+      const int lS = gy > 0 ? l - 1 : l, lN = gy < ny - 1 ? l + 1 : l;
+      T ku, kv;
+      rk_rhs<T, HYPER>(fu, fv, s_lu, s_lv, l, lW, lE, lS, lN, s_nt[l], a, ku, kv);
+      const T u0 = s_u[l], v0 = s_v[l], iD = s_iD[l];
+      T nu_, nv_;
+      if constexpr (FAST) {
+        if (s == 1) {
+          nu_ = (u0 + a.dt * ku) * iD;
+          nv_ = (v0 + a.dt * kv) * iD;
+          // k1 is needed again at the final stage (margin 0 within margin R1)
+          s_bu[l] = ku;
+          s_bv[l] = kv;
+        } else {
+          const T k1u = s_bu[l], k1v = s_bv[l];
+          nu_ = (u0 + T(0.5) * a.dt * (k1u + ku)) * iD;
+          nv_ = (v0 + T(0.5) * a.dt * (k1v + kv)) * iD;
+        }
+      } else {
+        if (s == 1) {
+          nu_ = (u0 + a.dt * ku) * iD;
+          nv_ = (v0 + a.dt * kv) * iD;
+        } else {
+          const T us = u0 + a.dt * ku, vs = v0 + a.dt * kv;
+          if (s == 2) {
+            nu_ = (T(0.75) * u0 + T(0.25) * us) * iD;
+            nv_ = (T(0.75) * v0 + T(0.25) * vs) * iD;
+          } else {
+            nu_ = (T(1.0 / 3.0) * u0 + T(2.0 / 3.0) * us) * iD;
+            nv_ = (T(1.0 / 3.0) * v0 + T(2.0 / 3.0) * vs) * iD;
+          }
+        }
+      }
+      if (s < S) {
+        ou[l] = nu_;
+        ov[l] = nv_;
+        continue;
+      }
+      // ---- final stage: hard mask, then BCs (rk3_simple_new.py:332-336,
+      // 264-279).  Boundary cells copy the masked value of an interior
+      // source; each source writes its own targets so every boundary cell
+      // is written exactly once.
+      const bool inlet = (gx == 0);
+      const bool interior = gx > 0 && gx < nx - 1 && gy > 0 && gy < ny - 1;
+      const long g = (long)gx * ny + gy;
+      if (inlet) {
+        const bool wall = (gy == 0 || gy == ny - 1);
+        a.u_out[g] = (wall && !a.slip) ? T(0) : a.U_inf;
+        a.v_out[g] = T(0);
+        continue;
+      }
+      if (!interior) continue;
+      const T m0 = a.mask[g];
+      const T fu_ = masked(nu_, m0), fv_ = masked(nv_, m0);
+      a.u_out[g] = fu_;
+      a.v_out[g] = fv_;
+      const T uw = a.slip ? fu_ : T(0);
+      const bool out_row = (gx == nx - 2);
+      if (out_row) {
+        a.u_out[g + ny] = fu_;
+        a.v_out[g + ny] = fv_;
+      }
+      if (gy == 1) {
+        a.u_out[g - 1] = uw;
+        a.v_out[g - 1] = T(0);
+        if (out_row) { a.u_out[g + ny - 1] = uw; a.v_out[g + ny - 1] = T(0); }
+      }
+      if (gy == ny - 2) {
+        a.u_out[g + 1] = uw;
+        a.v_out[g + 1] = T(0);
+        if (out_row) { a.u_out[g + ny + 1] = uw; a.v_out[g + ny + 1] = T(0); }
+      }
+    }
+    __syncthreads();
+    fu = ou;
+    fv = ov;
+  }
+}
+
+template <typename T, int TX, int TY, bool FAST, bool HYPER>
+static int launch_rk(const RkArgs<T>& a, hipStream_t st) {
+  constexpr int S = FAST ? 2 : 3;
+  constexpr int R1 = HYPER ? 2 : 1;
+  constexpr int H = S * R1;
+  constexpr size_t RN = (size_t)(TX + 2 * H) * (TY + 2 * H);
+  constexpr size_t NARR = HYPER ? 10 : 8;
+  const size_t smem = RN * NARR * sizeof(T);
+  static_assert(RN * NARR * sizeof(T) <= 65536, "LDS budget");
+  dim3 grid(cdiv(a.ny, TY), cdiv(a.nx, TX));
+  hipLaunchKernelGGL((k_rk_fused<T, TX, TY, FAST, HYPER>), grid, dim3(256), smem, st, a);
+  AJ_LAUNCH_CHECK("k_rk_fused");
+  return AJ_OK;
+}
+
+template <typename T>
+static int rk_step_t(const void* u, const void* v, const void* mask, const void* sdf,
+                     const void* nu_sgs, void* u_out, void* v_out, int nx, int ny,
+                     const aj_rk_params* p, hipStream_t st) {
+  RkArgs<T> a;
+  a.u = (const T*)u; a.v = (const T*)v; a.mask = (const T*)mask;
+  a.sdf = (const T*)sdf; a.nu_sgs = (const T*)nu_sgs;
+  a.u_out = (T*)u_out; a.v_out = (T*)v_out;
+  a.nx = nx; a.ny = ny;
+  a.dt = T(p->dt); a.nu = T(p->nu);
+  a.inv_dx = T(1.0 / p->dx); a.inv_dy = T(1.0 / p->dy);
+  a.inv_dx2 = T(1.0 / (p->dx * p->dx)); a.inv_dy2 = T(1.0 / (p->dy * p->dy));
+  a.U_inf = T(p->U_inf); a.inv_eta = T(1.0 / p->brinkman_eta);
+  a.sdf_scale = T(1.0 / (0.1 * p->dx));
+  a.nu_h = T(p->nu * p->nu_hyper_ratio * (p->dx * p->dy));
+  a.slip = p->slip_walls ? 1 : 0;
+  const bool hyper = p->nu_hyper_ratio > 0.0;
+  const bool fast = p->fast_mode != 0;
+  if constexpr (sizeof(T) == 8) {
+    if (!fast && !hyper) return launch_rk<T, 8, 64, false, false>(a, st);
+    if (fast && !hyper) return launch_rk<T, 8, 64, true, false>(a, st);
+    if (!fast && hyper) return launch_rk<T, 4, 32, false, true>(a, st);
+    return launch_rk<T, 4, 32, true, true>(a, st);
+  } else {
+    if (!fast && !hyper) return launch_rk<T, 16, 64, false, false>(a, st);
+    if (fast && !hyper) return launch_rk<T, 16, 64, true, false>(a, st);
+    if (!fast && hyper) return launch_rk<T, 8, 64, false, true>(a, st);
+    return launch_rk<T, 8, 64, true, true>(a, st);
+  }
+}
+
+// --------------------------------------------------------------------------
+// Smagorinsky (solver/les_models.py:22-54, 179-200)
+// --------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_les(const T* __restrict__ u, const T* __restrict__ v,
+                                             T* __restrict__ nsgs, int nx, int ny, T inv_2dx,
+                                             T inv_2dy, T coef) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= ny) return;
+  // periodic jnp.roll neighbours (solver/operators.py:11-14, 35-38)
+  const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
+  const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
+  const long c = (long)i * ny;
+  const T du_dx = (u[(long)ip * ny + j] - u[(long)im * ny + j]) * inv_2dx;
+  const T du_dy = (u[c + jp] - u[c + jm]) * inv_2dy;
+  const T dv_dx = (v[(long)ip * ny + j] - v[(long)im * ny + j]) * inv_2dx;
+  const T dv_dy = (v[c + jp] - v[c + jm]) * inv_2dy;
+  const T sxy = T(0.5) * (du_dy + dv_dx);
+  const T smag = sqrt(T(2) * (du_dx * du_dx + dv_dy * dv_dy + T(2) * sxy * sxy) + T(1e-10));
+  nsgs[c + j] = coef * smag;
+}
+
+// --------------------------------------------------------------------------
+// rhs = divergence_nonperiodic(u*, v*) * mask / dt (operators.py:81-96,
+// step.py:194-199): zero on every boundary cell.
+// --------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_div_rhs(const T* __restrict__ us, const T* __restrict__ vs,
+                                                 const T* __restrict__ mask, T* __restrict__ rhs,
+                                                 int nx, int ny, T inv_2dx, T inv_2dy, T inv_dt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= ny) return;
+  const long g = (long)i * ny + j;
+  T r = T(0);
+  if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
+    const T d = (us[g + ny] - us[g - ny]) * inv_2dx + (vs[g + 1] - vs[g - 1]) * inv_2dy;
+    r = mask ? d * mask[g] * inv_dt : d * inv_dt;
+  }
+  rhs[g] = r;
+}
+
+// --------------------------------------------------------------------------
+// Projection (step.py:257-264, 305-349, 364-377, 394-397)
+// --------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_project(const T* __restrict__ us, const T* __restrict__ vs,
+                                                 const T* __restrict__ p, const T* __restrict__ mask,
+                                                 T* __restrict__ uo, T* __restrict__ vo, int nx,
+                                                 int ny, T dt, T inv_2dx, T inv_2dy, T iface_thr,
+                                                 T U_inf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= ny) return;
+  const long g = (long)i * ny + j;
+  if (i == 0) {
+    // inlet: u = U_inf (walls copy it), v = 0, then the final mask
+    const T m0 = mask[g];
+    uo[g] = masked(U_inf, m0);
+    vo[g] = T(0);
+    return;
+  }
+  if (!(i < nx - 1 && j > 0 && j < ny - 1)) return;  // written by a source
+  const T gx0 = (p[g + ny] - p[g - ny]) * inv_2dx;
+  const T gy0 = (p[g + 1] - p[g - 1]) * inv_2dy;
+  const T mx = (mask[g + ny] - mask[g - ny]) * inv_2dx;
+  const T my = (mask[g + 1] - mask[g - 1]) * inv_2dy;
+  const T gm = sqrt(mx * mx + my * my);
+  T gx = gx0, gy = gy0;
+  if (gm > iface_thr) {
+    const T nxv = mx / (gm + T(1e-8));
+    const T nyv = my / (gm + T(1e-8));
+    const T dpn = gx0 * nxv + gy0 * nyv;
+    gx = gx0 - dpn * nxv;
+    gy = gy0 - dpn * nyv;
+  }
+  const T m0 = mask[g];
+  const T un = masked(us[g] - dt * gx, m0);
+  const T vn = masked(vs[g] - dt * gy, m0);
+  uo[g] = un;  // second masking is idempotent on the source cell
+  vo[g] = vn;
+  const bool out_row = (i == nx - 2);
+  if (out_row) {
+    uo[g + ny] = masked(un, mask[g + ny]);
+    vo[g + ny] = masked(vn, mask[g + ny]);
+  }
+  if (j == 1) {
+    uo[g - 1] = masked(un, mask[g - 1]);
+    vo[g - 1] = T(0);
+    if (out_row) { uo[g + ny - 1] = masked(un, mask[g + ny - 1]); vo[g + ny - 1] = T(0); }
+  }
+  if (j == ny - 2) {
+    uo[g + 1] = masked(un, mask[g + 1]);
+    vo[g + 1] = T(0);
+    if (out_row) { uo[g + ny + 1] = masked(un, mask[g + ny + 1]); vo[g + ny + 1] = T(0); }
+  }
+}
+
+static dim3 row_grid(int nx, int ny) { return dim3(cdiv(ny, 256), nx); }
+
+static int check_grid(int nx, int ny) {
+  if (nx < 4 || ny < 4) return fail(AJ_ERR_INVALID, "grid must be at least 4x4");
+  if ((long)nx * ny > (1L << 31)) return fail(AJ_ERR_INVALID, "grid too large");
+  return AJ_OK;
+}
+
+}  // namespace aj
+
+using namespace aj;
+
+extern "C" int aj_rk_step(int dtype, const void* u, const void* v, const void* mask,
+                          const void* sdf, const void* nu_sgs, void* u_out, void* v_out,
+                          int nx, int ny, const aj_rk_params* prm, void* stream) {
+  if (!u || !v || !mask || !u_out || !v_out || !prm) return fail(AJ_ERR_INVALID, "aj_rk_step: null pointer");
+  if (int rc = check_grid(nx, ny)) return rc;
+  if (!(prm->brinkman_eta > 0)) return fail(AJ_ERR_INVALID, "aj_rk_step: brinkman_eta must be > 0");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == AJ_F32) return rk_step_t<float>(u, v, mask, sdf, nu_sgs, u_out, v_out, nx, ny, prm, st);
+  if (dtype == AJ_F64) return rk_step_t<double>(u, v, mask, sdf, nu_sgs, u_out, v_out, nx, ny, prm, st);
+  return fail(AJ_ERR_INVALID, "aj_rk_step: bad dtype");
+}
+
+extern "C" int aj_les_smagorinsky(int dtype, const void* u, const void* v, void* nu_sgs, int nx,
+                                  int ny, double dx, double dy, double delta, double c_s,
+                                  void* stream) {
+  if (!u || !v || !nu_sgs) return fail(AJ_ERR_INVALID, "aj_les_smagorinsky: null pointer");
+  if (int rc = check_grid(nx, ny)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const double coef = (c_s * delta) * (c_s * delta);
+  if (dtype == AJ_F32) {
+    hipLaunchKernelGGL(k_les<float>, row_grid(nx, ny), dim3(256), 0, st, (const float*)u,
+                       (const float*)v, (float*)nu_sgs, nx, ny, float(1.0 / (2.0 * dx)),
+                       float(1.0 / (2.0 * dy)), float(coef));
+  } else if (dtype == AJ_F64) {
+    hipLaunchKernelGGL(k_les<double>, row_grid(nx, ny), dim3(256), 0, st, (const double*)u,
+                       (const double*)v, (double*)nu_sgs, nx, ny, 1.0 / (2.0 * dx),
+                       1.0 / (2.0 * dy), coef);
+  } else {
+    return fail(AJ_ERR_INVALID, "aj_les_smagorinsky: bad dtype");
+  }
+  AJ_LAUNCH_CHECK("k_les");
+  return AJ_OK;
+}
+
+extern "C" int aj_divergence_rhs(int dtype, const void* us, const void* vs, const void* mask,
+                                 void* rhs, int nx, int ny, double dx, double dy, double dt,
+                                 void* stream) {
+  if (!us || !vs || !rhs) return fail(AJ_ERR_INVALID, "aj_divergence_rhs: null pointer");
+  if (int rc = check_grid(nx, ny)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == AJ_F32) {
+    hipLaunchKernelGGL(k_div_rhs<float>, row_grid(nx, ny), dim3(256), 0, st, (const float*)us,
+                       (const float*)vs, (const float*)mask, (float*)rhs, nx, ny,
+                       float(1.0 / (2.0 * dx)), float(1.0 / (2.0 * dy)), float(1.0 / dt));
+  } else if (dtype == AJ_F64) {
+    hipLaunchKernelGGL(k_div_rhs<double>, row_grid(nx, ny), dim3(256), 0, st, (const double*)us,
+                       (const double*)vs, (const double*)mask, (double*)rhs, nx, ny,
+                       1.0 / (2.0 * dx), 1.0 / (2.0 * dy), 1.0 / dt);
+  } else {
+    return fail(AJ_ERR_INVALID, "aj_divergence_rhs: bad dtype");
+  }
+  AJ_LAUNCH_CHECK("k_div_rhs");
+  return AJ_OK;
+}
+
+extern "C" int aj_project(int dtype, const void* us, const void* vs, const void* p,
+                          const void* mask, void* uo, void* vo, int nx, int ny, double dt,
+                          double dx, double dy, double U_inf, void* stream) {
+  if (!us || !vs || !p || !mask || !uo || !vo) return fail(AJ_ERR_INVALID, "aj_project: null pointer");
+  if (int rc = check_grid(nx, ny)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == AJ_F32) {
+    hipLaunchKernelGGL(k_project<float>, row_grid(nx, ny), dim3(256), 0, st, (const float*)us,
+                       (const float*)vs, (const float*)p, (const float*)mask, (float*)uo,
+                       (float*)vo, nx, ny, float(dt), float(1.0 / (2.0 * dx)),
+                       float(1.0 / (2.0 * dy)), float(0.1 / dx), float(U_inf));
+  } else if (dtype == AJ_F64) {
+    hipLaunchKernelGGL(k_project<double>, row_grid(nx, ny), dim3(256), 0, st, (const double*)us,
+                       (const double*)vs, (const double*)p, (const double*)mask, (double*)uo,
+                       (double*)vo, nx, ny, dt, 1.0 / (2.0 * dx), 1.0 / (2.0 * dy), 0.1 / dx,
+                       U_inf);
+  } else {
+    return fail(AJ_ERR_INVALID, "aj_project: bad dtype");
+  }
+  AJ_LAUNCH_CHECK("k_project");
+  return AJ_OK;
+}

@@ -1,0 +1,170 @@
+"""One projection timestep: ``step_pure`` with the reference signature and a
+preallocated ``CollocatedStepper`` for repeated stepping.
+
+Mirrors solver/pure_functions/step.py:49-467 (collocated grid, von Karman
+flow): [LES] -> rk_step_unified -> predictor BCs -> rhs = div*mask/dt ->
+poisson_multigrid -> projection.  Every arithmetic stage is a HIP kernel.
+
+The predictor BCs of step.py:182-191 (inlet + slip walls on u*, v*) are
+already the last thing ``rk_step_unified`` applies (rk3_simple_new.py:
+264-279 with slip walls), so re-imposing them is the identity and no kernel
+is spent on it.  With ``use_les`` the reference computes nu_sgs and then
+passes ``nu_sgs=None`` to the RK step (step.py:177-180), i.e. the eddy
+viscosity never reaches the flow; ``les_coupling="discard"`` (default)
+reproduces that and skips the dead computation, ``"apply"`` feeds it to the
+RK step (the RK function's own contract, rk3_simple_new.py:304).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from . import _arrays as A
+from .advection_schemes import rk_step_unified
+from .operators import constant_smagorinsky, pressure_rhs, project
+from .pressure_solvers import get_plan
+
+
+@dataclass
+class SimState:
+    """Simulation state (solver/params.py:11-30)."""
+    u: torch.Tensor
+    v: torch.Tensor
+    p: torch.Tensor
+    u_prev: torch.Tensor
+    v_prev: torch.Tensor
+    c: torch.Tensor
+    dt: float
+    iteration: int
+    grid_type: str = "collocated"
+    integral: float = 0.0
+    prev_error: float = 0.0
+
+
+def initial_state(u, v, dt: float, device=None) -> SimState:
+    ut = A.as_device(u, A.torch_dtype_of(u), device, "u")
+    vt = A.as_device(v, ut.dtype, ut.device, "v")
+    z = torch.zeros_like(ut)
+    return SimState(u=ut, v=vt, p=z.clone(), u_prev=ut.clone(), v_prev=vt.clone(), c=z.clone(),
+                    dt=float(dt), iteration=0)
+
+
+def step_pure(state: SimState, mask, sdf, dx: float, dy: float, nu: float, U_inf: float,
+              use_les: bool = False, smagorinsky_constant: float = 0.1,
+              weno_epsilon: float = 1e-6, eps: float = 0.01, adaptive_dt: bool = False,
+              dt_min: float = 5e-4, dt_max: float = 0.01, target_div: float = 1e-4,
+              Kp: float = 0.5, Ki: float = 0.05, Kd: float = 0.1,
+              flow_type: str = "von_karman", advection_scheme: str = "rk3",
+              pressure_solver: str = "multigrid", les_model: str = "smagorinsky",
+              limiter: str = "minmod", v_cycles: int = 1, fast_mode: bool = False,
+              brinkman_eta: float = 0.005, grid_type: str = "collocated",
+              verbose: bool = False, nn_pressure_model=None, *,
+              les_coupling: str = "discard", mg_variant: str = "reference",
+              tolerance: float = 1e-6, max_cycles: int = 40, warm_start=None) -> SimState:
+    """One timestep; arguments as step_pure (step.py:49-66).
+
+    Extensions (keyword-only): ``les_coupling`` ("discard" | "apply"),
+    ``mg_variant`` ("reference": ``v_cycles`` reference V-cycles;
+    "textbook": solve to ``||div|| <= tolerance``), ``warm_start``: callable
+    ``(rhs, sdf) -> p0`` used as the multigrid initial guess.
+    """
+    if grid_type != "collocated":
+        raise NotImplementedError("only grid_type='collocated' is implemented")
+    if flow_type != "von_karman":
+        raise NotImplementedError("only flow_type='von_karman' is implemented")
+    if adaptive_dt:
+        raise NotImplementedError("adaptive_dt: the PID dt controller is not implemented")
+    if use_les and les_model != "smagorinsky":
+        raise NotImplementedError("only les_model='smagorinsky' is implemented")
+    u, v, dt = state.u, state.v, state.dt
+    mt = A.as_device(mask, u.dtype, u.device, "mask")
+    st = A.as_device(sdf, u.dtype, u.device, "sdf")
+    nu_sgs = None
+    if use_les and les_coupling == "apply":
+        delta = (dx * dy) ** 0.5
+        nu_sgs = constant_smagorinsky(u, v, dx, dy, delta, smagorinsky_constant)
+    us, vs = rk_step_unified(u, v, dt, nu, dx, dy, mt, sdf=st, U_inf=U_inf, nu_sgs=nu_sgs,
+                             nu_hyper_ratio=0.0, slip_walls=True, fast_mode=fast_mode,
+                             brinkman_eta=brinkman_eta, flow_type=flow_type)
+    rhs = pressure_rhs(us, vs, mt, dt, dx, dy)
+    nx, ny = u.shape
+    if pressure_solver == "nn" and nn_pressure_model is not None:
+        p = nn_pressure_model(rhs, mt)
+    elif pressure_solver == "multigrid":
+        p0 = warm_start(rhs, st) if warm_start is not None else None
+        p = torch.empty_like(rhs)
+        if mg_variant == "reference":
+            plan = get_plan(nx, ny, dx, dy, u.dtype, "reference", 4, u.device)
+            plan.solve(rhs, p, p0, v_cycles, tolerance)
+        else:
+            plan = get_plan(nx, ny, dx, dy, u.dtype, "textbook", 0, u.device)
+            plan.solve(rhs, p, p0, max_cycles, tolerance)
+    else:
+        raise NotImplementedError(f"pressure_solver={pressure_solver!r}")
+    un, vn = project(us, vs, p, mt, dt, dx, dy, U_inf)
+    return SimState(u=un, v=vn, p=p, u_prev=u, v_prev=v, c=state.c, dt=dt,
+                    iteration=state.iteration + 1, integral=state.integral,
+                    prev_error=state.prev_error)
+
+
+class CollocatedStepper:
+    """Repeated steps on preallocated device buffers (no allocation per
+    step): ping-pong velocity buffers, one multigrid plan."""
+
+    def __init__(self, nx, ny, dx, dy, mask, sdf, nu, U_inf, dt, *, dtype=torch.float32,
+                 device="cuda", brinkman_eta=0.005, v_cycles=3, mg_variant="reference",
+                 tolerance=1e-6, max_cycles=40, fast_mode=False, use_les=False,
+                 les_coupling="discard", smagorinsky_constant=0.1):
+        A.require_gpu()
+        self.nx, self.ny, self.dx, self.dy = nx, ny, float(dx), float(dy)
+        self.nu, self.U_inf, self.dt = float(nu), float(U_inf), float(dt)
+        self.eta, self.v_cycles = float(brinkman_eta), int(v_cycles)
+        self.mg_variant, self.tolerance, self.max_cycles = mg_variant, tolerance, max_cycles
+        self.fast_mode = fast_mode
+        self.les = use_les and les_coupling == "apply"
+        self.cs = smagorinsky_constant
+        dev = torch.device(device)
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        self.mask = A.as_device(mask, dtype, dev, "mask")
+        self.sdf = A.as_device(sdf, dtype, dev, "sdf")
+        e = lambda: torch.empty((nx, ny), dtype=dtype, device=dev)  # noqa: E731
+        self.us, self.vs, self.rhs, self.p = e(), e(), e(), e()
+        self.nsgs = e() if self.les else None
+        self.u = [e(), e()]
+        self.v = [e(), e()]
+        self.cur = 0
+        variant = "reference" if mg_variant == "reference" else "textbook"
+        self.plan = get_plan(nx, ny, dx, dy, dtype, variant, 4 if variant == "reference" else 0,
+                             dev)
+
+    def load(self, u, v):
+        self.u[self.cur].copy_(A.as_device(u, self.u[0].dtype, self.device, "u"))
+        self.v[self.cur].copy_(A.as_device(v, self.u[0].dtype, self.device, "v"))
+
+    @property
+    def state(self):
+        return self.u[self.cur], self.v[self.cur]
+
+    def step(self, p0=None):
+        u, v = self.u[self.cur], self.v[self.cur]
+        un, vn = self.u[1 - self.cur], self.v[1 - self.cur]
+        nsgs = None
+        if self.les:
+            delta = (self.dx * self.dy) ** 0.5
+            constant_smagorinsky(u, v, self.dx, self.dy, delta, self.cs, out=self.nsgs)
+            nsgs = self.nsgs
+        rk_step_unified(u, v, self.dt, self.nu, self.dx, self.dy, self.mask, sdf=self.sdf,
+                        U_inf=self.U_inf, nu_sgs=nsgs, fast_mode=self.fast_mode,
+                        brinkman_eta=self.eta, out=(self.us, self.vs))
+        pressure_rhs(self.us, self.vs, self.mask, self.dt, self.dx, self.dy, out=self.rhs)
+        if self.mg_variant == "reference":
+            self.plan.solve(self.rhs, self.p, p0, self.v_cycles, self.tolerance)
+        else:
+            self.plan.solve(self.rhs, self.p, p0, self.max_cycles, self.tolerance)
+        project(self.us, self.vs, self.p, self.mask, self.dt, self.dx, self.dy, self.U_inf,
+                out=(un, vn))
+        self.cur = 1 - self.cur
+        return un, vn

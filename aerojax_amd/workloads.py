@@ -1,0 +1,72 @@
+"""Synthetic workloads (seeded inputs) for the benchmark and the tests.
+
+Each function returns a dict with the grid, the obstacle SDF and mask
+(float64 host arrays), the initial velocity (float64, seeded noise), the
+viscosity and the time step.  Domains the configurations leave open are
+fixed here: the 2048x512 learned-warm-start case uses a 4x1 channel
+(dx = dy = 1/512).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import geometry as G
+
+
+def small_airfoil(noise_sigma=1e-3, seed=0):
+    """512x96 on 4x0.75, NACA 0012 chord 0.5 at 10 deg, LE at (1, 0.375),
+    mask eps 0.05, Re 1000 (nu = U c / Re = 5e-4), dt 1e-3, 3 V-cycles."""
+    grid = G.GridParams(512, 96, 4.0, 0.75)
+    geo = G.naca_geometry(grid, "0012", chord=0.5, aoa_deg=10.0, x_le=1.0, y_le=0.375, eps=0.05)
+    u, v = G.von_karman_ic(grid.nx, grid.ny, 1.0, noise_sigma, seed)
+    return dict(name="naca0012_512x96", grid=grid, sdf=np.nan_to_num(geo.sdf, nan=0.0),
+                mask=geo.mask, u=u, v=v, nu=G.viscosity(1.0, 0.5, 1000.0), dt=1e-3,
+                U_inf=1.0, v_cycles=3, les=False)
+
+
+def large_cylinder(noise_sigma=1e-3, seed=0):
+    """4096x1024 on 8x2, cylinder r = 0.1 at (1.5, 1.0), eps 0.05,
+    Re 1e5 (nu = U 2r / Re = 2e-6), Smagorinsky Cs = 0.17, dt 1e-4."""
+    grid = G.GridParams(4096, 1024, 8.0, 2.0)
+    geo = G.cylinder_geometry(grid, 1.5, 1.0, 0.1, eps=0.05)
+    u, v = G.von_karman_ic(grid.nx, grid.ny, 1.0, noise_sigma, seed)
+    return dict(name="cylinder_4096x1024", grid=grid, sdf=geo.sdf, mask=geo.mask, u=u, v=v,
+                nu=G.viscosity(1.0, 0.2, 1e5), dt=1e-4, U_inf=1.0, les=True, cs=0.17)
+
+
+def random_obstacle(grid, rng, eps=0.05):
+    """Cylinder with radius in [0.05, 0.15] or NACA 4-digit with thickness
+    6-18 % and AoA in [-15, 15] deg (drawn from ``rng``)."""
+    if rng.random() < 0.5:
+        r = rng.uniform(0.05, 0.15)
+        cx = rng.uniform(0.6, 1.2)
+        cy = grid.ly / 2 + rng.uniform(-0.1, 0.1)
+        geo = G.cylinder_geometry(grid, cx, cy, r, eps=eps)
+        desc = f"cylinder r={r:.3f} at ({cx:.2f},{cy:.2f})"
+    else:
+        tpct = int(rng.integers(6, 19))
+        m = int(rng.integers(0, 7))
+        p = int(rng.integers(2, 7)) if m > 0 else 0
+        des = f"{m}{p}{tpct:02d}"
+        aoa = rng.uniform(-15.0, 15.0)
+        chord = rng.uniform(0.3, 0.6)
+        x_le = rng.uniform(0.5, 1.0)
+        geo = G.naca_geometry(grid, des, chord=chord, aoa_deg=aoa, x_le=x_le,
+                              y_le=grid.ly / 2, eps=eps)
+        desc = f"NACA {des} c={chord:.2f} aoa={aoa:.1f}"
+    return geo, desc
+
+
+def warm_start_case(seed=0, nx=2048, ny=512, noise_sigma=1e-3):
+    """2048x512 on a 4x1 channel, geometry drawn per seed, Re log-uniform in
+    [500, 5e4], dt 1e-3, solve to ||div|| <= 1e-6."""
+    rng = np.random.default_rng(seed)
+    grid = G.GridParams(nx, ny, 4.0, 1.0)
+    geo, desc = random_obstacle(grid, rng)
+    Re = math.exp(rng.uniform(math.log(500.0), math.log(5e4)))
+    u, v = G.von_karman_ic(nx, ny, 1.0, noise_sigma, seed)
+    return dict(name=f"warmstart_{nx}x{ny}_seed{seed}", desc=desc, grid=grid,
+                sdf=np.nan_to_num(geo.sdf, nan=0.0), mask=geo.mask, u=u, v=v,
+                nu=G.viscosity(1.0, geo.L_char, Re), Re=Re, dt=1e-3, U_inf=1.0, tol=1e-6)

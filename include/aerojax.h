@@ -1,0 +1,154 @@
+/*
+ * aerojax.h -- C ABI of the MI355X (gfx950) implementation of AeroJAX's
+ * collocated projection-method timestep.
+ *
+ * The reference (arriemeijer-creator/AeroJAX) is pure Python/JAX; its
+ * "plugin API" for this path is two Python call signatures resolved by
+ * import:
+ *   rk_step_unified   advection_schemes/rk3_simple_new.py:175-180
+ *   poisson_multigrid pressure_solvers/multigrid_solver.py:55-56
+ * composed by step_pure (solver/pure_functions/step.py:49-467).  Each entry
+ * point below replaces one jitted JAX expression of that path; the Python
+ * package aerojax_amd binds them through ctypes and keeps the reference
+ * signatures.
+ *
+ * Conventions
+ *   - Every field is (nx, ny) row-major with y contiguous (index i*ny + j),
+ *     i = 0 inlet row, i = nx-1 outlet row, j = 0 / ny-1 wall columns
+ *     (solver/params.py:54-56, indexing='ij').
+ *   - Pointers are device pointers (hipMalloc / torch tensors); buffers are
+ *     caller-owned, outputs must not alias inputs.  `dtype` selects the
+ *     element type of every field argument: AJ_F32 (float) or AJ_F64
+ *     (double).
+ *   - `stream` is a hipStream_t (NULL = default stream).  Calls are
+ *     stream-ordered and asynchronous unless stated otherwise.
+ *   - Return value: AJ_OK (0) or an AJ_ERR_* code; aj_last_error() gives the
+ *     message of the last failure on the calling thread.
+ */
+#ifndef AEROJAX_H_
+#define AEROJAX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  AJ_OK = 0,
+  AJ_ERR_INVALID = 1,     /* bad argument (null pointer, size, dtype)      */
+  AJ_ERR_GRID = 2,        /* grid not suitable for multigrid               */
+  AJ_ERR_HIP = 3,         /* HIP runtime error                             */
+  AJ_ERR_NONFINITE = 4,   /* NaN/Inf detected                              */
+  AJ_ERR_UNSUPPORTED = 5  /* option outside this implementation            */
+};
+
+enum { AJ_F32 = 0, AJ_F64 = 1 };
+
+/* Library version string, e.g. "aerojax-amd 0.1.0 gfx950". */
+const char* aj_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* aj_last_error(void);
+
+/* ---- RK predictor ------------------------------------------------------
+ * Replaces rk_step_unified (advection_schemes/rk3_simple_new.py:175-358),
+ * flow_type='von_karman': RK3 (fast_mode=0) or RK2/Heun (fast_mode=1) of
+ * first-order upwind advection + nu*Laplacian (+ nu_sgs, + hyperviscosity
+ * -nu*ratio*dx*dy*lap(lap)), implicit Brinkman factor 1 + dt*chi/eta with
+ * chi = sigmoid(-sdf/(0.1 dx)) (sdf != NULL) or 1 - mask, hard mask
+ * (mask > 0.5) and inlet/outlet/wall BCs.  All stages run in one fused
+ * kernel on LDS tiles.  sdf and nu_sgs may be NULL.
+ */
+typedef struct aj_rk_params {
+  double dt, nu, dx, dy, U_inf, brinkman_eta, nu_hyper_ratio;
+  int slip_walls; /* 1 = slip walls (reference default), 0 = no-slip */
+  int fast_mode;  /* 0 = RK3, 1 = RK2 */
+} aj_rk_params;
+
+int aj_rk_step(int dtype, const void* u, const void* v, const void* mask,
+               const void* sdf, const void* nu_sgs, void* u_out, void* v_out,
+               int nx, int ny, const aj_rk_params* prm, void* stream);
+
+/* ---- Smagorinsky eddy viscosity ----------------------------------------
+ * Replaces constant_smagorinsky + compute_strain_rate
+ * (solver/les_models.py:22-54, 179-200): nu_sgs = (c_s*delta)^2 *
+ * sqrt(2(Sxx^2+Syy^2+2Sxy^2) + 1e-10), periodic central differences.
+ */
+int aj_les_smagorinsky(int dtype, const void* u, const void* v, void* nu_sgs,
+                       int nx, int ny, double dx, double dy, double delta,
+                       double c_s, void* stream);
+
+/* ---- Pressure right-hand side -----------------------------------------
+ * Replaces divergence_nonperiodic (solver/operators.py:81-96) followed by
+ * rhs = div*mask/dt (solver/pure_functions/step.py:194-199).  mask == NULL
+ * and dt == 1 give the bare divergence.
+ */
+int aj_divergence_rhs(int dtype, const void* u_star, const void* v_star,
+                      const void* mask, void* rhs, int nx, int ny, double dx,
+                      double dy, double dt, void* stream);
+
+/* ---- Projection ---------------------------------------------------------
+ * Replaces the projection tail of step_pure (step.py:257-264, 305-349,
+ * 364-377, 394-397): grad p, tangential projection at interface cells
+ * (|grad mask| > 0.1/dx), dp/dx = 0 on the inlet row, u = u* - dt grad p,
+ * mask, slip/inlet/outlet BCs, mask.
+ */
+int aj_project(int dtype, const void* u_star, const void* v_star, const void* p,
+               const void* mask, void* u_out, void* v_out, int nx, int ny,
+               double dt, double dx, double dy, double U_inf, void* stream);
+
+/* ---- Multigrid pressure Poisson ----------------------------------------
+ * A plan owns the level hierarchy workspace (allocated once, hipMalloc)
+ * for one grid shape and spacing.
+ *
+ * AJ_MG_REFERENCE: restates poisson_multigrid
+ *   (pressure_solvers/multigrid_solver.py:55-241): b[-1,:] = 0, V-cycles of
+ *   2+2 Jacobi sweeps with fine-grid spacing on every level, 2x2 box
+ *   restriction, the reference prolongation, 10 sweeps on the coarsest
+ *   level, outlet clamp, convergence test sum(r^2)*dx*dy < tol^2 before
+ *   every further cycle (evaluated on the device, no host sync).
+ *   `levels` as in the reference (default 4).
+ * AJ_MG_TEXTBOOK: this project's to-tolerance solver for the consistent
+ *   outlet-Dirichlet operator: FP32 V(2,2) cycles (red-black Gauss-Seidel,
+ *   cell-centred transfers, dense coarsest solve) inside an FP64 defect
+ *   correction; stops when sqrt(sum_{i<nx-1} r^2 dx dy) <= tolerance with
+ *   r = b - A p in FP64.  `levels` is ignored (coarsens to <= 64 cells).
+ */
+typedef struct aj_mg_plan aj_mg_plan;
+enum { AJ_MG_REFERENCE = 0, AJ_MG_TEXTBOOK = 1 };
+
+int aj_mg_create(aj_mg_plan** plan, int dtype, int variant, int nx, int ny,
+                 double dx, double dy, int levels);
+int aj_mg_destroy(aj_mg_plan* plan);
+size_t aj_mg_workspace_bytes(const aj_mg_plan* plan);
+/* Number of levels of the plan's hierarchy (coarsest included). */
+int aj_mg_num_levels(const aj_mg_plan* plan);
+
+#define AJ_MG_MAX_NORMS 64
+typedef struct aj_mg_stats {
+  int cycles;      /* V-cycles executed                                  */
+  int outer;       /* textbook: defect-correction iterations; reference: */
+                   /* residual evaluations (one per cycle slot)       */
+  int converged;   /* tolerance reached                                  */
+  int nonfinite;   /* a residual norm was NaN/Inf                        */
+  double norm;     /* last residual norm (reference: sum r^2 dx dy;       */
+                   /* textbook: sqrt(sum r^2 dx dy))                      */
+  double norms[AJ_MG_MAX_NORMS]; /* reference: per cycle slot; textbook: */
+                                 /* [0] initial, [k] after outer iter k  */
+} aj_mg_stats;
+
+/* Solve A p = rhs.  p0 (may be NULL = zero guess, the reference behaviour,
+ * multigrid_solver.py:219) is the initial guess; p_out receives p.
+ * REFERENCE: `max_cycles` = v_cycles.  TEXTBOOK: `max_cycles` bounds the
+ * number of FP32 V-cycles (2 per outer iteration). */
+int aj_poisson_mg(aj_mg_plan* plan, const void* rhs, const void* p0,
+                  void* p_out, int max_cycles, double tolerance, void* stream);
+/* Copy the statistics of the last aj_poisson_mg call on this plan
+ * (synchronises `stream`). */
+int aj_mg_get_stats(aj_mg_plan* plan, aj_mg_stats* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AEROJAX_H_ */

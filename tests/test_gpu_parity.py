@@ -1,0 +1,287 @@
+"""GPU parity of every HIP entry point against the CPU oracle.
+
+Bar: relative L2 <= 1e-5 for FP32 kernels and <= 1e-12 for FP64 kernels,
+against the FP64 oracle (FP32 results are also compared with the FP32
+oracle); obstacle masks are shared inputs, and the hard-mask zero pattern
+must match exactly."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import reference_step as R
+from oracle import textbook_mg as T
+
+pytestmark = pytest.mark.gpu
+
+TOL = {torch.float32: 1e-5, torch.float64: 1e-12}
+NP = {torch.float32: np.float32, torch.float64: np.float64}
+
+
+def dev(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def host(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def case(nx, ny, lx, ly, kind="naca", seed=0, noise=1e-3):
+    dx, dy, X, Y = R.grid_coords(nx, ny, lx, ly)
+    if kind == "naca":
+        m, p, t = R.parse_naca_4digit("0012")
+        sdf = R.naca_sdf(X, Y, 0.5 * lx / 4, 10.0, 0.25 * lx, 0.5 * ly, m, p, t)
+    else:
+        sdf = R.cylinder_sdf(X, Y, 0.3 * lx, 0.5 * ly, 0.1 * ly)
+    sdf = np.nan_to_num(sdf, nan=0.0)
+    mask = R.mask_from_sdf(sdf, 0.05 * ly)
+    u, v = R.von_karman_ic(nx, ny, 1.0, noise, seed=seed, dtype=np.float64)
+    # a wake-like perturbation so that advection and diffusion are exercised
+    rng = np.random.default_rng(seed + 1)
+    u = u + 0.2 * np.sin(3 * X) * np.cos(2 * Y) + 0.01 * rng.normal(size=u.shape)
+    v = v + 0.1 * np.cos(5 * X) * np.sin(4 * Y)
+    return dict(dx=dx, dy=dy, X=X, Y=Y, sdf=sdf, mask=mask, u=u, v=v)
+
+
+SHAPES = [(512, 96, 4.0, 0.75), (130, 70, 2.0, 1.0), (2048, 512, 4.0, 1.0)]
+DTYPES = [torch.float32, torch.float64]
+
+
+def check(got, ref64, ref_dt, dtype, what):
+    tol = TOL[dtype]
+    e64 = R.rel_l2(got, ref64)
+    print(f"{what}: rel_l2 vs fp64 oracle = {e64:.3e}", flush=True)
+    assert e64 <= tol, (what, e64)
+    if ref_dt is not None:
+        assert R.rel_l2(got, ref_dt) <= tol, what
+
+
+# ------------------------------------------------------------------ RK
+
+@pytest.mark.parametrize("shape", SHAPES[:2])
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("variant", ["sdf", "nosdf", "fast", "nsgs", "hyper", "noslip"])
+def test_rk_step_parity(gpu, shape, dtype, variant):
+    import aerojax_amd as aj
+    c = case(*shape)
+    dt, nu, eta = 1e-3, 5e-4, 0.005
+    kw = dict(U_inf=1.0, brinkman_eta=eta)
+    sdf = None if variant == "nosdf" else c["sdf"]
+    nsgs = None
+    if variant == "nsgs":
+        nsgs = R.constant_smagorinsky(c["u"], c["v"], c["dx"], c["dy"], math.sqrt(c["dx"] * c["dy"]), 0.17)
+        nsgs = nsgs * 50.0  # make the term matter
+    if variant == "fast":
+        kw["fast_mode"] = True
+    if variant == "hyper":
+        kw["nu_hyper_ratio"] = 0.5
+    if variant == "noslip":
+        kw["slip_walls"] = False
+    ref = R.rk_step_unified(c["u"], c["v"], dt, nu, c["dx"], c["dy"], c["mask"], sdf=sdf,
+                            nu_sgs=nsgs, **kw)
+    f = NP[dtype]
+    refd = R.rk_step_unified(c["u"].astype(f), c["v"].astype(f), dt, nu, c["dx"], c["dy"],
+                             c["mask"].astype(f), sdf=None if sdf is None else sdf.astype(f),
+                             nu_sgs=None if nsgs is None else nsgs.astype(f), **kw)
+    out = aj.rk_step_unified(dev(c["u"], dtype), dev(c["v"], dtype), dt, nu, c["dx"], c["dy"],
+                             dev(c["mask"], dtype), sdf=None if sdf is None else dev(sdf, dtype),
+                             nu_sgs=None if nsgs is None else dev(nsgs, dtype), **kw)
+    torch.cuda.synchronize()
+    for k, name in enumerate(("u*", "v*")):
+        g = host(out[k])
+        check(g, ref[k], refd[k], dtype, f"rk {variant} {shape[:2]} {dtype} {name}")
+        solid = c["mask"] <= 0.5
+        assert np.all(g[1:-1, 1:-1][solid[1:-1, 1:-1]] == 0.0)
+
+
+def test_rk_full_size_uniform_flow_fixed_point(gpu):
+    import aerojax_amd as aj
+    nx, ny = 2048, 512
+    u = torch.full((nx, ny), 1.0, device="cuda")
+    v = torch.zeros((nx, ny), device="cuda")
+    m = torch.ones((nx, ny), device="cuda")
+    us, vs = aj.rk_step_unified(u, v, 1e-3, 1e-3, 1 / 512, 1 / 512, m, U_inf=1.0)
+    assert torch.equal(us, u) and torch.equal(vs, v)
+
+
+# ------------------------------------------------------------------ operators
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_les_div_project_parity(gpu, shape, dtype):
+    import aerojax_amd as aj
+    c = case(*shape)
+    dx, dy, dt = c["dx"], c["dy"], 1e-3
+    f = NP[dtype]
+    delta = math.sqrt(dx * dy)
+    ref = R.constant_smagorinsky(c["u"], c["v"], dx, dy, delta, 0.17)
+    got = aj.constant_smagorinsky(dev(c["u"], dtype), dev(c["v"], dtype), dx, dy, delta, 0.17)
+    check(host(got), ref, None, dtype, f"les {shape[:2]}")
+    ref = R.pressure_rhs(c["u"], c["v"], c["mask"], dt, dx, dy)
+    got = aj.pressure_rhs(dev(c["u"], dtype), dev(c["v"], dtype), dev(c["mask"], dtype), dt, dx, dy)
+    check(host(got), ref, None, dtype, f"rhs {shape[:2]}")
+    got = aj.divergence_nonperiodic(dev(c["u"], dtype), dev(c["v"], dtype), dx, dy)
+    check(host(got), R.divergence_nonperiodic(c["u"], c["v"], dx, dy), None, dtype, "div")
+    p = 5.0 * np.sin(2 * c["X"]) * np.cos(3 * c["Y"])
+    ref = R.project(c["u"], c["v"], p, c["mask"], dt, dx, dy, 1.0)
+    refd = R.project(c["u"].astype(f), c["v"].astype(f), p.astype(f), c["mask"].astype(f), dt, dx,
+                     dy, 1.0)
+    got = aj.project(dev(c["u"], dtype), dev(c["v"], dtype), dev(p, dtype), dev(c["mask"], dtype),
+                     dt, dx, dy, 1.0)
+    for k in range(2):
+        check(host(got[k]), ref[k], refd[k], dtype, f"project {shape[:2]} {k}")
+
+
+# ------------------------------------------------------------------ multigrid
+
+@pytest.mark.parametrize("shape", [(512, 96), (2048, 512), (64, 32)])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_reference_multigrid_parity(gpu, shape, dtype):
+    import aerojax_amd as aj
+    nx, ny = shape
+    dx, dy = 4.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(3)
+    x = np.linspace(0, 1, nx)[:, None]
+    y = np.linspace(0, 1, ny)[None, :]
+    b = 300 * np.sin(7 * x) * np.cos(5 * y) + 30 * rng.normal(size=(nx, ny))
+    ref, st_ref = R.poisson_multigrid(b, None, dx, dy, levels=4, v_cycles=3, return_stats=True)
+    got, st = aj.poisson_multigrid(dev(b, dtype), None, dx, dy, levels=4, v_cycles=3,
+                                   return_stats=True)
+    check(host(got), ref, None, dtype, f"mg-ref {shape} {dtype}")
+    assert st["cycles"] == st_ref["cycles"] == 3
+    assert not st["nonfinite"]
+    assert np.allclose(st["norms"][:3], st_ref["norm_sq"], rtol=1e-4)
+    # warm start
+    p0 = 0.5 * ref
+    ref2 = R.poisson_multigrid(b, None, dx, dy, v_cycles=2, p0=p0)
+    got2 = aj.poisson_multigrid(dev(b, dtype), None, dx, dy, v_cycles=2, p0=dev(p0, dtype))
+    check(host(got2), ref2, None, dtype, "mg-ref warm")
+
+
+def test_reference_multigrid_gate_and_errors(gpu):
+    import aerojax_amd as aj
+    b = torch.zeros((64, 32), device="cuda")
+    p, st = aj.poisson_multigrid(b, None, 0.1, 0.1, v_cycles=5, return_stats=True)
+    assert st["cycles"] == 1 and st["converged"] and torch.count_nonzero(p) == 0
+    with pytest.raises(ValueError, match="not suitable for multigrid"):
+        aj.poisson_multigrid(torch.zeros((7, 5), device="cuda"), None, 0.1, 0.1)
+    with pytest.raises(ValueError):
+        aj.poisson_multigrid(torch.zeros((32, 16), device="cuda"), None, 0.1, 0.1)
+    # levels=2 hierarchy (coarsest 4x4 through the LDS coarse kernel)
+    rng = np.random.default_rng(9)
+    bb = rng.normal(size=(16, 16))
+    ref = R.poisson_multigrid(bb, None, 0.1, 0.07, levels=2, v_cycles=2)
+    got = aj.poisson_multigrid(dev(bb, torch.float64), None, 0.1, 0.07, levels=2, v_cycles=2)
+    check(host(got), ref, None, torch.float64, "mg-ref levels=2")
+
+
+def test_reference_multigrid_large_coarse_level(gpu):
+    # 4096x1024 with levels=4 leaves a 256x64 coarsest grid (multi-launch path)
+    import aerojax_amd as aj
+    nx, ny = 4096, 1024
+    dx = dy = 1.0 / 512
+    rng = np.random.default_rng(4)
+    b = rng.normal(size=(nx, ny))
+    ref = R.poisson_multigrid(b, None, dx, dy, v_cycles=1, tolerance=0.0)
+    got = aj.poisson_multigrid(dev(b, torch.float32), None, dx, dy, v_cycles=1, tolerance=0.0)
+    check(host(got), ref, None, torch.float32, "mg-ref 4096x1024")
+
+
+@pytest.mark.parametrize("shape", [(64, 16), (512, 96), (2048, 512)])
+def test_textbook_multigrid_one_outer_iteration_parity(gpu, shape):
+    import aerojax_amd as aj
+    nx, ny = shape
+    dx, dy = 4.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(5)
+    b = rng.normal(size=(nx, ny)) * 500
+    mg = T.TextbookMG(nx, ny, dx, dy)
+    ref, st_ref = mg.solve(b, tol=0.0, max_outer=1)
+    got, st = aj.poisson_multigrid_to_tolerance(dev(b, torch.float64), dx, dy, tolerance=0.0,
+                                                max_cycles=2, return_stats=True)
+    assert st["cycles"] == 2 and st["outer"] == 1
+    check(host(got), ref, None, torch.float32, f"mg-textbook 1 outer {shape}")
+    assert np.allclose(st["norms"][:2], st_ref["norms"][:2], rtol=1e-4)
+
+
+@pytest.mark.parametrize("shape", [(512, 96), (2048, 512)])
+def test_textbook_multigrid_to_tolerance(gpu, shape):
+    import aerojax_amd as aj
+    nx, ny = shape
+    dx, dy = 4.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(6)
+    b = rng.normal(size=(nx, ny)) * 500
+    got, st = aj.poisson_multigrid_to_tolerance(dev(b, torch.float64), dx, dy, tolerance=1e-6,
+                                                max_cycles=40, return_stats=True)
+    print("textbook", shape, st["cycles"], ["%.2e" % x for x in st["norms"]], flush=True)
+    assert st["converged"] and st["norm"] <= 1e-6 and not st["nonfinite"]
+    p = host(got)
+    mgo = T.TextbookMG(nx, ny, dx, dy)
+    bb = b.copy()
+    bb[-1] = 0
+    assert mgo.norm(mgo.residual64(p, bb)) <= 1e-6
+    if nx * ny <= 60000:
+        pd = T.direct_solve(b, dx, dy)
+        assert R.rel_l2(p, pd) < 1e-9
+
+
+# ------------------------------------------------------------------ full step
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_step_pure_parity_config_small(gpu, dtype):
+    """512x96 on 4x0.75, NACA 0012 chord 0.5 at 10 deg, LE at (1, 0.375),
+    mask eps 0.05, Re 1000 (nu = 5e-4), dt 1e-3, 3 reference V-cycles."""
+    import aerojax_amd as aj
+    from aerojax_amd import geometry as G
+    nx, ny, lx, ly = 512, 96, 4.0, 0.75
+    grid = G.GridParams(nx, ny, lx, ly)
+    geo = G.naca_geometry(grid, "0012", chord=0.5, aoa_deg=10.0, x_le=1.0, y_le=0.375, eps=0.05)
+    sdf = np.nan_to_num(geo.sdf, nan=0.0)
+    mask = geo.mask
+    u, v = G.von_karman_ic(nx, ny, 1.0, 1e-3, seed=0)
+    ref = R.step_pure(u, v, mask, sdf, grid.dx, grid.dy, 5e-4, 1.0, 1e-3, v_cycles=3)
+    f = NP[dtype]
+    refd = R.step_pure(u.astype(f), v.astype(f), mask.astype(f), sdf.astype(f), grid.dx, grid.dy,
+                       5e-4, 1.0, 1e-3, v_cycles=3)
+    st = aj.initial_state(dev(u, dtype), dev(v, dtype), 1e-3)
+    new = aj.step_pure(st, dev(mask, dtype), dev(sdf, dtype), grid.dx, grid.dy, 5e-4, 1.0,
+                       v_cycles=3)
+    for got, r64, rd, name in ((new.u, ref[0], refd[0], "u"), (new.v, ref[1], refd[1], "v"),
+                               (new.p, ref[2], refd[2], "p")):
+        check(host(got), r64, rd, dtype, f"step {name} {dtype}")
+    assert new.iteration == 1 and new.u_prev is st.u
+
+
+def test_step_trajectory_error_growth(gpu):
+    """20 FP32 steps against the FP64 oracle: error is reported, and must
+    stay small (round-off growth, not a gate on bitwise agreement)."""
+    import aerojax_amd as aj
+    nx, ny = 256, 64
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder", noise=1e-3)
+    dx, dy = c["dx"], c["dy"]
+    u64, v64 = c["u"], c["v"]
+    stp = aj.CollocatedStepper(nx, ny, dx, dy, c["mask"], c["sdf"], 5e-4, 1.0, 1e-3, v_cycles=3)
+    stp.load(u64, v64)
+    errs = []
+    for k in range(20):
+        u64, v64, _ = R.step_pure(u64, v64, c["mask"], c["sdf"], dx, dy, 5e-4, 1.0, 1e-3,
+                                  v_cycles=3)
+        stp.step()
+        ug, vg = stp.state
+        errs.append(R.rel_l2(host(ug), u64))
+    print("trajectory rel err:", ["%.1e" % e for e in errs], flush=True)
+    assert errs[-1] < 1e-4
+
+
+def test_step_textbook_reaches_tolerance(gpu):
+    import aerojax_amd as aj
+    nx, ny = 512, 128
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder")
+    st = aj.initial_state(dev(c["u"], torch.float32), dev(c["v"], torch.float32), 1e-3)
+    new = aj.step_pure(st, dev(c["mask"], torch.float32), dev(c["sdf"], torch.float32), c["dx"],
+                       c["dy"], 5e-4, 1.0, mg_variant="textbook", tolerance=1e-6)
+    plan = aj.pressure_solvers.get_plan(nx, ny, c["dx"], c["dy"], torch.float32, "textbook", 0,
+                                        new.u.device)
+    s = plan.stats()
+    assert s["converged"] and s["norm"] <= 1e-6
+    assert torch.isfinite(new.u).all()

@@ -1,0 +1,389 @@
+"""Pins the CPU oracle of the reference step against closed forms,
+invariants, hand-computed values and brute force (the reference ships no
+tests or golden vectors for this path)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import reference_step as R
+
+
+# ---------------------------------------------------------------- operators
+
+def test_gradients_exact_on_affine_and_boundary_conventions():
+    nx, ny, dx, dy = 6, 5, 0.3, 0.2
+    i = np.arange(nx)[:, None] * np.ones((1, ny))
+    j = np.ones((nx, 1)) * np.arange(ny)[None, :]
+    f = 2.0 * i * dx - 3.0 * j * dy + 1.5  # affine: df/dx = 2, df/dy = -3
+    assert np.allclose(R.grad_x_nonperiodic(f, dx), 2.0, rtol=0, atol=1e-13)
+    assert np.allclose(R.grad_y_nonperiodic(f, dy), -3.0, rtol=0, atol=1e-13)
+    # quadratic: central difference exact in the interior, one-sided at the rows
+    q = (i * dx) ** 2
+    gx = R.grad_x_nonperiodic(q, dx)
+    assert np.allclose(gx[1:-1], 2 * i[1:-1] * dx, atol=1e-13)
+    assert np.allclose(gx[0], (dx ** 2 - 0) / dx)  # forward difference
+    xl, xm = (nx - 1) * dx, (nx - 2) * dx
+    assert np.allclose(gx[-1], (xl ** 2 - xm ** 2) / dx)  # backward difference
+
+
+def test_divergence_zero_on_boundary_and_exact_inside():
+    nx, ny, dx, dy = 7, 6, 0.1, 0.25
+    i = np.arange(nx)[:, None] * np.ones((1, ny))
+    j = np.ones((nx, 1)) * np.arange(ny)[None, :]
+    u = 3.0 * (i * dx) + 1.0
+    v = -0.5 * (j * dy) ** 2
+    d = R.divergence_nonperiodic(u, v, dx, dy)
+    assert np.all(d[0] == 0) and np.all(d[-1] == 0) and np.all(d[:, 0] == 0) and np.all(d[:, -1] == 0)
+    assert np.allclose(d[1:-1, 1:-1], 3.0 - (j[1:-1, 1:-1] * dy), atol=1e-12)
+
+
+def test_smagorinsky_linear_shear_closed_form():
+    nx, ny, dx, dy, a, Cs = 8, 10, 0.1, 0.05, 2.5, 0.17
+    j = np.ones((nx, 1)) * np.arange(ny)[None, :]
+    u = a * j * dy
+    v = np.zeros_like(u)
+    delta = math.sqrt(dx * dy)
+    nus = R.constant_smagorinsky(u, v, dx, dy, delta, Cs)
+    # interior columns: du/dy = a, |S| = sqrt(2 * 2 * (a/2)^2 + 1e-10) = sqrt(a^2 + 1e-10)
+    expect = (Cs * delta) ** 2 * math.sqrt(a * a + 1e-10)
+    assert np.allclose(nus[:, 1:-1], expect, rtol=1e-12)
+    # wall columns wrap around (jnp.roll): du/dy = (u[1] - u[-1]) / (2 dy)
+    du = (u[:, 1] - u[:, -1]) / (2 * dy)
+    expect0 = (Cs * delta) ** 2 * np.sqrt(2 * 2 * (0.5 * du) ** 2 + 1e-10)
+    assert np.allclose(nus[:, 0], expect0, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- RK step
+
+def _box(nx=12, ny=9, val=1.0):
+    return np.full((nx, ny), val)
+
+
+def test_rk_uniform_flow_is_fixed_point():
+    nx, ny = 12, 9
+    u, v = _box(nx, ny, 1.3), _box(nx, ny, 0.0)
+    mask = np.ones((nx, ny))
+    for fast in (False, True):
+        us, vs = R.rk_step_unified(u, v, 1e-3, 1e-3, 0.1, 0.1, mask, sdf=None, U_inf=1.3,
+                                   fast_mode=fast)
+        assert np.array_equal(us, u) and np.array_equal(vs, v)
+
+
+def test_rk_uniform_chi_single_division_discriminates_ssp():
+    # constant fields -> rhs == 0 at every stage; the reference restarts
+    # stages 2 and 3 from u, so u* = u / D exactly once (SSP-RK3 would give
+    # a different value); BCs then copy the interior.
+    nx, ny, dt, eta = 10, 8, 2e-3, 0.01
+    u0, v0 = 0.7, 0.2
+    u, v = _box(nx, ny, u0), _box(nx, ny, v0)
+    sdf = np.full((nx, ny), 0.004)  # uniform chi
+    mask = np.ones((nx, ny))
+    dx = 0.05
+    chi = 1.0 / (1.0 + math.exp(0.004 / (0.1 * dx)))
+    D = 1.0 + dt * chi / eta
+    us, vs = R.rk_step_unified(u, v, dt, 1e-3, dx, dx, mask, sdf=sdf, U_inf=1.0, brinkman_eta=eta)
+    assert np.allclose(us[1:-1, 1:-1], u0 / D, rtol=1e-14)
+    assert np.allclose(vs[1:-1, 1:-1], v0 / D, rtol=1e-14)
+    # SSP-RK3 with the same implicit factor would give u0/D**3-ish: not equal
+    assert abs(u0 / D - u0 / D ** 2) > 1e-6
+    # RK2: (u + 0.5 dt (k1 + k2)) / D = u / D as well
+    us2, _ = R.rk_step_unified(u, v, dt, 1e-3, dx, dx, mask, sdf=sdf, fast_mode=True,
+                               brinkman_eta=eta)
+    assert np.allclose(us2[1:-1, 1:-1], u0 / D, rtol=1e-14)
+
+
+def test_rk_chi_from_mask_when_no_sdf():
+    nx, ny, dt, eta = 8, 8, 1e-3, 0.005
+    u, v = _box(nx, ny, 1.0), _box(nx, ny, 0.0)
+    mask = np.full((nx, ny), 0.75)  # chi = 0.25, mask > 0.5 keeps the value
+    us, _ = R.rk_step_unified(u, v, dt, 0.0, 0.1, 0.1, mask, sdf=None, brinkman_eta=eta)
+    assert np.allclose(us[1:-1, 1:-1], 1.0 / (1.0 + dt * 0.25 / eta), rtol=1e-14)
+
+
+def test_rk_upwind_first_stage_hand_formula():
+    # one stage of forward Euler is visible through RK2's first stage: use a
+    # 1-D profile in x, v = 0, nu = 0, chi = 0 and compare k1 with the upwind
+    # formula by running RK2 with dt tiny and extracting (u* - u)/dt ~ k.
+    nx, ny, dx = 9, 5, 0.2
+    x = np.arange(nx)[:, None] * np.ones((1, ny))
+    u = 0.5 + 0.1 * np.sin(x)
+    v = np.zeros_like(u)
+    mask = np.ones_like(u)
+    dt = 1e-9
+    us, _ = R.rk_step_unified(u, v, dt, 0.0, dx, dx, mask, sdf=None, fast_mode=True)
+    k = (us - u) / dt
+    # u > 0 everywhere -> backward difference, edge pad at i=0
+    up = np.concatenate([u[:1], u[:-1]], axis=0)
+    k_hand = -u * (u - up) / dx
+    assert np.allclose(k[1:-1, 1:-1], k_hand[1:-1, 1:-1], rtol=1e-5, atol=1e-8)
+
+
+def test_rk_bcs_and_mask():
+    nx, ny = 10, 7
+    rng = np.random.default_rng(3)
+    u = 1 + 0.1 * rng.normal(size=(nx, ny))
+    v = 0.1 * rng.normal(size=(nx, ny))
+    mask = np.ones((nx, ny))
+    mask[4:6, 3] = 0.2
+    us, vs = R.rk_step_unified(u, v, 1e-3, 1e-3, 0.1, 0.1, mask, U_inf=1.0)
+    assert np.all(us[0] == 1.0) and np.all(vs[0] == 0)
+    assert np.array_equal(us[-1, 1:-1], us[-2, 1:-1])
+    assert np.array_equal(us[:, 0], us[:, 1]) and np.array_equal(us[:, -1], us[:, -2])
+    assert np.all(vs[:, 0] == 0) and np.all(vs[:, -1] == 0)
+    assert np.all(us[4:6, 3] == 0) and np.all(vs[4:6, 3] == 0)
+    us_ns, _ = R.rk_step_unified(u, v, 1e-3, 1e-3, 0.1, 0.1, mask, U_inf=1.0, slip_walls=False)
+    assert np.all(us_ns[:, 0] == 0) and np.all(us_ns[:, -1] == 0)
+
+
+# ---------------------------------------------------------------- multigrid
+
+def test_mg_restrict_prolong_hand_values():
+    f = np.arange(16, dtype=np.float64).reshape(4, 4)
+    r = R.mg_restrict(f)
+    assert np.array_equal(r, np.array([[2.5, 4.5], [10.5, 12.5]]))
+    c = np.array([[1.0, 2.0], [3.0, 5.0]])
+    F = R.mg_prolong(c)
+    expect = np.array([
+        [1.0, 1.5, 2.0, 0.0],
+        [2.0, 2.75, 3.5, 0.0],
+        [3.0, 4.0, 5.0, 0.0],
+        [0.0, 0.0, 0.0, 0.0],
+    ])
+    assert np.array_equal(F, expect)
+    # degenerate restrict: a side below 2 collapses to fine[:1, :1]
+    assert np.array_equal(R.mg_restrict(np.arange(6.0).reshape(2, 3)), np.array([[0.0]]))
+
+
+def test_mg_hierarchy_and_errors():
+    ml, shapes = R.mg_level_shapes(512, 96, 4)
+    assert ml == 4 and shapes[-1] == (32, 6)
+    ml, shapes = R.mg_level_shapes(2048, 512, 4)
+    assert shapes[-1] == (128, 32)
+    with pytest.raises(ValueError, match="not suitable for multigrid"):
+        R.mg_level_shapes(7, 5, 4)
+    with pytest.raises(ValueError):
+        R.mg_level_shapes(32, 16, 4)  # reference fails on p + e broadcast
+
+
+def _edge_lap(p, ax, ay):
+    pp = np.pad(p, 1, mode="edge")
+    return ax * (pp[2:, 1:-1] + pp[:-2, 1:-1] - 2 * p) + ay * (pp[1:-1, 2:] + pp[1:-1, :-2] - 2 * p)
+
+
+def test_mg_exact_solution_is_fixed_point():
+    # p* with its last two rows zero: b = A p* has b[-1] = 0, p* is a Jacobi
+    # fixed point of the edge-padded operator, the residual vanishes and the
+    # coarse correction is zero -> one V-cycle from p* returns p*.
+    nx, ny, dx, dy = 64, 32, 0.05, 0.04
+    rng = np.random.default_rng(7)
+    ps = rng.normal(size=(nx, ny))
+    ps[-2:] = 0.0
+    b = _edge_lap(ps, 1 / dx ** 2, 1 / dy ** 2)
+    p, st = R.poisson_multigrid(b, None, dx, dy, v_cycles=2, tolerance=1e-6, p0=ps,
+                                return_stats=True)
+    assert np.allclose(p, ps, rtol=0, atol=1e-10)
+    assert st["converged"] and st["cycles"] == 1
+    # a sign error in the smoother would move p away from p*
+    p_bad = ps.copy()
+    p_bad[1:-2] += 1e-3
+    q = R.poisson_multigrid(b, None, dx, dy, v_cycles=1, tolerance=0.0, p0=p_bad)
+    assert not np.allclose(q, ps, atol=1e-10)
+
+
+def test_mg_linearity_and_outlet_row():
+    nx, ny, dx, dy = 64, 32, 1 / 16, 1 / 32
+    rng = np.random.default_rng(1)
+    b1, b2 = rng.normal(size=(nx, ny)), rng.normal(size=(nx, ny))
+    p1 = R.poisson_multigrid(b1, None, dx, dy, v_cycles=3, tolerance=0.0)
+    p2 = R.poisson_multigrid(b2, None, dx, dy, v_cycles=3, tolerance=0.0)
+    p12 = R.poisson_multigrid(2 * b1 - 3 * b2, None, dx, dy, v_cycles=3, tolerance=0.0)
+    assert np.allclose(p12, 2 * p1 - 3 * p2, rtol=1e-12, atol=1e-12)
+    assert np.all(p1[-1] == 0)
+
+
+def _dense(n_rows, n_cols, fn):
+    N = n_rows * n_cols
+    cols = []
+    for k in range(N):
+        e = np.zeros(N)
+        e[k] = 1.0
+        cols.append(fn(e.reshape(n_rows, n_cols)).reshape(-1))
+    return np.array(cols).T
+
+
+def test_mg_vcycle_matches_dense_composition():
+    # Build every operator of one reference V-cycle as an explicit matrix from
+    # its element-wise definition and compose them; compare with the oracle.
+    nx, ny, dx, dy = 16, 16, 0.1, 0.07
+    ax, ay = 1 / dx ** 2, 1 / dy ** 2
+    N, Nc = nx * ny, (nx // 2) * (ny // 2)
+
+    def idx(i, j, m):
+        return i * m + j
+
+    def jacobi_mats(n, m):
+        J = np.zeros((n * m, n * m))
+        for i in range(n):
+            for j in range(m):
+                r = idx(i, j, m)
+                for (ii, jj, w) in ((min(i + 1, n - 1), j, ax), (max(i - 1, 0), j, ax),
+                                    (i, min(j + 1, m - 1), ay), (i, max(j - 1, 0), ay)):
+                    J[r, idx(ii, jj, m)] += w / (2 * (ax + ay))
+        return J, -np.eye(n * m) / (2 * (ax + ay))
+
+    def lap_mat(n, m):
+        L = np.zeros((n * m, n * m))
+        for i in range(n):
+            for j in range(m):
+                r = idx(i, j, m)
+                for (ii, jj, w) in ((min(i + 1, n - 1), j, ax), (max(i - 1, 0), j, ax),
+                                    (i, min(j + 1, m - 1), ay), (i, max(j - 1, 0), ay)):
+                    L[r, idx(ii, jj, m)] += w
+                L[r, r] -= 2 * (ax + ay)
+        return L
+
+    Rm = np.zeros((Nc, N))
+    for I in range(nx // 2):
+        for Jj in range(ny // 2):
+            for (a, b_) in ((0, 0), (1, 0), (0, 1), (1, 1)):
+                Rm[idx(I, Jj, ny // 2), idx(2 * I + a, 2 * Jj + b_, ny)] = 0.25
+    Pm = np.zeros((N, Nc))
+    nc, mc = nx // 2, ny // 2
+    for i in range(nx):
+        for j in range(ny):
+            if i == nx - 1 or j == ny - 1:
+                continue
+            I, Jj = i // 2, j // 2
+            pts = [(I + a, Jj + b_) for a in range(1 + i % 2) for b_ in range(1 + j % 2)]
+            for (a, b_) in pts:
+                Pm[idx(i, j, ny), idx(a, b_, mc)] = 1.0 / len(pts)
+    C = np.eye(N)
+    for j in range(ny):
+        C[idx(nx - 1, j, ny), idx(nx - 1, j, ny)] = 0.0
+    Bz = np.eye(N)
+    for j in range(ny):
+        Bz[idx(nx - 1, j, ny), idx(nx - 1, j, ny)] = 0.0
+    Jf, Kf = jacobi_mats(nx, ny)
+    Jc, Kc = jacobi_mats(nc, mc)
+    L = lap_mat(nx, ny)
+
+    def smooth_lin(Jm, Km, nsw, n):
+        # affine map p -> Sp p + Sb b after nsw sweeps
+        Sp, Sb = np.eye(n), np.zeros((n, n))
+        for _ in range(nsw):
+            Sp, Sb = Jm @ Sp, Jm @ Sb + Km
+        return Sp, Sb
+
+    Sp2, Sb2 = smooth_lin(Jf, Kf, 2, N)
+    _, Sbc = smooth_lin(Jc, Kc, 10, Nc)
+    # levels=2 on 16x16: max_levels = 2 -> level 0 and level 1 smooth, level 2 coarse?
+    ml, shapes = R.mg_level_shapes(nx, ny, 2)
+    assert ml == 2
+    # level 1 (8x8) V-cycle operator on its rhs, with a coarse 4x4 solve
+    n1, m1 = shapes[1]
+    n2, m2 = shapes[2]
+    L1 = np.zeros((n1 * m1, n1 * m1))
+    J1, K1 = jacobi_mats(n1, m1)
+    L1 = lap_mat(n1, m1)
+    R1 = np.zeros((n2 * m2, n1 * m1))
+    for I in range(n2):
+        for Jj in range(m2):
+            for (a, b_) in ((0, 0), (1, 0), (0, 1), (1, 1)):
+                R1[idx(I, Jj, m2), idx(2 * I + a, 2 * Jj + b_, m1)] = 0.25
+    P1 = np.zeros((n1 * m1, n2 * m2))
+    for i in range(n1):
+        for j in range(m1):
+            if i == n1 - 1 or j == m1 - 1:
+                continue
+            I, Jj = i // 2, j // 2
+            pts = [(I + a, Jj + b_) for a in range(1 + i % 2) for b_ in range(1 + j % 2)]
+            for (a, b_) in pts:
+                P1[idx(i, j, m1), idx(a, b_, m2)] = 1.0 / len(pts)
+    J2, K2 = jacobi_mats(n2, m2)
+    _, Sb10 = smooth_lin(J2, K2, 10, n2 * m2)
+    C1 = np.eye(n1 * m1)
+    for j in range(m1):
+        C1[idx(n1 - 1, j, m1), idx(n1 - 1, j, m1)] = 0.0
+    Sp2_1, Sb2_1 = smooth_lin(J1, K1, 2, n1 * m1)
+    pre1 = C1 @ Sb2_1                          # b1 -> p after pre-smooth
+    r1 = np.eye(n1 * m1) - L1 @ pre1          # b1 -> residual
+    e1 = P1 @ Sb10 @ R1 @ r1                   # b1 -> correction
+    V1 = C1 @ (Sp2_1 @ (pre1 + e1) + Sb2_1)   # b1 -> level-1 result
+    pre0 = C @ Sb2 @ Bz
+    r0 = Bz - L @ pre0
+    e0 = Pm @ V1 @ Rm @ r0
+    V0 = C @ (Sp2 @ (pre0 + e0) + Sb2 @ Bz)
+    rng = np.random.default_rng(5)
+    b = rng.normal(size=(nx, ny))
+    p = R.poisson_multigrid(b, None, dx, dy, levels=2, v_cycles=1, tolerance=0.0)
+    assert np.allclose(p.reshape(-1), V0 @ b.reshape(-1), rtol=1e-10, atol=1e-10)
+
+
+def test_mg_zero_rhs_converges_after_one_cycle():
+    b = np.zeros((64, 32))
+    p, st = R.poisson_multigrid(b, None, 0.1, 0.1, v_cycles=5, return_stats=True)
+    assert st["cycles"] == 1 and st["converged"] and np.all(p == 0)
+
+
+# ---------------------------------------------------------------- geometry
+
+def test_naca_sdf_hand_points():
+    t = 0.12
+    c = 0.5
+    X = np.array([[1.0 + 0.3 * c, 0.9, 1.0 + 0.3 * c]])
+    Y = np.array([[0.375, 0.375, 0.375 + 0.2]])
+    m, p, tt = R.parse_naca_4digit("0012")
+    assert (m, p, tt) == (0.0, 0.0, 0.12)
+    sdf = R.naca_sdf(X, Y, c, 0.0, 1.0, 0.375, m, p, tt)
+    xs = 0.3
+    yt = 5 * t * (0.2969 * math.sqrt(xs) - 0.1260 * xs - 0.3516 * xs ** 2 + 0.2843 * xs ** 3
+                  - 0.1015 * xs ** 4)
+    assert abs(yt / t - 0.50014) < 1e-4
+    assert np.isclose(sdf[0, 0], -yt * c, rtol=1e-12)        # on the chord line, inside
+    assert np.isclose(sdf[0, 1], 0.1, rtol=1e-12)            # ahead of the LE: distance to LE
+    assert np.isclose(sdf[0, 2], 0.2 - yt * c, rtol=1e-12)   # above: vertical distance
+    # symmetric section: NaN only exactly at x/c == 0
+    s0 = R.naca_sdf(np.array([[1.0]]), np.array([[0.375]]), c, 0.0, 1.0, 0.375, m, p, tt)
+    assert np.isnan(s0[0, 0])
+    # rotation: a point on the rotated chord line at AoA stays inside
+    a = math.radians(10.0)
+    Xr = np.array([[1.0 + 0.3 * c * math.cos(a)]])
+    Yr = np.array([[0.375 - 0.3 * c * math.sin(a)]])
+    sr = R.naca_sdf(Xr, Yr, c, 10.0, 1.0, 0.375, m, p, tt)
+    assert np.isclose(sr[0, 0], -yt * c, rtol=1e-9)
+
+
+def test_cylinder_mask():
+    X = np.array([[1.5, 1.6, 1.8]])
+    Y = np.array([[1.0, 1.0, 1.0]])
+    s = R.cylinder_sdf(X, Y, 1.5, 1.0, 0.1)
+    assert np.allclose(s, [[-0.1, 0.0, 0.2]])
+    m = R.mask_from_sdf(s, 0.05)
+    assert np.isclose(m[0, 1], 0.5) and m[0, 0] < 0.2 and m[0, 2] > 0.98
+
+
+# ---------------------------------------------------------------- composition
+
+def test_step_uniform_flow_fixed_point():
+    nx, ny = 64, 32
+    u = np.full((nx, ny), 1.0)
+    v = np.zeros((nx, ny))
+    mask = np.ones((nx, ny))
+    un, vn, p = R.step_pure(u, v, mask, None, 0.05, 0.05, 1e-3, 1.0, 1e-3, v_cycles=3)
+    assert np.array_equal(un, u) and np.array_equal(vn, v) and np.all(p == 0)
+
+
+def test_step_fp32_close_to_fp64():
+    nx, ny, lx, ly = 128, 32, 4.0, 1.0
+    dx, dy, X, Y = R.grid_coords(nx, ny, lx, ly)
+    sdf = R.cylinder_sdf(X, Y, 1.0, 0.5, 0.1)
+    mask = R.mask_from_sdf(sdf, 0.05)
+    u, v = R.von_karman_ic(nx, ny, 1.0, 1e-3, seed=0, dtype=np.float64)
+    out64 = R.step_pure(u, v, mask, sdf, dx, dy, 5e-4, 1.0, 1e-3, v_cycles=3)
+    f = np.float32
+    out32 = R.step_pure(u.astype(f), v.astype(f), mask.astype(f), sdf.astype(f), dx, dy, 5e-4, 1.0,
+                        1e-3, v_cycles=3)
+    for a, b in zip(out32, out64):
+        assert a.dtype == np.float32
+        assert R.rel_l2(a, b) < 1e-5
