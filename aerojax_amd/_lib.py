@@ -25,13 +25,19 @@ AJ_F64 = 1
 AJ_MG_REFERENCE = 0
 AJ_MG_TEXTBOOK = 1
 AJ_MG_MAX_NORMS = 64
+AJ_MG_OPT_ADAPTIVE = 0
+AJ_MG_OPT_GRAPH = 1
 
 # Every symbol the header declares; tests check the library exports them.
 EXPORTS = (
     "aj_version", "aj_last_error", "aj_rk_step", "aj_les_smagorinsky", "aj_divergence_rhs",
     "aj_project", "aj_mg_create", "aj_mg_destroy", "aj_mg_workspace_bytes",
-    "aj_mg_num_levels", "aj_poisson_mg", "aj_mg_get_stats",
+    "aj_mg_num_levels", "aj_mg_set_option", "aj_poisson_mg", "aj_mg_get_stats",
+    "aj_profile_begin", "aj_profile_end", "aj_profile_read",
 )
+
+PROF_TAGS = ("rk", "rhs", "project", "mg_pre0", "mg_post0", "mg_pre", "mg_post", "mg_bottom",
+             "mg_resid")
 
 
 class AeroJaxLibraryError(RuntimeError):
@@ -86,8 +92,12 @@ _SIGS = {
     "aj_mg_destroy": (_i, [_vp]),
     "aj_mg_workspace_bytes": (ctypes.c_size_t, [_vp]),
     "aj_mg_num_levels": (_i, [_vp]),
+    "aj_mg_set_option": (_i, [_vp, _i, _i]),
     "aj_poisson_mg": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
     "aj_mg_get_stats": (_i, [_vp, ctypes.POINTER(MgStats), _vp]),
+    "aj_profile_begin": (_i, [_i]),
+    "aj_profile_end": (_i, []),
+    "aj_profile_read": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
 }
 
 
@@ -120,6 +130,29 @@ def check(rc: int, what: str):
         if rc == AJ_ERR_GRID:
             raise ValueError(msg)
         raise AeroJaxError(rc, f"{what}: {msg}")
+
+
+def profile_begin(max_records: int = 20000):
+    check(load().aj_profile_begin(int(max_records)), "aj_profile_begin")
+
+
+def profile_end():
+    check(load().aj_profile_end(), "aj_profile_end")
+
+
+def profile_read() -> dict:
+    """{tag: {"count", "ms", "bytes", "avg_us", "gbps"}} of the recorded launches."""
+    out = {}
+    lib = load()
+    for k, name in enumerate(PROF_TAGS):
+        c, ms, by = _i(), _d(), _d()
+        check(lib.aj_profile_read(k, ctypes.byref(c), ctypes.byref(ms), ctypes.byref(by)),
+              "aj_profile_read")
+        if c.value:
+            out[name] = {"count": c.value, "ms": ms.value, "bytes": by.value,
+                         "avg_us": 1e3 * ms.value / c.value,
+                         "gbps": by.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else None}
+    return out
 
 
 def version() -> str:

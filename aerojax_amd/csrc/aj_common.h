@@ -32,6 +32,25 @@ int hip_fail(hipError_t e, const char* what);
 
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// ---- live kernel timing (HIP events around selected launches) -----------
+// Enabled by aj_profile_begin; every instrumented launch records an event
+// pair on its stream and the algorithmic bytes it moves.
+bool prof_on();
+int prof_begin_launch(int tag, hipStream_t st);  // returns a slot or -1
+void prof_end_launch(int slot, double bytes, hipStream_t st);
+
+struct ProfScope {
+  int slot;
+  double bytes;
+  hipStream_t st;
+  ProfScope(int tag, double b, hipStream_t s) : slot(-1), bytes(b), st(s) {
+    if (prof_on()) slot = prof_begin_launch(tag, s);
+  }
+  ~ProfScope() {
+    if (slot >= 0) prof_end_launch(slot, bytes, st);
+  }
+};
+
 // ---- device helpers -----------------------------------------------------
 __device__ __forceinline__ int clampi(int x, int lo, int hi) {
   return x < lo ? lo : (x > hi ? hi : x);

@@ -35,6 +35,14 @@ struct aj_mg_plan {
   int npartial = 0;
   aj_mg_stats* stats = nullptr;     // device copy, also the cycle gate
   int* state = nullptr;             // textbook: [0] = current p64 buffer
+  aj_mg_stats* host_stats = nullptr;  // pinned copy read by the adaptive driver
+  int adaptive = 1;                 // textbook: read convergence between iterations
+  int last_outer = 4;               // textbook: iterations the previous solve needed
+  int use_graph = 1;                // textbook: replay a captured outer iteration
+  void* graph = nullptr;            // hipGraph_t of one outer iteration
+  void* graph_exec = nullptr;       // hipGraphExec_t
+  double* dtol = nullptr;           // textbook: tolerance of the current solve
+  void* bcopy = nullptr;            // textbook: plan-owned copy of the rhs
 };
 
 namespace aj {

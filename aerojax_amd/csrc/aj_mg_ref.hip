@@ -56,23 +56,36 @@ __device__ __forceinline__ T prolong_ref(const T* __restrict__ c, int ncx, int n
 // NS Jacobi sweeps (smooth, multigrid_solver.py:158-171) on one tile.
 template <typename T, int TX, int TY, int NS>
 __global__ __launch_bounds__(256) void k_ref_jacobi(RefJacobiArgs<T> a) {
-  if (gated(a.gate)) return;
   constexpr int RH = TX + 2 * NS, RW = TY + 2 * NS, RN = RH * RW;
   __shared__ T s_p[2][RN];
   __shared__ T s_b[RN];
   const int nx = a.nx, ny = a.ny;
   const int gx0 = blockIdx.y * TX - NS, gy0 = blockIdx.x * TY - NS;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < RN; idx += 256) {
-    const int lx = idx / RW, ly = idx - lx * RW;
-    const int gx = gx0 + lx, gy = gy0 + ly;
-    if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-    const long g = (long)gx * ny + gy;
-    T pv = a.p_in ? a.p_in[g] : T(0);
-    if (a.e_c) pv = pv + prolong_ref(a.e_c, a.ncx, a.ncy, gx, gy);
-    s_p[0][idx] = pv;
-    if (lx >= 1 && lx < RH - 1 && ly >= 1 && ly < RW - 1)
-      s_b[idx] = (a.b_zero_last && gx == nx - 1) ? T(0) : a.b[g];
+  {
+    // all loads of a thread in flight together (clamped, always in bounds)
+    constexpr int NIT = (RN + 255) / 256;
+    T rp[NIT], rb[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int idx = min(tid + it * 256, RN - 1);
+      const int lx = idx / RW, ly = idx - lx * RW;
+      const int gx = clampi(gx0 + lx, 0, nx - 1), gy = clampi(gy0 + ly, 0, ny - 1);
+      const long g = (long)gx * ny + gy;
+      T pv = a.p_in ? a.p_in[g] : T(0);
+      if (a.e_c) pv = pv + prolong_ref(a.e_c, a.ncx, a.ncy, gx, gy);
+      rp[it] = pv;
+      rb[it] = (a.b_zero_last && gx == nx - 1) ? T(0) : a.b[g];
+    }
+    if (gated(a.gate)) return;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int idx = tid + it * 256;
+      if (idx < RN) {
+        s_p[0][idx] = rp[it];
+        s_b[idx] = rb[it];
+      }
+    }
   }
   __syncthreads();
   int cur = 0;
@@ -438,6 +451,13 @@ extern "C" int aj_mg_create(aj_mg_plan** out, int dtype, int variant, int nx, in
 
 extern "C" int aj_mg_destroy(aj_mg_plan* p) {
   if (!p) return AJ_OK;
+  if (p->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy((hipGraph_t)p->graph);
+  p->graph_exec = p->graph = nullptr;
+  if (p->host_stats) {
+    (void)hipHostFree(p->host_stats);
+    p->host_stats = nullptr;
+  }
   if (p->ws) {
     hipError_t e = hipFree(p->ws);
     p->ws = nullptr;
@@ -447,6 +467,15 @@ extern "C" int aj_mg_destroy(aj_mg_plan* p) {
   }
   delete p;
   return AJ_OK;
+}
+
+extern "C" int aj_mg_set_option(aj_mg_plan* p, int option, int value) {
+  if (!p) return fail(AJ_ERR_INVALID, "aj_mg_set_option: null plan");
+  switch (option) {
+    case AJ_MG_OPT_ADAPTIVE: p->adaptive = value ? 1 : 0; return AJ_OK;
+    case AJ_MG_OPT_GRAPH: p->use_graph = value ? 1 : 0; return AJ_OK;
+    default: return fail(AJ_ERR_INVALID, "aj_mg_set_option: unknown option");
+  }
 }
 
 extern "C" size_t aj_mg_workspace_bytes(const aj_mg_plan* p) { return p ? p->ws_bytes : 0; }

@@ -29,7 +29,6 @@
 
 namespace aj {
 
-constexpr int kTbNPartial = 1024;
 constexpr int kBottomCells = 4096;
 constexpr int kBottomThreads = 1024;
 constexpr int kTbCoarseCells = 64;
@@ -49,23 +48,23 @@ struct TbLevel {
   float axl, ayl;  // coefficients
   float gam;       // outlet ghost factor (level >= 1)
   float dE_last;   // (1 - gamma) * axl
+  float inv_int;   // 1 / (2 axl + 2 ayl): inverse diagonal of interior rows
 };
 
 // Exact row solve of the level operator at (i, j) given a neighbour fetch.
 // Neighbour values outside the domain are never read.
 template <typename F>
-__device__ __forceinline__ float tb_relax(const TbLevel& L, int i, int j, float pc, float bv,
-                                          F&& nb) {
+__device__ __forceinline__ float tb_relax(const TbLevel& L, int i, int j, float bv, F&& nb) {
   const bool hasW = i > 0, hasS = j > 0, hasN = j < L.m - 1;
   const bool last = (i == L.n - 1);
   const float E = (!last) ? nb(1, 0) : 0.0f;
   const float W = hasW ? nb(-1, 0) : 0.0f;
   const float S = hasS ? nb(0, -1) : 0.0f;
   const float N = hasN ? nb(0, 1) : 0.0f;
+  const float off = L.axl * (E + W) + L.ayl * (N + S);
+  if (hasW && hasS && hasN && !last) return (off - bv) * L.inv_int;
   const float dE = (L.level > 0 && last) ? L.dE_last : L.axl;
   const float diag = ((dE + (hasW ? L.axl : 0.0f)) + (hasS ? L.ayl : 0.0f)) + (hasN ? L.ayl : 0.0f);
-  const float off = L.axl * (E + W) + L.ayl * (N + S);
-  (void)pc;
   return (off - bv) / diag;
 }
 
@@ -103,50 +102,77 @@ __device__ __forceinline__ float tb_prolong(const float* __restrict__ e, int nc,
   return 0.5625f * own + 0.1875f * (xn + yn) + 0.0625f * dn;
 }
 
+// Four red-black half-sweeps on an LDS tile whose (0, 0) is global
+// (gx0, gy0); half-sweep k updates the cells of colour (k-1)&1 on the
+// region that extends MG0 - k cells beyond the tile.  Only cells of the
+// active colour are visited (W/2 per row).
+template <int TX, int TY, int H, int MG0, int NTH>
+__device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, const float* s_b,
+                                               int gx0, int gy0) {
+  constexpr int RW = TY + 2 * H;
+  const int tid = threadIdx.x;
+  const int active_rows = (L.level == 0) ? L.n - 1 : L.n;
+#pragma unroll
+  for (int k = 1; k <= 4; ++k) {
+    const int mg = MG0 - k;
+    const int color = (k - 1) & 1;
+    const int W2 = (TY + 2 * mg) / 2, Hh = TX + 2 * mg;
+    for (int idx = tid; idx < W2 * Hh; idx += NTH) {
+      const int lx = H - mg + idx / W2;
+      const int gx = gx0 + lx;
+      const int ly0 = H - mg;
+      const int par = (color - gx - gy0 - ly0) & 1;
+      const int ly = ly0 + par + 2 * (idx % W2);
+      const int gy = gy0 + ly;
+      if (gx < 0 || gx >= active_rows || gy < 0 || gy >= L.m) continue;
+      const int l = lx * RW + ly;
+      s_p[l] = tb_relax(L, gx, gy, s_b[l], [&](int di, int dj) { return s_p[l + di * RW + dj]; });
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // pre: 2 RB-GS sweeps + residual + restriction (tile TX x TY, both even)
 // ---------------------------------------------------------------------------
-template <int TX, int TY>
-__global__ __launch_bounds__(256) void k_tb_pre(TbLevel L, const float* __restrict__ p_in,
+template <int TX, int TY, int NTH>
+__global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restrict__ p_in,
                                                 const float* __restrict__ b,
                                                 float* __restrict__ Y, float* __restrict__ Bc,
                                                 const aj_mg_stats* gate) {
-  if (tb_gated(gate)) return;
   constexpr int H = 5, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
   __shared__ float s_p[RN];
   __shared__ float s_b[RN];
   __shared__ float s_r[TX * TY];
+  const int gated = gate->converged;
   const int n = L.n, m = L.m;
   const int gx0 = blockIdx.y * TX - H, gy0 = blockIdx.x * TY - H;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < RN; idx += 256) {
-    const int lx = idx / RW, ly = idx - lx * RW;
-    const int gx = gx0 + lx, gy = gy0 + ly;
-    if (gx < 0 || gx >= n || gy < 0 || gy >= m) continue;
-    const long g = (long)gx * m + gy;
-    s_p[idx] = p_in ? p_in[g] : 0.0f;
-    if (lx >= 1 && lx < RH - 1 && ly >= 1 && ly < RW - 1) s_b[idx] = b[g];
+  {
+    constexpr int NIT = (RN + NTH - 1) / NTH;
+    float rp[NIT], rb[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int idx = min(tid + it * NTH, RN - 1);
+      const int lx = idx / RW, ly = idx - lx * RW;
+      const long g = (long)clampi(gx0 + lx, 0, n - 1) * m + clampi(gy0 + ly, 0, m - 1);
+      rp[it] = p_in ? p_in[g] : 0.0f;
+      rb[it] = b[g];
+    }
+    if (gated) return;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int idx = tid + it * NTH;
+      if (idx < RN) {
+        s_p[idx] = rp[it];
+        s_b[idx] = rb[it];
+      }
+    }
   }
   __syncthreads();
-  const int active_rows = (L.level == 0) ? n - 1 : n;
-#pragma unroll
-  for (int k = 1; k <= 4; ++k) {
-    const int mg = H - k;
-    const int color = (k - 1) & 1;
-    const int W = TY + 2 * mg, Hh = TX + 2 * mg;
-    for (int idx = tid; idx < W * Hh; idx += 256) {
-      const int lx = H - mg + idx / W, ly = H - mg + idx % W;
-      const int gx = gx0 + lx, gy = gy0 + ly;
-      if (gx < 0 || gx >= active_rows || gy < 0 || gy >= m) continue;
-      if (((gx + gy) & 1) != color) continue;
-      const int l = lx * RW + ly;
-      s_p[l] = tb_relax(L, gx, gy, s_p[l], s_b[l],
-                        [&](int di, int dj) { return s_p[l + di * RW + dj]; });
-    }
-    __syncthreads();
-  }
+  tb_tile_sweeps<TX, TY, H, H, NTH>(L, s_p, s_b, gx0, gy0);
   // residual on the tile, write Y
-  for (int idx = tid; idx < TX * TY; idx += 256) {
+  for (int idx = tid; idx < TX * TY; idx += NTH) {
     const int tx = idx / TY, ty = idx % TY;
     const int lx = H + tx, ly = H + ty;
     const int gx = gx0 + lx, gy = gy0 + ly;
@@ -158,7 +184,7 @@ __global__ __launch_bounds__(256) void k_tb_pre(TbLevel L, const float* __restri
   }
   __syncthreads();
   const int mc = m >> 1;
-  for (int idx = tid; idx < (TX / 2) * (TY / 2); idx += 256) {
+  for (int idx = tid; idx < (TX / 2) * (TY / 2); idx += NTH) {
     const int cx = idx / (TY / 2), cy = idx % (TY / 2);
     const int gx = gx0 + H + 2 * cx, gy = gy0 + H + 2 * cy;
     if (gx >= n || gy >= m) continue;
@@ -171,48 +197,46 @@ __global__ __launch_bounds__(256) void k_tb_pre(TbLevel L, const float* __restri
 // ---------------------------------------------------------------------------
 // post: Y + prolong(E_coarse), 2 RB-GS sweeps
 // ---------------------------------------------------------------------------
-template <int TX, int TY>
-__global__ __launch_bounds__(256) void k_tb_post(TbLevel L, const float* __restrict__ Yin,
+template <int TX, int TY, int NTH>
+__global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restrict__ Yin,
                                                  const float* __restrict__ Ec, int nc, int mc,
                                                  float gamc, const float* __restrict__ b,
                                                  float* __restrict__ E,
                                                  const aj_mg_stats* gate) {
-  if (tb_gated(gate)) return;
   constexpr int H = 4, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
   __shared__ float s_p[RN];
   __shared__ float s_b[RN];
+  const int gated = gate->converged;
   const int n = L.n, m = L.m;
   const int gx0 = blockIdx.y * TX - H, gy0 = blockIdx.x * TY - H;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < RN; idx += 256) {
-    const int lx = idx / RW, ly = idx - lx * RW;
-    const int gx = gx0 + lx, gy = gy0 + ly;
-    if (gx < 0 || gx >= n || gy < 0 || gy >= m) continue;
-    const long g = (long)gx * m + gy;
-    float pv = Yin[g];
-    if (!(L.level == 0 && gx == n - 1)) pv = pv + tb_prolong(Ec, nc, mc, gamc, gx, gy);
-    s_p[idx] = pv;
-    if (lx >= 1 && lx < RH - 1 && ly >= 1 && ly < RW - 1) s_b[idx] = b[g];
+  {
+    constexpr int NIT = (RN + NTH - 1) / NTH;
+    float rp[NIT], rb[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int idx = min(tid + it * NTH, RN - 1);
+      const int lx = idx / RW, ly = idx - lx * RW;
+      const int gx = clampi(gx0 + lx, 0, n - 1), gy = clampi(gy0 + ly, 0, m - 1);
+      const long g = (long)gx * m + gy;
+      float pv = Yin[g];
+      if (!(L.level == 0 && gx == n - 1)) pv = pv + tb_prolong(Ec, nc, mc, gamc, gx, gy);
+      rp[it] = pv;
+      rb[it] = b[g];
+    }
+    if (gated) return;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int idx = tid + it * NTH;
+      if (idx < RN) {
+        s_p[idx] = rp[it];
+        s_b[idx] = rb[it];
+      }
+    }
   }
   __syncthreads();
-  const int active_rows = (L.level == 0) ? n - 1 : n;
-#pragma unroll
-  for (int k = 1; k <= 4; ++k) {
-    const int mg = H - k;
-    const int color = (k - 1) & 1;
-    const int W = TY + 2 * mg, Hh = TX + 2 * mg;
-    for (int idx = tid; idx < W * Hh; idx += 256) {
-      const int lx = H - mg + idx / W, ly = H - mg + idx % W;
-      const int gx = gx0 + lx, gy = gy0 + ly;
-      if (gx < 0 || gx >= active_rows || gy < 0 || gy >= m) continue;
-      if (((gx + gy) & 1) != color) continue;
-      const int l = lx * RW + ly;
-      s_p[l] = tb_relax(L, gx, gy, s_p[l], s_b[l],
-                        [&](int di, int dj) { return s_p[l + di * RW + dj]; });
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < TX * TY; idx += 256) {
+  tb_tile_sweeps<TX, TY, H, H, NTH>(L, s_p, s_b, gx0, gy0);
+  for (int idx = tid; idx < TX * TY; idx += NTH) {
     const int lx = H + idx / TY, ly = H + idx % TY;
     const int gx = gx0 + lx, gy = gy0 + ly;
     if (gx >= n || gy >= m) continue;
@@ -229,32 +253,45 @@ struct TbBottom {
   TbLevel lev[kMaxLevels];
   int off[kMaxLevels];      // LDS offsets of each level's arrays
   float gam_next[kMaxLevels];
+  float inv_m[kMaxLevels];  // 1/m for the index split
   int coarse_rows;          // unknown rows of the coarsest level
   int total;                // total cells
+  int tpr;                  // threads per row of the dense solve (power of 2, <= 64)
 };
+
+// c / m for 0 <= c < 2^20 via a float reciprocal and one correction step
+__device__ __forceinline__ int fdiv_idx(int c, int m, float inv_m) {
+  int q = (int)((float)c * inv_m);
+  if (q * m > c) --q;
+  else if ((q + 1) * m <= c) ++q;
+  return q;
+}
 
 __device__ void tb_bottom_rbgs(const TbLevel& L, float* p, const float* b, int sweeps) {
   const int n = L.n, m = L.m;
   const int active_rows = (L.level == 0) ? n - 1 : n;
-  const int cells = active_rows * m;
+  const int m2 = (m + 1) / 2;  // colour slots per row
+  const int rpp = blockDim.x / m2;  // rows per pass (m2 <= blockDim.x)
+  const int r0 = threadIdx.x / m2, q = threadIdx.x - r0 * m2;
   for (int s = 0; s < 2 * sweeps; ++s) {
     const int color = s & 1;
-    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-      const int i = c / m, j = c - i * m;
-      if (((i + j) & 1) != color) continue;
-      p[c] = tb_relax(L, i, j, p[c], b[c], [&](int di, int dj) { return p[c + di * m + dj]; });
+    if (r0 < rpp) {
+      for (int i = r0; i < active_rows; i += rpp) {
+        const int j = ((color - i) & 1) + 2 * q;
+        if (j >= m) continue;
+        const int c = i * m + j;
+        p[c] = tb_relax(L, i, j, b[c], [&](int di, int dj) { return p[c + di * m + dj]; });
+      }
     }
     __syncthreads();
   }
 }
 
-// p_in (may be null = zero guess, may alias e_out) is the top level's start
-__global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(TbBottom B,
-                                                              const float* b_in,
+__global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(TbBottom B, const float* b_in,
                                                               const float* p_in, float* e_out,
                                                               const double* __restrict__ Ainv,
                                                               const aj_mg_stats* gate) {
-  if (tb_gated(gate)) return;
+  if (gate->converged) return;
   extern __shared__ float sm[];
   float* sp = sm;               // iterates
   float* sb = sm + B.total;     // right-hand sides
@@ -278,7 +315,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(TbBottom B,
     float* bc = sb + B.off[k + 1];
     const int m = L.m;
     for (int c = threadIdx.x; c < C.n * C.m; c += blockDim.x) {
-      const int I = c / C.m, J = c - I * C.m;
+      const int I = fdiv_idx(c, C.m, B.inv_m[k + 1]), J = c - I * C.m;
       const int i = 2 * I, j = 2 * J;
       auto R = [&](int ii, int jj) {
         const int q = ii * m + jj;
@@ -290,16 +327,22 @@ __global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(TbBottom B,
     }
     __syncthreads();
   }
-  // coarsest: dense FP64 inverse
+  // coarsest: dense FP64 inverse, B.tpr threads per row, shuffle reduction
   {
     const TbLevel& L = B.lev[nl - 1];
     float* p = sp + B.off[nl - 1];
     const float* bb = sb + B.off[nl - 1];
     const int N = B.coarse_rows * L.m;
-    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    const int tpr = B.tpr;
+    const int rows_per_pass = blockDim.x / tpr;
+    const int lane = threadIdx.x % tpr;
+    for (int r0 = 0; r0 < N; r0 += rows_per_pass) {
+      const int r = r0 + threadIdx.x / tpr;
       double acc = 0.0;
-      for (int k = 0; k < N; ++k) acc += Ainv[(long)r * N + k] * (double)bb[k];
-      p[r] = (float)acc;
+      if (r < N)
+        for (int k = lane; k < N; k += tpr) acc += Ainv[(long)r * N + k] * (double)bb[k];
+      for (int off = tpr / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if (r < N && lane == 0) p[r] = (float)acc;
     }
     for (int r = N + threadIdx.x; r < L.n * L.m; r += blockDim.x) p[r] = 0.0f;
   }
@@ -311,7 +354,7 @@ __global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(TbBottom B,
     float* p = sp + B.off[k];
     const float* e = sp + B.off[k + 1];
     for (int c = threadIdx.x; c < L.n * L.m; c += blockDim.x) {
-      const int i = c / L.m, j = c - i * L.m;
+      const int i = fdiv_idx(c, L.m, B.inv_m[k]), j = c - i * L.m;
       if (L.level == 0 && i == L.n - 1) continue;
       p[c] = p[c] + tb_prolong(e, C.n, C.m, B.gam_next[k], i, j);
     }
@@ -328,12 +371,14 @@ __global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(TbBottom B,
 // FP64 defect correction
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, double* __restrict__ P,
+__global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, const T* __restrict__ rhs,
+                                                 double* __restrict__ P, T* __restrict__ bcopy,
                                                  int nx, int ny) {
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= (long)nx * ny) return;
   const int i = (int)(c / ny);
   P[c] = (p0 && i < nx - 1) ? (double)p0[c] : 0.0;
+  bcopy[c] = rhs[c];
 }
 
 // p' = P[cur] + e (e may be null), r = b - A p' in FP64 on rows < nx-1.
@@ -351,34 +396,37 @@ __global__ __launch_bounds__(256) void k_tb_resid(double* __restrict__ P0, doubl
   double* dst = cur ? P0 : P1;
   const long n = (long)nx * ny;
   double acc = 0.0;
-  for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long)gridDim.x * 256) {
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c < n) {
     const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
     if (i == nx - 1) {
       r32[c] = 0.0f;
       if (e) dst[c] = 0.0;
-      continue;
+    } else {
+      auto P = [&](long q) { return e ? src[q] + (double)e[q] : src[q]; };
+      const double pc = P(c);
+      const double W = i > 0 ? P(c - ny) : pc;
+      const double E = (i + 1 < nx - 1) ? P(c + ny) : 0.0;
+      const double S = j > 0 ? P(c - 1) : pc;
+      const double N = j < ny - 1 ? P(c + 1) : pc;
+      const double Ap = ax * ((E - pc) + (W - pc)) + ay * ((N - pc) + (S - pc));
+      const double r = (double)b[c] - Ap;
+      r32[c] = (float)r;
+      if (e) dst[c] = pc;
+      acc = r * r;
     }
-    auto P = [&](long q) { return e ? src[q] + (double)e[q] : src[q]; };
-    const double pc = P(c);
-    const double W = i > 0 ? P(c - ny) : pc;
-    const double E = (i + 1 < nx - 1) ? P(c + ny) : 0.0;
-    const double S = j > 0 ? P(c - 1) : pc;
-    const double N = j < ny - 1 ? P(c + 1) : pc;
-    const double Ap = ax * ((E - pc) + (W - pc)) + ay * ((N - pc) + (S - pc));
-    const double r = (double)b[c] - Ap;
-    r32[c] = (float)r;
-    if (e) dst[c] = pc;
-    acc += r * r;
   }
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-__global__ void k_tb_norm_final(const double* __restrict__ partial, int np, double dxdy, double tol,
-                                int flip, int cycles_added, aj_mg_stats* st, int* state) {
+__global__ void k_tb_norm_final(const double* __restrict__ partial, int np, double dxdy,
+                                const double* __restrict__ dtol, int flip, int cycles_added,
+                                aj_mg_stats* st, int* state) {
   if (st->converged) return;  // this outer iteration was skipped
   __shared__ double sh[16];
   double acc = 0.0;
+#pragma unroll 8
   for (int k = threadIdx.x; k < np; k += 256) acc += partial[k];
   const double tot = block_sum(acc, sh);
   if (threadIdx.x == 0) {
@@ -393,7 +441,7 @@ __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, doub
     }
     st->norm = nrm;
     if (!isfinite(nrm)) st->nonfinite = 1;
-    st->converged = (nrm <= tol) ? 1 : 0;
+    st->converged = (nrm <= dtol[0]) ? 1 : 0;
   }
 }
 
@@ -407,8 +455,9 @@ __global__ __launch_bounds__(256) void k_tb_out(const double* __restrict__ P0,
   out[c] = (T)(state[0] ? P1[c] : P0[c]);
 }
 
-__global__ void k_tb_begin(aj_mg_stats* st, int* state) {
+__global__ void k_tb_begin(aj_mg_stats* st, int* state, double* dtol, double tol) {
   if (threadIdx.x == 0) {
+    dtol[0] = tol;
     st->cycles = 0;
     st->outer = 0;
     st->converged = 0;
@@ -424,6 +473,7 @@ __global__ void k_tb_begin(aj_mg_stats* st, int* state) {
 // ---------------------------------------------------------------------------
 constexpr int kPX = 32, kPY = 64;  // pre tile
 constexpr int kQX = 32, kQY = 64;  // post tile
+constexpr int kTileThreads = 1024;
 
 static TbLevel make_level(const aj_mg_plan* p, int l) {
   TbLevel L;
@@ -437,6 +487,7 @@ static TbLevel make_level(const aj_mg_plan* p, int l) {
   const double g = l > 0 ? outlet_gamma(l) : 1.0;
   L.gam = (float)g;
   L.dE_last = (float)(1.0 - g) * L.axl;
+  L.inv_int = 1.0f / (2.0f * L.axl + 2.0f * L.ayl);
   return L;
 }
 
@@ -554,16 +605,19 @@ int tb_create(aj_mg_plan* p) {
     p->p64[0] = (double*)cv.take(n0 * 8);
     p->p64[1] = (double*)cv.take(n0 * 8);
     p->coarse_inv = (double*)cv.take((size_t)N * N * 8);
-    p->partial = (double*)cv.take(kTbNPartial * 8);
-    p->npartial = kTbNPartial;
+    p->npartial = (int)cdiv(n0, 256);
+    p->partial = (double*)cv.take((size_t)p->npartial * 8);
     p->stats = (aj_mg_stats*)cv.take(sizeof(aj_mg_stats));
     p->state = (int*)cv.take(64);
+    p->dtol = (double*)cv.take(64);
+    p->bcopy = cv.take(n0 * (p->dtype == AJ_F64 ? 8 : 4));
     if (pass == 0) {
       p->ws_bytes = cv.off;
       AJ_HIP(hipMalloc(&p->ws, p->ws_bytes));
     }
   }
   AJ_HIP(hipMemcpy(p->coarse_inv, A.data(), (size_t)N * N * 8, hipMemcpyHostToDevice));
+  AJ_HIP(hipHostMalloc((void**)&p->host_stats, sizeof(aj_mg_stats), hipHostMallocDefault));
   return AJ_OK;
 }
 
@@ -580,10 +634,18 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
       B.off[k] = off;
       off += B.lev[k].n * B.lev[k].m;
       B.gam_next[k] = (bot + k + 1 <= L) ? (float)outlet_gamma(bot + k + 1) : 1.0f;
+      B.inv_m[k] = 1.0f / (float)B.lev[k].m;
+    }
+    {
+      const int N = p->coarse_rows * p->lny[L];
+      int tpr = 1;
+      while (tpr < 64 && tpr * 2 * N <= kBottomThreads) tpr *= 2;
+      B.tpr = tpr;
     }
     B.total = off;
     B.coarse_rows = p->coarse_rows;
     const size_t smem = (size_t)2 * off * sizeof(float);
+    ProfScope prof(AJ_PROF_MG_BOTTOM, (double)B.lev[0].n * B.lev[0].m * (warm ? 12 : 8), st);
     hipLaunchKernelGGL(k_tb_bottom, dim3(1), dim3(kBottomThreads), smem, st, B,
                        (const float*)p->tB[lev], warm ? (const float*)p->tE[lev] : (const float*)nullptr,
                        p->tE[lev], (const double*)p->coarse_inv,
@@ -594,18 +656,77 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
   const TbLevel Lv = make_level(p, lev);
   const int n = Lv.n, m = Lv.m;
   dim3 gpre(cdiv(m, kPY), cdiv(n, kPX));
-  hipLaunchKernelGGL((k_tb_pre<kPX, kPY>), gpre, dim3(256), 0, st, Lv,
+  {
+  // p_in (warm), b read; Y and the quarter-size coarse rhs written
+  ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, (double)n * m * (warm ? 13 : 9), st);
+  hipLaunchKernelGGL((k_tb_pre<kPX, kPY, kTileThreads>), gpre, dim3(kTileThreads), 0, st, Lv,
                      warm ? (const float*)p->tE[lev] : (const float*)nullptr,
                      (const float*)p->tB[lev], p->tY[lev], p->tB[lev + 1],
                      (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_pre");
+  }
   if (int rc = tb_vcycle(p, lev + 1, false, st)) return rc;
   dim3 gpost(cdiv(m, kQY), cdiv(n, kQX));
-  hipLaunchKernelGGL((k_tb_post<kQX, kQY>), gpost, dim3(256), 0, st, Lv, (const float*)p->tY[lev],
+  {
+  // Y, quarter-size coarse correction, b read; E written
+  ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, (double)n * m * 13, st);
+  hipLaunchKernelGGL((k_tb_post<kQX, kQY, kTileThreads>), gpost, dim3(kTileThreads), 0, st, Lv,
+                     (const float*)p->tY[lev],
                      (const float*)p->tE[lev + 1], p->lnx[lev + 1], p->lny[lev + 1],
                      (float)outlet_gamma(lev + 1), (const float*)p->tB[lev], p->tE[lev],
                      (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_post");
+  }
+  return AJ_OK;
+}
+
+// launches of one defect-correction outer iteration (2 V-cycles, FP64
+// residual with the update p64 += e, norm and gate); only plan-owned buffers
+template <typename T>
+static int tb_outer_launches(aj_mg_plan* p, hipStream_t st) {
+  const int nx = p->nx, ny = p->ny;
+  const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
+  const int nb = p->npartial;
+  for (int c = 0; c < kCyclesPerOuter; ++c)
+    if (int rc = tb_vcycle(p, 0, c > 0, st)) return rc;
+  // p64, e, b read; p64', r32 written
+  ProfScope prof(AJ_PROF_MG_RESID, (double)nx * ny * (8 + 4 + sizeof(T) + 8 + 4), st);
+  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
+                     (const int*)p->state, (const float*)p->tE[0], (const T*)p->bcopy, p->tB[0],
+                     p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
+  AJ_LAUNCH_CHECK("k_tb_resid");
+  hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
+                     p->dx * p->dy, (const double*)p->dtol, 1, kCyclesPerOuter, p->stats, p->state);
+  AJ_LAUNCH_CHECK("k_tb_norm_final");
+  return AJ_OK;
+}
+
+template <typename T>
+static int tb_outer_iteration(aj_mg_plan* p, hipStream_t st) {
+  // live event timing needs the individual launches, so it bypasses the graph
+  if (!p->use_graph || prof_on()) return tb_outer_launches<T>(p, st);
+  if (!p->graph_exec) {
+    // capture once per plan on a private stream; replays launch on `st`
+    hipStream_t cs;
+    AJ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int rc = AJ_OK;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      rc = tb_outer_launches<T>(p, cs);
+      hipGraph_t g = nullptr;
+      e = hipStreamEndCapture(cs, &g);
+      if (e == hipSuccess && rc == AJ_OK) {
+        e = hipGraphInstantiate((hipGraphExec_t*)&p->graph_exec, g, nullptr, nullptr, 0);
+        p->graph = g;
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc != AJ_OK) return rc;
+    if (e != hipSuccess) return hip_fail(e, "capture of the multigrid outer iteration");
+  }
+  AJ_HIP(hipGraphLaunch((hipGraphExec_t)p->graph_exec, st));
   return AJ_OK;
 }
 
@@ -615,31 +736,40 @@ static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int ma
   const int nx = p->nx, ny = p->ny;
   const long n = (long)nx * ny;
   const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
-  const double dxdy = p->dx * p->dy;
-  const int nb = (int)std::min<long>(kTbNPartial, cdiv(n, 256));
-  hipLaunchKernelGGL(k_tb_begin, dim3(1), dim3(64), 0, st, p->stats, p->state);
+  const int nb = p->npartial;
+  hipLaunchKernelGGL(k_tb_begin, dim3(1), dim3(64), 0, st, p->stats, p->state, p->dtol, tol);
   AJ_LAUNCH_CHECK("k_tb_begin");
-  hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p0, p->p64[0], nx, ny);
+  hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p0, rhs, p->p64[0],
+                     (T*)p->bcopy, nx, ny);
   AJ_LAUNCH_CHECK("k_tb_init");
   // outer iteration 0: residual of the initial guess
   hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
-                     (const int*)p->state, (const float*)nullptr, rhs, p->tB[0], p->partial, nx,
-                     ny, ax, ay, (const aj_mg_stats*)p->stats);
+                     (const int*)p->state, (const float*)nullptr, (const T*)p->bcopy, p->tB[0],
+                     p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_resid");
   hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
-                     dxdy, tol, 0, 0, p->stats, p->state);
+                     p->dx * p->dy, (const double*)p->dtol, 0, 0, p->stats, p->state);
   AJ_LAUNCH_CHECK("k_tb_norm_final");
   const int max_outer = (max_cycles + kCyclesPerOuter - 1) / kCyclesPerOuter;
-  for (int k = 0; k < max_outer; ++k) {
-    for (int c = 0; c < kCyclesPerOuter; ++c)
-      if (int rc = tb_vcycle(p, 0, c > 0, st)) return rc;
-    hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
-                       (const int*)p->state, (const float*)p->tE[0], rhs, p->tB[0], p->partial, nx,
-                       ny, ax, ay, (const aj_mg_stats*)p->stats);
-    AJ_LAUNCH_CHECK("k_tb_resid");
-    hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
-                       dxdy, tol, 1, kCyclesPerOuter, p->stats, p->state);
-    AJ_LAUNCH_CHECK("k_tb_norm_final");
+  if (!p->adaptive) {
+    // fully stream-ordered: every iteration is enqueued, those after
+    // convergence exit at their first instruction
+    for (int k = 0; k < max_outer; ++k)
+      if (int rc = tb_outer_iteration<T>(p, st)) return rc;
+  } else {
+    // enqueue the number of iterations the previous solve needed, then read
+    // the device state once and top up one iteration at a time
+    int done = std::min(std::max(p->last_outer, 1), max_outer);
+    for (int k = 0; k < done; ++k)
+      if (int rc = tb_outer_iteration<T>(p, st)) return rc;
+    for (;;) {
+      AJ_HIP(hipMemcpyAsync(p->host_stats, p->stats, sizeof(aj_mg_stats), hipMemcpyDeviceToHost, st));
+      AJ_HIP(hipStreamSynchronize(st));
+      if (p->host_stats->converged || p->host_stats->nonfinite || done >= max_outer) break;
+      if (int rc = tb_outer_iteration<T>(p, st)) return rc;
+      ++done;
+    }
+    p->last_outer = std::max(p->host_stats->outer, 1);
   }
   hipLaunchKernelGGL(k_tb_out<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, (const double*)p->p64[0],
                      (const double*)p->p64[1], (const int*)p->state, p_out, n);

@@ -68,8 +68,8 @@ __device__ __forceinline__ T masked(T val, T m) {
 
 // Fused RK kernel.  FAST = RK2 (2 stages), else RK3.  HYPER adds the
 // biharmonic term (per-stage stencil radius 2).
-template <typename T, int TX, int TY, bool FAST, bool HYPER>
-__global__ __launch_bounds__(256) void k_rk_fused(RkArgs<T> a) {
+template <typename T, int TX, int TY, bool FAST, bool HYPER, int NTH>
+__global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
   constexpr int S = FAST ? 2 : 3;
   constexpr int R1 = HYPER ? 2 : 1;  // per-stage stencil radius
   constexpr int H = S * R1;          // total halo
@@ -98,25 +98,43 @@ __global__ __launch_bounds__(256) void k_rk_fused(RkArgs<T> a) {
   const bool has_nsgs = a.nu_sgs != nullptr;
   const bool has_sdf = a.sdf != nullptr;
 
+  // mask of this thread's tile cells (final stage), loaded up front
+  constexpr int NT = (TX * TY + NTH - 1) / NTH;
+  T rmask[NT];
+#pragma unroll
+  for (int it = 0; it < NT; ++it) {
+    const int idx = min(tid + it * NTH, TX * TY - 1);
+    const int gx = clampi(x0 + idx / TY, 0, nx - 1), gy = clampi(y0 + idx % TY, 0, ny - 1);
+    rmask[it] = a.mask[(long)gx * ny + gy];
+  }
   // ---- load u, v on margin H; chi factor and nu_tot on margin H - R1 ----
-  for (int idx = tid; idx < RN; idx += 256) {
-    const int lx = idx / RW, ly = idx - lx * RW;
-    const int gx = gx0 + lx, gy = gy0 + ly;
-    if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-    const long g = (long)gx * ny + gy;
-    s_u[idx] = a.u[g];
-    s_v[idx] = a.v[g];
-    const int m = H - R1;
-    if (lx >= H - m && lx < H + TX + m && ly >= H - m && ly < H + TY + m) {
-      T chi;
-      if (has_sdf) {
-        // sigmoid(-sdf / (0.1 dx)) (rk3_simple_new.py:198)
-        chi = T(1) / (T(1) + exp(a.sdf[g] * a.sdf_scale));
-      } else {
-        chi = T(1) - a.mask[g];
-      }
+  // All loads of a thread are issued before any is consumed (addresses are
+  // clamped into the domain, so every load is in bounds; slots outside the
+  // domain hold edge copies and are never read by the stencils).
+  {
+    constexpr int NIT = (RN + NTH - 1) / NTH;
+    T ru[NIT], rv[NIT], rc[NIT], rn[NIT];
+    const T* csrc = has_sdf ? a.sdf : a.mask;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int idx = min(tid + it * NTH, RN - 1);
+      const int lx = idx / RW, ly = idx - lx * RW;
+      const long g = (long)clampi(gx0 + lx, 0, nx - 1) * ny + clampi(gy0 + ly, 0, ny - 1);
+      ru[it] = a.u[g];
+      rv[it] = a.v[g];
+      rc[it] = csrc[g];
+      rn[it] = has_nsgs ? a.nu_sgs[g] : T(0);
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int idx = tid + it * NTH;
+      if (idx >= RN) break;
+      s_u[idx] = ru[it];
+      s_v[idx] = rv[it];
+      // sigmoid(-sdf / (0.1 dx)) (rk3_simple_new.py:198), else 1 - mask
+      const T chi = has_sdf ? T(1) / (T(1) + exp(rc[it] * a.sdf_scale)) : T(1) - rc[it];
       s_iD[idx] = T(1) / (T(1) + a.dt * chi * a.inv_eta);
-      s_nt[idx] = has_nsgs ? a.nu + a.nu_sgs[g] : a.nu;
+      s_nt[idx] = has_nsgs ? a.nu + rn[it] : a.nu;
     }
   }
   __syncthreads();
@@ -130,7 +148,7 @@ __global__ __launch_bounds__(256) void k_rk_fused(RkArgs<T> a) {
       // laplacian of the stage input on margin m + 1
       const int mm = m + 1;
       const int W = TY + 2 * mm, Hh = TX + 2 * mm;
-      for (int idx = tid; idx < W * Hh; idx += 256) {
+      for (int idx = tid; idx < W * Hh; idx += NTH) {
         const int lx = H - mm + idx / W, ly = H - mm + idx % W;
         const int gx = gx0 + lx, gy = gy0 + ly;
         if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
@@ -146,18 +164,13 @@ __global__ __launch_bounds__(256) void k_rk_fused(RkArgs<T> a) {
     }
     T* ou = (s == 1) ? s_au : s_bu;
     T* ov = (s == 1) ? s_av : s_bv;
-    const int W = TY + 2 * m, Hh = TX + 2 * m;
-    for (int idx = tid; idx < W * Hh; idx += 256) {
-      const int lx = H - m + idx / W, ly = H - m + idx % W;
-      const int gx = gx0 + lx, gy = gy0 + ly;
-      if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-      const int l = lx * RW + ly;
+    // stage value at LDS cell l (RK3: rk3_simple_new.py:315-331; RK2: :339-347)
+    auto stage = [&](int l, int gx, int gy, T& nu_, T& nv_) {
       const int lW = gx > 0 ? l - RW : l, lE = gx < nx - 1 ? l + RW : l;
       const int lS = gy > 0 ? l - 1 : l, lN = gy < ny - 1 ? l + 1 : l;
       T ku, kv;
       rk_rhs<T, HYPER>(fu, fv, s_lu, s_lv, l, lW, lE, lS, lN, s_nt[l], a, ku, kv);
       const T u0 = s_u[l], v0 = s_v[l], iD = s_iD[l];
-      T nu_, nv_;
       if constexpr (FAST) {
         if (s == 1) {
           nu_ = (u0 + a.dt * ku) * iD;
@@ -185,26 +198,47 @@ __global__ __launch_bounds__(256) void k_rk_fused(RkArgs<T> a) {
           }
         }
       }
-      if (s < S) {
+    };
+    if (s < S) {
+      const int W = TY + 2 * m, Hh = TX + 2 * m;
+      for (int idx = tid; idx < W * Hh; idx += NTH) {
+        const int lx = H - m + idx / W, ly = H - m + idx % W;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+        const int l = lx * RW + ly;
+        T nu_, nv_;
+        stage(l, gx, gy, nu_, nv_);
         ou[l] = nu_;
         ov[l] = nv_;
-        continue;
       }
-      // ---- final stage: hard mask, then BCs (rk3_simple_new.py:332-336,
-      // 264-279).  Boundary cells copy the masked value of an interior
-      // source; each source writes its own targets so every boundary cell
-      // is written exactly once.
-      const bool inlet = (gx == 0);
-      const bool interior = gx > 0 && gx < nx - 1 && gy > 0 && gy < ny - 1;
+      __syncthreads();
+      fu = ou;
+      fv = ov;
+      continue;
+    }
+    // ---- final stage on the tile: hard mask, then BCs
+    // (rk3_simple_new.py:332-336, 264-279).  Boundary cells copy the masked
+    // value of an interior source; each source writes its own targets so
+    // every boundary cell is written exactly once.
+#pragma unroll
+    for (int it = 0; it < NT; ++it) {
+      const int idx = tid + it * NTH;
+      if (idx >= TX * TY) break;
+      const int lx = H + idx / TY, ly = H + idx % TY;
+      const int gx = gx0 + lx, gy = gy0 + ly;
+      if (gx >= nx || gy >= ny) continue;
       const long g = (long)gx * ny + gy;
-      if (inlet) {
+      if (gx == 0) {
         const bool wall = (gy == 0 || gy == ny - 1);
         a.u_out[g] = (wall && !a.slip) ? T(0) : a.U_inf;
         a.v_out[g] = T(0);
         continue;
       }
+      const bool interior = gx < nx - 1 && gy > 0 && gy < ny - 1;
       if (!interior) continue;
-      const T m0 = a.mask[g];
+      T nu_, nv_;
+      stage(lx * RW + ly, gx, gy, nu_, nv_);
+      const T m0 = rmask[it];
       const T fu_ = masked(nu_, m0), fv_ = masked(nv_, m0);
       a.u_out[g] = fu_;
       a.v_out[g] = fv_;
@@ -225,9 +259,6 @@ __global__ __launch_bounds__(256) void k_rk_fused(RkArgs<T> a) {
         if (out_row) { a.u_out[g + ny + 1] = uw; a.v_out[g + ny + 1] = T(0); }
       }
     }
-    __syncthreads();
-    fu = ou;
-    fv = ov;
   }
 }
 
@@ -241,7 +272,11 @@ static int launch_rk(const RkArgs<T>& a, hipStream_t st) {
   const size_t smem = RN * NARR * sizeof(T);
   static_assert(RN * NARR * sizeof(T) <= 65536, "LDS budget");
   dim3 grid(cdiv(a.ny, TY), cdiv(a.nx, TX));
-  hipLaunchKernelGGL((k_rk_fused<T, TX, TY, FAST, HYPER>), grid, dim3(256), smem, st, a);
+  constexpr int NTH = 512;
+  // algorithmic bytes: u, v, chi source, mask (+ nu_sgs) read, u*, v* written
+  const double bytes = (double)a.nx * a.ny * sizeof(T) * (6 + (a.nu_sgs ? 1 : 0));
+  ProfScope prof(AJ_PROF_RK, bytes, st);
+  hipLaunchKernelGGL((k_rk_fused<T, TX, TY, FAST, HYPER, NTH>), grid, dim3(NTH), smem, st, a);
   AJ_LAUNCH_CHECK("k_rk_fused");
   return AJ_OK;
 }
@@ -431,6 +466,8 @@ extern "C" int aj_divergence_rhs(int dtype, const void* us, const void* vs, cons
   if (!us || !vs || !rhs) return fail(AJ_ERR_INVALID, "aj_divergence_rhs: null pointer");
   if (int rc = check_grid(nx, ny)) return rc;
   hipStream_t st = (hipStream_t)stream;
+  // u*, v* (+ mask) read, rhs written
+  ProfScope prof(AJ_PROF_RHS, (double)nx * ny * (dtype == AJ_F64 ? 8 : 4) * (mask ? 4 : 3), st);
   if (dtype == AJ_F32) {
     hipLaunchKernelGGL(k_div_rhs<float>, row_grid(nx, ny), dim3(256), 0, st, (const float*)us,
                        (const float*)vs, (const float*)mask, (float*)rhs, nx, ny,
@@ -452,6 +489,8 @@ extern "C" int aj_project(int dtype, const void* us, const void* vs, const void*
   if (!us || !vs || !p || !mask || !uo || !vo) return fail(AJ_ERR_INVALID, "aj_project: null pointer");
   if (int rc = check_grid(nx, ny)) return rc;
   hipStream_t st = (hipStream_t)stream;
+  // u*, v*, p, mask read; u, v written
+  ProfScope prof(AJ_PROF_PROJECT, (double)nx * ny * (dtype == AJ_F64 ? 8 : 4) * 6, st);
   if (dtype == AJ_F32) {
     hipLaunchKernelGGL(k_project<float>, row_grid(nx, ny), dim3(256), 0, st, (const float*)us,
                        (const float*)vs, (const float*)p, (const float*)mask, (float*)uo,

@@ -50,6 +50,19 @@ class MultigridPlan:
     def workspace_bytes(self) -> int:
         return int(_lib.load().aj_mg_workspace_bytes(self.handle))
 
+    def set_option(self, adaptive: bool | None = None, graph: bool | None = None):
+        """Textbook plans: ``adaptive`` reads convergence between outer
+        iterations (host sync, default) instead of enqueueing all
+        ``max_cycles``; ``graph`` replays a captured outer iteration
+        (default) instead of launching its kernels one by one."""
+        lib = _lib.load()
+        if adaptive is not None:
+            _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_ADAPTIVE, int(adaptive)),
+                       "aj_mg_set_option")
+        if graph is not None:
+            _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_GRAPH, int(graph)),
+                       "aj_mg_set_option")
+
     @property
     def num_levels(self) -> int:
         return int(_lib.load().aj_mg_num_levels(self.handle))

@@ -73,22 +73,21 @@ def main():
     log(f"plain MG: {steps_per_s:.1f} steps/s, V-cycles/step (warmup) {cycles}, last norm "
         f"{s['norm']:.2e}")
 
-    # live per-kernel timing of the fused RK kernel with HIP events on its stream
-    rk_ms = None
-    st = torch.cuda.current_stream(dev)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    u, v = stp.state
-    reps = 50
-    ev0.record(st)
-    for _ in range(reps):
-        aj.rk_step_unified(u, v, stp.dt, stp.nu, stp.dx, stp.dy, stp.mask, sdf=stp.sdf,
-                           U_inf=stp.U_inf, brinkman_eta=stp.eta, out=(stp.us, stp.vs))
-    ev1.record(st)
+    # live per-kernel timing: HIP event pairs around every instrumented launch
+    # on the stream it runs on (multigrid iterations launched eagerly)
+    from aerojax_amd import _lib
+    _lib.profile_begin(20000)
+    for _ in range(min(args.steps, 10)):
+        stp.step()
     torch.cuda.synchronize()
-    rk_ms = ev0.elapsed_time(ev1) / reps
-    rk_bytes = 24 * nx * ny  # reads u, v, sdf, mask; writes u*, v* (FP32)
-    achieved = rk_bytes / (rk_ms * 1e-3) / 1e9
-
+    _lib.profile_end()
+    prof = _lib.profile_read()
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    bytes_per_launch = d["bytes"] / d["count"]
+    achieved = d["gbps"]
+    log("kernel profile: " + ", ".join(f"{k} {v['avg_us']:.1f}us x{v['count']} "
+                                         f"{v['gbps']:.0f}GB/s" for k, v in prof.items()))
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         from oracle import reference_step as R
@@ -128,9 +127,12 @@ def main():
         "config": {"workload": case["name"], "geometry": case["desc"], "Re": round(case["Re"], 1),
                    "grid": [nx, ny], "tolerance": case["tol"], "solver": "plain MG (zero guess)",
                    "vcycles_per_step": cycles[-1] if cycles else None},
-        "roofline": {"kernel": "k_rk_fused", "bound": "hbm", "achieved": achieved, "peak": 8000.0,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
                      "unit": "GB/s", "frac": achieved / 8000.0, "traffic": None,
-                     "kernel_ms": rk_ms, "bytes_per_launch": rk_bytes},
+                     "avg_launch_us": d["avg_us"], "bytes_per_launch": bytes_per_launch},
+        "kernels": {k: {"avg_us": round(v["avg_us"], 2), "launches_per_step":
+                        v["count"] / min(args.steps, 10), "gbps": round(v["gbps"], 1)}
+                    for k, v in prof.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -124,6 +124,19 @@ int aj_mg_destroy(aj_mg_plan* plan);
 size_t aj_mg_workspace_bytes(const aj_mg_plan* plan);
 /* Number of levels of the plan's hierarchy (coarsest included). */
 int aj_mg_num_levels(const aj_mg_plan* plan);
+/* Plan options (no counterpart in the reference, whose solver always runs
+ * v_cycles cycles).  TEXTBOOK plans only:
+ *   AJ_MG_OPT_ADAPTIVE = 1 (default): aj_poisson_mg enqueues as many outer
+ *     iterations as the previous solve on this plan needed, synchronises
+ *     `stream` to read the convergence state and tops up one iteration at
+ *     a time (host-blocking).  0: every iteration up to max_cycles is
+ *     enqueued and the call never blocks (iterations after convergence
+ *     exit at their first instruction); use 0 under stream capture.
+ *   AJ_MG_OPT_GRAPH = 1 (default): one outer iteration is captured once
+ *     into a hipGraph (plan-owned buffers only) and replayed; 0: eager
+ *     kernel launches. */
+enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1 };
+int aj_mg_set_option(aj_mg_plan* plan, int option, int value);
 
 #define AJ_MG_MAX_NORMS 64
 typedef struct aj_mg_stats {
@@ -147,6 +160,31 @@ int aj_poisson_mg(aj_mg_plan* plan, const void* rhs, const void* p0,
 /* Copy the statistics of the last aj_poisson_mg call on this plan
  * (synchronises `stream`). */
 int aj_mg_get_stats(aj_mg_plan* plan, aj_mg_stats* out, void* stream);
+
+/* ---- Live kernel timing -------------------------------------------------
+ * No counterpart in the reference (its profiling is wall-clock around the
+ * jitted step, viewer/simulation_controller.py:383-408).  Between
+ * aj_profile_begin and aj_profile_end every instrumented launch records a
+ * HIP event pair on the stream it runs on (up to max_records launches; a
+ * textbook plan launches its kernels eagerly instead of replaying its
+ * graph).  aj_profile_read sums, over the recorded launches with `tag`,
+ * the event time and the algorithmic bytes (compulsory reads + writes) the
+ * launch moves; it waits for those events. */
+enum {
+  AJ_PROF_RK = 0,          /* fused RK predictor                        */
+  AJ_PROF_RHS = 1,         /* divergence -> rhs                         */
+  AJ_PROF_PROJECT = 2,     /* projection                                */
+  AJ_PROF_MG_PRE0 = 3,     /* textbook pre-smooth + restrict, level 0   */
+  AJ_PROF_MG_POST0 = 4,    /* textbook prolong + post-smooth, level 0   */
+  AJ_PROF_MG_PRE = 5,      /* same, levels >= 1                         */
+  AJ_PROF_MG_POST = 6,
+  AJ_PROF_MG_BOTTOM = 7,   /* single-workgroup coarse levels            */
+  AJ_PROF_MG_RESID = 8,    /* FP64 defect-correction residual           */
+  AJ_PROF_NUM_TAGS = 9
+};
+int aj_profile_begin(int max_records);
+int aj_profile_end(void);
+int aj_profile_read(int tag, int* count, double* total_ms, double* total_bytes);
 
 #ifdef __cplusplus
 }

@@ -201,7 +201,11 @@ def test_textbook_multigrid_one_outer_iteration_parity(gpu, shape):
                                                 max_cycles=2, return_stats=True)
     assert st["cycles"] == 2 and st["outer"] == 1
     check(host(got), ref, None, torch.float32, f"mg-textbook 1 outer {shape}")
-    assert np.allclose(st["norms"][:2], st_ref["norms"][:2], rtol=1e-4)
+    # initial residual: same FP64 arithmetic on the same b
+    assert np.isclose(st["norms"][0], st_ref["norms"][0], rtol=1e-12)
+    # after two FP32 V-cycles the residual is ~1e-3 of b: FP32 rounding of p
+    # (rel ~1e-7) is amplified by ||A|| ||p|| / ||r||, so compare loosely
+    assert np.isclose(st["norms"][1], st_ref["norms"][1], rtol=2e-2)
 
 
 @pytest.mark.parametrize("shape", [(512, 96), (2048, 512)])
@@ -285,3 +289,33 @@ def test_step_textbook_reaches_tolerance(gpu):
     s = plan.stats()
     assert s["converged"] and s["norm"] <= 1e-6
     assert torch.isfinite(new.u).all()
+
+
+def test_textbook_graph_replays_match_eager(gpu):
+    """The captured outer iteration, replayed across solves with eager work
+    and synchronisations in between, gives the eager launches' result."""
+    import aerojax_amd as aj
+    nx, ny = 512, 128
+    dx, dy = 4.0 / nx, 1.0 / ny
+    plans = {}
+    for name, (adaptive, graph) in {"eager": (True, False), "graph": (True, True),
+                                    "graph_noadapt": (False, True)}.items():
+        pl = aj.MultigridPlan(nx, ny, dx, dy, torch.float32, "textbook", 0)
+        pl.set_option(adaptive=adaptive, graph=graph)
+        plans[name] = pl
+    rng = np.random.default_rng(12)
+    for k in range(4):
+        b = dev(rng.normal(size=(nx, ny)) * 300 * (1 + k), torch.float32)
+        outs, stats = {}, {}
+        for name, pl in plans.items():
+            o = torch.empty_like(b)
+            pl.solve(b, o, None, 40, 1e-6)
+            torch.cuda.synchronize()
+            outs[name], stats[name] = o.clone(), pl.stats()
+            b.mul_(1.0)  # eager work between replays
+        for name in ("graph", "graph_noadapt"):
+            assert torch.equal(outs[name], outs["eager"]), (k, name)
+            assert stats[name]["cycles"] == stats["eager"]["cycles"]
+        assert stats["eager"]["converged"]
+    for pl in plans.values():
+        pl.close()
