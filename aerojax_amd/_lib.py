@@ -157,3 +157,23 @@ def profile_read() -> dict:
 
 def version() -> str:
     return load().aj_version().decode()
+
+
+_NN_SIGS = {
+    "aj_nn_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i]),
+    "aj_nn_destroy": (_i, [_vp]),
+    "aj_nn_num_params": (ctypes.c_long, [_vp]),
+    "aj_nn_get_params": (_i, [_vp, _vp, ctypes.c_long, _i]),
+    "aj_nn_set_params": (_i, [_vp, _vp, ctypes.c_long]),
+    "aj_nn_forward": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "aj_nn_loss_grad": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
+    "aj_nn_adam": (_i, [_vp, ctypes.c_float, ctypes.c_float, _vp]),
+    "aj_nn_train_step": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_float, _vp, _vp]),
+    "aj_nn_make_sample": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_float, ctypes.c_float,
+                               _vp, _vp, _vp]),
+    "aj_nn_warm_start": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_float, ctypes.c_float,
+                              _vp]),
+}
+_SIGS.update(_NN_SIGS)
+EXPORTS = EXPORTS + tuple(_NN_SIGS)
+PROF_TAGS = PROF_TAGS + ("nn_conv", "nn_dw")

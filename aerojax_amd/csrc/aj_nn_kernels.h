@@ -1,0 +1,327 @@
+// Kernels of the pressure warm-start network: BF16 MFMA implicit-GEMM 3x3
+// convolutions (forward, input gradient, weight gradient), GELU, 2x2
+// average pooling, nearest upsampling, 1x1 head, MSE, Adam.
+//
+// Activations are NHWC bf16 (channels contiguous) with an explicit channel
+// stride, so a decoder input is one buffer whose first channels come from
+// the upsampled coarser level and whose last channels are the encoder skip
+// written there directly (no concatenation copy).  Convolutions are
+// "same" 3x3 with zero padding.
+#pragma once
+
+#include "aj_common.h"
+
+namespace aj {
+namespace nn {
+
+typedef unsigned short bf16_t;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float((unsigned)b << 16); }
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (bf16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float x) {
+  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+__device__ __forceinline__ bf16x8 load8(const bf16_t* p) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, implicit GEMM on v_mfma_f32_16x16x32_bf16.
+// D[pixel][co] = sum_{tap, ci} in[pixel + tap][ci] * w[co][tap][ci]
+// A operand (16 pixels x 32 ci): lane l holds pixel l%16, ci 8*(l/16)..+7.
+// B operand (32 ci x 16 co):     lane l holds co l%16,    ci 8*(l/16)..+7.
+// D (16 pixels x 16 co):         lane l holds co l%16, pixels 4*(l/16)..+3.
+// A workgroup (4 waves) covers 64 pixels x NT output channels.
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+  const bf16_t* in;
+  int ldi, ci;          // input channel stride, channels (multiple of 32)
+  const bf16_t* w;      // [co][9][ci]
+  const float* bias;    // [co] or null
+  int co;               // output channels (multiple of NT)
+  int B, H, W;
+  bf16_t* out;          // post-activation bf16 (may be null)
+  int ldo;
+  bf16_t* pre;          // pre-activation bf16 (may be null)
+  int ldp;
+  float* outf;          // fp32 output (input-gradient mode, may be null)
+  int ldf, accum;
+  int act;              // 1 = GELU
+};
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+  constexpr int NJ = NT / 16;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int HW = a.H * a.W;
+  const long P = (long)a.B * HW;
+  const long pbase = (long)blockIdx.x * 64 + wv * 16;
+  const long pa = pbase + (lane & 15);  // A-operand pixel of this lane
+  const int kq = (lane >> 4) * 8;       // channel offset of this lane's k-slice
+  int b = 0, h = 0, w = 0;
+  const bool pv = pa < P;
+  if (pv) {
+    b = (int)(pa / HW);
+    const int r = (int)(pa - (long)b * HW);
+    h = r / a.W;
+    w = r - h * a.W;
+  }
+  const int co0 = blockIdx.y * NT;
+  floatx4 acc[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < 9; ++t) {
+    const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
+    const bool ok = pv && hh >= 0 && hh < a.H && ww >= 0 && ww < a.W;
+    const bf16_t* src = a.in + ((long)(b * a.H + (ok ? hh : 0)) * a.W + (ok ? ww : 0)) * a.ldi + kq;
+    const bf16_t* wrow = a.w + ((long)(co0 + (lane & 15)) * 9 + t) * a.ci + kq;
+    for (int c0 = 0; c0 < a.ci; c0 += 32) {
+      const bf16x8 A = ok ? load8(src + c0) : zero8();
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const bf16x8 Bm = load8(wrow + (long)j * 16 * 9 * a.ci + c0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bm, acc[j], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int co = co0 + 16 * j + (lane & 15);
+    const float bv = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long p = pbase + (lane >> 4) * 4 + i;
+      if (p >= P) continue;
+      const float v = acc[j][i] + bv;
+      if (a.pre) a.pre[p * a.ldp + co] = f2bf(v);
+      if (a.out) a.out[p * a.ldo + co] = f2bf(a.act ? gelu(v) : v);
+      if (a.outf) {
+        float* o = a.outf + p * a.ldf + co;
+        *o = a.accum ? *o + v : v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[co][ci][kh][kw] += sum_p g[p][co] * x[p + tap][ci]
+// (fp32, torch OIHW layout, atomics across pixel chunks).
+// A (16 co x 32 pixels) and B (32 pixels x 16 ci) are staged transposed in
+// LDS: sg[co][px], sx[ci][row][px] for 32 pixels of one image row and the
+// 3 x 34 input pixels their taps touch.  Wave wv owns output channels
+// co0 + 16 wv .. +15 and keeps 9 taps x 2 ci sub-tiles of accumulators.
+// ---------------------------------------------------------------------------
+struct DwArgs {
+  const bf16_t* g;   // dPre, NHWC, stride ldg, co channels
+  int ldg, co;
+  const bf16_t* x;   // conv input, NHWC, stride ldx, ci_real channels read
+  int ldx, ci;       // ci: channels of the weight (real, not padded)
+  float* dw;         // [co][ci][3][3]
+  int B, H, W;
+  int rows_per_block;
+};
+
+__global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
+  __shared__ bf16_t sg[64][40];      // [co][px] (32 used, padded rows)
+  __shared__ bf16_t sx[32][3][48];   // [ci][dh][px + 1] (34 used)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int co0 = blockIdx.y * 64, ci0 = blockIdx.z * 32;
+  const int nseg = (a.W + 31) / 32;
+  const long nrows = (long)a.B * a.H;
+  floatx4 acc[9][2];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) acc[t][0] = acc[t][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const long r0 = (long)blockIdx.x * a.rows_per_block;
+  const long r1 = r0 + a.rows_per_block < nrows ? r0 + a.rows_per_block : nrows;
+  for (long row = r0; row < r1; ++row) {
+    const int b = (int)(row / a.H), h = (int)(row - (long)b * a.H);
+    for (int s = 0; s < nseg; ++s) {
+      const int w0 = s * 32;
+      __syncthreads();
+      // stage g: 32 px x 64 co (transposed)
+      for (int e = threadIdx.x; e < 32 * 64; e += 256) {
+        const int px = e >> 6, c = e & 63;
+        const int ww = w0 + px, co = co0 + c;
+        bf16_t v = 0;
+        if (ww < a.W && co < a.co) v = a.g[((long)(b * a.H + h) * a.W + ww) * a.ldg + co];
+        sg[c][px] = v;
+      }
+      // stage x: 3 rows x 34 px x 32 ci (transposed)
+      for (int e = threadIdx.x; e < 3 * 34 * 32; e += 256) {
+        const int c = e & 31, q = e >> 5;
+        const int dh = q / 34, px = q - dh * 34;
+        const int hh = h + dh - 1, ww = w0 + px - 1, ci = ci0 + c;
+        bf16_t v = 0;
+        if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W && ci < a.ci)
+          v = a.x[((long)(b * a.H + hh) * a.W + ww) * a.ldx + ci];
+        sx[c][dh][px] = v;
+      }
+      __syncthreads();
+      if (co0 + wv * 16 >= a.co) continue;
+      const int kq = (lane >> 4) * 8;
+      const bf16x8 A = *reinterpret_cast<const bf16x8*>(&sg[wv * 16 + (lane & 15)][kq]);
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const int dh = t / 3, dw = t % 3;
+#pragma unroll
+        for (int nsub = 0; nsub < 2; ++nsub) {
+          const bf16_t* src = &sx[nsub * 16 + (lane & 15)][dh][kq + dw];
+          bf16x8 Bm;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) Bm[k] = __builtin_bit_cast(__bf16, src[k]);
+          acc[t][nsub] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bm, acc[t][nsub], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (co0 + wv * 16 >= a.co) return;
+  // D (16 co x 16 ci): lane holds ci lane%16, co rows 4*(lane/16)..+3
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int nsub = 0; nsub < 2; ++nsub) {
+      const int ci = ci0 + nsub * 16 + (lane & 15);
+      if (ci >= a.ci) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int co = co0 + wv * 16 + (lane >> 4) * 4 + i;
+        if (co >= a.co) continue;
+        atomicAdd(&a.dw[((long)co * a.ci + ci) * 9 + t], acc[t][nsub][i]);
+      }
+    }
+}
+
+// bias gradient db[c] += sum_p g[p][c]
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16_t* g, int ldg, int C, long P,
+                                                   float* db, int chunk) {
+  const int c = threadIdx.x % C;  // C <= 256 divides 256 or C == 256
+  const int lanes_per_c = 256 / C;
+  const int sub = threadIdx.x / C;
+  const long p0 = (long)blockIdx.x * chunk;
+  const long p1 = p0 + chunk < P ? p0 + chunk : P;
+  float acc = 0.f;
+  for (long p = p0 + sub; p < p1; p += lanes_per_c) acc += bf2f(g[p * ldg + c]);
+  atomicAdd(&db[c], acc);
+}
+
+// dPre = dPost * gelu'(pre)  (fp32 in, bf16 out)
+__global__ __launch_bounds__(256) void k_gelu_back(const float* dpost, int ldd,
+                                                   const bf16_t* pre, int ldp, bf16_t* dpre,
+                                                   int ldo, int C, long P) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= P * C) return;
+  const long p = e / C;
+  const int c = (int)(e - p * C);
+  const float x = bf2f(pre[p * ldp + c]);
+  dpre[p * ldo + c] = f2bf(dpost[p * ldd + c] * gelu_grad(x));
+}
+
+// 2x2 average pool of a channel slice: out[B][H/2][W/2][C]
+__global__ __launch_bounds__(256) void k_pool(const bf16_t* in, int ldi, bf16_t* out, int ldo,
+                                              int C, int B, int H, int W) {
+  const int Ho = H / 2, Wo = W / 2;
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long)B * Ho * Wo * C) return;
+  const long po = e / C;
+  const int c = (int)(e - po * C);
+  const int b = (int)(po / (Ho * Wo));
+  const int r = (int)(po - (long)b * Ho * Wo);
+  const int ho = r / Wo, wo = r - ho * Wo;
+  const long p00 = ((long)(b * H + 2 * ho) * W + 2 * wo);
+  const float s = bf2f(in[p00 * ldi + c]) + bf2f(in[(p00 + 1) * ldi + c]) +
+                  bf2f(in[(p00 + W) * ldi + c]) + bf2f(in[(p00 + W + 1) * ldi + c]);
+  out[po * ldo + c] = f2bf(0.25f * s);
+}
+
+// pool backward: din[2ho+a][2wo+b][c] += dout[ho][wo][c] / 4  (fp32)
+__global__ __launch_bounds__(256) void k_pool_back(const float* dout, int ldo, float* din,
+                                                   int ldi, int C, int B, int H, int W) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long)B * H * W * C) return;
+  const long p = e / C;
+  const int c = (int)(e - p * C);
+  const int b = (int)(p / ((long)H * W));
+  const int r = (int)(p - (long)b * H * W);
+  const int h = r / W, w = r - h * W;
+  const long po = ((long)(b * (H / 2) + h / 2) * (W / 2) + w / 2);
+  din[p * ldi + c] += 0.25f * dout[po * ldo + c];
+}
+
+// nearest 2x upsample: out[h][w][c] = in[h/2][w/2][c]
+__global__ __launch_bounds__(256) void k_up(const bf16_t* in, int ldi, bf16_t* out, int ldo,
+                                            int C, int B, int H, int W) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long)B * H * W * C) return;
+  const long p = e / C;
+  const int c = (int)(e - p * C);
+  const int b = (int)(p / ((long)H * W));
+  const int r = (int)(p - (long)b * H * W);
+  const int h = r / W, w = r - h * W;
+  const long pi = ((long)(b * (H / 2) + h / 2) * (W / 2) + w / 2);
+  out[p * ldo + c] = in[pi * ldi + c];
+}
+
+// upsample backward: din[ho][wo][c] = sum of the 2x2 children (fp32)
+__global__ __launch_bounds__(256) void k_up_back(const float* dout, int ldo, float* din, int ldi,
+                                                 int C, int B, int Ho, int Wo) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long)B * Ho * Wo * C) return;
+  const long po = e / C;
+  const int c = (int)(e - po * C);
+  const int b = (int)(po / ((long)Ho * Wo));
+  const int r = (int)(po - (long)b * Ho * Wo);
+  const int ho = r / Wo, wo = r - ho * Wo;
+  const int W = 2 * Wo;
+  const long p00 = ((long)(b * 2 * Ho + 2 * ho) * W + 2 * wo);
+  din[po * ldi + c] = dout[p00 * ldo + c] + dout[(p00 + 1) * ldo + c] +
+                      dout[(p00 + W) * ldo + c] + dout[(p00 + W + 1) * ldo + c];
+}
+
+// 1x1 head: out[p] = sum_c x[p][c] w[c] + b
+__global__ __launch_bounds__(256) void k_head(const bf16_t* x, int ldx, int C, const float* w,
+                                              const float* b, float* out, long P) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  float s = b[0];
+  for (int c = 0; c < C; ++c) s += bf2f(x[p * ldx + c]) * w[c];
+  out[p] = s;
+}
+
+// head backward: dx[p][c] = dout[p] w[c] (fp32), dw[c] += sum_p dout[p] x[p][c], db += sum dout
+__global__ __launch_bounds__(256) void k_head_back(const bf16_t* x, int ldx, int C, const float* w,
+                                                   const float* dout, float* dx, int ldd,
+                                                   float* dw, float* db, long P) {
+  __shared__ float sw[256];
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const float g = p < P ? dout[p] : 0.f;
+  for (int c = 0; c < C; ++c) {
+    if (p < P) dx[p * ldd + c] = g * w[c];
+    float v = p < P ? g * bf2f(x[p * ldx + c]) : 0.f;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&dw[c], sw[0] + sw[1] + sw[2] + sw[3]);
+    __syncthreads();
+  }
+  float v = g;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(db, v);
+}
+
+}  // namespace nn
+}  // namespace aj

@@ -180,11 +180,58 @@ enum {
   AJ_PROF_MG_POST = 6,
   AJ_PROF_MG_BOTTOM = 7,   /* single-workgroup coarse levels            */
   AJ_PROF_MG_RESID = 8,    /* FP64 defect-correction residual           */
-  AJ_PROF_NUM_TAGS = 9
+  AJ_PROF_NN_CONV = 9,     /* network 3x3 conv (forward / input grad);  */
+                           /* "bytes" holds FLOPs for this tag          */
+  AJ_PROF_NN_DW = 10,      /* network weight gradient (FLOPs)           */
+  AJ_PROF_NUM_TAGS = 11
 };
 int aj_profile_begin(int max_records);
 int aj_profile_end(void);
 int aj_profile_read(int tag, int* count, double* total_ms, double* total_bytes);
+
+/* ---- Learned warm start -------------------------------------------------
+ * Replaces the reference's learned pressure operators
+ * (neural_operators/nn_pressure_operator_*.py, trained by
+ * neural_operators/training_utils.py:84-253, called in place of the solve
+ * at step.py:201-203), used here as the multigrid's initial guess: a
+ * `levels`-level U-Net (3x3 convs, GELU, 2x2 average pooling, nearest
+ * upsampling, skips) with `base_channels` channels at full resolution,
+ * 2 inputs (rhs, sdf), 1 output (p).  BF16 weights/activations on MFMA
+ * with FP32 accumulation; FP32 master weights and Adam (clip +-1, MSE).
+ *
+ * Parameters are one flat FP32 vector: for each conv in the order enc
+ * level 0..L-1 (a, b), dec level L-2..0 (a, b): weight [co][ci][3][3]
+ * then bias [co]; then the 1x1 head weight [c0], bias [1].
+ * Network inputs x are FP32 [B][H][W][2] (rhs/s, tanh(sdf/sdf_ref)),
+ * targets / outputs FP32 [B][H][W] (p/(s*pscale)); idx (device int32[B],
+ * may be NULL = 0..B-1) selects samples of x and t.  H, W divisible by
+ * 2^(levels-1). */
+typedef struct aj_nn aj_nn;
+int aj_nn_create(aj_nn** net, int base_channels, int levels, int seed);
+int aj_nn_destroy(aj_nn* net);
+long aj_nn_num_params(const aj_nn* net);
+/* which = 0: parameters, 1: gradients of the last aj_nn_loss_grad (host copy) */
+int aj_nn_get_params(aj_nn* net, float* host, long count, int which);
+int aj_nn_set_params(aj_nn* net, const float* host, long count);
+int aj_nn_forward(aj_nn* net, const float* x, const int* idx, float* out, int B, int H, int W,
+                  void* stream);
+/* forward + MSE + backward; *loss (device float, may be NULL) = batch MSE */
+int aj_nn_loss_grad(aj_nn* net, const float* x, const float* t, const int* idx, int B, int H,
+                    int W, float* loss, void* stream);
+int aj_nn_adam(aj_nn* net, float lr, float clip, void* stream);
+int aj_nn_train_step(aj_nn* net, const float* x, const float* t, const int* idx, int B, int H,
+                     int W, float lr, float* loss, void* stream);
+/* One training pair from solver fields (nx x ny, FP32): rhs, sdf, p
+ * restricted r times (2x2 averages) and normalised with s = RMS of the
+ * restricted rhs: x_out [H][W][2], t_out [H][W] (p may be NULL). */
+int aj_nn_make_sample(aj_nn* net, const float* rhs, const float* sdf, const float* p, int nx,
+                      int ny, int r, float sdf_ref, float pscale, float* x_out, float* t_out,
+                      void* stream);
+/* p0 (nx x ny) = prolong^r(s*pscale*net(x)) with x built as in
+ * aj_nn_make_sample; the prolongation is the textbook multigrid's
+ * cell-centred bilinear one (outlet row 0).  Stream-ordered, no sync. */
+int aj_nn_warm_start(aj_nn* net, const float* rhs, const float* sdf, float* p0, int nx, int ny,
+                     int r, float sdf_ref, float pscale, void* stream);
 
 #ifdef __cplusplus
 }
