@@ -62,15 +62,21 @@ class RkParams(ctypes.Structure):
 class MgStats(ctypes.Structure):
     _fields_ = [("cycles", ctypes.c_int), ("outer", ctypes.c_int), ("converged", ctypes.c_int),
                 ("nonfinite", ctypes.c_int), ("norm", ctypes.c_double),
-                ("norms", ctypes.c_double * AJ_MG_MAX_NORMS)]
+                ("norms", ctypes.c_double * AJ_MG_MAX_NORMS),
+                ("guess_norm", ctypes.c_double), ("zero_norm", ctypes.c_double),
+                ("used_guess", ctypes.c_int), ("reserved", ctypes.c_int)]
 
     def as_dict(self, textbook=False):
         # reference: one norm per cycle evaluation (outer = evaluations);
         # textbook: initial residual + one norm per outer iteration
         k = self.outer + 1 if textbook else self.outer
-        return {"cycles": self.cycles, "outer": self.outer, "converged": bool(self.converged),
-                "nonfinite": bool(self.nonfinite), "norm": self.norm,
-                "norms": [self.norms[i] for i in range(min(k, AJ_MG_MAX_NORMS))]}
+        d = {"cycles": self.cycles, "outer": self.outer, "converged": bool(self.converged),
+             "nonfinite": bool(self.nonfinite), "norm": self.norm,
+             "norms": [self.norms[i] for i in range(min(k, AJ_MG_MAX_NORMS))]}
+        if textbook:
+            d.update(guess_norm=self.guess_norm, zero_norm=self.zero_norm,
+                     used_guess=bool(self.used_guess))
+        return d
 
 
 _lock = threading.Lock()

@@ -32,6 +32,7 @@ struct aj_mg_plan {
 
   // reductions and device-side state
   double* partial = nullptr;
+  double* partial2 = nullptr;       // textbook: zero-guess residual partials
   int npartial = 0;
   aj_mg_stats* stats = nullptr;     // device copy, also the cycle gate
   int* state = nullptr;             // textbook: [0] = current p64 buffer

@@ -420,6 +420,88 @@ __global__ __launch_bounds__(256) void k_tb_resid(double* __restrict__ P0, doubl
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// Initial guess test: partial sums of r(p0)^2 (r32 <- r(p0)) and of the
+// zero-guess residual b^2, rows < nx-1.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tb_resid_guess(const double* __restrict__ P,
+                                                        const T* __restrict__ b,
+                                                        float* __restrict__ r32,
+                                                        double* __restrict__ part_g,
+                                                        double* __restrict__ part_z, int nx, int ny,
+                                                        double ax, double ay) {
+  __shared__ double sh[16];
+  const long n = (long)nx * ny;
+  double ag = 0.0, az = 0.0;
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c < n) {
+    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+    if (i == nx - 1) {
+      r32[c] = 0.0f;
+    } else {
+      const double pc = P[c];
+      const double W = i > 0 ? P[c - ny] : pc;
+      const double E = (i + 1 < nx - 1) ? P[c + ny] : 0.0;
+      const double S = j > 0 ? P[c - 1] : pc;
+      const double N = j < ny - 1 ? P[c + 1] : pc;
+      const double Ap = ax * ((E - pc) + (W - pc)) + ay * ((N - pc) + (S - pc));
+      const double bv = (double)b[c];
+      const double r = bv - Ap;
+      r32[c] = (float)r;
+      ag = r * r;
+      az = bv * bv;
+    }
+  }
+  const double tg = block_sum(ag, sh);
+  const double tz = block_sum(az, sh);
+  if (threadIdx.x == 0) {
+    part_g[blockIdx.x] = tg;
+    part_z[blockIdx.x] = tz;
+  }
+}
+
+// keep the guess only if its residual is not larger than the zero guess's
+__global__ void k_tb_guess_final(const double* __restrict__ part_g,
+                                 const double* __restrict__ part_z, int np, double dxdy,
+                                 const double* __restrict__ dtol, aj_mg_stats* st, int* state) {
+  __shared__ double sh[16];
+  double ag = 0.0, az = 0.0;
+#pragma unroll 8
+  for (int k = threadIdx.x; k < np; k += 256) {
+    ag += part_g[k];
+    az += part_z[k];
+  }
+  const double tg = block_sum(ag, sh);
+  const double tz = block_sum(az, sh);
+  if (threadIdx.x == 0) {
+    const double ng = sqrt(tg * dxdy), nz = sqrt(tz * dxdy);
+    const int use_guess = (ng <= nz) ? 1 : 0;
+    st->guess_norm = ng;
+    st->zero_norm = nz;
+    st->used_guess = use_guess;
+    state[1] = use_guess ? 0 : 1;
+    const double nrm = use_guess ? ng : nz;
+    st->norms[0] = nrm;
+    st->norm = nrm;
+    if (!isfinite(ng) || !isfinite(nz)) st->nonfinite = 1;
+    st->converged = (nrm <= dtol[0]) ? 1 : 0;
+  }
+}
+
+// fallback to the zero guess: p64 = 0, r32 = b (rows < nx-1)
+template <typename T>
+__global__ __launch_bounds__(256) void k_tb_fallback(double* __restrict__ P,
+                                                     const T* __restrict__ b,
+                                                     float* __restrict__ r32,
+                                                     const int* __restrict__ state, int nx,
+                                                     int ny) {
+  if (state[1] == 0) return;
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= (long)nx * ny) return;
+  const int i = (int)(c / ny);
+  P[c] = 0.0;
+  r32[c] = (i < nx - 1) ? (float)b[c] : 0.0f;
+}
+
 __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, double dxdy,
                                 const double* __restrict__ dtol, int flip, int cycles_added,
                                 aj_mg_stats* st, int* state) {
@@ -463,7 +545,11 @@ __global__ void k_tb_begin(aj_mg_stats* st, int* state, double* dtol, double tol
     st->converged = 0;
     st->nonfinite = 0;
     st->norm = 0.0;
+    st->guess_norm = 0.0;
+    st->zero_norm = 0.0;
+    st->used_guess = 0;
     state[0] = 0;
+    state[1] = 0;
   }
   for (int k = threadIdx.x; k < AJ_MG_MAX_NORMS; k += blockDim.x) st->norms[k] = 0.0;
 }
@@ -607,6 +693,7 @@ int tb_create(aj_mg_plan* p) {
     p->coarse_inv = (double*)cv.take((size_t)N * N * 8);
     p->npartial = (int)cdiv(n0, 256);
     p->partial = (double*)cv.take((size_t)p->npartial * 8);
+    p->partial2 = (double*)cv.take((size_t)p->npartial * 8);
     p->stats = (aj_mg_stats*)cv.take(sizeof(aj_mg_stats));
     p->state = (int*)cv.take(64);
     p->dtol = (double*)cv.take(64);
@@ -743,13 +830,27 @@ static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int ma
                      (T*)p->bcopy, nx, ny);
   AJ_LAUNCH_CHECK("k_tb_init");
   // outer iteration 0: residual of the initial guess
-  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
-                     (const int*)p->state, (const float*)nullptr, (const T*)p->bcopy, p->tB[0],
-                     p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
-  AJ_LAUNCH_CHECK("k_tb_resid");
-  hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
-                     p->dx * p->dy, (const double*)p->dtol, 0, 0, p->stats, p->state);
-  AJ_LAUNCH_CHECK("k_tb_norm_final");
+  if (p0) {
+    // a guess is kept only if its residual is not above the zero guess's
+    hipLaunchKernelGGL(k_tb_resid_guess<T>, dim3(nb), dim3(256), 0, st, (const double*)p->p64[0],
+                       (const T*)p->bcopy, p->tB[0], p->partial, p->partial2, nx, ny, ax, ay);
+    AJ_LAUNCH_CHECK("k_tb_resid_guess");
+    hipLaunchKernelGGL(k_tb_guess_final, dim3(1), dim3(256), 0, st, (const double*)p->partial,
+                       (const double*)p->partial2, nb, p->dx * p->dy, (const double*)p->dtol,
+                       p->stats, p->state);
+    AJ_LAUNCH_CHECK("k_tb_guess_final");
+    hipLaunchKernelGGL(k_tb_fallback<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p->p64[0],
+                       (const T*)p->bcopy, p->tB[0], (const int*)p->state, nx, ny);
+    AJ_LAUNCH_CHECK("k_tb_fallback");
+  } else {
+    hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
+                       (const int*)p->state, (const float*)nullptr, (const T*)p->bcopy, p->tB[0],
+                       p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
+    AJ_LAUNCH_CHECK("k_tb_resid");
+    hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
+                       p->dx * p->dy, (const double*)p->dtol, 0, 0, p->stats, p->state);
+    AJ_LAUNCH_CHECK("k_tb_norm_final");
+  }
   const int max_outer = (max_cycles + kCyclesPerOuter - 1) / kCyclesPerOuter;
   if (!p->adaptive) {
     // fully stream-ordered: every iteration is enqueued, those after

@@ -115,13 +115,14 @@ class CollocatedStepper:
     def __init__(self, nx, ny, dx, dy, mask, sdf, nu, U_inf, dt, *, dtype=torch.float32,
                  device="cuda", brinkman_eta=0.005, v_cycles=3, mg_variant="reference",
                  tolerance=1e-6, max_cycles=40, fast_mode=False, use_les=False,
-                 les_coupling="discard", smagorinsky_constant=0.1):
+                 les_coupling="discard", smagorinsky_constant=0.1, warm_start=None):
         A.require_gpu()
         self.nx, self.ny, self.dx, self.dy = nx, ny, float(dx), float(dy)
         self.nu, self.U_inf, self.dt = float(nu), float(U_inf), float(dt)
         self.eta, self.v_cycles = float(brinkman_eta), int(v_cycles)
         self.mg_variant, self.tolerance, self.max_cycles = mg_variant, tolerance, max_cycles
         self.fast_mode = fast_mode
+        self.warm_start = warm_start  # callable (rhs, sdf) -> p0, e.g. WarmStartNet
         self.les = use_les and les_coupling == "apply"
         self.cs = smagorinsky_constant
         dev = torch.device(device)
@@ -160,6 +161,8 @@ class CollocatedStepper:
                         U_inf=self.U_inf, nu_sgs=nsgs, fast_mode=self.fast_mode,
                         brinkman_eta=self.eta, out=(self.us, self.vs))
         pressure_rhs(self.us, self.vs, self.mask, self.dt, self.dx, self.dy, out=self.rhs)
+        if p0 is None and self.warm_start is not None:
+            p0 = self.warm_start(self.rhs, self.sdf)
         if self.mg_variant == "reference":
             self.plan.solve(self.rhs, self.p, p0, self.v_cycles, self.tolerance)
         else:

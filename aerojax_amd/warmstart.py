@@ -1,0 +1,75 @@
+"""Solver-generated training data and training of the warm-start network.
+
+Training pairs are (rhs, sdf) -> p taken from the project's own to-tolerance
+solver (textbook multigrid, ||div|| <= 1e-6) while it advances flows with
+randomly drawn obstacles and Reynolds numbers, restricted to the network's
+resolution and normalised (see ``WarmStartNet.make_sample``).  All of it
+runs on the GPU; the host only draws the geometry of each flow.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import workloads
+from .nn import WarmStartNet
+from .step import CollocatedStepper
+
+
+def generate_pairs(net: WarmStartNet, n_pairs: int, seeds, nx=2048, ny=512, steps_per_seed=40,
+                   stride=2, skip=2, log=None):
+    """Advance one flow per seed with the plain solver and record every
+    ``stride``-th step after ``skip`` steps.  Returns device tensors
+    x [N, H, W, 2] and t [N, H, W] (float32)."""
+    dev = net.device
+    H, W = nx >> net.r, ny >> net.r
+    X = torch.empty((n_pairs, H, W, 2), dtype=torch.float32, device=dev)
+    T = torch.empty((n_pairs, H, W), dtype=torch.float32, device=dev)
+    k = 0
+    t0 = time.perf_counter()
+    for seed in seeds:
+        if k >= n_pairs:
+            break
+        case = workloads.warm_start_case(int(seed), nx, ny)
+        g = case["grid"]
+        stp = CollocatedStepper(nx, ny, g.dx, g.dy, case["mask"], case["sdf"], case["nu"],
+                                case["U_inf"], case["dt"], device=dev, mg_variant="textbook",
+                                tolerance=case["tol"], max_cycles=60)
+        stp.load(case["u"], case["v"])
+        for s in range(steps_per_seed):
+            stp.step()
+            if s >= skip and (s - skip) % stride == 0 and k < n_pairs:
+                net.make_sample(stp.rhs, stp.sdf, stp.p, X[k], T[k])
+                k += 1
+        if log:
+            log(f"  seed {seed}: {case['desc']}, Re={case['Re']:.0f}; {k}/{n_pairs} pairs, "
+                f"{time.perf_counter() - t0:.1f}s")
+    torch.cuda.synchronize(dev)
+    return X[:k], T[:k]
+
+
+def train(net: WarmStartNet, X, T, epochs=2, batch=16, lr=5e-4, lr_min=1e-5, seed=0, log=None):
+    """Adam on shuffled mini-batches with a cosine learning-rate schedule.
+    Returns the per-epoch mean losses (one host sync per epoch)."""
+    n = X.shape[0]
+    steps = max(1, (n // batch) * epochs)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    hist = []
+    it = 0
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(torch.int32).to(net.device)
+        acc = torch.zeros((), dtype=torch.float32, device=net.device)
+        nb = n // batch
+        for b in range(nb):
+            idx = perm[b * batch:(b + 1) * batch]
+            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * it / steps))
+            acc += net.train_step(X, T, idx, cur)[0]
+            it += 1
+        hist.append(float(acc.item()) / max(nb, 1))
+        if log:
+            log(f"  epoch {ep}: loss {hist[-1]:.4e} ({time.perf_counter() - t0:.1f}s)")
+    return hist

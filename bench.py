@@ -1,12 +1,17 @@
-"""Benchmark: projection steps/s at 2048x512 solved to ||div|| <= 1e-6.
+"""Benchmark: projection steps/s at 2048x512 solved to ||div|| <= 1e-6,
+NN-warm-started multigrid vs plain multigrid.
 
 One step = RK3 predictor (fused HIP kernel) -> pressure rhs -> textbook
 multigrid (FP32 V(2,2) cycles in FP64 defect correction) until
 sqrt(sum (b - A p)^2 dx dy) <= 1e-6 -> projection, on a 2048x512 FP32 grid
-(4x1 channel, obstacle and Re drawn from seed 0).  Inputs are resident in
-HBM before timing starts.  Prints one JSON line.
+(4x1 channel; obstacle and Re drawn from seed 0).  Arms:
+  plain : zero initial guess (the reference's poisson_multigrid start)
+  nn    : initial guess from the warm-start U-Net (trained beforehand on
+          pairs generated by the same solver on other seeds; zero guess
+          whenever the network's guess has the larger residual)
+Inputs are resident in HBM before timing starts.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-nn]
 """
 from __future__ import annotations
 
@@ -24,6 +29,45 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def run_arm(aj, case, dev, steps, warmup, warm_start=None):
+    """Warmup then `steps` timed steps; returns (steps/s, stepper, per-step stats)."""
+    import torch
+    g = case["grid"]
+    stp = aj.CollocatedStepper(g.nx, g.ny, g.dx, g.dy, case["mask"], case["sdf"], case["nu"],
+                               case["U_inf"], case["dt"], dtype=torch.float32, device=dev,
+                               mg_variant="textbook", tolerance=case["tol"], max_cycles=60,
+                               warm_start=warm_start)
+    stp.load(case["u"], case["v"])
+    stats = []
+    for _ in range(warmup):
+        stp.step()
+        s = stp.plan.stats()
+        assert s["converged"], s
+        stats.append(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stp.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s = stp.plan.stats()
+    assert s["converged"] and s["norm"] <= case["tol"], s
+    return steps / elapsed, elapsed, stp, stats
+
+
+def cycles_per_step(aj, stp, n):
+    """Untimed: V-cycles and guess use over n further steps."""
+    cyc, used, ratio = [], [], []
+    for _ in range(n):
+        stp.step()
+        s = stp.plan.stats()
+        cyc.append(s["cycles"])
+        if "used_guess" in s and s.get("zero_norm", 0) > 0 and stp.warm_start is not None:
+            used.append(s["used_guess"])
+            ratio.append(s["guess_norm"] / s["zero_norm"])
+    return cyc, used, ratio
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -32,17 +76,22 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-nn", action="store_true")
+    ap.add_argument("--nn-pairs", type=int, default=2000)
+    ap.add_argument("--nn-epochs", type=int, default=3)
+    ap.add_argument("--nn-r", type=int, default=2, help="network runs on the grid restricted r times")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the hot path is single-GPU; see the sharding design notes)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(single-GPU hot path; the sharding design is documented, not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
     import torch
 
     import aerojax_amd as aj
-    from aerojax_amd import workloads
+    from aerojax_amd import _lib, workloads
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -50,50 +99,67 @@ def main():
     g = case["grid"]
     nx, ny = g.nx, g.ny
     log(f"workload {case['name']}: {case['desc']}, Re={case['Re']:.0f}")
-    stp = aj.CollocatedStepper(nx, ny, g.dx, g.dy, case["mask"], case["sdf"], case["nu"],
-                               case["U_inf"], case["dt"], dtype=torch.float32, device=dev,
-                               mg_variant="textbook", tolerance=case["tol"], max_cycles=40)
-    stp.load(case["u"], case["v"])
-    torch.cuda.synchronize()
-    cycles = []
-    for _ in range(args.warmup):
-        stp.step()
-        s = stp.plan.stats()
-        cycles.append(s["cycles"])
-        assert s["converged"], s
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stp.step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    s = stp.plan.stats()
-    assert s["converged"] and s["norm"] <= case["tol"], s
-    steps_per_s = args.steps / elapsed
-    log(f"plain MG: {steps_per_s:.1f} steps/s, V-cycles/step (warmup) {cycles}, last norm "
-        f"{s['norm']:.2e}")
 
-    # live per-kernel timing: HIP event pairs around every instrumented launch
-    # on the stream it runs on (multigrid iterations launched eagerly)
-    from aerojax_amd import _lib
+    # ---- plain multigrid arm
+    sps_plain, el_plain, stp_plain, _ = run_arm(aj, case, dev, args.steps, args.warmup)
+    cyc_plain, _, _ = cycles_per_step(aj, stp_plain, 5)
+    log(f"plain MG: {sps_plain:.1f} steps/s, V-cycles/step {cyc_plain}")
+
+    # ---- NN warm-start arm
+    nn_info = None
+    sps_nn = None
+    if not args.no_nn:
+        from aerojax_amd.nn import WarmStartNet
+        from aerojax_amd.warmstart import generate_pairs, train
+        net = WarmStartNet(base_channels=32, levels=4, seed=0, r=args.nn_r, device=dev)
+        t0 = time.perf_counter()
+        X, T = generate_pairs(net, args.nn_pairs, range(1000, 100000), nx, ny,
+                              steps_per_seed=40, stride=2, skip=2)
+        t_gen = time.perf_counter() - t0
+        log(f"generated {X.shape[0]} pairs at {X.shape[1]}x{X.shape[2]} in {t_gen:.1f}s")
+        t0 = time.perf_counter()
+        hist = train(net, X, T, epochs=args.nn_epochs, batch=16, lr=5e-4, log=log)
+        t_train = time.perf_counter() - t0
+        del X, T
+        torch.cuda.empty_cache()
+        sps_nn, el_nn, stp_nn, _ = run_arm(aj, case, dev, args.steps, args.warmup, warm_start=net)
+        cyc_nn, used, ratio = cycles_per_step(aj, stp_nn, 5)
+        log(f"NN warm start: {sps_nn:.1f} steps/s, V-cycles/step {cyc_nn}, guess used {used}, "
+            f"guess/zero residual {['%.2f' % r for r in ratio]}")
+        nn_info = {"steps_per_s": sps_nn, "vcycles_per_step": cyc_nn, "guess_used": used,
+                   "guess_over_zero_residual": ratio, "train_pairs": args.nn_pairs,
+                   "train_epochs": args.nn_epochs, "train_loss": hist, "gen_s": round(t_gen, 1),
+                   "train_s": round(t_train, 1), "net_resolution": [nx >> args.nn_r, ny >> args.nn_r],
+                   "params": net.num_params}
+
+    # ---- live per-kernel timing (HIP event pairs on the launching stream)
+    stp = stp_nn if sps_nn is not None else stp_plain
     _lib.profile_begin(20000)
-    for _ in range(min(args.steps, 10)):
+    nprof = 10
+    for _ in range(nprof):
         stp.step()
     torch.cuda.synchronize()
     _lib.profile_end()
     prof = _lib.profile_read()
-    dom = max(prof, key=lambda k: prof[k]["ms"])
-    d = prof[dom]
-    bytes_per_launch = d["bytes"] / d["count"]
-    achieved = d["gbps"]
-    log("kernel profile: " + ", ".join(f"{k} {v['avg_us']:.1f}us x{v['count']} "
-                                         f"{v['gbps']:.0f}GB/s" for k, v in prof.items()))
+    hbm = {k: v for k, v in prof.items() if not k.startswith("nn_")}
+    dom = max(hbm, key=lambda k: hbm[k]["ms"])
+    d = hbm[dom]
+    log("kernel profile: " + ", ".join(f"{k} {v['avg_us']:.1f}us x{v['count']}"
+                                         for k, v in prof.items()))
+    mfma = None
+    if "nn_conv" in prof:
+        c = prof["nn_conv"]
+        mfma = {"kernel": "k_conv3x3 (inference)", "achieved_tflops": c["gbps"] / 1e3,
+                "peak_tflops": 2500.0, "frac": c["gbps"] / 1e3 / 2500.0,
+                "avg_launch_us": c["avg_us"]}
+
+    # ---- CPU baseline: NumPy restatement of the same step, bounded sample
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         from oracle import reference_step as R
-        from oracle import textbook_mg as T
+        from oracle import textbook_mg as TM
         f = np.float32
-        mg = T.TextbookMG(nx, ny, g.dx, g.dy)
+        mg = TM.TextbookMG(nx, ny, g.dx, g.dy)
         uu, vv = case["u"].astype(f), case["v"].astype(f)
         mask, sdf = case["mask"].astype(f), case["sdf"].astype(f)
         t0 = time.perf_counter()
@@ -104,16 +170,18 @@ def main():
             rhs = R.pressure_rhs(us, vs, mask, case["dt"], g.dx, g.dy)
             p64, _ = mg.solve(rhs, tol=case["tol"])
             uu, vv = R.project(us, vs, p64.astype(f), mask, case["dt"], g.dx, g.dy, 1.0)
-        cpu_elapsed = time.perf_counter() - t0
-        cpu = {"value": args.cpu_steps / cpu_elapsed, "unit": "steps/s", "cores": 1,
-               "kind": "port",
+        cpu_el = time.perf_counter() - t0
+        cpu = {"value": args.cpu_steps / cpu_el, "unit": "steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the NumPy restatement (FP32 fields, FP64 "
-                         f"defect correction) of the same 2048x512 to-tolerance step"}
+                         "defect correction, plain multigrid) of the same 2048x512 "
+                         "to-tolerance step, first steps from the initial condition"}
         log(f"cpu baseline: {cpu['value']:.3f} steps/s")
 
+    value = sps_nn if sps_nn is not None else sps_plain
+    elapsed = el_nn if sps_nn is not None else el_plain
     line = {
         "metric": "steps/s at 2048x512 to ||div||<=1e-6: NN-warm-start vs plain MG; HBM GB/s",
-        "value": steps_per_s,
+        "value": value,
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -125,13 +193,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": case["name"], "geometry": case["desc"], "Re": round(case["Re"], 1),
-                   "grid": [nx, ny], "tolerance": case["tol"], "solver": "plain MG (zero guess)",
-                   "vcycles_per_step": cycles[-1] if cycles else None},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
-                     "unit": "GB/s", "frac": achieved / 8000.0, "traffic": None,
-                     "avg_launch_us": d["avg_us"], "bytes_per_launch": bytes_per_launch},
-        "kernels": {k: {"avg_us": round(v["avg_us"], 2), "launches_per_step":
-                        v["count"] / min(args.steps, 10), "gbps": round(v["gbps"], 1)}
+                   "grid": [nx, ny], "tolerance": case["tol"],
+                   "arm": "nn_warm_start" if sps_nn is not None else "plain_mg",
+                   "plain_mg_steps_per_s": sps_plain, "plain_vcycles_per_step": cyc_plain,
+                   "nn_over_plain": (sps_nn / sps_plain) if sps_nn else None, "nn": nn_info},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gbps"], "peak": 8000.0,
+                     "unit": "GB/s", "frac": d["gbps"] / 8000.0, "traffic": None,
+                     "avg_launch_us": d["avg_us"], "bytes_per_launch": d["bytes"] / d["count"]},
+        "mfma": mfma,
+        "kernels": {k: {"avg_us": round(v["avg_us"], 2), "launches_per_step": v["count"] / nprof,
+                        ("tflops" if k.startswith("nn_") else "gbps"):
+                            round(v["gbps"] / (1e3 if k.startswith("nn_") else 1), 2)}
                     for k, v in prof.items()},
         "cpu_baseline": cpu,
     }

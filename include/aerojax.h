@@ -149,6 +149,12 @@ typedef struct aj_mg_stats {
                    /* textbook: sqrt(sum r^2 dx dy))                      */
   double norms[AJ_MG_MAX_NORMS]; /* reference: per cycle slot; textbook: */
                                  /* [0] initial, [k] after outer iter k  */
+  /* textbook with p0: residual norms of the guess and of the zero guess;
+   * the guess is used only if guess_norm <= zero_norm */
+  double guess_norm;
+  double zero_norm;
+  int used_guess;
+  int reserved;
 } aj_mg_stats;
 
 /* Solve A p = rhs.  p0 (may be NULL = zero guess, the reference behaviour,

@@ -96,19 +96,33 @@ def test_adam_step_matches_torch(net):
     net.set_params(flat0)
 
 
-def test_training_reduces_loss(gpu):
+def test_training_tracks_oracle_and_reduces_loss(gpu):
+    """Same batches, same Adam schedule: the GPU loss trajectory follows the
+    FP64 oracle's for the first steps, and training lowers the loss.  (At
+    lr 2e-3 this U-Net without normalisation layers diverges on this data in
+    the FP64 oracle too, so 5e-4 is used.)"""
     from aerojax_amd.nn import WarmStartNet
     n = WarmStartNet(base_channels=32, levels=4, seed=5)
-    B, H, W, N = 4, 32, 16, 16
+    B, H, W, N, lr = 4, 32, 16, 16, 5e-4
     x, t = batch(N, H, W, 7)
     xt, tt = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
-    losses = []
+    p = torch.nn.Parameter(torch.from_numpy(n.get_params().astype(np.float64)))
+    opt = torch.optim.Adam([p], lr=lr, betas=(0.9, 0.999), eps=1e-8)
+    losses, ref = [], []
     rng = np.random.default_rng(0)
-    for it in range(60):
-        idx = torch.from_numpy(rng.choice(N, B, replace=False).astype(np.int32)).cuda()
-        losses.append(float(n.train_step(xt, tt, idx, 2e-3).item()))
-    print("nn training loss:", ["%.3e" % v for v in losses[::10]], flush=True)
-    assert np.mean(losses[-10:]) < 0.5 * np.mean(losses[:5])
+    for it in range(80):
+        sel = rng.choice(N, B, replace=False)
+        idx = torch.from_numpy(sel.astype(np.int32)).cuda()
+        losses.append(float(n.train_step(xt, tt, idx, lr).item()))
+        if it < 8:
+            lo, g, _ = NR.loss_and_grads(p.detach().numpy(), x[sel], t[sel])
+            p.grad = torch.from_numpy(np.clip(g, -1, 1))
+            opt.step()
+            ref.append(lo)
+    print("nn training loss:", ["%.3e" % v for v in losses[::10]], "oracle:",
+          ["%.3e" % v for v in ref], flush=True)
+    assert np.allclose(losses[:8], ref, rtol=5e-2)
+    assert np.mean(losses[-10:]) < 0.7 * np.mean(losses[:5])
     n.close()
 
 

@@ -57,7 +57,7 @@ def test_version_and_error_string_without_gpu(lib):
 def test_struct_layouts_match_header():
     from aerojax_amd import _lib
     assert ctypes.sizeof(_lib.RkParams) == 7 * 8 + 2 * 4 + 0  # 64 bytes, no tail padding
-    assert ctypes.sizeof(_lib.MgStats) == 4 * 4 + 8 + 64 * 8
+    assert ctypes.sizeof(_lib.MgStats) == 4 * 4 + 8 + 64 * 8 + 2 * 8 + 2 * 4
 
 
 def test_no_cpu_fallback():
