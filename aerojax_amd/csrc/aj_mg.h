@@ -26,7 +26,9 @@ struct aj_mg_plan {
   // textbook variant
   std::vector<float*> tB, tY, tE;   // FP32 V-cycle buffers per level
   double* p64[2] = {nullptr, nullptr};  // FP64 iterate (device-selected ping-pong)
-  double* coarse_inv = nullptr;     // dense inverse of the coarsest operator
+  double* coarse_dct = nullptr;     // coarsest level: orthonormal DCT-II [m][m]
+  double* coarse_l = nullptr;       // coarsest level: per-mode LU multipliers [m][rows]
+  double* coarse_dinv = nullptr;    // coarsest level: per-mode 1/diagonal of U [m][rows]
   int coarse_rows = 0;              // unknown rows of the coarsest level
   int bottom_level = 0;             // first level handled by the bottom kernel
 

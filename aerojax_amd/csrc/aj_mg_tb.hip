@@ -5,12 +5,13 @@
 //   ax (p[i+1] + p[i-1] - 2p) + ay (p[j+1] + p[j-1] - 2p) = b
 // with edge (Neumann) ghosts at the inlet row and both walls.
 //
-// Hierarchy: cell-centred 2x coarsening down to <= 64 cells; coarse level l
-// has coefficients ax/4^l, ay/4^l and an outlet ghost gamma_l * last that
+// Hierarchy: cell-centred 2x coarsening down to <= 1024 cells; coarse level
+// l has coefficients ax/4^l, ay/4^l and an outlet ghost gamma_l * last that
 // puts its Dirichlet point where the fine grid has it
 // (gamma_l = 1 - 1/(1/2 - 2^-(l+1))).  Smoother: red-black Gauss-Seidel
 // with the exact row solve.  Restriction: 2x2 average.  Prolongation:
-// cell-centred bilinear.  Coarsest level: dense FP64 inverse.
+// cell-centred bilinear.  Coarsest level: exact FP64 solve (cosine
+// transform along y, one tridiagonal solve per mode along x).
 //
 // Kernels per V-cycle on the large levels (temporal blocking in LDS):
 //   pre  : 2 RB-GS sweeps (4 half-sweeps, halo 5) + residual + restriction,
@@ -31,7 +32,7 @@ namespace aj {
 
 constexpr int kBottomCells = 4096;
 constexpr int kBottomThreads = 1024;
-constexpr int kTbCoarseCells = 64;
+constexpr int kTbCoarseCells = 1024;
 constexpr int kCyclesPerOuter = 2;
 
 __device__ __forceinline__ bool tb_gated(const aj_mg_stats* g) {
@@ -244,128 +245,7 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
   }
 }
 
-// ---------------------------------------------------------------------------
-// bottom: levels bot..L of the V-cycle in one workgroup (LDS resident)
-// ---------------------------------------------------------------------------
-constexpr int kMaxLevels = 24;
-struct TbBottom {
-  int nlev;                 // number of levels handled (bot..L)
-  TbLevel lev[kMaxLevels];
-  int off[kMaxLevels];      // LDS offsets of each level's arrays
-  float gam_next[kMaxLevels];
-  float inv_m[kMaxLevels];  // 1/m for the index split
-  int coarse_rows;          // unknown rows of the coarsest level
-  int total;                // total cells
-  int tpr;                  // threads per row of the dense solve (power of 2, <= 64)
-};
-
-// c / m for 0 <= c < 2^20 via a float reciprocal and one correction step
-__device__ __forceinline__ int fdiv_idx(int c, int m, float inv_m) {
-  int q = (int)((float)c * inv_m);
-  if (q * m > c) --q;
-  else if ((q + 1) * m <= c) ++q;
-  return q;
-}
-
-__device__ void tb_bottom_rbgs(const TbLevel& L, float* p, const float* b, int sweeps) {
-  const int n = L.n, m = L.m;
-  const int active_rows = (L.level == 0) ? n - 1 : n;
-  const int m2 = (m + 1) / 2;  // colour slots per row
-  const int rpp = blockDim.x / m2;  // rows per pass (m2 <= blockDim.x)
-  const int r0 = threadIdx.x / m2, q = threadIdx.x - r0 * m2;
-  for (int s = 0; s < 2 * sweeps; ++s) {
-    const int color = s & 1;
-    if (r0 < rpp) {
-      for (int i = r0; i < active_rows; i += rpp) {
-        const int j = ((color - i) & 1) + 2 * q;
-        if (j >= m) continue;
-        const int c = i * m + j;
-        p[c] = tb_relax(L, i, j, b[c], [&](int di, int dj) { return p[c + di * m + dj]; });
-      }
-    }
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(TbBottom B, const float* b_in,
-                                                              const float* p_in, float* e_out,
-                                                              const double* __restrict__ Ainv,
-                                                              const aj_mg_stats* gate) {
-  if (gate->converged) return;
-  extern __shared__ float sm[];
-  float* sp = sm;               // iterates
-  float* sb = sm + B.total;     // right-hand sides
-  const int nl = B.nlev;
-  {
-    const int n0 = B.lev[0].n * B.lev[0].m;
-    for (int c = threadIdx.x; c < n0; c += blockDim.x) {
-      sb[c] = b_in[c];
-      sp[c] = p_in ? p_in[c] : 0.0f;
-    }
-  }
-  __syncthreads();
-  // down
-  for (int k = 0; k < nl - 1; ++k) {
-    const TbLevel& L = B.lev[k];
-    float* p = sp + B.off[k];
-    float* bb = sb + B.off[k];
-    tb_bottom_rbgs(L, p, bb, 2);
-    const TbLevel& C = B.lev[k + 1];
-    float* pc = sp + B.off[k + 1];
-    float* bc = sb + B.off[k + 1];
-    const int m = L.m;
-    for (int c = threadIdx.x; c < C.n * C.m; c += blockDim.x) {
-      const int I = fdiv_idx(c, C.m, B.inv_m[k + 1]), J = c - I * C.m;
-      const int i = 2 * I, j = 2 * J;
-      auto R = [&](int ii, int jj) {
-        const int q = ii * m + jj;
-        return tb_resid(L, ii, jj, p[q], bb[q], [&](int di, int dj) { return p[q + di * m + dj]; });
-      };
-      const float r00 = R(i, j), r10 = R(i + 1, j), r01 = R(i, j + 1), r11 = R(i + 1, j + 1);
-      bc[c] = 0.25f * ((r00 + r10) + (r01 + r11));
-      pc[c] = 0.0f;
-    }
-    __syncthreads();
-  }
-  // coarsest: dense FP64 inverse, B.tpr threads per row, shuffle reduction
-  {
-    const TbLevel& L = B.lev[nl - 1];
-    float* p = sp + B.off[nl - 1];
-    const float* bb = sb + B.off[nl - 1];
-    const int N = B.coarse_rows * L.m;
-    const int tpr = B.tpr;
-    const int rows_per_pass = blockDim.x / tpr;
-    const int lane = threadIdx.x % tpr;
-    for (int r0 = 0; r0 < N; r0 += rows_per_pass) {
-      const int r = r0 + threadIdx.x / tpr;
-      double acc = 0.0;
-      if (r < N)
-        for (int k = lane; k < N; k += tpr) acc += Ainv[(long)r * N + k] * (double)bb[k];
-      for (int off = tpr / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-      if (r < N && lane == 0) p[r] = (float)acc;
-    }
-    for (int r = N + threadIdx.x; r < L.n * L.m; r += blockDim.x) p[r] = 0.0f;
-  }
-  __syncthreads();
-  // up
-  for (int k = nl - 2; k >= 0; --k) {
-    const TbLevel& L = B.lev[k];
-    const TbLevel& C = B.lev[k + 1];
-    float* p = sp + B.off[k];
-    const float* e = sp + B.off[k + 1];
-    for (int c = threadIdx.x; c < L.n * L.m; c += blockDim.x) {
-      const int i = fdiv_idx(c, L.m, B.inv_m[k]), j = c - i * L.m;
-      if (L.level == 0 && i == L.n - 1) continue;
-      p[c] = p[c] + tb_prolong(e, C.n, C.m, B.gam_next[k], i, j);
-    }
-    __syncthreads();
-    tb_bottom_rbgs(L, p, sb + B.off[k], 2);
-  }
-  {
-    const int n0 = B.lev[0].n * B.lev[0].m;
-    for (int c = threadIdx.x; c < n0; c += blockDim.x) e_out[c] = sp[c];
-  }
-}
+#include "aj_mg_tb_bottom.h"
 
 // ---------------------------------------------------------------------------
 // FP64 defect correction
@@ -577,60 +457,6 @@ static TbLevel make_level(const aj_mg_plan* p, int l) {
   return L;
 }
 
-// dense operator of level l on its unknown rows (row-major, cell index i*m+j)
-static std::vector<double> dense_op(const aj_mg_plan* p, int l, int* rows_out) {
-  const int n = p->lnx[l], m = p->lny[l];
-  const int rows = (l == 0) ? n - 1 : n;
-  const int N = rows * m;
-  const double s = std::ldexp(1.0, 2 * l);
-  const double ax = 1.0 / (p->dx * p->dx) / s, ay = 1.0 / (p->dy * p->dy) / s;
-  const double g = l > 0 ? outlet_gamma(l) : 0.0;
-  std::vector<double> A((size_t)N * N, 0.0);
-  auto at = [&](int r, int c) -> double& { return A[(size_t)r * N + c]; };
-  for (int i = 0; i < rows; ++i)
-    for (int j = 0; j < m; ++j) {
-      const int r = i * m + j;
-      // east
-      if (i + 1 < rows) { at(r, r + m) += ax; at(r, r) -= ax; }
-      else if (l == 0) { at(r, r) -= ax; }                 // Dirichlet row value 0
-      else { at(r, r) += ax * (g - 1.0); }                 // ghost g * p
-      if (i > 0) { at(r, r - m) += ax; at(r, r) -= ax; }
-      if (j > 0) { at(r, r - 1) += ay; at(r, r) -= ay; }
-      if (j < m - 1) { at(r, r + 1) += ay; at(r, r) -= ay; }
-    }
-  *rows_out = rows;
-  return A;
-}
-
-static bool invert(std::vector<double>& A, int N) {
-  std::vector<double> I((size_t)N * N, 0.0);
-  for (int k = 0; k < N; ++k) I[(size_t)k * N + k] = 1.0;
-  for (int c = 0; c < N; ++c) {
-    int piv = c;
-    for (int r = c + 1; r < N; ++r)
-      if (std::fabs(A[(size_t)r * N + c]) > std::fabs(A[(size_t)piv * N + c])) piv = r;
-    if (A[(size_t)piv * N + c] == 0.0) return false;
-    if (piv != c)
-      for (int k = 0; k < N; ++k) {
-        std::swap(A[(size_t)c * N + k], A[(size_t)piv * N + k]);
-        std::swap(I[(size_t)c * N + k], I[(size_t)piv * N + k]);
-      }
-    const double d = A[(size_t)c * N + c];
-    for (int k = 0; k < N; ++k) { A[(size_t)c * N + k] /= d; I[(size_t)c * N + k] /= d; }
-    for (int r = 0; r < N; ++r) {
-      if (r == c) continue;
-      const double f = A[(size_t)r * N + c];
-      if (f == 0.0) continue;
-      for (int k = 0; k < N; ++k) {
-        A[(size_t)r * N + k] -= f * A[(size_t)c * N + k];
-        I[(size_t)r * N + k] -= f * I[(size_t)c * N + k];
-      }
-    }
-  }
-  A.swap(I);
-  return true;
-}
-
 struct Carve {
   char* base;
   size_t off = 0;
@@ -657,10 +483,10 @@ int tb_create(aj_mg_plan* p) {
   if (p->nx < 3 || p->ny < 2) return fail(AJ_ERR_GRID, "textbook multigrid needs nx >= 3, ny >= 2");
   const int L = (int)p->lnx.size() - 1;
   if (L + 1 > kMaxLevels) return fail(AJ_ERR_GRID, "textbook multigrid: too many levels");
-  int rows = 0;
-  std::vector<double> A = dense_op(p, L, &rows);
-  const int N = rows * p->lny[L];
-  if (N > 1024) {
+  const int nL = p->lnx[L], mL = p->lny[L];
+  const int rows = (L == 0) ? nL - 1 : nL;
+  const int N = rows * mL;
+  if ((long)nL * mL > kTbCoarseCells || rows < 1) {
     char msg[200];
     std::snprintf(msg, sizeof(msg),
                   "Grid dimensions (%d, %d) not suitable for multigrid: coarsest level has %d "
@@ -668,8 +494,39 @@ int tb_create(aj_mg_plan* p) {
                   p->nx, p->ny, N);
     return fail(AJ_ERR_GRID, msg);
   }
-  if (!invert(A, N)) return fail(AJ_ERR_GRID, "textbook multigrid: singular coarse operator");
   p->coarse_rows = rows;
+  // exact coarsest solve: orthonormal DCT-II along y, LU factors of the
+  // per-mode tridiagonal along x (rows 0..rows-1 of level L)
+  std::vector<double> Cm((size_t)mL * mL), Lf((size_t)mL * rows), Dinv((size_t)mL * rows);
+  {
+    const double pi = 3.14159265358979323846;
+    const double s = std::ldexp(1.0, 2 * L);
+    const double ax = 1.0 / (p->dx * p->dx) / s, ay = 1.0 / (p->dy * p->dy) / s;
+    const double g = L > 0 ? outlet_gamma(L) : 0.0;
+    for (int k = 0; k < mL; ++k)
+      for (int j = 0; j < mL; ++j)
+        Cm[(size_t)k * mL + j] =
+            std::sqrt((k == 0 ? 1.0 : 2.0) / mL) * std::cos(pi * k * (j + 0.5) / mL);
+    for (int k = 0; k < mL; ++k) {
+      const double sk = std::sin(pi * k / (2.0 * mL));
+      const double mu = -4.0 * ay * sk * sk;
+      double dprev = 0.0;
+      for (int i = 0; i < rows; ++i) {
+        const double dW = i > 0 ? ax : 0.0;
+        const double dE = (i < rows - 1 || L == 0) ? ax : ax * (1.0 - g);
+        const double diag = -(dW + dE) + mu;
+        double d = diag, l = 0.0;
+        if (i > 0) {
+          l = ax / dprev;
+          d = diag - l * ax;
+        }
+        if (d == 0.0) return fail(AJ_ERR_GRID, "textbook multigrid: singular coarse operator");
+        Lf[(size_t)k * rows + i] = l;
+        Dinv[(size_t)k * rows + i] = 1.0 / d;
+        dprev = d;
+      }
+    }
+  }
   // first level handled by the bottom kernel
   int bot = L;
   while (bot > 0 && (long)p->lnx[bot - 1] * p->lny[bot - 1] <= kBottomCells) --bot;
@@ -690,7 +547,9 @@ int tb_create(aj_mg_plan* p) {
     }
     p->p64[0] = (double*)cv.take(n0 * 8);
     p->p64[1] = (double*)cv.take(n0 * 8);
-    p->coarse_inv = (double*)cv.take((size_t)N * N * 8);
+    p->coarse_dct = (double*)cv.take((size_t)mL * mL * 8);
+    p->coarse_l = (double*)cv.take((size_t)mL * rows * 8);
+    p->coarse_dinv = (double*)cv.take((size_t)mL * rows * 8);
     p->npartial = (int)cdiv(n0, 256);
     p->partial = (double*)cv.take((size_t)p->npartial * 8);
     p->partial2 = (double*)cv.take((size_t)p->npartial * 8);
@@ -703,7 +562,9 @@ int tb_create(aj_mg_plan* p) {
       AJ_HIP(hipMalloc(&p->ws, p->ws_bytes));
     }
   }
-  AJ_HIP(hipMemcpy(p->coarse_inv, A.data(), (size_t)N * N * 8, hipMemcpyHostToDevice));
+  AJ_HIP(hipMemcpy(p->coarse_dct, Cm.data(), Cm.size() * 8, hipMemcpyHostToDevice));
+  AJ_HIP(hipMemcpy(p->coarse_l, Lf.data(), Lf.size() * 8, hipMemcpyHostToDevice));
+  AJ_HIP(hipMemcpy(p->coarse_dinv, Dinv.data(), Dinv.size() * 8, hipMemcpyHostToDevice));
   AJ_HIP(hipHostMalloc((void**)&p->host_stats, sizeof(aj_mg_stats), hipHostMallocDefault));
   return AJ_OK;
 }
@@ -723,20 +584,14 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
       B.gam_next[k] = (bot + k + 1 <= L) ? (float)outlet_gamma(bot + k + 1) : 1.0f;
       B.inv_m[k] = 1.0f / (float)B.lev[k].m;
     }
-    {
-      const int N = p->coarse_rows * p->lny[L];
-      int tpr = 1;
-      while (tpr < 64 && tpr * 2 * N <= kBottomThreads) tpr *= 2;
-      B.tpr = tpr;
-    }
     B.total = off;
     B.coarse_rows = p->coarse_rows;
-    const size_t smem = (size_t)2 * off * sizeof(float);
+    const size_t smem = (size_t)p->lnx[L] * p->lny[L] * sizeof(double) + (size_t)2 * off * sizeof(float);
     ProfScope prof(AJ_PROF_MG_BOTTOM, (double)B.lev[0].n * B.lev[0].m * (warm ? 12 : 8), st);
     hipLaunchKernelGGL(k_tb_bottom, dim3(1), dim3(kBottomThreads), smem, st, B,
                        (const float*)p->tB[lev], warm ? (const float*)p->tE[lev] : (const float*)nullptr,
-                       p->tE[lev], (const double*)p->coarse_inv,
-                       (const aj_mg_stats*)p->stats);
+                       p->tE[lev], (const double*)p->coarse_dct, (const double*)p->coarse_l,
+                       (const double*)p->coarse_dinv, (const aj_mg_stats*)p->stats);
     AJ_LAUNCH_CHECK("k_tb_bottom");
     return AJ_OK;
   }

@@ -24,8 +24,9 @@ Hierarchy (level 0 = fine grid):
     Keeping every level's Dirichlet point where the fine grid has it gives a
     grid-independent V(2,2) factor of ~0.085 (a face-Dirichlet coarse
     outlet gives ~0.167).
-  * coarsening stops when n*m <= COARSE_CELLS or a side stops being even;
-    the coarsest level is solved exactly (dense FP64 inverse).
+  * coarsening stops when n*m <= COARSE_CELLS (1024) or a side stops being
+    even; the coarsest level is solved exactly (dense FP64 inverse here; a
+    cosine transform + tridiagonal solves on the GPU, same linear map).
 Smoother: red-black Gauss-Seidel (red = (i+j) even first), exact row solve
 (clamped Neumann neighbours drop out of the row and the diagonal).
 Restriction: 2x2 average (cell-centred full weighting; level-0 outlet row
@@ -39,7 +40,7 @@ from __future__ import annotations
 
 import numpy as np
 
-COARSE_CELLS = 64
+COARSE_CELLS = 1024
 
 
 def hierarchy(nx, ny):

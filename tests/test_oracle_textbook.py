@@ -35,7 +35,7 @@ def test_manufactured_solution_direct_and_mg():
 
 
 def test_convergence_factor_grid_independent():
-    for (nx, ny) in [(64, 16), (256, 64)]:
+    for (nx, ny) in [(256, 64), (512, 128)]:
         dx, dy = 4.0 / nx, 1.0 / ny
         mg = T.TextbookMG(nx, ny, dx, dy, dtype=np.float64)
         rng = np.random.default_rng(0)
@@ -68,7 +68,8 @@ def test_defect_correction_reaches_1e6_from_fp32_cycles():
 
 
 def test_hierarchy_and_gamma():
-    assert T.hierarchy(2048, 512)[-1] == (16, 4)
-    assert T.hierarchy(512, 96)[-1] == (16, 3)
+    assert T.hierarchy(2048, 512)[-1] == (64, 16)
+    assert T.hierarchy(512, 96)[-1] == (64, 12)
+    assert T.hierarchy(64, 16) == [(64, 16)]
     assert np.isclose(T.outlet_gamma(1), -3.0)
     assert -1.0 > T.outlet_gamma(8) > -1.01
