@@ -362,7 +362,8 @@ __global__ void k_tb_guess_final(const double* __restrict__ part_g,
     const double nrm = use_guess ? ng : nz;
     st->norms[0] = nrm;
     st->norm = nrm;
-    if (!isfinite(ng) || !isfinite(nz)) st->nonfinite = 1;
+    // a rejected (e.g. NaN) guess does not make the solve non-finite
+    if (!isfinite(nrm)) st->nonfinite = 1;
     st->converged = (nrm <= dtol[0]) ? 1 : 0;
   }
 }

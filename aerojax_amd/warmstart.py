@@ -42,6 +42,9 @@ def generate_pairs(net: WarmStartNet, n_pairs: int, seeds, nx=2048, ny=512, step
         for s in range(steps_per_seed):
             stp.step()
             if s >= skip and (s - skip) % stride == 0 and k < n_pairs:
+                st = stp.plan.stats()
+                if not st["converged"] or st["nonfinite"]:
+                    break  # the flow left the scheme's stability range: drop the seed
                 net.make_sample(stp.rhs, stp.sdf, stp.p, X[k], T[k])
                 k += 1
         if log:

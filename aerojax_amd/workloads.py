@@ -61,7 +61,12 @@ def random_obstacle(grid, rng, eps=0.05):
 
 def warm_start_case(seed=0, nx=2048, ny=512, noise_sigma=1e-3):
     """2048x512 on a 4x1 channel, geometry drawn per seed, Re log-uniform in
-    [500, 5e4], dt 1e-3, solve to ||div|| <= 1e-6."""
+    [500, 5e4], solve to ||div|| <= 1e-6.
+
+    dt = 2.5e-4 gives the advective CFL number U dt/dx = 0.128 of the
+    512x96 / dt 1e-3 configuration; at dt = 1e-3 (CFL 0.51) the step with a
+    converged pressure blows up for steep airfoils within ~10 steps once the
+    local CFL passes ~1 (the CPU restatement does the same)."""
     rng = np.random.default_rng(seed)
     grid = G.GridParams(nx, ny, 4.0, 1.0)
     geo, desc = random_obstacle(grid, rng)
@@ -69,4 +74,4 @@ def warm_start_case(seed=0, nx=2048, ny=512, noise_sigma=1e-3):
     u, v = G.von_karman_ic(nx, ny, 1.0, noise_sigma, seed)
     return dict(name=f"warmstart_{nx}x{ny}_seed{seed}", desc=desc, grid=grid,
                 sdf=np.nan_to_num(geo.sdf, nan=0.0), mask=geo.mask, u=u, v=v,
-                nu=G.viscosity(1.0, geo.L_char, Re), Re=Re, dt=1e-3, U_inf=1.0, tol=1e-6)
+                nu=G.viscosity(1.0, geo.L_char, Re), Re=Re, dt=2.5e-4, U_inf=1.0, tol=1e-6)
