@@ -26,9 +26,9 @@ struct aj_mg_plan {
   // textbook variant
   std::vector<float*> tB, tY, tE;   // FP32 V-cycle buffers per level
   double* p64[2] = {nullptr, nullptr};  // FP64 iterate (device-selected ping-pong)
-  double* coarse_dct = nullptr;     // coarsest level: orthonormal DCT-II [m][m]
-  double* coarse_l = nullptr;       // coarsest level: per-mode LU multipliers [m][rows]
-  double* coarse_dinv = nullptr;    // coarsest level: per-mode 1/diagonal of U [m][rows]
+  float* coarse_dct = nullptr;      // coarsest level: orthonormal DCT-II [m][m]
+  float* coarse_l = nullptr;        // coarsest level: per-mode LU multipliers [m][rows]
+  float* coarse_dinv = nullptr;     // coarsest level: per-mode 1/diagonal of U [m][rows]
   int coarse_rows = 0;              // unknown rows of the coarsest level
   int bottom_level = 0;             // first level handled by the bottom kernel
 
@@ -42,6 +42,7 @@ struct aj_mg_plan {
   int adaptive = 1;                 // textbook: read convergence between iterations
   int last_outer = 4;               // textbook: iterations the previous solve needed
   int use_graph = 1;                // textbook: replay a captured outer iteration
+  int guess_test = 1;               // textbook: keep p0 only if ||r(p0)|| <= ||b||
   void* graph = nullptr;            // hipGraph_t of one outer iteration
   void* graph_exec = nullptr;       // hipGraphExec_t
   double* dtol = nullptr;           // textbook: tolerance of the current solve

@@ -31,6 +31,8 @@
 namespace aj {
 
 constexpr int kBottomCells = 4096;
+// dynamic LDS available to the bottom kernel without a function attribute
+constexpr size_t kBottomLdsBytes = 64 * 1024;
 constexpr int kBottomThreads = 1024;
 constexpr int kTbCoarseCells = 1024;
 constexpr int kCyclesPerOuter = 2;
@@ -340,9 +342,11 @@ __global__ __launch_bounds__(256) void k_tb_resid_guess(const double* __restrict
 }
 
 // keep the guess only if its residual is not larger than the zero guess's
+// (test = 0: keep any finite guess)
 __global__ void k_tb_guess_final(const double* __restrict__ part_g,
                                  const double* __restrict__ part_z, int np, double dxdy,
-                                 const double* __restrict__ dtol, aj_mg_stats* st, int* state) {
+                                 int test, const double* __restrict__ dtol, aj_mg_stats* st,
+                                 int* state) {
   __shared__ double sh[16];
   double ag = 0.0, az = 0.0;
 #pragma unroll 8
@@ -354,7 +358,7 @@ __global__ void k_tb_guess_final(const double* __restrict__ part_g,
   const double tz = block_sum(az, sh);
   if (threadIdx.x == 0) {
     const double ng = sqrt(tg * dxdy), nz = sqrt(tz * dxdy);
-    const int use_guess = (ng <= nz) ? 1 : 0;
+    const int use_guess = (test ? (ng <= nz) : isfinite(ng)) ? 1 : 0;
     st->guess_norm = ng;
     st->zero_norm = nz;
     st->used_guess = use_guess;
@@ -548,9 +552,9 @@ int tb_create(aj_mg_plan* p) {
     }
     p->p64[0] = (double*)cv.take(n0 * 8);
     p->p64[1] = (double*)cv.take(n0 * 8);
-    p->coarse_dct = (double*)cv.take((size_t)mL * mL * 8);
-    p->coarse_l = (double*)cv.take((size_t)mL * rows * 8);
-    p->coarse_dinv = (double*)cv.take((size_t)mL * rows * 8);
+    p->coarse_dct = (float*)cv.take((size_t)mL * mL * 4);
+    p->coarse_l = (float*)cv.take((size_t)mL * rows * 4);
+    p->coarse_dinv = (float*)cv.take((size_t)mL * rows * 4);
     p->npartial = (int)cdiv(n0, 256);
     p->partial = (double*)cv.take((size_t)p->npartial * 8);
     p->partial2 = (double*)cv.take((size_t)p->npartial * 8);
@@ -563,9 +567,16 @@ int tb_create(aj_mg_plan* p) {
       AJ_HIP(hipMalloc(&p->ws, p->ws_bytes));
     }
   }
-  AJ_HIP(hipMemcpy(p->coarse_dct, Cm.data(), Cm.size() * 8, hipMemcpyHostToDevice));
-  AJ_HIP(hipMemcpy(p->coarse_l, Lf.data(), Lf.size() * 8, hipMemcpyHostToDevice));
-  AJ_HIP(hipMemcpy(p->coarse_dinv, Dinv.data(), Dinv.size() * 8, hipMemcpyHostToDevice));
+  {
+    // the tables feed the FP32 V-cycle's coarsest correction: stored as float
+    auto up = [](float* dst, const std::vector<double>& src) {
+      std::vector<float> f(src.begin(), src.end());
+      return hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice);
+    };
+    AJ_HIP(up(p->coarse_dct, Cm));
+    AJ_HIP(up(p->coarse_l, Lf));
+    AJ_HIP(up(p->coarse_dinv, Dinv));
+  }
   AJ_HIP(hipHostMalloc((void**)&p->host_stats, sizeof(aj_mg_stats), hipHostMallocDefault));
   return AJ_OK;
 }
@@ -587,12 +598,16 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
     }
     B.total = off;
     B.coarse_rows = p->coarse_rows;
-    const size_t smem = (size_t)p->lnx[L] * p->lny[L] * sizeof(double) + (size_t)2 * off * sizeof(float);
+    const size_t base = (size_t)p->lnx[L] * p->lny[L] * sizeof(double) + (size_t)2 * off * sizeof(float);
+    const size_t tables = (size_t)p->lny[L] * (p->lny[L] + 2 * p->coarse_rows) * sizeof(float);
+    const bool stage = base + tables <= kBottomLdsBytes;
+    const size_t smem = base + (stage ? tables : 0);
     ProfScope prof(AJ_PROF_MG_BOTTOM, (double)B.lev[0].n * B.lev[0].m * (warm ? 12 : 8), st);
-    hipLaunchKernelGGL(k_tb_bottom, dim3(1), dim3(kBottomThreads), smem, st, B,
-                       (const float*)p->tB[lev], warm ? (const float*)p->tE[lev] : (const float*)nullptr,
-                       p->tE[lev], (const double*)p->coarse_dct, (const double*)p->coarse_l,
-                       (const double*)p->coarse_dinv, (const aj_mg_stats*)p->stats);
+    hipLaunchKernelGGL(stage ? k_tb_bottom<true> : k_tb_bottom<false>, dim3(1), dim3(kBottomThreads),
+                       smem, st, B, (const float*)p->tB[lev],
+                       warm ? (const float*)p->tE[lev] : (const float*)nullptr, p->tE[lev],
+                       (const float*)p->coarse_dct, (const float*)p->coarse_l,
+                       (const float*)p->coarse_dinv, (const aj_mg_stats*)p->stats);
     AJ_LAUNCH_CHECK("k_tb_bottom");
     return AJ_OK;
   }
@@ -692,8 +707,8 @@ static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int ma
                        (const T*)p->bcopy, p->tB[0], p->partial, p->partial2, nx, ny, ax, ay);
     AJ_LAUNCH_CHECK("k_tb_resid_guess");
     hipLaunchKernelGGL(k_tb_guess_final, dim3(1), dim3(256), 0, st, (const double*)p->partial,
-                       (const double*)p->partial2, nb, p->dx * p->dy, (const double*)p->dtol,
-                       p->stats, p->state);
+                       (const double*)p->partial2, nb, p->dx * p->dy, p->guess_test,
+                       (const double*)p->dtol, p->stats, p->state);
     AJ_LAUNCH_CHECK("k_tb_guess_final");
     hipLaunchKernelGGL(k_tb_fallback<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p->p64[0],
                        (const T*)p->bcopy, p->tB[0], (const int*)p->state, nx, ny);

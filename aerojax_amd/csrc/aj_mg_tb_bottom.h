@@ -49,36 +49,38 @@ __device__ __forceinline__ void tb_bottom_rbgs(const TbLevel L, float* p, const 
 }
 
 // Exact solve of the coarsest level: b (rows x m, float, LDS) -> p (n x m,
-// float, LDS; rows beyond `rows` = 0).  C: [m][m] orthonormal DCT-II,
-// lf/invd: [m][rows] LU factors of the per-mode tridiagonal, sup = ax.
+// float, LDS; rows beyond `rows` = 0).  sC: [m][m] orthonormal DCT-II,
+// sl/sd: [m][rows] LU factors of the per-mode tridiagonal (all staged in
+// LDS at kernel start), sup = ax.  Transforms accumulate in FP64.
 __device__ __forceinline__ void tb_coarse_exact(const float* b, float* p, int n, int rows, int m,
-                                                double* w, const double* __restrict__ C,
-                                                const double* __restrict__ lf,
-                                                const double* __restrict__ invd, double sup) {
+                                                double* w, const float* sC, const float* sl,
+                                                const float* sd, double sup) {
   const int N = rows * m;
   // forward transform along y: w[i][k] = sum_j C[k][j] b[i][j]
   for (int c = threadIdx.x; c < N; c += blockDim.x) {
     const int i = c / m, k = c - i * m;
-    const double* Ck = C + (long)k * m;
+    const float* Ck = sC + k * m;
     const float* bi = b + i * m;
     double acc = 0.0;
-    for (int j = 0; j < m; ++j) acc += Ck[j] * (double)bi[j];
+    for (int j = 0; j < m; ++j) acc += (double)Ck[j] * (double)bi[j];
     w[c] = acc;
   }
   __syncthreads();
   // one tridiagonal solve along x per mode (forward / backward substitution)
   for (int k = threadIdx.x; k < m; k += blockDim.x) {
-    const double* l = lf + (long)k * rows;
-    const double* d = invd + (long)k * rows;
+    const float* l = sl + k * rows;
+    const float* d = sd + k * rows;
     double y = w[k];
+#pragma unroll 8
     for (int i = 1; i < rows; ++i) {
-      y = w[i * m + k] - l[i] * y;
+      y = w[i * m + k] - (double)l[i] * y;
       w[i * m + k] = y;
     }
-    double x = y * d[rows - 1];
+    double x = y * (double)d[rows - 1];
     w[(rows - 1) * m + k] = x;
+#pragma unroll 8
     for (int i = rows - 2; i >= 0; --i) {
-      x = (w[i * m + k] - sup * x) * d[i];
+      x = (w[i * m + k] - sup * x) * (double)d[i];
       w[i * m + k] = x;
     }
   }
@@ -89,7 +91,7 @@ __device__ __forceinline__ void tb_coarse_exact(const float* b, float* p, int n,
     float v = 0.0f;
     if (i < rows) {
       double acc = 0.0;
-      for (int k = 0; k < m; ++k) acc += C[(long)k * m + j] * w[i * m + k];
+      for (int k = 0; k < m; ++k) acc += (double)sC[k * m + j] * w[i * m + k];
       v = (float)acc;
     }
     p[c] = v;
@@ -97,23 +99,39 @@ __device__ __forceinline__ void tb_coarse_exact(const float* b, float* p, int n,
   __syncthreads();
 }
 
-// p_in (may be null = zero guess, may alias e_out) is the top level's start
+// p_in (may be null = zero guess, may alias e_out) is the top level's start.
+// kStage: copy the coarsest-solve tables into LDS (when they fit) so the
+// serial per-mode substitutions read LDS instead of L2.
+template <bool kStage>
 __global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(
     TbBottom B, const float* b_in, const float* p_in, float* e_out,
-    const double* __restrict__ C, const double* __restrict__ lf, const double* __restrict__ invd,
+    const float* __restrict__ C, const float* __restrict__ lf, const float* __restrict__ invd,
     const aj_mg_stats* gate) {
   if (gate->converged) return;
   extern __shared__ double smd[];
   const int nl = B.nlev;
   const TbLevel Lc = B.lev[nl - 1];
-  double* w = smd;                                          // coarsest transform workspace
-  float* sp = reinterpret_cast<float*>(smd + Lc.n * Lc.m);  // iterates
-  float* sb = sp + B.total;                                 // right-hand sides
+  const int mc = Lc.m, rc = B.coarse_rows;
+  double* w = smd;                                           // coarsest transform workspace
+  float* tab = reinterpret_cast<float*>(smd + Lc.n * Lc.m);  // staged tables (kStage)
+  const int ntab = kStage ? mc * (mc + 2 * rc) : 0;
+  float* sp = tab + ntab;                                    // iterates
+  float* sb = sp + B.total;                                  // right-hand sides
+  const float* sC = kStage ? tab : C;                        // DCT table [m][m]
+  const float* sl = kStage ? tab + mc * mc : lf;             // LU multipliers [m][rows]
+  const float* sd = kStage ? tab + mc * (mc + rc) : invd;    // 1/diag [m][rows]
   {
     const int n0 = B.lev[0].n * B.lev[0].m;
     for (int c = threadIdx.x; c < n0; c += blockDim.x) {
       sb[c] = b_in[c];
       sp[c] = p_in ? p_in[c] : 0.0f;
+    }
+    if constexpr (kStage) {
+      for (int c = threadIdx.x; c < mc * mc; c += blockDim.x) tab[c] = C[c];
+      for (int c = threadIdx.x; c < mc * rc; c += blockDim.x) {
+        tab[mc * mc + c] = lf[c];
+        tab[mc * (mc + rc) + c] = invd[c];
+      }
     }
   }
   __syncthreads();
@@ -140,8 +158,8 @@ __global__ __launch_bounds__(kBottomThreads) void k_tb_bottom(
     }
     __syncthreads();
   }
-  tb_coarse_exact(sb + B.off[nl - 1], sp + B.off[nl - 1], Lc.n, B.coarse_rows, Lc.m, w, C, lf,
-                  invd, (double)Lc.axl);
+  tb_coarse_exact(sb + B.off[nl - 1], sp + B.off[nl - 1], Lc.n, rc, mc, w, sC, sl, sd,
+                  (double)Lc.axl);
   // up: prolong, correct, smooth
   for (int k = nl - 2; k >= 0; --k) {
     const TbLevel L = B.lev[k];

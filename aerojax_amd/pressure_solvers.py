@@ -50,12 +50,18 @@ class MultigridPlan:
     def workspace_bytes(self) -> int:
         return int(_lib.load().aj_mg_workspace_bytes(self.handle))
 
-    def set_option(self, adaptive: bool | None = None, graph: bool | None = None):
+    def set_option(self, adaptive: bool | None = None, graph: bool | None = None,
+                   guess_test: bool | None = None):
         """Textbook plans: ``adaptive`` reads convergence between outer
         iterations (host sync, default) instead of enqueueing all
         ``max_cycles``; ``graph`` replays a captured outer iteration
-        (default) instead of launching its kernels one by one."""
+        (default) instead of launching its kernels one by one;
+        ``guess_test`` keeps an initial guess only if its residual is not
+        above the zero guess's (default) instead of keeping any finite one."""
         lib = _lib.load()
+        if guess_test is not None:
+            _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_GUESS_TEST,
+                                            int(guess_test)), "aj_mg_set_option")
         if adaptive is not None:
             _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_ADAPTIVE, int(adaptive)),
                        "aj_mg_set_option")

@@ -54,6 +54,18 @@ def generate_pairs(net: WarmStartNet, n_pairs: int, seeds, nx=2048, ny=512, step
     return X[:k], T[:k]
 
 
+def normalise_targets(net: WarmStartNet, T):
+    """Scale the targets to unit RMS in place and fold the factor into the
+    network's output scale (``pscale``), so that the regression target is
+    O(1) whatever the pressure's magnitude.  Returns the factor."""
+    q = float(torch.sqrt(torch.mean(T.double() ** 2)).item())
+    if not math.isfinite(q) or q <= 0.0:
+        raise ValueError(f"training targets have RMS {q}")
+    T.div_(q)
+    net.pscale *= q
+    return q
+
+
 def train(net: WarmStartNet, X, T, epochs=2, batch=16, lr=5e-4, lr_min=1e-5, seed=0, log=None):
     """Adam on shuffled mini-batches with a cosine learning-rate schedule.
     Returns the per-epoch mean losses (one host sync per epoch)."""

@@ -134,8 +134,12 @@ int aj_mg_num_levels(const aj_mg_plan* plan);
  *     exit at their first instruction); use 0 under stream capture.
  *   AJ_MG_OPT_GRAPH = 1 (default): one outer iteration is captured once
  *     into a hipGraph (plan-owned buffers only) and replayed; 0: eager
- *     kernel launches. */
-enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1 };
+ *     kernel launches.
+ *   AJ_MG_OPT_GUESS_TEST = 1 (default): an initial guess p0 is kept only if
+ *     its FP64 residual norm is not above the zero guess's (||b||), else
+ *     the solve restarts from zero; 0: a finite guess is always kept (used
+ *     to measure what a guess is worth). */
+enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1, AJ_MG_OPT_GUESS_TEST = 2 };
 int aj_mg_set_option(aj_mg_plan* plan, int option, int value);
 
 #define AJ_MG_MAX_NORMS 64
