@@ -5,7 +5,7 @@
 //   ax (p[i+1] + p[i-1] - 2p) + ay (p[j+1] + p[j-1] - 2p) = b
 // with edge (Neumann) ghosts at the inlet row and both walls.
 //
-// Hierarchy: cell-centred 2x coarsening down to <= 1024 cells; coarse level
+// Hierarchy: cell-centred 2x coarsening down to <= 64 cells; coarse level
 // l has coefficients ax/4^l, ay/4^l and an outlet ghost gamma_l * last that
 // puts its Dirichlet point where the fine grid has it
 // (gamma_l = 1 - 1/(1/2 - 2^-(l+1))).  Smoother: red-black Gauss-Seidel
@@ -34,7 +34,8 @@ constexpr int kBottomCells = 4096;
 // dynamic LDS available to the bottom kernel without a function attribute
 constexpr size_t kBottomLdsBytes = 64 * 1024;
 constexpr int kBottomThreads = 1024;
-constexpr int kTbCoarseCells = 1024;
+constexpr int kTbCoarseCells = 64;     // coarsening stops at or below this many cells
+constexpr int kTbMaxCoarse = 1024;     // unknowns the exact coarsest solve accepts
 constexpr int kCyclesPerOuter = 2;
 
 __device__ __forceinline__ bool tb_gated(const aj_mg_stats* g) {
@@ -491,7 +492,7 @@ int tb_create(aj_mg_plan* p) {
   const int nL = p->lnx[L], mL = p->lny[L];
   const int rows = (L == 0) ? nL - 1 : nL;
   const int N = rows * mL;
-  if ((long)nL * mL > kTbCoarseCells || rows < 1) {
+  if ((long)N > kTbMaxCoarse || rows < 1) {
     char msg[200];
     std::snprintf(msg, sizeof(msg),
                   "Grid dimensions (%d, %d) not suitable for multigrid: coarsest level has %d "

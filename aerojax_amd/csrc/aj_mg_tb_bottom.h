@@ -1,6 +1,7 @@
 // Bottom of the textbook V-cycle: every level with <= kBottomCells cells
 // runs in one workgroup with its iterates and right-hand sides in LDS; the
-// coarsest level (<= 1024 cells) is solved exactly in FP64 by a cosine
+// coarsest level (<= 1024 unknowns; <= 64 cells wherever the sides stay
+// even) is solved exactly in FP64 by a cosine
 // transform along y (the walls are Neumann, so DCT-II diagonalises the y
 // operator: eigenvalues -4 ay sin^2(pi k / 2m)) and one tridiagonal solve
 // along x per cosine mode (LU factors precomputed on the host at plan

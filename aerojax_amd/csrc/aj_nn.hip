@@ -213,15 +213,25 @@ int ensure_act(aj_nn* n, int B, int H, int W) {
   return AJ_OK;
 }
 
-int conv_launch(const ConvArgs& c, hipStream_t st) {
-  const long P = (long)c.B * c.H * c.W;
-  const int nt = (c.co % 64 == 0) ? 64 : (c.co % 32 == 0 ? 32 : 16);
-  dim3 grid(blocks(P) > 0 ? (unsigned)((P + 63) / 64) : 1, c.co / nt);
-  if (nt == 64) hipLaunchKernelGGL(k_conv3x3<64>, grid, dim3(256), 0, st, c);
-  else if (nt == 32) hipLaunchKernelGGL(k_conv3x3<32>, grid, dim3(256), 0, st, c);
-  else hipLaunchKernelGGL(k_conv3x3<16>, grid, dim3(256), 0, st, c);
+// tile choice: 8x32-pixel patches where the layer has enough of them to
+// fill the chip, 4x16 otherwise; NT = 64 / 32 / 16 output channels
+template <int PH, int PW>
+static int conv_launch_t(const ConvArgs& c, int nt, hipStream_t st) {
+  const long tiles = (long)c.B * ((c.H + PH - 1) / PH) * ((c.W + PW - 1) / PW);
+  dim3 grid((unsigned)tiles, c.co / nt);
+  if (nt == 64) hipLaunchKernelGGL((k_conv3x3<PH, PW, 64>), grid, dim3(256), 0, st, c);
+  else if (nt == 32) hipLaunchKernelGGL((k_conv3x3<PH, PW, 32>), grid, dim3(256), 0, st, c);
+  else hipLaunchKernelGGL((k_conv3x3<PH, PW, 16>), grid, dim3(256), 0, st, c);
   AJ_LAUNCH_CHECK("k_conv3x3");
   return AJ_OK;
+}
+
+int conv_launch(const ConvArgs& c, hipStream_t st) {
+  if (c.ci % 32 || c.co % 16) return fail(AJ_ERR_INVALID, "conv: ci % 32 or co % 16 != 0");
+  const int nt = (c.co % 64 == 0) ? 64 : (c.co % 32 == 0 ? 32 : 16);
+  const long big = (long)c.B * ((c.H + 7) / 8) * ((c.W + 31) / 32) * (c.co / nt);
+  if (big >= 512 && c.W >= 32) return conv_launch_t<8, 32>(c, nt, st);
+  return conv_launch_t<4, 16>(c, nt, st);
 }
 
 // forward conv + GELU: out (post) and pre both stored

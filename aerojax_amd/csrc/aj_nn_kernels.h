@@ -45,8 +45,15 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 // D[pixel][co] = sum_{tap, ci} in[pixel + tap][ci] * w[co][tap][ci]
 // A operand (16 pixels x 32 ci): lane l holds pixel l%16, ci 8*(l/16)..+7.
 // B operand (32 ci x 16 co):     lane l holds co l%16,    ci 8*(l/16)..+7.
-// D (16 pixels x 16 co):         lane l holds co l%16, pixels 4*(l/16)..+3.
-// A workgroup (4 waves) covers 64 pixels x NT output channels.
+// D (16 pixels x 16 co):         lane l holds co l%16, pixels 4*(l/16)+i.
+//
+// A workgroup (4 waves) computes a PH x PW pixel patch of one image for NT
+// output channels.  Per 32-channel slice of the input it stages the
+// (PH+2) x (PW+2) input patch and the 9 x NT x 32 weights in LDS, then each
+// wave runs its PH/4 pixel rows: per tap, NT/16 B fragments are read once
+// and reused across the wave's (PH/4)(PW/16) pixel sub-tiles.  LDS rows are
+// 32 bf16 + 8 pad (80 B): the 16 lanes of a quarter-wave then hit 16
+// distinct 4-bank groups, so the 16-byte fragment reads are conflict-free.
 // ---------------------------------------------------------------------------
 struct ConvArgs {
   const bf16_t* in;
@@ -64,55 +71,93 @@ struct ConvArgs {
   int act;              // 1 = GELU
 };
 
-template <int NT>
+constexpr int kLdsRow = 40;  // bf16 per LDS row (32 channels + pad)
+
+template <int PH, int PW, int NT>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
-  constexpr int NJ = NT / 16;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int HW = a.H * a.W;
-  const long P = (long)a.B * HW;
-  const long pbase = (long)blockIdx.x * 64 + wv * 16;
-  const long pa = pbase + (lane & 15);  // A-operand pixel of this lane
-  const int kq = (lane >> 4) * 8;       // channel offset of this lane's k-slice
-  int b = 0, h = 0, w = 0;
-  const bool pv = pa < P;
-  if (pv) {
-    b = (int)(pa / HW);
-    const int r = (int)(pa - (long)b * HW);
-    h = r / a.W;
-    w = r - h * a.W;
-  }
+  static_assert(PH % 4 == 0 && PW % 16 == 0 && NT % 16 == 0, "conv tile");
+  constexpr int RW = PW + 2, RP = (PH + 2) * RW;
+  constexpr int WR = PH / 4;         // pixel rows per wave
+  constexpr int SX = PW / 16;        // 16-pixel sub-tiles per row
+  constexpr int MS = WR * SX, NS = NT / 16;
+  __shared__ __attribute__((aligned(16))) bf16_t sx[RP * kLdsRow];
+  __shared__ __attribute__((aligned(16))) bf16_t sw[9 * NT * kLdsRow];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tilesW = (a.W + PW - 1) / PW, tilesH = (a.H + PH - 1) / PH;
+  int t = blockIdx.x;
+  const int tw = t % tilesW;
+  t /= tilesW;
+  const int th = t % tilesH;
+  const int b = t / tilesH;
+  const int h0 = th * PH, w0 = tw * PW;
   const int co0 = blockIdx.y * NT;
-  floatx4 acc[NJ];
+  const int kq = (lane >> 4) * 8;
+  floatx4 acc[MS][NS];
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int t = 0; t < 9; ++t) {
-    const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
-    const bool ok = pv && hh >= 0 && hh < a.H && ww >= 0 && ww < a.W;
-    const bf16_t* src = a.in + ((long)(b * a.H + (ok ? hh : 0)) * a.W + (ok ? ww : 0)) * a.ldi + kq;
-    const bf16_t* wrow = a.w + ((long)(co0 + (lane & 15)) * 9 + t) * a.ci + kq;
-    for (int c0 = 0; c0 < a.ci; c0 += 32) {
-      const bf16x8 A = ok ? load8(src + c0) : zero8();
+  for (int m = 0; m < MS; ++m)
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const bf16x8 Bm = load8(wrow + (long)j * 16 * 9 * a.ci + c0);
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bm, acc[j], 0, 0, 0);
+    for (int j = 0; j < NS; ++j) acc[m][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const bf16_t* img = a.in + (long)b * a.H * a.W * a.ldi;
+  for (int c0 = 0; c0 < a.ci; c0 += 32) {
+    // input patch with a one-pixel halo, zero outside the image
+    for (int e = tid; e < RP * 4; e += 256) {
+      const int px = e >> 2, q = e & 3;
+      const int rr = px / RW, cc = px - rr * RW;
+      const int hh = h0 + rr - 1, ww = w0 + cc - 1;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
+        v = *reinterpret_cast<const uint4*>(img + ((long)hh * a.W + ww) * a.ldi + c0 + q * 8);
+      *reinterpret_cast<uint4*>(sx + px * kLdsRow + q * 8) = v;
+    }
+    // weights of the slice: row (tap, co) holds 32 channels
+    for (int e = tid; e < 9 * NT * 4; e += 256) {
+      const int q = e & 3, r = e >> 2;
+      const int col = r / 9, tap = r - col * 9;
+      const uint4 v = *reinterpret_cast<const uint4*>(a.w + ((long)(co0 + col) * 9 + tap) * a.ci +
+                                                      c0 + q * 8);
+      *reinterpret_cast<uint4*>(sw + (tap * NT + col) * kLdsRow + q * 8) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap - (tap / 3) * 3;
+      bf16x8 Bf[NS];
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+        Bf[j] = *reinterpret_cast<const bf16x8*>(sw + (tap * NT + 16 * j + (lane & 15)) * kLdsRow + kq);
+#pragma unroll
+      for (int m = 0; m < MS; ++m) {
+        const int row = wv * WR + m / SX;
+        const int col = (m % SX) * 16 + (lane & 15);
+        const bf16x8 A =
+            *reinterpret_cast<const bf16x8*>(sx + ((row + dy) * RW + col + dx) * kLdsRow + kq);
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+          acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bf[j], acc[m][j], 0, 0, 0);
       }
     }
+    __syncthreads();
   }
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) {
+  for (int j = 0; j < NS; ++j) {
     const int co = co0 + 16 * j + (lane & 15);
     const float bv = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const long p = pbase + (lane >> 4) * 4 + i;
-      if (p >= P) continue;
-      const float v = acc[j][i] + bv;
-      if (a.pre) a.pre[p * a.ldp + co] = f2bf(v);
-      if (a.out) a.out[p * a.ldo + co] = f2bf(a.act ? gelu(v) : v);
-      if (a.outf) {
-        float* o = a.outf + p * a.ldf + co;
-        *o = a.accum ? *o + v : v;
+    for (int m = 0; m < MS; ++m) {
+      const int hh = h0 + wv * WR + m / SX;
+      if (hh >= a.H) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ww = w0 + (m % SX) * 16 + (lane >> 4) * 4 + i;
+        if (ww >= a.W) continue;
+        const long p = ((long)b * a.H + hh) * a.W + ww;
+        const float v = acc[m][j][i] + bv;
+        if (a.pre) a.pre[p * a.ldp + co] = f2bf(v);
+        if (a.out) a.out[p * a.ldo + co] = f2bf(a.act ? gelu(v) : v);
+        if (a.outf) {
+          float* o = a.outf + p * a.ldf + co;
+          *o = a.accum ? *o + v : v;
+        }
       }
     }
   }

@@ -68,8 +68,10 @@ def test_defect_correction_reaches_1e6_from_fp32_cycles():
 
 
 def test_hierarchy_and_gamma():
-    assert T.hierarchy(2048, 512)[-1] == (64, 16)
-    assert T.hierarchy(512, 96)[-1] == (64, 12)
-    assert T.hierarchy(64, 16) == [(64, 16)]
+    assert T.hierarchy(2048, 512)[-1] == (16, 4)
+    assert T.hierarchy(512, 96)[-1] == (16, 3)
+    assert T.hierarchy(64, 16) == [(64, 16), (32, 8), (16, 4)]
+    assert T.hierarchy(8, 8) == [(8, 8)]
+    assert T.hierarchy(96, 33) == [(96, 33)]  # odd side: no coarsening
     assert np.isclose(T.outlet_gamma(1), -3.0)
     assert -1.0 > T.outlet_gamma(8) > -1.01
