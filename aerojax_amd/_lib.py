@@ -37,7 +37,7 @@ EXPORTS = (
     "aj_profile_begin", "aj_profile_end", "aj_profile_read",
 )
 
-PROF_TAGS = ("rk", "rhs", "project", "mg_pre0", "mg_post0", "mg_pre", "mg_post", "mg_bottom",
+PROF_TAGS = ("rk", "rhs", "project", "mg_pre0", "mg_post0", "mg_pre", "mg_post", "mg_coarse",
              "mg_resid")
 
 

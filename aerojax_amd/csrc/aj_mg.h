@@ -9,7 +9,7 @@ struct aj_mg_plan {
   int dtype = AJ_F32;
   int variant = AJ_MG_REFERENCE;
   int nx = 0, ny = 0;
-  int levels = 4;     // reference: requested levels
+  int levels = 4;     // requested levels (textbook: 0 = automatic)
   int ml = 0;         // reference: recursion depth (coarsest level index)
   double dx = 0, dy = 0;
   std::vector<int> lnx, lny;  // level shapes, level 0 = fine grid
@@ -26,11 +26,14 @@ struct aj_mg_plan {
   // textbook variant
   std::vector<float*> tB, tY, tE;   // FP32 V-cycle buffers per level
   double* p64[2] = {nullptr, nullptr};  // FP64 iterate (device-selected ping-pong)
-  float* coarse_dct = nullptr;      // coarsest level: orthonormal DCT-II [m][m]
-  float* coarse_l = nullptr;        // coarsest level: per-mode LU multipliers [m][rows]
-  float* coarse_dinv = nullptr;     // coarsest level: per-mode 1/diagonal of U [m][rows]
+  double* coarse_C = nullptr;       // coarsest level: orthonormal DCT-II [m][m]
+  float* coarse_Cf = nullptr;       // same, FP32
+  double* coarse_coef = nullptr;    // coarsest level: PCR multipliers [steps][m][2][rows]
+  double* coarse_dinv = nullptr;    // coarsest level: 1 / reduced diagonal [m][rows]
+  double* coarse_W = nullptr;       // coarsest level: transformed rhs [m][rows]
+  double* coarse_X = nullptr;       // coarsest level: per-mode solution [m][rows]
   int coarse_rows = 0;              // unknown rows of the coarsest level
-  int bottom_level = 0;             // first level handled by the bottom kernel
+  int coarse_steps = 0;             // PCR steps (ceil log2 rows)
 
   // reductions and device-side state
   double* partial = nullptr;

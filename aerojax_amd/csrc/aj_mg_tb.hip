@@ -5,20 +5,21 @@
 //   ax (p[i+1] + p[i-1] - 2p) + ay (p[j+1] + p[j-1] - 2p) = b
 // with edge (Neumann) ghosts at the inlet row and both walls.
 //
-// Hierarchy: cell-centred 2x coarsening down to <= 64 cells; coarse level
+// Hierarchy: cell-centred 2x coarsening down to <= 65536 cells (or exactly
+// the requested number of levels); coarse level
 // l has coefficients ax/4^l, ay/4^l and an outlet ghost gamma_l * last that
 // puts its Dirichlet point where the fine grid has it
 // (gamma_l = 1 - 1/(1/2 - 2^-(l+1))).  Smoother: red-black Gauss-Seidel
 // with the exact row solve.  Restriction: 2x2 average.  Prolongation:
 // cell-centred bilinear.  Coarsest level: exact FP64 solve (cosine
-// transform along y, one tridiagonal solve per mode along x).
+// transform along y, one tridiagonal solve per mode along x, three kernels).
 //
-// Kernels per V-cycle on the large levels (temporal blocking in LDS):
+// Kernels per V-cycle on every level above the coarsest (temporal blocking
+// in LDS):
 //   pre  : 2 RB-GS sweeps (4 half-sweeps, halo 5) + residual + restriction,
 //          writes the smoothed iterate Y and the coarse rhs
 //   post : Y + prolong(E_coarse), 2 RB-GS sweeps (halo 4), writes E
-// Levels with <= kBottomCells cells run in one workgroup entirely in LDS
-// (bottom kernel).  Defect correction per outer iteration:
+// Defect correction per outer iteration:
 //   resid: p64' = p64 + e (on the fly), r = b - A p64' in FP64, writes
 //          r32 and p64', FP64 partial sums of r^2
 //   final: norm = sqrt(sum r^2 dx dy), convergence gate, buffer flip
@@ -30,12 +31,8 @@
 
 namespace aj {
 
-constexpr int kBottomCells = 4096;
-// dynamic LDS available to the bottom kernel without a function attribute
-constexpr size_t kBottomLdsBytes = 64 * 1024;
-constexpr int kBottomThreads = 1024;
-constexpr int kTbCoarseCells = 64;     // coarsening stops at or below this many cells
-constexpr int kTbMaxCoarse = 1024;     // unknowns the exact coarsest solve accepts
+constexpr int kTbCoarseCells = 65536;  // coarsening stops at or below this many cells
+constexpr int kMaxLevels = 24;
 constexpr int kCyclesPerOuter = 2;
 
 __device__ __forceinline__ bool tb_gated(const aj_mg_stats* g) {
@@ -248,7 +245,7 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
   }
 }
 
-#include "aj_mg_tb_bottom.h"
+#include "aj_mg_tb_coarse.h"
 
 // ---------------------------------------------------------------------------
 // FP64 defect correction
@@ -474,69 +471,89 @@ struct Carve {
 };
 
 int tb_create(aj_mg_plan* p) {
-  // hierarchy
+  // hierarchy: levels = 0 coarsens while n*m > kTbCoarseCells; levels = L > 0
+  // asks for exactly L levels
+  if (p->nx < 3 || p->ny < 2) return fail(AJ_ERR_GRID, "textbook multigrid needs nx >= 3, ny >= 2");
   p->lnx.assign(1, p->nx);
   p->lny.assign(1, p->ny);
   {
     int n = p->nx, m = p->ny;
-    while ((long)n * m > kTbCoarseCells && n % 2 == 0 && m % 2 == 0 && n >= 4 && m >= 2) {
+    auto more = [&]() {
+      return p->levels > 0 ? (int)p->lnx.size() < p->levels
+                           : ((long)n * m > kTbCoarseCells || n > kCMaxRows);
+    };
+    while (more() && n % 2 == 0 && m % 2 == 0 && n >= 4 && m >= 2) {
       n /= 2;
       m /= 2;
       p->lnx.push_back(n);
       p->lny.push_back(m);
     }
   }
-  if (p->nx < 3 || p->ny < 2) return fail(AJ_ERR_GRID, "textbook multigrid needs nx >= 3, ny >= 2");
   const int L = (int)p->lnx.size() - 1;
-  if (L + 1 > kMaxLevels) return fail(AJ_ERR_GRID, "textbook multigrid: too many levels");
   const int nL = p->lnx[L], mL = p->lny[L];
   const int rows = (L == 0) ? nL - 1 : nL;
-  const int N = rows * mL;
-  if ((long)N > kTbMaxCoarse || rows < 1) {
-    char msg[200];
+  if ((p->levels > 0 && L + 1 != p->levels) || L + 1 > kMaxLevels || rows < 1 ||
+      rows > kCMaxRows) {
+    char msg[240];
     std::snprintf(msg, sizeof(msg),
-                  "Grid dimensions (%d, %d) not suitable for multigrid: coarsest level has %d "
-                  "unknowns",
-                  p->nx, p->ny, N);
+                  "Grid dimensions (%d, %d) not suitable for multigrid: %d levels (requested "
+                  "%d), coarsest level %dx%d with %d unknown rows (at most %d)",
+                  p->nx, p->ny, L + 1, p->levels, nL, mL, rows, kCMaxRows);
     return fail(AJ_ERR_GRID, msg);
   }
   p->coarse_rows = rows;
-  // exact coarsest solve: orthonormal DCT-II along y, LU factors of the
-  // per-mode tridiagonal along x (rows 0..rows-1 of level L)
-  std::vector<double> Cm((size_t)mL * mL), Lf((size_t)mL * rows), Dinv((size_t)mL * rows);
+  int steps = 0;
+  while ((1 << steps) < rows) ++steps;
+  p->coarse_steps = steps;
+  // exact coarsest solve: orthonormal DCT-II along y; per cosine mode the
+  // tridiagonal T_k along x and its parallel-cyclic-reduction multipliers
+  std::vector<double> Cm((size_t)mL * mL), Coef((size_t)std::max(steps, 1) * mL * 2 * rows),
+      Dinv((size_t)mL * rows);
+  std::vector<float> Cf((size_t)mL * mL);
   {
     const double pi = 3.14159265358979323846;
-    const double s = std::ldexp(1.0, 2 * L);
-    const double ax = 1.0 / (p->dx * p->dx) / s, ay = 1.0 / (p->dy * p->dy) / s;
-    const double g = L > 0 ? outlet_gamma(L) : 0.0;
+    const double sl = std::ldexp(1.0, 2 * L);
+    const double axL = 1.0 / (p->dx * p->dx) / sl, ayL = 1.0 / (p->dy * p->dy) / sl;
+    const double dE_last = (L == 0) ? axL : axL * (1.0 - outlet_gamma(L));
     for (int k = 0; k < mL; ++k)
-      for (int j = 0; j < mL; ++j)
-        Cm[(size_t)k * mL + j] =
-            std::sqrt((k == 0 ? 1.0 : 2.0) / mL) * std::cos(pi * k * (j + 0.5) / mL);
+      for (int j = 0; j < mL; ++j) {
+        const double v = std::sqrt((k == 0 ? 1.0 : 2.0) / mL) * std::cos(pi * k * (j + 0.5) / mL);
+        Cm[(size_t)k * mL + j] = v;
+        Cf[(size_t)k * mL + j] = (float)v;
+      }
+    std::vector<double> a(rows), d(rows), c(rows), na(rows), nd(rows), nc(rows);
     for (int k = 0; k < mL; ++k) {
       const double sk = std::sin(pi * k / (2.0 * mL));
-      const double mu = -4.0 * ay * sk * sk;
-      double dprev = 0.0;
+      const double mu = -4.0 * ayL * sk * sk;
       for (int i = 0; i < rows; ++i) {
-        const double dW = i > 0 ? ax : 0.0;
-        const double dE = (i < rows - 1 || L == 0) ? ax : ax * (1.0 - g);
-        const double diag = -(dW + dE) + mu;
-        double d = diag, l = 0.0;
-        if (i > 0) {
-          l = ax / dprev;
-          d = diag - l * ax;
+        const double dW = i > 0 ? axL : 0.0;
+        const double dE = i < rows - 1 ? axL : dE_last;
+        a[i] = i > 0 ? axL : 0.0;
+        c[i] = i < rows - 1 ? axL : 0.0;
+        d[i] = mu - (dW + dE);
+      }
+      for (int t = 0; t < steps; ++t) {
+        const int s = 1 << t;
+        double* al = &Coef[(((size_t)t * mL + k) * 2) * rows];
+        double* ga = al + rows;
+        for (int i = 0; i < rows; ++i) {
+          al[i] = i >= s ? -a[i] / d[i - s] : 0.0;
+          ga[i] = i + s < rows ? -c[i] / d[i + s] : 0.0;
+          na[i] = i >= s ? al[i] * a[i - s] : 0.0;
+          nc[i] = i + s < rows ? ga[i] * c[i + s] : 0.0;
+          nd[i] = d[i] + (i >= s ? al[i] * c[i - s] : 0.0) + (i + s < rows ? ga[i] * a[i + s] : 0.0);
         }
-        if (d == 0.0) return fail(AJ_ERR_GRID, "textbook multigrid: singular coarse operator");
-        Lf[(size_t)k * rows + i] = l;
-        Dinv[(size_t)k * rows + i] = 1.0 / d;
-        dprev = d;
+        a.swap(na);
+        c.swap(nc);
+        d.swap(nd);
+      }
+      for (int i = 0; i < rows; ++i) {
+        if (d[i] == 0.0 || !std::isfinite(d[i]))
+          return fail(AJ_ERR_GRID, "textbook multigrid: singular coarse operator");
+        Dinv[(size_t)k * rows + i] = 1.0 / d[i];
       }
     }
   }
-  // first level handled by the bottom kernel
-  int bot = L;
-  while (bot > 0 && (long)p->lnx[bot - 1] * p->lny[bot - 1] <= kBottomCells) --bot;
-  p->bottom_level = bot;
   p->ml = L;
 
   const long n0 = (long)p->nx * p->ny;
@@ -553,9 +570,12 @@ int tb_create(aj_mg_plan* p) {
     }
     p->p64[0] = (double*)cv.take(n0 * 8);
     p->p64[1] = (double*)cv.take(n0 * 8);
-    p->coarse_dct = (float*)cv.take((size_t)mL * mL * 4);
-    p->coarse_l = (float*)cv.take((size_t)mL * rows * 4);
-    p->coarse_dinv = (float*)cv.take((size_t)mL * rows * 4);
+    p->coarse_C = (double*)cv.take(Cm.size() * 8);
+    p->coarse_Cf = (float*)cv.take(Cf.size() * 4);
+    p->coarse_coef = (double*)cv.take(Coef.size() * 8);
+    p->coarse_dinv = (double*)cv.take(Dinv.size() * 8);
+    p->coarse_W = (double*)cv.take((size_t)mL * rows * 8);
+    p->coarse_X = (double*)cv.take((size_t)mL * rows * 8);
     p->npartial = (int)cdiv(n0, 256);
     p->partial = (double*)cv.take((size_t)p->npartial * 8);
     p->partial2 = (double*)cv.take((size_t)p->npartial * 8);
@@ -568,50 +588,48 @@ int tb_create(aj_mg_plan* p) {
       AJ_HIP(hipMalloc(&p->ws, p->ws_bytes));
     }
   }
-  {
-    // the tables feed the FP32 V-cycle's coarsest correction: stored as float
-    auto up = [](float* dst, const std::vector<double>& src) {
-      std::vector<float> f(src.begin(), src.end());
-      return hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice);
-    };
-    AJ_HIP(up(p->coarse_dct, Cm));
-    AJ_HIP(up(p->coarse_l, Lf));
-    AJ_HIP(up(p->coarse_dinv, Dinv));
-  }
+  AJ_HIP(hipMemcpy(p->coarse_C, Cm.data(), Cm.size() * 8, hipMemcpyHostToDevice));
+  AJ_HIP(hipMemcpy(p->coarse_Cf, Cf.data(), Cf.size() * 4, hipMemcpyHostToDevice));
+  AJ_HIP(hipMemcpy(p->coarse_coef, Coef.data(), Coef.size() * 8, hipMemcpyHostToDevice));
+  AJ_HIP(hipMemcpy(p->coarse_dinv, Dinv.data(), Dinv.size() * 8, hipMemcpyHostToDevice));
   AJ_HIP(hipHostMalloc((void**)&p->host_stats, sizeof(aj_mg_stats), hipHostMallocDefault));
   return AJ_OK;
 }
 
-// one FP32 V-cycle for A_0 e = tB[0]; result in tE[0]; warm = use tE[0] as start
+// exact solve of the coarsest level: tB[L] -> tE[L]
+static int tb_coarse(aj_mg_plan* p, hipStream_t st) {
+  const int L = p->ml;
+  TbCoarse c;
+  c.n = p->lnx[L];
+  c.m = p->lny[L];
+  c.rows = p->coarse_rows;
+  c.steps = p->coarse_steps;
+  c.C = p->coarse_C;
+  c.Cf = p->coarse_Cf;
+  c.coef = p->coarse_coef;
+  c.dinv = p->coarse_dinv;
+  c.W = p->coarse_W;
+  c.X = p->coarse_X;
+  const aj_mg_stats* gate = p->stats;
+  // b read, e written (the per-mode FP64 intermediates stay in L2)
+  ProfScope prof(AJ_PROF_MG_COARSE, (double)c.n * c.m * 8, st);
+  hipLaunchKernelGGL(k_tb_cfwd, dim3(cdiv(c.rows, kCT), cdiv(c.m, kCT)), dim3(256), 0, st, c,
+                     (const float*)p->tB[L], gate);
+  AJ_LAUNCH_CHECK("k_tb_cfwd");
+  const int nt = c.rows <= 128 ? 64 : (c.rows <= 256 ? 128 : (c.rows <= 512 ? 256 : 512));
+  hipLaunchKernelGGL(k_tb_cpcr, dim3(c.m), dim3(nt), (size_t)2 * c.rows * sizeof(double), st, c,
+                     gate);
+  AJ_LAUNCH_CHECK("k_tb_cpcr");
+  hipLaunchKernelGGL(k_tb_cinv, dim3(cdiv(c.n, kCT), cdiv(c.m, kCT)), dim3(256), 0, st, c,
+                     p->tE[L], gate);
+  AJ_LAUNCH_CHECK("k_tb_cinv");
+  return AJ_OK;
+}
+
+// one FP32 V-cycle for A_lev e = tB[lev]; result in tE[lev]; warm = use
+// tE[lev] as the start (the exact coarsest solve ignores it)
 static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
-  const int bot = p->bottom_level;
-  if (lev == bot) {
-    TbBottom B;
-    const int L = p->ml;
-    B.nlev = L - bot + 1;
-    int off = 0;
-    for (int k = 0; k < B.nlev; ++k) {
-      B.lev[k] = make_level(p, bot + k);
-      B.off[k] = off;
-      off += B.lev[k].n * B.lev[k].m;
-      B.gam_next[k] = (bot + k + 1 <= L) ? (float)outlet_gamma(bot + k + 1) : 1.0f;
-      B.inv_m[k] = 1.0f / (float)B.lev[k].m;
-    }
-    B.total = off;
-    B.coarse_rows = p->coarse_rows;
-    const size_t base = (size_t)p->lnx[L] * p->lny[L] * sizeof(double) + (size_t)2 * off * sizeof(float);
-    const size_t tables = (size_t)p->lny[L] * (p->lny[L] + 2 * p->coarse_rows) * sizeof(float);
-    const bool stage = base + tables <= kBottomLdsBytes;
-    const size_t smem = base + (stage ? tables : 0);
-    ProfScope prof(AJ_PROF_MG_BOTTOM, (double)B.lev[0].n * B.lev[0].m * (warm ? 12 : 8), st);
-    hipLaunchKernelGGL(stage ? k_tb_bottom<true> : k_tb_bottom<false>, dim3(1), dim3(kBottomThreads),
-                       smem, st, B, (const float*)p->tB[lev],
-                       warm ? (const float*)p->tE[lev] : (const float*)nullptr, p->tE[lev],
-                       (const float*)p->coarse_dct, (const float*)p->coarse_l,
-                       (const float*)p->coarse_dinv, (const aj_mg_stats*)p->stats);
-    AJ_LAUNCH_CHECK("k_tb_bottom");
-    return AJ_OK;
-  }
+  if (lev == p->ml) return tb_coarse(p, st);
   const TbLevel Lv = make_level(p, lev);
   const int n = Lv.n, m = Lv.m;
   dim3 gpre(cdiv(m, kPY), cdiv(n, kPX));

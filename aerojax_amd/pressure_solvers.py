@@ -144,17 +144,19 @@ def poisson_multigrid(rhs, mask, dx: float, dy: float, levels: int = 4, v_cycles
 
 def poisson_multigrid_to_tolerance(rhs, dx: float, dy: float, tolerance: float = 1e-6,
                                    max_cycles: int = 40, *, p0=None, out=None,
-                                   return_stats: bool = False):
+                                   return_stats: bool = False, levels: int = 0):
     """Textbook multigrid to ``sqrt(sum_{i<nx-1} (b - A p)^2 dx dy) <=
     tolerance`` on the consistent outlet-Dirichlet operator (FP32 V(2,2)
-    cycles, FP64 residual/norm/accumulation)."""
+    cycles, FP64 residual/norm/accumulation, exact FP64 coarsest solve).
+    ``levels`` = 0 coarsens down to <= 65536 cells; L > 0 asks for exactly
+    L levels."""
     A.require_gpu()
     want_np = A.is_numpy(rhs)
     dt_ = A.torch_dtype_of(rhs)
     bt = A.as_device(rhs, dt_, name="rhs")
     nx, ny = bt.shape
     pt = A.as_device(p0, dt_, bt.device, "p0")
-    plan = get_plan(nx, ny, dx, dy, dt_, "textbook", 0, bt.device)
+    plan = get_plan(nx, ny, dx, dy, dt_, "textbook", levels, bt.device)
     po = torch.empty_like(bt) if out is None else out
     plan.solve(bt, po, pt, max_cycles, tolerance)
     res = A.out_like(po, want_np)

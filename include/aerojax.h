@@ -188,7 +188,7 @@ enum {
   AJ_PROF_MG_POST0 = 4,    /* textbook prolong + post-smooth, level 0   */
   AJ_PROF_MG_PRE = 5,      /* same, levels >= 1                         */
   AJ_PROF_MG_POST = 6,
-  AJ_PROF_MG_BOTTOM = 7,   /* single-workgroup coarse levels            */
+  AJ_PROF_MG_COARSE = 7,   /* exact coarsest-level solve (3 kernels)    */
   AJ_PROF_MG_RESID = 8,    /* FP64 defect-correction residual           */
   AJ_PROF_NN_CONV = 9,     /* network 3x3 conv (forward / input grad);  */
                            /* "bytes" holds FLOPs for this tag          */

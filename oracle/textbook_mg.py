@@ -24,12 +24,15 @@ Hierarchy (level 0 = fine grid):
     Keeping every level's Dirichlet point where the fine grid has it gives a
     grid-independent V(2,2) factor of ~0.085 (a face-Dirichlet coarse
     outlet gives ~0.167).
-  * coarsening stops when n*m <= COARSE_CELLS (64) or a side stops being
-    even; the coarsest level (at most MAX_COARSE_UNKNOWNS = 1024 unknowns)
-    is solved exactly (dense FP64 inverse here; a cosine transform +
-    tridiagonal solves on the GPU, same linear map).  Coarsening below 1024
-    cells leaves the V(2,2) factor unchanged (0.083 at 1024x256 with a 64x16,
-    32x8, 16x4 or 8x2 coarsest level) and shortens the exact solve.
+  * coarsening stops when n*m <= COARSE_CELLS (65536) and n <=
+    MAX_COARSE_ROWS (1024), or when a side stops being even (or after
+    exactly ``levels`` levels when that is given); the
+    coarsest level is solved exactly in FP64: the walls are Neumann, so the
+    orthonormal DCT-II along y diagonalises the y operator (eigenvalues
+    -4 ay sin^2(pi k / 2m)), leaving one tridiagonal system along x per
+    cosine mode (Thomas here; parallel cyclic reduction on the GPU; the same
+    linear map).  The exact coarse solve is the "coarse-grid solve" of the
+    V-cycle; a 512x128 coarsest level replaces five further levels.
 Smoother: red-black Gauss-Seidel (red = (i+j) even first), exact row solve
 (clamped Neumann neighbours drop out of the row and the diagonal).
 Restriction: 2x2 average (cell-centred full weighting; level-0 outlet row
@@ -43,18 +46,29 @@ from __future__ import annotations
 
 import numpy as np
 
-COARSE_CELLS = 64
-MAX_COARSE_UNKNOWNS = 1024
+COARSE_CELLS = 65536
+MAX_COARSE_ROWS = 1024
 
 
-def hierarchy(nx, ny):
-    """Level shapes ``[(n0, m0), (n1, m1), ...]`` down to the coarsest."""
+def hierarchy(nx, ny, levels=0):
+    """Level shapes ``[(n0, m0), (n1, m1), ...]`` down to the coarsest.
+
+    ``levels`` = 0: coarsen while n*m > COARSE_CELLS or n > MAX_COARSE_ROWS,
+    and both sides are even; ``levels`` = L > 0: exactly L levels
+    (ValueError if the sides run out)."""
     shapes = [(nx, ny)]
     n, m = nx, ny
-    while n * m > COARSE_CELLS and n % 2 == 0 and m % 2 == 0 and n >= 4 and m >= 2:
+
+    def more():
+        return len(shapes) < levels if levels > 0 else (n * m > COARSE_CELLS or
+                                                         n > MAX_COARSE_ROWS)
+    while more() and n % 2 == 0 and m % 2 == 0 and n >= 4 and m >= 2:
         n //= 2
         m //= 2
         shapes.append((n, m))
+    if levels > 0 and len(shapes) != levels:
+        raise ValueError(f"Grid dimensions ({nx}, {ny}) not suitable for multigrid: "
+                         f"{levels} levels requested, {len(shapes)} possible")
     return shapes
 
 
@@ -173,6 +187,48 @@ def prolong(e, fine_level, fine_shape):
     return out
 
 
+def coarse_tables(level, shape, ax, ay):
+    """FP64 tables of the exact coarse solve: orthonormal DCT-II matrix
+    C[k, j] and the per-mode tridiagonal (off-diagonal ``off``, diagonal
+    ``diag[k, i]``) on the level's unknown rows."""
+    n, m = shape
+    rows = n - 1 if level == 0 else n
+    axl, ayl = _coefs(level, ax, ay)
+    k = np.arange(m)[:, None]
+    j = np.arange(m)[None, :]
+    C = np.sqrt(np.where(k == 0, 1.0, 2.0) / m) * np.cos(np.pi * k * (j + 0.5) / m)
+    mu = -4.0 * ayl * np.sin(np.pi * np.arange(m) / (2.0 * m)) ** 2
+    i = np.arange(rows)
+    dW = np.where(i > 0, axl, 0.0)
+    dE = np.full(rows, axl)
+    if level > 0:
+        dE[-1] = axl * (1.0 - outlet_gamma(level))
+    diag = -(dW + dE)[None, :] + mu[:, None]
+    return C, axl, diag
+
+
+def coarse_solve(b, level, shape, tables):
+    """Exact FP64 solve of ``A_level x = b`` on the unknown rows (the level-0
+    Dirichlet row of the result is 0): DCT along y, Thomas along x per mode,
+    inverse DCT."""
+    C, off, diag = tables
+    n, m = shape
+    rows = diag.shape[1]
+    W = np.asarray(b[:rows], dtype=np.float64) @ C.T      # W[i, k]
+    dd = diag.T.copy()                                    # [i, k]
+    X = W.copy()
+    for i in range(1, rows):                              # forward elimination
+        f = off / dd[i - 1]
+        dd[i] -= f * off
+        X[i] -= f * X[i - 1]
+    X[rows - 1] /= dd[rows - 1]
+    for i in range(rows - 2, -1, -1):                     # back substitution
+        X[i] = (X[i] - off * X[i + 1]) / dd[i]
+    out = np.zeros((n, m))
+    out[:rows] = X @ C
+    return out
+
+
 def dense_operator(level, shape, ax, ay):
     """Dense FP64 matrix of the level operator on its unknowns."""
     n, m = shape
@@ -189,20 +245,21 @@ def dense_operator(level, shape, ax, ay):
 class TextbookMG:
     """FP32 V(2,2) multigrid + FP64 defect correction for one grid shape."""
 
-    def __init__(self, nx, ny, dx, dy, pre=2, post=2, dtype=np.float32):
+    def __init__(self, nx, ny, dx, dy, pre=2, post=2, dtype=np.float32, levels=0):
         self.nx, self.ny = nx, ny
         self.ax = 1.0 / (dx * dx)
         self.ay = 1.0 / (dy * dy)
         self.dx, self.dy = dx, dy
-        self.shapes = hierarchy(nx, ny)
+        self.shapes = hierarchy(nx, ny, levels)
         self.pre, self.post = pre, post
         self.dtype = dtype
         L = len(self.shapes) - 1
-        A = dense_operator(L, self.shapes[L], self.ax, self.ay)
-        if A.shape[0] > MAX_COARSE_UNKNOWNS:
+        nL = self.shapes[L][0]
+        rows = nL - 1 if L == 0 else nL
+        if rows < 1 or rows > MAX_COARSE_ROWS:
             raise ValueError(f"Grid dimensions ({nx}, {ny}) not suitable for multigrid: "
-                             f"coarsest level has {A.shape[0]} unknowns")
-        self.coarse_inv = np.linalg.inv(A)
+                             f"coarsest level has {rows} rows")
+        self.coarse = coarse_tables(L, self.shapes[L], self.ax, self.ay)
 
     def vcycle(self, b, level=0, p=None):
         """One V-cycle on ``A_level e = b`` (zero initial guess if p is None)."""
@@ -210,11 +267,7 @@ class TextbookMG:
         shape = self.shapes[level]
         L = len(self.shapes) - 1
         if level == L:
-            rows = shape[0] - 1 if level == 0 else shape[0]
-            x = (self.coarse_inv @ b[:rows].reshape(-1).astype(np.float64)).astype(T)
-            out = np.zeros(shape, dtype=T)
-            out[:rows] = x.reshape(rows, shape[1])
-            return out
+            return coarse_solve(b, level, shape, self.coarse).astype(T)
         if p is None:
             p = np.zeros(shape, dtype=T)
         p = rbgs(p, b, level, self.ax, self.ay, self.pre)

@@ -188,24 +188,31 @@ def test_reference_multigrid_large_coarse_level(gpu):
     check(host(got), ref, None, torch.float32, "mg-ref 4096x1024")
 
 
-@pytest.mark.parametrize("shape", [(64, 16), (512, 96), (2048, 512)])
-def test_textbook_multigrid_one_outer_iteration_parity(gpu, shape):
+@pytest.mark.parametrize("shape,levels", [((64, 16), 0), ((64, 16), 3), ((512, 96), 0),
+                                          ((512, 96), 4), ((2048, 512), 0), ((256, 64), 6)])
+def test_textbook_multigrid_one_outer_iteration_parity(gpu, shape, levels):
     import aerojax_amd as aj
     nx, ny = shape
     dx, dy = 4.0 / nx, 1.0 / ny
     rng = np.random.default_rng(5)
     b = rng.normal(size=(nx, ny)) * 500
-    mg = T.TextbookMG(nx, ny, dx, dy)
+    mg = T.TextbookMG(nx, ny, dx, dy, levels=levels)
     ref, st_ref = mg.solve(b, tol=0.0, max_outer=1)
     got, st = aj.poisson_multigrid_to_tolerance(dev(b, torch.float64), dx, dy, tolerance=0.0,
-                                                max_cycles=2, return_stats=True)
+                                                max_cycles=2, return_stats=True, levels=levels)
     assert st["cycles"] == 2 and st["outer"] == 1
     check(host(got), ref, None, torch.float32, f"mg-textbook 1 outer {shape}")
     # initial residual: same FP64 arithmetic on the same b
     assert np.isclose(st["norms"][0], st_ref["norms"][0], rtol=1e-12)
-    # after two FP32 V-cycles the residual is ~1e-3 of b: FP32 rounding of p
-    # (rel ~1e-7) is amplified by ||A|| ||p|| / ||r||, so compare loosely
-    assert np.isclose(st["norms"][1], st_ref["norms"][1], rtol=2e-2)
+    if len(mg.shapes) == 1:
+        # the coarsest level is the fine grid: one exact solve whose FP32
+        # output rounding alone sets the residual, differently on each side
+        assert st["norms"][1] < 1e-3 * st["norms"][0]
+        assert st_ref["norms"][1] < 1e-3 * st_ref["norms"][0]
+    else:
+        # after two FP32 V-cycles the residual is ~1e-3 of b: FP32 rounding of
+        # p (rel ~1e-7) is amplified by ||A|| ||p|| / ||r||, so compare loosely
+        assert np.isclose(st["norms"][1], st_ref["norms"][1], rtol=2e-2)
 
 
 @pytest.mark.parametrize("shape", [(512, 96), (2048, 512)])
