@@ -46,8 +46,8 @@ struct aj_mg_plan {
   int last_outer = 4;               // textbook: iterations the previous solve needed
   int use_graph = 1;                // textbook: replay a captured outer iteration
   int guess_test = 1;               // textbook: keep p0 only if ||r(p0)|| <= ||b||
-  void* graph = nullptr;            // hipGraph_t of one outer iteration
-  void* graph_exec = nullptr;       // hipGraphExec_t
+  void* graph[2] = {nullptr, nullptr};       // hipGraph_t of an even / odd outer iteration
+  void* graph_exec[2] = {nullptr, nullptr};  // hipGraphExec_t
   double* dtol = nullptr;           // textbook: tolerance of the current solve
   void* bcopy = nullptr;            // textbook: plan-owned copy of the rhs
 };

@@ -451,9 +451,11 @@ extern "C" int aj_mg_create(aj_mg_plan** out, int dtype, int variant, int nx, in
 
 extern "C" int aj_mg_destroy(aj_mg_plan* p) {
   if (!p) return AJ_OK;
-  if (p->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec);
-  if (p->graph) (void)hipGraphDestroy((hipGraph_t)p->graph);
-  p->graph_exec = p->graph = nullptr;
+  for (int k = 0; k < 2; ++k) {
+    if (p->graph_exec[k]) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec[k]);
+    if (p->graph[k]) (void)hipGraphDestroy((hipGraph_t)p->graph[k]);
+    p->graph_exec[k] = p->graph[k] = nullptr;
+  }
   if (p->host_stats) {
     (void)hipHostFree(p->host_stats);
     p->host_stats = nullptr;

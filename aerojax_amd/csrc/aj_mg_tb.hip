@@ -35,10 +35,6 @@ constexpr int kTbCoarseCells = 65536;  // coarsening stops at or below this many
 constexpr int kMaxLevels = 24;
 constexpr int kCyclesPerOuter = 2;
 
-__device__ __forceinline__ bool tb_gated(const aj_mg_stats* g) {
-  return *(volatile const int*)&g->converged != 0;
-}
-
 static double outlet_gamma(int level) {
   return 1.0 - 1.0 / (0.5 - std::ldexp(1.0, -level - 1));
 }
@@ -262,42 +258,51 @@ __global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, const
 }
 
 // p' = P[cur] + e (e may be null), r = b - A p' in FP64 on rows < nx-1.
+// The hardware deals consecutive workgroup ids out to the 8 XCDs in turn;
+// renumbering so that each XCD gets one contiguous range of blocks keeps
+// the +-row neighbours of a block in the same L2.
+__device__ __forceinline__ long xcd_block(long b, long nb) {
+  if (nb % 8) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+// r = b - A (src + e) in FP64 on rows < nx-1 (r32 <- r, FP64 partial sums
+// of r^2); with e, dst <- src + e.  src / dst are fixed per captured outer
+// iteration (even / odd), so no device state decides them.  Every load is
+// issued before the convergence gate is tested.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tb_resid(double* __restrict__ P0, double* __restrict__ P1,
-                                                  const int* __restrict__ state,
+__global__ __launch_bounds__(256) void k_tb_resid(const double* __restrict__ src,
+                                                  double* __restrict__ dst,
                                                   const float* __restrict__ e,
                                                   const T* __restrict__ b, float* __restrict__ r32,
                                                   double* __restrict__ partial, int nx, int ny,
                                                   double ax, double ay, const aj_mg_stats* gate) {
-  if (tb_gated(gate)) return;
   __shared__ double sh[16];
-  const int cur = state[0];
-  const double* src = cur ? P1 : P0;
-  double* dst = cur ? P0 : P1;
+  const int gated = gate->converged;
   const long n = (long)nx * ny;
-  double acc = 0.0;
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
-  if (c < n) {
-    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
-    if (i == nx - 1) {
-      r32[c] = 0.0f;
-      if (e) dst[c] = 0.0;
-    } else {
-      auto P = [&](long q) { return e ? src[q] + (double)e[q] : src[q]; };
-      const double pc = P(c);
-      const double W = i > 0 ? P(c - ny) : pc;
-      const double E = (i + 1 < nx - 1) ? P(c + ny) : 0.0;
-      const double S = j > 0 ? P(c - 1) : pc;
-      const double N = j < ny - 1 ? P(c + 1) : pc;
-      const double Ap = ax * ((E - pc) + (W - pc)) + ay * ((N - pc) + (S - pc));
-      const double r = (double)b[c] - Ap;
-      r32[c] = (float)r;
-      if (e) dst[c] = pc;
-      acc = r * r;
-    }
+  const long blk = xcd_block(blockIdx.x, gridDim.x);
+  const long c = blk * 256 + threadIdx.x;
+  double r = 0.0, pc = 0.0;
+  const bool in = c < n;
+  const int i = in ? (int)(c / ny) : 0, j = in ? (int)(c - (long)i * ny) : 0;
+  const bool eq = in && i < nx - 1;
+  if (eq) {
+    auto P = [&](long q) { return e ? src[q] + (double)e[q] : src[q]; };
+    pc = P(c);
+    const double W = i > 0 ? P(c - ny) : pc;
+    const double E = (i + 1 < nx - 1) ? P(c + ny) : 0.0;
+    const double S = j > 0 ? P(c - 1) : pc;
+    const double N = j < ny - 1 ? P(c + 1) : pc;
+    const double Ap = ax * ((E - pc) + (W - pc)) + ay * ((N - pc) + (S - pc));
+    r = (double)b[c] - Ap;
   }
-  const double t = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  if (gated) return;  // uniform across the grid
+  if (in) {
+    r32[c] = (float)r;
+    if (e) dst[c] = pc;
+  }
+  const double t = block_sum(r * r, sh);
+  if (threadIdx.x == 0) partial[blk] = t;
 }
 
 // Initial guess test: partial sums of r(p0)^2 (r32 <- r(p0)) and of the
@@ -388,11 +393,12 @@ __global__ __launch_bounds__(256) void k_tb_fallback(double* __restrict__ P,
 __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, double dxdy,
                                 const double* __restrict__ dtol, int flip, int cycles_added,
                                 aj_mg_stats* st, int* state) {
-  if (st->converged) return;  // this outer iteration was skipped
   __shared__ double sh[16];
+  const int skipped = st->converged;  // this outer iteration was skipped
   double acc = 0.0;
 #pragma unroll 8
   for (int k = threadIdx.x; k < np; k += 256) acc += partial[k];
+  if (skipped) return;
   const double tot = block_sum(acc, sh);
   if (threadIdx.x == 0) {
     const double nrm = sqrt(tot * dxdy);
@@ -417,7 +423,9 @@ __global__ __launch_bounds__(256) void k_tb_out(const double* __restrict__ P0,
                                                 long n) {
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= n) return;
-  out[c] = (T)(state[0] ? P1[c] : P0[c]);
+  const int s = state[0];
+  const double a0 = P0[c], a1 = P1[c];  // both loads in flight with the state
+  out[c] = (T)(s ? a1 : a0);
 }
 
 __global__ void k_tb_begin(aj_mg_stats* st, int* state, double* dtol, double tol) {
@@ -658,9 +666,12 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
 }
 
 // launches of one defect-correction outer iteration (2 V-cycles, FP64
-// residual with the update p64 += e, norm and gate); only plan-owned buffers
+// residual with the update p64 += e, norm and gate); only plan-owned
+// buffers.  Iteration k of a solve reads p64[k & 1] and writes p64[~k & 1]:
+// iterations run in order and only those after convergence are skipped, so
+// the parity of the enqueue count names the live buffer.
 template <typename T>
-static int tb_outer_launches(aj_mg_plan* p, hipStream_t st) {
+static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
   const int nx = p->nx, ny = p->ny;
   const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
   const int nb = p->npartial;
@@ -668,8 +679,8 @@ static int tb_outer_launches(aj_mg_plan* p, hipStream_t st) {
     if (int rc = tb_vcycle(p, 0, c > 0, st)) return rc;
   // p64, e, b read; p64', r32 written
   ProfScope prof(AJ_PROF_MG_RESID, (double)nx * ny * (8 + 4 + sizeof(T) + 8 + 4), st);
-  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
-                     (const int*)p->state, (const float*)p->tE[0], (const T*)p->bcopy, p->tB[0],
+  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, (const double*)p->p64[parity],
+                     p->p64[parity ^ 1], (const float*)p->tE[0], (const T*)p->bcopy, p->tB[0],
                      p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_resid");
   hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
@@ -679,22 +690,22 @@ static int tb_outer_launches(aj_mg_plan* p, hipStream_t st) {
 }
 
 template <typename T>
-static int tb_outer_iteration(aj_mg_plan* p, hipStream_t st) {
+static int tb_outer_iteration(aj_mg_plan* p, int parity, hipStream_t st) {
   // live event timing needs the individual launches, so it bypasses the graph
-  if (!p->use_graph || prof_on()) return tb_outer_launches<T>(p, st);
-  if (!p->graph_exec) {
+  if (!p->use_graph || prof_on()) return tb_outer_launches<T>(p, parity, st);
+  if (!p->graph_exec[parity]) {
     // capture once per plan on a private stream; replays launch on `st`
     hipStream_t cs;
     AJ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     int rc = AJ_OK;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      rc = tb_outer_launches<T>(p, cs);
+      rc = tb_outer_launches<T>(p, parity, cs);
       hipGraph_t g = nullptr;
       e = hipStreamEndCapture(cs, &g);
       if (e == hipSuccess && rc == AJ_OK) {
-        e = hipGraphInstantiate((hipGraphExec_t*)&p->graph_exec, g, nullptr, nullptr, 0);
-        p->graph = g;
+        e = hipGraphInstantiate((hipGraphExec_t*)&p->graph_exec[parity], g, nullptr, nullptr, 0);
+        p->graph[parity] = g;
       } else if (g) {
         (void)hipGraphDestroy(g);
       }
@@ -703,7 +714,7 @@ static int tb_outer_iteration(aj_mg_plan* p, hipStream_t st) {
     if (rc != AJ_OK) return rc;
     if (e != hipSuccess) return hip_fail(e, "capture of the multigrid outer iteration");
   }
-  AJ_HIP(hipGraphLaunch((hipGraphExec_t)p->graph_exec, st));
+  AJ_HIP(hipGraphLaunch((hipGraphExec_t)p->graph_exec[parity], st));
   return AJ_OK;
 }
 
@@ -733,8 +744,8 @@ static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int ma
                        (const T*)p->bcopy, p->tB[0], (const int*)p->state, nx, ny);
     AJ_LAUNCH_CHECK("k_tb_fallback");
   } else {
-    hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, p->p64[0], p->p64[1],
-                       (const int*)p->state, (const float*)nullptr, (const T*)p->bcopy, p->tB[0],
+    hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, (const double*)p->p64[0],
+                       p->p64[1], (const float*)nullptr, (const T*)p->bcopy, p->tB[0],
                        p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
     AJ_LAUNCH_CHECK("k_tb_resid");
     hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
@@ -746,18 +757,18 @@ static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int ma
     // fully stream-ordered: every iteration is enqueued, those after
     // convergence exit at their first instruction
     for (int k = 0; k < max_outer; ++k)
-      if (int rc = tb_outer_iteration<T>(p, st)) return rc;
+      if (int rc = tb_outer_iteration<T>(p, k & 1, st)) return rc;
   } else {
     // enqueue the number of iterations the previous solve needed, then read
     // the device state once and top up one iteration at a time
     int done = std::min(std::max(p->last_outer, 1), max_outer);
     for (int k = 0; k < done; ++k)
-      if (int rc = tb_outer_iteration<T>(p, st)) return rc;
+      if (int rc = tb_outer_iteration<T>(p, k & 1, st)) return rc;
     for (;;) {
       AJ_HIP(hipMemcpyAsync(p->host_stats, p->stats, sizeof(aj_mg_stats), hipMemcpyDeviceToHost, st));
       AJ_HIP(hipStreamSynchronize(st));
       if (p->host_stats->converged || p->host_stats->nonfinite || done >= max_outer) break;
-      if (int rc = tb_outer_iteration<T>(p, st)) return rc;
+      if (int rc = tb_outer_iteration<T>(p, done & 1, st)) return rc;
       ++done;
     }
     p->last_outer = std::max(p->host_stats->outer, 1);

@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--nn-pairs", type=int, default=2000)
     ap.add_argument("--nn-epochs", type=int, default=3)
     ap.add_argument("--nn-r", type=int, default=2, help="network runs on the grid restricted r times")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MiB device buffer 5 times (reference traffic for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -89,6 +92,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 5 x 256 MiB")
+        return
 
     import aerojax_amd as aj
     from aerojax_amd import _lib, workloads
@@ -141,8 +153,15 @@ def main():
     torch.cuda.synchronize()
     _lib.profile_end()
     prof = _lib.profile_read()
-    hbm = {k: v for k, v in prof.items() if not k.startswith("nn_")}
+    # roofline kernel: the single HBM kernel with the largest total time
+    # ("mg_coarse" times a group of three small-level kernels, "nn_*" are MFMA)
+    hbm = {k: v for k, v in prof.items() if not k.startswith("nn_") and k != "mg_coarse"}
     dom = max(hbm, key=lambda k: hbm[k]["ms"])
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_2048x512.json")
+    if os.path.exists(tfile) and (nx, ny) == (2048, 512):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("kernels", {}).get(dom, {}).get("hbm_bytes_per_launch")
     d = hbm[dom]
     log("kernel profile: " + ", ".join(f"{k} {v['avg_us']:.1f}us x{v['count']}"
                                          for k, v in prof.items()))
@@ -198,7 +217,7 @@ def main():
                    "plain_mg_steps_per_s": sps_plain, "plain_vcycles_per_step": cyc_plain,
                    "nn_over_plain": (sps_nn / sps_plain) if sps_nn else None, "nn": nn_info},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gbps"], "peak": 8000.0,
-                     "unit": "GB/s", "frac": d["gbps"] / 8000.0, "traffic": None,
+                     "unit": "GB/s", "frac": d["gbps"] / 8000.0, "traffic": traffic,
                      "avg_launch_us": d["avg_us"], "bytes_per_launch": d["bytes"] / d["count"]},
         "mfma": mfma,
         "kernels": {k: {"avg_us": round(v["avg_us"], 2), "launches_per_step": v["count"] / nprof,
