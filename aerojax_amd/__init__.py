@@ -5,20 +5,25 @@ Public API (reference signatures):
   rk_step_unified        advection_schemes/rk3_simple_new.py:175-180
   poisson_multigrid      pressure_solvers/multigrid_solver.py:55-56
   step_pure              solver/pure_functions/step.py:49-66
+  step_collocated        solver/simulation/step_handlers.py:62-185 (GUI step)
+  get_re_parameters      solver/params.py:308-413
 plus ``poisson_multigrid_to_tolerance`` (textbook multigrid, FP64 defect
 correction) and ``CollocatedStepper`` (preallocated repeated stepping).
 All arithmetic runs in hand-written HIP kernels in ``libaerojax_hip.so``.
 """
 from ._lib import AeroJaxError, AeroJaxLibraryError, load as load_library, version
 from .advection_schemes import rk_step_unified
-from .operators import constant_smagorinsky, divergence_nonperiodic, pressure_rhs, project
+from .operators import (constant_smagorinsky, divergence_nonperiodic, pressure_rhs, project,
+                        project_collocated, smagorinsky_total)
+from .params import get_re_parameters
 from .pressure_solvers import (MultigridPlan, poisson_multigrid,
                                poisson_multigrid_to_tolerance)
-from .step import CollocatedStepper, SimState, initial_state, step_pure
+from .step import CollocatedStepper, SimState, initial_state, step_collocated, step_pure
 
 __all__ = [
     "AeroJaxError", "AeroJaxLibraryError", "load_library", "version", "rk_step_unified",
     "constant_smagorinsky", "divergence_nonperiodic", "pressure_rhs", "project",
+    "project_collocated", "smagorinsky_total", "get_re_parameters",
     "MultigridPlan", "poisson_multigrid", "poisson_multigrid_to_tolerance",
-    "CollocatedStepper", "SimState", "initial_state", "step_pure",
+    "CollocatedStepper", "SimState", "initial_state", "step_collocated", "step_pure",
 ]

@@ -31,8 +31,9 @@ AJ_MG_OPT_GUESS_TEST = 2
 
 # Every symbol the header declares; tests check the library exports them.
 EXPORTS = (
-    "aj_version", "aj_last_error", "aj_rk_step", "aj_les_smagorinsky", "aj_divergence_rhs",
-    "aj_project", "aj_mg_create", "aj_mg_destroy", "aj_mg_workspace_bytes",
+    "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
+    "aj_les_smagorinsky_total", "aj_divergence_rhs", "aj_project", "aj_project_collocated",
+    "aj_mg_create", "aj_mg_destroy", "aj_mg_workspace_bytes",
     "aj_mg_num_levels", "aj_mg_set_option", "aj_poisson_mg", "aj_mg_get_stats",
     "aj_profile_begin", "aj_profile_end", "aj_profile_read",
 )
@@ -92,9 +93,14 @@ _SIGS = {
     "aj_last_error": (ctypes.c_char_p, []),
     "aj_rk_step": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
                         ctypes.POINTER(RkParams), _vp]),
+    "aj_rk_step_nu_field": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
+                                 ctypes.POINTER(RkParams), _vp]),
     "aj_les_smagorinsky": (_i, [_i, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
+    "aj_les_smagorinsky_total": (_i, [_i, _vp, _vp, _d, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
     "aj_divergence_rhs": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _vp]),
     "aj_project": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
+    "aj_project_collocated": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d,
+                                   _vp]),
     "aj_mg_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _d, _d, _i]),
     "aj_mg_destroy": (_i, [_vp]),
     "aj_mg_workspace_bytes": (ctypes.c_size_t, [_vp]),

@@ -24,7 +24,11 @@ def rk_step_unified(u, v, dt: float, nu: float, dx: float, dy: float, mask, sdf=
     penalisation, hard mask and von Karman BCs.
 
     ``u, v, mask, sdf, nu_sgs``: (nx, ny) torch CUDA tensors (float32 or
-    float64) or NumPy arrays (then NumPy is returned).  ``out``: optional
+    float64) or NumPy arrays (then NumPy is returned).  ``nu`` is a scalar,
+    or a field as the GUI step passes it (nu + nu_sgs,
+    solver/simulation/step_handlers.py:76-90): the Laplacian then uses
+    nu + nu_sgs per cell and the biharmonic term nu * nu_hyper_ratio * dx dy
+    per cell, as the reference's array arithmetic does.  ``out``: optional
     pair of preallocated output tensors.
     """
     if flow_type != "von_karman":
@@ -39,11 +43,15 @@ def rk_step_unified(u, v, dt: float, nu: float, dx: float, dy: float, mask, sdf=
     st = A.as_device(sdf, dt_, dev, "sdf")
     nt = A.as_device(nu_sgs, dt_, dev, "nu_sgs") if isinstance(nu_sgs, (torch.Tensor,)) or (
         nu_sgs is not None and not isinstance(nu_sgs, (int, float))) else None
-    nu_total = float(nu)
+    field = not isinstance(nu, (int, float))
+    nf = A.as_device(nu, dt_, dev, "nu") if field else None
+    nu_total = 0.0 if field else float(nu)
     if nu_sgs is not None and isinstance(nu_sgs, (int, float)):
+        if field:
+            raise ValueError("a scalar nu_sgs with a field nu is not supported")
         nu_total = float(nu) + float(nu_sgs)
     nx, ny = ut.shape
-    for name, t in (("v", vt), ("mask", mt), ("sdf", st), ("nu_sgs", nt)):
+    for name, t in (("v", vt), ("mask", mt), ("sdf", st), ("nu_sgs", nt), ("nu", nf)):
         if t is not None and tuple(t.shape) != (nx, ny):
             raise ValueError(f"{name} shape {tuple(t.shape)} != u shape {(nx, ny)}")
     if out is None:
@@ -56,8 +64,13 @@ def rk_step_unified(u, v, dt: float, nu: float, dx: float, dy: float, mask, sdf=
                         int(bool(fast_mode)))
     lib = _lib.load()
     with torch.cuda.device(dev):
-        rc = lib.aj_rk_step(A.dtype_code(ut), A.ptr(ut), A.ptr(vt), A.ptr(mt), A.ptr(st),
-                            A.ptr(nt), A.ptr(uo), A.ptr(vo), nx, ny, ctypes.byref(prm),
-                            A.stream_handle(dev))
+        if field:
+            rc = lib.aj_rk_step_nu_field(A.dtype_code(ut), A.ptr(ut), A.ptr(vt), A.ptr(mt),
+                                         A.ptr(st), A.ptr(nf), A.ptr(nt), A.ptr(uo), A.ptr(vo),
+                                         nx, ny, ctypes.byref(prm), A.stream_handle(dev))
+        else:
+            rc = lib.aj_rk_step(A.dtype_code(ut), A.ptr(ut), A.ptr(vt), A.ptr(mt), A.ptr(st),
+                                A.ptr(nt), A.ptr(uo), A.ptr(vo), nx, ny, ctypes.byref(prm),
+                                A.stream_handle(dev))
     _lib.check(rc, "aj_rk_step")
     return A.out_like(uo, want_np), A.out_like(vo, want_np)

@@ -25,10 +25,12 @@ struct RkArgs {
   const T* mask;
   const T* sdf;     // may be null -> chi = 1 - mask
   const T* nu_sgs;  // may be null
+  const T* nu_field;  // may be null: per-cell viscosity replacing nu
   T* u_out;
   T* v_out;
   int nx, ny;
-  T dt, nu, inv_dx, inv_dy, inv_dx2, inv_dy2, U_inf, inv_eta, sdf_scale, nu_h;
+  T dt, nu, inv_dx, inv_dy, inv_dx2, inv_dy2, U_inf, inv_eta, sdf_scale;
+  T hyper;  // nu_hyper_ratio * dx * dy (the biharmonic coefficient is nu * hyper)
   int slip;
 };
 
@@ -37,7 +39,7 @@ struct RkArgs {
 template <typename T, bool HYPER>
 __device__ __forceinline__ void rk_rhs(const T* fu, const T* fv, const T* lu, const T* lv,
                                        int l, int lW, int lE, int lS, int lN, T nu_tot,
-                                       const RkArgs<T>& a, T& ru, T& rv) {
+                                       T nu_h, const RkArgs<T>& a, T& ru, T& rv) {
   const T uc = fu[l], vc = fv[l];
   const T uW = fu[lW], uE = fu[lE], uS = fu[lS], uN = fu[lN];
   const T vW = fv[lW], vE = fv[lE], vS = fv[lS], vN = fv[lN];
@@ -56,8 +58,8 @@ __device__ __forceinline__ void rk_rhs(const T* fu, const T* fv, const T* lu, co
                  (lu[lN] + lu[lS] - T(2) * lu[l]) * a.inv_dy2;
     const T bv = (lv[lE] + lv[lW] - T(2) * lv[l]) * a.inv_dx2 +
                  (lv[lN] + lv[lS] - T(2) * lv[l]) * a.inv_dy2;
-    ru = ru - a.nu_h * bu;
-    rv = rv - a.nu_h * bv;
+    ru = ru - nu_h * bu;
+    rv = rv - nu_h * bv;
   }
 }
 
@@ -89,6 +91,7 @@ __global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
   T* s_bv = s_bu + RN;      // RK3: stage-2 output; RK2: k1
   T* s_lu = s_bv + RN;      // HYPER only: laplacians of the stage input
   T* s_lv = s_lu + RN;
+  T* s_nh = s_lv + RN;      // HYPER only: biharmonic coefficient nu * hyper
 
   const int nx = a.nx, ny = a.ny;
   const int y0 = blockIdx.x * TY;
@@ -96,6 +99,7 @@ __global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
   const int gx0 = x0 - H, gy0 = y0 - H;  // global index of LDS (0, 0)
   const int tid = threadIdx.x;
   const bool has_nsgs = a.nu_sgs != nullptr;
+  const bool has_nuf = a.nu_field != nullptr;
   const bool has_sdf = a.sdf != nullptr;
 
   // mask of this thread's tile cells (final stage), loaded up front
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
   // domain hold edge copies and are never read by the stencils).
   {
     constexpr int NIT = (RN + NTH - 1) / NTH;
-    T ru[NIT], rv[NIT], rc[NIT], rn[NIT];
+    T ru[NIT], rv[NIT], rc[NIT], rn[NIT], rf[NIT];
     const T* csrc = has_sdf ? a.sdf : a.mask;
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -124,6 +128,7 @@ __global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
       rv[it] = a.v[g];
       rc[it] = csrc[g];
       rn[it] = has_nsgs ? a.nu_sgs[g] : T(0);
+      rf[it] = has_nuf ? a.nu_field[g] : a.nu;
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -134,7 +139,10 @@ __global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
       // sigmoid(-sdf / (0.1 dx)) (rk3_simple_new.py:198), else 1 - mask
       const T chi = has_sdf ? T(1) / (T(1) + exp(rc[it] * a.sdf_scale)) : T(1) - rc[it];
       s_iD[idx] = T(1) / (T(1) + a.dt * chi * a.inv_eta);
-      s_nt[idx] = has_nsgs ? a.nu + rn[it] : a.nu;
+      // nu_total = nu (+ nu_sgs) with nu a scalar or a field
+      // (rk3_simple_new.py:304-309)
+      s_nt[idx] = has_nsgs ? rf[it] + rn[it] : rf[it];
+      if constexpr (HYPER) s_nh[idx] = rf[it] * a.hyper;
     }
   }
   __syncthreads();
@@ -169,7 +177,8 @@ __global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
       const int lW = gx > 0 ? l - RW : l, lE = gx < nx - 1 ? l + RW : l;
       const int lS = gy > 0 ? l - 1 : l, lN = gy < ny - 1 ? l + 1 : l;
       T ku, kv;
-      rk_rhs<T, HYPER>(fu, fv, s_lu, s_lv, l, lW, lE, lS, lN, s_nt[l], a, ku, kv);
+      rk_rhs<T, HYPER>(fu, fv, s_lu, s_lv, l, lW, lE, lS, lN, s_nt[l], HYPER ? s_nh[l] : T(0), a,
+                       ku, kv);
       const T u0 = s_u[l], v0 = s_v[l], iD = s_iD[l];
       if constexpr (FAST) {
         if (s == 1) {
@@ -268,9 +277,18 @@ static int launch_rk(const RkArgs<T>& a, hipStream_t st) {
   constexpr int R1 = HYPER ? 2 : 1;
   constexpr int H = S * R1;
   constexpr size_t RN = (size_t)(TX + 2 * H) * (TY + 2 * H);
-  constexpr size_t NARR = HYPER ? 10 : 8;
+  constexpr size_t NARR = HYPER ? 11 : 8;
   const size_t smem = RN * NARR * sizeof(T);
-  static_assert(RN * NARR * sizeof(T) <= 65536, "LDS budget");
+  static_assert(RN * NARR * sizeof(T) <= 160 * 1024, "LDS budget");
+  if (smem > 65536) {
+    // dynamic LDS beyond 64 KiB has to be enabled per kernel (gfx950: 160 KiB)
+    static bool raised = false;
+    if (!raised) {
+      AJ_HIP(hipFuncSetAttribute((const void*)k_rk_fused<T, TX, TY, FAST, HYPER, 512>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+      raised = true;
+    }
+  }
   dim3 grid(cdiv(a.ny, TY), cdiv(a.nx, TX));
   constexpr int NTH = 512;
   // algorithmic bytes: u, v, chi source, mask (+ nu_sgs) read, u*, v* written
@@ -283,11 +301,11 @@ static int launch_rk(const RkArgs<T>& a, hipStream_t st) {
 
 template <typename T>
 static int rk_step_t(const void* u, const void* v, const void* mask, const void* sdf,
-                     const void* nu_sgs, void* u_out, void* v_out, int nx, int ny,
-                     const aj_rk_params* p, hipStream_t st) {
+                     const void* nu_field, const void* nu_sgs, void* u_out, void* v_out, int nx,
+                     int ny, const aj_rk_params* p, hipStream_t st) {
   RkArgs<T> a;
   a.u = (const T*)u; a.v = (const T*)v; a.mask = (const T*)mask;
-  a.sdf = (const T*)sdf; a.nu_sgs = (const T*)nu_sgs;
+  a.sdf = (const T*)sdf; a.nu_sgs = (const T*)nu_sgs; a.nu_field = (const T*)nu_field;
   a.u_out = (T*)u_out; a.v_out = (T*)v_out;
   a.nx = nx; a.ny = ny;
   a.dt = T(p->dt); a.nu = T(p->nu);
@@ -295,7 +313,7 @@ static int rk_step_t(const void* u, const void* v, const void* mask, const void*
   a.inv_dx2 = T(1.0 / (p->dx * p->dx)); a.inv_dy2 = T(1.0 / (p->dy * p->dy));
   a.U_inf = T(p->U_inf); a.inv_eta = T(1.0 / p->brinkman_eta);
   a.sdf_scale = T(1.0 / (0.1 * p->dx));
-  a.nu_h = T(p->nu * p->nu_hyper_ratio * (p->dx * p->dy));
+  a.hyper = T(p->nu_hyper_ratio * (p->dx * p->dy));
   a.slip = p->slip_walls ? 1 : 0;
   const bool hyper = p->nu_hyper_ratio > 0.0;
   const bool fast = p->fast_mode != 0;
@@ -318,8 +336,8 @@ static int rk_step_t(const void* u, const void* v, const void* mask, const void*
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_les(const T* __restrict__ u, const T* __restrict__ v,
-                                             T* __restrict__ nsgs, int nx, int ny, T inv_2dx,
-                                             T inv_2dy, T coef) {
+                                             T* __restrict__ nsgs, T* __restrict__ ntot, T nu,
+                                             int nx, int ny, T inv_2dx, T inv_2dy, T coef) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= ny) return;
@@ -333,7 +351,9 @@ __global__ __launch_bounds__(256) void k_les(const T* __restrict__ u, const T* _
   const T dv_dy = (v[c + jp] - v[c + jm]) * inv_2dy;
   const T sxy = T(0.5) * (du_dy + dv_dx);
   const T smag = sqrt(T(2) * (du_dx * du_dx + dv_dy * dv_dy + T(2) * sxy * sxy) + T(1e-10));
-  nsgs[c + j] = coef * smag;
+  const T s = coef * smag;
+  nsgs[c + j] = s;
+  if (ntot) ntot[c + j] = nu + s;  // nu_total = nu + nu_sgs (step_handlers.py:76)
 }
 
 // --------------------------------------------------------------------------
@@ -422,42 +442,108 @@ static int check_grid(int nx, int ny) {
   return AJ_OK;
 }
 
+// Projection tail of the GUI step (_step_collocated,
+// solver/simulation/step_handlers.py:131-176): grad_*_nonperiodic of p,
+// dp/dx[0, :] = 0 (the corner taper of solver/pure_functions/
+// corner_smoothing.py:39-61 then scales zeros), u = u* - dt grad p; inlet
+// u[0, 1:-1] = U_in, v[0, :] = 0; outlet row nx-1 copies row nx-2; no-slip
+// walls u = v = 0 on j = 0, ny-1 (corners included); then the hard mask.
+// Every cell computes its own value (row nx-1 recomputes row nx-2's
+// correction, whose gradients are central).
+template <typename T>
+__global__ __launch_bounds__(256) void k_project_gui(const T* __restrict__ us,
+                                                     const T* __restrict__ vs,
+                                                     const T* __restrict__ p,
+                                                     const T* __restrict__ mask,
+                                                     T* __restrict__ uo, T* __restrict__ vo,
+                                                     int nx, int ny, T dt, T inv_2dx, T inv_2dy,
+                                                     T U_in) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= ny) return;
+  const long g = (long)i * ny + j;
+  T un = T(0), vn = T(0);
+  if (j > 0 && j < ny - 1) {
+    if (i == 0) {
+      un = U_in;
+    } else {
+      const long s = (long)(i == nx - 1 ? nx - 2 : i) * ny + j;
+      const T gx = (p[s + ny] - p[s - ny]) * inv_2dx;
+      const T gy = (p[s + 1] - p[s - 1]) * inv_2dy;
+      un = us[s] - dt * gx;
+      vn = vs[s] - dt * gy;
+    }
+  }
+  const T m0 = mask[g];
+  uo[g] = masked(un, m0);
+  vo[g] = masked(vn, m0);
+}
+
 }  // namespace aj
 
 using namespace aj;
 
-extern "C" int aj_rk_step(int dtype, const void* u, const void* v, const void* mask,
-                          const void* sdf, const void* nu_sgs, void* u_out, void* v_out,
-                          int nx, int ny, const aj_rk_params* prm, void* stream) {
+static int rk_entry(int dtype, const void* u, const void* v, const void* mask, const void* sdf,
+                    const void* nu_field, const void* nu_sgs, void* u_out, void* v_out, int nx,
+                    int ny, const aj_rk_params* prm, void* stream) {
   if (!u || !v || !mask || !u_out || !v_out || !prm) return fail(AJ_ERR_INVALID, "aj_rk_step: null pointer");
   if (int rc = check_grid(nx, ny)) return rc;
   if (!(prm->brinkman_eta > 0)) return fail(AJ_ERR_INVALID, "aj_rk_step: brinkman_eta must be > 0");
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == AJ_F32) return rk_step_t<float>(u, v, mask, sdf, nu_sgs, u_out, v_out, nx, ny, prm, st);
-  if (dtype == AJ_F64) return rk_step_t<double>(u, v, mask, sdf, nu_sgs, u_out, v_out, nx, ny, prm, st);
+  if (dtype == AJ_F32)
+    return rk_step_t<float>(u, v, mask, sdf, nu_field, nu_sgs, u_out, v_out, nx, ny, prm, st);
+  if (dtype == AJ_F64)
+    return rk_step_t<double>(u, v, mask, sdf, nu_field, nu_sgs, u_out, v_out, nx, ny, prm, st);
   return fail(AJ_ERR_INVALID, "aj_rk_step: bad dtype");
 }
 
-extern "C" int aj_les_smagorinsky(int dtype, const void* u, const void* v, void* nu_sgs, int nx,
-                                  int ny, double dx, double dy, double delta, double c_s,
-                                  void* stream) {
+extern "C" int aj_rk_step(int dtype, const void* u, const void* v, const void* mask,
+                          const void* sdf, const void* nu_sgs, void* u_out, void* v_out,
+                          int nx, int ny, const aj_rk_params* prm, void* stream) {
+  return rk_entry(dtype, u, v, mask, sdf, nullptr, nu_sgs, u_out, v_out, nx, ny, prm, stream);
+}
+
+extern "C" int aj_rk_step_nu_field(int dtype, const void* u, const void* v, const void* mask,
+                                   const void* sdf, const void* nu_field, const void* nu_sgs,
+                                   void* u_out, void* v_out, int nx, int ny,
+                                   const aj_rk_params* prm, void* stream) {
+  if (!nu_field) return fail(AJ_ERR_INVALID, "aj_rk_step_nu_field: null nu_field");
+  return rk_entry(dtype, u, v, mask, sdf, nu_field, nu_sgs, u_out, v_out, nx, ny, prm, stream);
+}
+
+static int les_entry(int dtype, const void* u, const void* v, double nu, void* nu_sgs,
+                     void* nu_total, int nx, int ny, double dx, double dy, double delta,
+                     double c_s, void* stream) {
   if (!u || !v || !nu_sgs) return fail(AJ_ERR_INVALID, "aj_les_smagorinsky: null pointer");
   if (int rc = check_grid(nx, ny)) return rc;
   hipStream_t st = (hipStream_t)stream;
   const double coef = (c_s * delta) * (c_s * delta);
   if (dtype == AJ_F32) {
     hipLaunchKernelGGL(k_les<float>, row_grid(nx, ny), dim3(256), 0, st, (const float*)u,
-                       (const float*)v, (float*)nu_sgs, nx, ny, float(1.0 / (2.0 * dx)),
-                       float(1.0 / (2.0 * dy)), float(coef));
+                       (const float*)v, (float*)nu_sgs, (float*)nu_total, float(nu), nx, ny,
+                       float(1.0 / (2.0 * dx)), float(1.0 / (2.0 * dy)), float(coef));
   } else if (dtype == AJ_F64) {
     hipLaunchKernelGGL(k_les<double>, row_grid(nx, ny), dim3(256), 0, st, (const double*)u,
-                       (const double*)v, (double*)nu_sgs, nx, ny, 1.0 / (2.0 * dx),
-                       1.0 / (2.0 * dy), coef);
+                       (const double*)v, (double*)nu_sgs, (double*)nu_total, nu, nx, ny,
+                       1.0 / (2.0 * dx), 1.0 / (2.0 * dy), coef);
   } else {
     return fail(AJ_ERR_INVALID, "aj_les_smagorinsky: bad dtype");
   }
   AJ_LAUNCH_CHECK("k_les");
   return AJ_OK;
+}
+
+extern "C" int aj_les_smagorinsky(int dtype, const void* u, const void* v, void* nu_sgs, int nx,
+                                  int ny, double dx, double dy, double delta, double c_s,
+                                  void* stream) {
+  return les_entry(dtype, u, v, 0.0, nu_sgs, nullptr, nx, ny, dx, dy, delta, c_s, stream);
+}
+
+extern "C" int aj_les_smagorinsky_total(int dtype, const void* u, const void* v, double nu,
+                                        void* nu_sgs, void* nu_total, int nx, int ny, double dx,
+                                        double dy, double delta, double c_s, void* stream) {
+  if (!nu_total) return fail(AJ_ERR_INVALID, "aj_les_smagorinsky_total: null nu_total");
+  return les_entry(dtype, u, v, nu, nu_sgs, nu_total, nx, ny, dx, dy, delta, c_s, stream);
 }
 
 extern "C" int aj_divergence_rhs(int dtype, const void* us, const void* vs, const void* mask,
@@ -505,5 +591,30 @@ extern "C" int aj_project(int dtype, const void* us, const void* vs, const void*
     return fail(AJ_ERR_INVALID, "aj_project: bad dtype");
   }
   AJ_LAUNCH_CHECK("k_project");
+  return AJ_OK;
+}
+
+extern "C" int aj_project_collocated(int dtype, const void* us, const void* vs, const void* p,
+                                     const void* mask, void* uo, void* vo, int nx, int ny,
+                                     double dt, double dx, double dy, double U_in, void* stream) {
+  if (!us || !vs || !p || !mask || !uo || !vo)
+    return fail(AJ_ERR_INVALID, "aj_project_collocated: null pointer");
+  if (int rc = check_grid(nx, ny)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  ProfScope prof(AJ_PROF_PROJECT, (double)nx * ny * (dtype == AJ_F64 ? 8 : 4) * 6, st);
+  if (dtype == AJ_F32) {
+    hipLaunchKernelGGL(k_project_gui<float>, row_grid(nx, ny), dim3(256), 0, st,
+                       (const float*)us, (const float*)vs, (const float*)p, (const float*)mask,
+                       (float*)uo, (float*)vo, nx, ny, float(dt), float(1.0 / (2.0 * dx)),
+                       float(1.0 / (2.0 * dy)), float(U_in));
+  } else if (dtype == AJ_F64) {
+    hipLaunchKernelGGL(k_project_gui<double>, row_grid(nx, ny), dim3(256), 0, st,
+                       (const double*)us, (const double*)vs, (const double*)p,
+                       (const double*)mask, (double*)uo, (double*)vo, nx, ny, dt,
+                       1.0 / (2.0 * dx), 1.0 / (2.0 * dy), U_in);
+  } else {
+    return fail(AJ_ERR_INVALID, "aj_project_collocated: bad dtype");
+  }
+  AJ_LAUNCH_CHECK("k_project_gui");
   return AJ_OK;
 }

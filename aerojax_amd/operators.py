@@ -38,7 +38,8 @@ def divergence_nonperiodic(u, v, dx: float, dy: float, *, out=None):
 
 
 def pressure_rhs(u_star, v_star, mask, dt: float, dx: float, dy: float, *, out=None):
-    """rhs = divergence_nonperiodic(u*, v*) * mask / dt (step.py:194-199)."""
+    """rhs = divergence_nonperiodic(u*, v*) * mask / dt (step.py:194-199);
+    ``mask=None`` gives div / dt (step_handlers.py:92-98)."""
     want_np, ut, (vt, mt) = _prep(u_star, v_star, mask, names=("u_star", "v_star", "mask"))
     nx, ny = ut.shape
     o = torch.empty_like(ut) if out is None else out
@@ -81,4 +82,44 @@ def project(u_star, v_star, p, mask, dt: float, dx: float, dy: float, U_inf: flo
                                     A.ptr(uo), A.ptr(vo), nx, ny, float(dt), float(dx), float(dy),
                                     float(U_inf), A.stream_handle(ut.device))
     _lib.check(rc, "aj_project")
+    return A.out_like(uo, want_np), A.out_like(vo, want_np)
+
+
+def smagorinsky_total(u, v, nu: float, dx: float, dy: float, delta: float, C_s: float = 0.1, *,
+                      out=None):
+    """``(nu_sgs, nu + nu_sgs)`` in one launch: the LES block of the GUI step
+    (solver/simulation/step_handlers.py:67-77, constant model)."""
+    want_np, ut, (vt,) = _prep(u, v, names=("u", "v"))
+    nx, ny = ut.shape
+    if out is None:
+        s, t = torch.empty_like(ut), torch.empty_like(ut)
+    else:
+        s, t = out
+    with torch.cuda.device(ut.device):
+        rc = _lib.load().aj_les_smagorinsky_total(A.dtype_code(ut), A.ptr(ut), A.ptr(vt),
+                                                  float(nu), A.ptr(s), A.ptr(t), nx, ny,
+                                                  float(dx), float(dy), float(delta), float(C_s),
+                                                  A.stream_handle(ut.device))
+    _lib.check(rc, "aj_les_smagorinsky_total")
+    return A.out_like(s, want_np), A.out_like(t, want_np)
+
+
+def project_collocated(u_star, v_star, p, mask, dt: float, dx: float, dy: float, U_in: float, *,
+                       out=None):
+    """Pressure correction and BCs of the GUI step (step_handlers.py:131-176):
+    no interface projection, dp/dx = 0 on the inlet row, inlet velocity
+    ``U_in`` on rows 1..ny-2 of the inlet, no-slip walls, hard mask."""
+    want_np, ut, (vt, pt, mt) = _prep(u_star, v_star, p, mask,
+                                      names=("u_star", "v_star", "p", "mask"))
+    nx, ny = ut.shape
+    if out is None:
+        uo, vo = torch.empty_like(ut), torch.empty_like(ut)
+    else:
+        uo, vo = out
+    with torch.cuda.device(ut.device):
+        rc = _lib.load().aj_project_collocated(A.dtype_code(ut), A.ptr(ut), A.ptr(vt), A.ptr(pt),
+                                               A.ptr(mt), A.ptr(uo), A.ptr(vo), nx, ny, float(dt),
+                                               float(dx), float(dy), float(U_in),
+                                               A.stream_handle(ut.device))
+    _lib.check(rc, "aj_project_collocated")
     return A.out_like(uo, want_np), A.out_like(vo, want_np)

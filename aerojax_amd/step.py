@@ -22,7 +22,8 @@ import torch
 
 from . import _arrays as A
 from .advection_schemes import rk_step_unified
-from .operators import constant_smagorinsky, pressure_rhs, project
+from .operators import (constant_smagorinsky, pressure_rhs, project, project_collocated,
+                        smagorinsky_total)
 from .pressure_solvers import get_plan
 
 
@@ -106,6 +107,73 @@ def step_pure(state: SimState, mask, sdf, dx: float, dy: float, nu: float, U_inf
     return SimState(u=un, v=vn, p=p, u_prev=u, v_prev=v, c=state.c, dt=dt,
                     iteration=state.iteration + 1, integral=state.integral,
                     prev_error=state.prev_error)
+
+
+def inlet_velocity(U_inf: float, iteration: int, startup_ramp_steps: int) -> float:
+    """Inlet speed with the cubic startup ramp (step_handlers.py:145-150);
+    the reference GUI sets startup_ramp_steps = 0 (initializers.py:29)."""
+    if iteration >= startup_ramp_steps:
+        return float(U_inf)
+    t = iteration / startup_ramp_steps
+    return float(U_inf) * (3.0 * t * t - 2.0 * t ** 3)
+
+
+def step_collocated(u, v, mask, dt: float, *, sdf, dx: float, dy: float, nu: float,
+                    U_inf: float, use_les: bool = False, les_model: str = "smagorinsky",
+                    smagorinsky_constant: float = 0.17, nu_hyper_ratio: float = 0.0,
+                    slip_walls: bool = True, fast_mode: bool = False,
+                    brinkman_eta: float = 0.01, v_cycles: int = 5, solver_dt: float = None,
+                    iteration: int = 0, startup_ramp_steps: int = 0,
+                    flow_type: str = "von_karman", pressure_solver: str = "multigrid",
+                    nn_pressure_model=None, levels: int = 4):
+    """One step of the GUI solver's ``_step_collocated``
+    (solver/simulation/step_handlers.py:62-185, von Karman branch); returns
+    ``(u, v, p)``.  The method's ``self.*`` attributes are keyword arguments
+    here: ``nu``, ``U_inf`` (self.flow), ``use_les``, ``les_model``,
+    ``smagorinsky_constant``, ``fast_mode``, ``brinkman_eta``, ``v_cycles``
+    (self.sim_params; multigrid_v_cycles defaults to 5, params.py:274),
+    ``nu_hyper_ratio`` (from ``get_re_parameters``), ``slip_walls`` (True,
+    base_solver.py:211), ``solver_dt`` (self.dt, used by the correction;
+    defaults to ``dt``), ``iteration`` / ``startup_ramp_steps`` (inlet ramp).
+
+    Differences from ``step_pure`` that this composition keeps: with LES the
+    RK step receives nu + nu_sgs as ``nu`` *and* nu_sgs (so the Laplacian
+    sees nu + 2 nu_sgs and the biharmonic coefficient scales with
+    nu + nu_sgs), no BC pass on u*, rhs = div/dt without the mask, no
+    interface projection, no-slip walls in the correction.
+    """
+    if flow_type != "von_karman":
+        raise NotImplementedError("only flow_type='von_karman' is implemented")
+    if use_les and les_model == "dynamic_smagorinsky":
+        raise NotImplementedError("les_model='dynamic_smagorinsky' is not implemented")
+    A.require_gpu()
+    want_np = A.is_numpy(u)
+    dt_ = A.torch_dtype_of(u)
+    ut = A.as_device(u, dt_, name="u")
+    vt = A.as_device(v, dt_, ut.device, "v")
+    mt = A.as_device(mask, dt_, ut.device, "mask")
+    st = A.as_device(sdf, dt_, ut.device, "sdf")
+    nu_arg, nu_sgs = nu, None
+    if use_les:
+        delta = (dx * dy) ** 0.5
+        nu_sgs, nu_arg = smagorinsky_total(ut, vt, nu, dx, dy, delta, smagorinsky_constant)
+    us, vs = rk_step_unified(ut, vt, dt, nu_arg, dx, dy, mt, sdf=st, U_inf=U_inf, nu_sgs=nu_sgs,
+                             nu_hyper_ratio=nu_hyper_ratio, slip_walls=slip_walls,
+                             fast_mode=fast_mode, brinkman_eta=brinkman_eta, flow_type=flow_type)
+    rhs = pressure_rhs(us, vs, None, dt, dx, dy)  # div / dt, no mask (step_handlers.py:92-98)
+    nx, ny = ut.shape
+    if pressure_solver == "nn" and nn_pressure_model is not None:
+        p = nn_pressure_model(rhs, mt)
+    elif pressure_solver == "multigrid":
+        p = torch.empty_like(rhs)
+        plan = get_plan(nx, ny, dx, dy, ut.dtype, "reference", levels, ut.device)
+        plan.solve(rhs, p, None, v_cycles, 1e-6)
+    else:
+        raise NotImplementedError(f"pressure_solver={pressure_solver!r}")
+    sdt = dt if solver_dt is None else solver_dt
+    un, vn = project_collocated(us, vs, p, mt, sdt, dx, dy,
+                                inlet_velocity(U_inf, iteration, startup_ramp_steps))
+    return A.out_like(un, want_np), A.out_like(vn, want_np), A.out_like(p, want_np)
 
 
 class CollocatedStepper:

@@ -70,6 +70,17 @@ int aj_rk_step(int dtype, const void* u, const void* v, const void* mask,
                const void* sdf, const void* nu_sgs, void* u_out, void* v_out,
                int nx, int ny, const aj_rk_params* prm, void* stream);
 
+/* Same step with the reference's `nu` argument given as a field: the GUI
+ * step passes nu_total = nu + nu_sgs as `nu` together with nu_sgs
+ * (solver/simulation/step_handlers.py:76-90), so the viscosity of the
+ * Laplacian is nu_field + nu_sgs and the biharmonic coefficient
+ * nu_field * nu_hyper_ratio * dx * dy (rk3_simple_new.py:304-309).
+ * prm->nu is ignored. */
+int aj_rk_step_nu_field(int dtype, const void* u, const void* v, const void* mask,
+                        const void* sdf, const void* nu_field, const void* nu_sgs,
+                        void* u_out, void* v_out, int nx, int ny, const aj_rk_params* prm,
+                        void* stream);
+
 /* ---- Smagorinsky eddy viscosity ----------------------------------------
  * Replaces constant_smagorinsky + compute_strain_rate
  * (solver/les_models.py:22-54, 179-200): nu_sgs = (c_s*delta)^2 *
@@ -78,6 +89,10 @@ int aj_rk_step(int dtype, const void* u, const void* v, const void* mask,
 int aj_les_smagorinsky(int dtype, const void* u, const void* v, void* nu_sgs,
                        int nx, int ny, double dx, double dy, double delta,
                        double c_s, void* stream);
+/* Same, also writing nu_total = nu + nu_sgs (step_handlers.py:67-77). */
+int aj_les_smagorinsky_total(int dtype, const void* u, const void* v, double nu,
+                             void* nu_sgs, void* nu_total, int nx, int ny, double dx,
+                             double dy, double delta, double c_s, void* stream);
 
 /* ---- Pressure right-hand side -----------------------------------------
  * Replaces divergence_nonperiodic (solver/operators.py:81-96) followed by
@@ -97,6 +112,19 @@ int aj_divergence_rhs(int dtype, const void* u_star, const void* v_star,
 int aj_project(int dtype, const void* u_star, const void* v_star, const void* p,
                const void* mask, void* u_out, void* v_out, int nx, int ny,
                double dt, double dx, double dy, double U_inf, void* stream);
+
+/* ---- Projection of the GUI step ----------------------------------------
+ * Replaces the pressure correction and BCs of _step_collocated
+ * (solver/simulation/step_handlers.py:131-176, von Karman branch):
+ * grad_x/y_nonperiodic of p, dp_dx[0,:] = 0 (+ the corner taper of
+ * solver/pure_functions/corner_smoothing.py:39-61, which then scales
+ * zeros), u = u* - dt grad p, inlet u[0,1:-1] = U_in (the caller applies the
+ * startup ramp), v[0,:] = 0, outlet copy of row nx-2, no-slip walls
+ * (corners included), hard mask.  No interface projection. */
+int aj_project_collocated(int dtype, const void* u_star, const void* v_star,
+                          const void* p, const void* mask, void* u_out, void* v_out,
+                          int nx, int ny, double dt, double dx, double dy, double U_in,
+                          void* stream);
 
 /* ---- Multigrid pressure Poisson ----------------------------------------
  * A plan owns the level hierarchy workspace (allocated once, hipMalloc)

@@ -264,7 +264,8 @@ def rk_step_unified(u, v, dt, nu, dx, dy, mask, sdf=None, U_inf=1.0, nu_sgs=None
     the end.  The sponge field is identically 1 (:282).
     """
     T = u.dtype.type
-    dt, nu, dx, dy = T(dt), T(nu), T(dx), T(dy)
+    dt, dx, dy = T(dt), T(dx), T(dy)
+    nu = nu if isinstance(nu, np.ndarray) else T(nu)  # a field in the GUI step
     U_inf, eta = T(U_inf), T(brinkman_eta)
     chi = brinkman_chi(sdf, mask, dx)
     D = 1 + dt * chi / eta
@@ -527,6 +528,80 @@ def step_pure(u, v, mask, sdf, dx, dy, nu, U_inf, dt, use_les=False,
     p = poisson_multigrid(rhs, mask, dx, dy, levels=levels, v_cycles=v_cycles,
                           tolerance=tolerance, p0=p0)
     un, vn = project(us, vs, p, mask, dt, dx, dy, U_inf)
+    if return_parts:
+        return un, vn, p, {"u_star": us, "v_star": vs, "rhs": rhs, "nu_sgs": nu_sgs}
+    return un, vn, p
+
+
+# ---------------------------------------------------------------------------
+# GUI step _step_collocated (solver/simulation/step_handlers.py:62-185),
+# von Karman branch
+# ---------------------------------------------------------------------------
+
+
+def corner_smooth_pressure_gradient(dp_dx, ny, width=10):
+    """apply_corner_smooth_pressure_gradient
+    (solver/pure_functions/corner_smoothing.py:39-61): row 0 entries j and
+    ny-1-j, j < width, scaled by sin^2(pi/2 * j/width)."""
+    T = dp_dx.dtype.type
+    dp_dx = dp_dx.copy()
+    j = np.arange(min(width, ny))  # narrower grids: in-range entries only
+    f = (np.sin(np.pi / 2 * (np.arange(width) / width)) ** 2)[j].astype(T)
+    dp_dx[0, j] = dp_dx[0, j] * f
+    top = ny - 1 - j
+    dp_dx[0, top] = dp_dx[0, top] * f
+    return dp_dx
+
+
+def project_collocated(u_star, v_star, p, mask, dt, dx, dy, U_in):
+    """Pressure correction and BCs of _step_collocated (step_handlers.py:
+    131-176): dp_dx[0, :] = 0 then the corner taper; u = u* - dt grad p;
+    inlet u[0, 1:-1] = U_in, v[0, :] = 0; outlet copies row nx-2; no-slip
+    walls (corners included); hard mask.  No interface projection."""
+    T = u_star.dtype.type
+    dt, dx, dy, U_in = T(dt), T(dx), T(dy), T(U_in)
+    ny = p.shape[1]
+    gx = grad_x_nonperiodic(p, dx)
+    gy = grad_y_nonperiodic(p, dy)
+    gx[0, :] = 0.0
+    gx = corner_smooth_pressure_gradient(gx, ny, 10)
+    un = u_star - dt * gx
+    vn = v_star - dt * gy
+    un[0, 1:-1] = U_in
+    vn[0, :] = 0.0
+    un[-1, :] = un[-2, :]
+    vn[-1, :] = vn[-2, :]
+    un[:, 0] = 0.0
+    un[:, -1] = 0.0
+    vn[:, 0] = 0.0
+    vn[:, -1] = 0.0
+    un = np.where(mask > 0.5, un, T(0))
+    vn = np.where(mask > 0.5, vn, T(0))
+    return un, vn
+
+
+def step_collocated(u, v, mask, sdf, dx, dy, nu, U_inf, dt, use_les=False,
+                    smagorinsky_constant=0.17, nu_hyper_ratio=0.0, slip_walls=True,
+                    fast_mode=False, brinkman_eta=0.01, v_cycles=5, solver_dt=None,
+                    U_in=None, levels=4, return_parts=False):
+    """_step_collocated (step_handlers.py:62-185), constant Smagorinsky,
+    multigrid pressure.  With LES the RK step gets nu + nu_sgs as ``nu`` and
+    nu_sgs as ``nu_sgs`` (:76-90); rhs = div / dt without the mask (:92-98).
+    Returns ``(u, v, p)``."""
+    T = u.dtype.type
+    nu_arg, nu_sgs = nu, None
+    if use_les:
+        delta = (dx * dy) ** 0.5
+        nu_sgs = constant_smagorinsky(u, v, dx, dy, delta, smagorinsky_constant)
+        nu_arg = T(nu) + nu_sgs
+    us, vs = rk_step_unified(u, v, dt, nu_arg, dx, dy, mask, sdf=sdf, U_inf=U_inf,
+                             nu_sgs=nu_sgs, nu_hyper_ratio=nu_hyper_ratio,
+                             slip_walls=slip_walls, fast_mode=fast_mode,
+                             brinkman_eta=brinkman_eta)
+    rhs = divergence_nonperiodic(us, vs, dx, dy) / T(dt)
+    p = poisson_multigrid(rhs, mask, dx, dy, levels=levels, v_cycles=v_cycles)
+    un, vn = project_collocated(us, vs, p, mask, dt if solver_dt is None else solver_dt, dx, dy,
+                                U_inf if U_in is None else U_in)
     if return_parts:
         return un, vn, p, {"u_star": us, "v_star": vs, "rhs": rhs, "nu_sgs": nu_sgs}
     return un, vn, p

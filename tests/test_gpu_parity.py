@@ -95,6 +95,53 @@ def test_rk_step_parity(gpu, shape, dtype, variant):
         assert np.all(g[1:-1, 1:-1][solid[1:-1, 1:-1]] == 0.0)
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("fast", [False, True])
+def test_rk_viscosity_field_parity(gpu, dtype, fast):
+    """``nu`` as a field with nu_sgs and hyper-viscosity (the GUI step's
+    call, step_handlers.py:76-90)."""
+    import aerojax_amd as aj
+    c = case(256, 64, 4.0, 1.0, kind="cylinder")
+    dx, dy, dt = c["dx"], c["dy"], 1e-3
+    nsgs = 50.0 * R.constant_smagorinsky(c["u"], c["v"], dx, dy, math.sqrt(dx * dy), 0.17)
+    nuf = 5e-4 + nsgs
+    kw = dict(U_inf=1.0, brinkman_eta=1e-4, nu_hyper_ratio=0.3, fast_mode=fast)
+    ref = R.rk_step_unified(c["u"], c["v"], dt, nuf, dx, dy, c["mask"], sdf=c["sdf"], nu_sgs=nsgs,
+                            **kw)
+    out = aj.rk_step_unified(dev(c["u"], dtype), dev(c["v"], dtype), dt, dev(nuf, dtype), dx, dy,
+                             dev(c["mask"], dtype), sdf=dev(c["sdf"], dtype),
+                             nu_sgs=dev(nsgs, dtype), **kw)
+    for k, name in enumerate(("u*", "v*")):
+        check(host(out[k]), ref[k], None, dtype, f"rk nu-field fast={fast} {dtype} {name}")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("Re,les,fast", [(800.0, False, False), (5e4, True, False),
+                                         (5e3, True, True)])
+def test_step_collocated_parity(gpu, dtype, Re, les, fast):
+    """The GUI step (_step_collocated) with the Re-dependent parameters."""
+    import aerojax_amd as aj
+    nx, ny = 256, 64
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder")
+    dx, dy, dt = c["dx"], c["dy"], 2e-4
+    prm = aj.get_re_parameters(Re, nx)
+    nu = 1.0 * 0.2 / Re
+    kw = dict(use_les=les, smagorinsky_constant=prm["C_s"], nu_hyper_ratio=prm["nu_hyper_ratio"],
+              fast_mode=fast, brinkman_eta=prm["brinkman_eta"], v_cycles=5)
+    ru, rv, rp = R.step_collocated(c["u"], c["v"], c["mask"], c["sdf"], dx, dy, nu, 1.0, dt,
+                                   **kw)
+    u, v, p = aj.step_collocated(dev(c["u"], dtype), dev(c["v"], dtype), dev(c["mask"], dtype),
+                                 dt, sdf=dev(c["sdf"], dtype), dx=dx, dy=dy, nu=nu, U_inf=1.0,
+                                 **kw)
+    torch.cuda.synchronize()
+    tag = f"step_collocated Re={Re:g} les={les} fast={fast} {dtype}"
+    check(host(p), rp, None, dtype, tag + " p")
+    check(host(u), ru, None, dtype, tag + " u")
+    check(host(v), rv, None, dtype, tag + " v")
+    solid = c["mask"] <= 0.5
+    assert np.all(host(u)[solid] == 0.0) and np.all(host(v)[solid] == 0.0)
+
+
 def test_rk_full_size_uniform_flow_fixed_point(gpu):
     import aerojax_amd as aj
     nx, ny = 2048, 512

@@ -387,3 +387,65 @@ def test_step_fp32_close_to_fp64():
     for a, b in zip(out32, out64):
         assert a.dtype == np.float32
         assert R.rel_l2(a, b) < 1e-5
+
+
+# ------------------------------------------------------------------ GUI step
+
+
+def test_project_collocated_hand_values():
+    """p = 10 i + j^2 on a 6x5 grid (dx = dy = 1): central dp/dx = 10,
+    dp/dy = 2j; u* = 3, v* = 1, dt = 0.5, U_in = 7; mask 0.2 at (2, 2)."""
+    nx, ny = 6, 5
+    i = np.arange(nx)[:, None]
+    j = np.arange(ny)[None, :]
+    p = (10.0 * i + j ** 2).astype(np.float64)
+    us = np.full((nx, ny), 3.0)
+    vs = np.full((nx, ny), 1.0)
+    mask = np.ones((nx, ny))
+    mask[2, 2] = 0.2
+    u, v = R.project_collocated(us, vs, p, mask, 0.5, 1.0, 1.0, 7.0)
+    ue = np.zeros((nx, ny))
+    ve = np.zeros((nx, ny))
+    for a in range(nx):
+        for b in range(1, ny - 1):
+            if a == 0:
+                ue[a, b], ve[a, b] = 7.0, 0.0
+            else:
+                ue[a, b], ve[a, b] = 3.0 - 0.5 * 10.0, 1.0 - 0.5 * 2.0 * b
+    ue[2, 2] = ve[2, 2] = 0.0
+    assert np.array_equal(u, ue) and np.array_equal(v, ve)
+
+
+def test_corner_taper_factors():
+    g = np.ones((3, 24))
+    t = R.corner_smooth_pressure_gradient(g, 24, 10)
+    assert t[0, 0] == 0.0 and t[0, 23] == 0.0
+    assert np.isclose(t[0, 5], 0.5) and np.isclose(t[0, 18], 0.5)   # sin^2(pi/4)
+    assert np.all(t[0, 10:14] == 1.0) and np.all(t[1:] == 1.0)
+
+
+def test_rk_field_nu_equals_scalar_when_uniform():
+    """A uniform viscosity field gives the scalar-nu step bit for bit."""
+    nx, ny = 40, 24
+    dx, dy, X, Y = R.grid_coords(nx, ny, 2.0, 1.0)
+    u, v = R.von_karman_ic(nx, ny, 1.0, 1e-2, seed=3, dtype=np.float64)
+    mask = np.ones((nx, ny))
+    a = R.rk_step_unified(u, v, 1e-3, 2e-3, dx, dy, mask, nu_hyper_ratio=0.2)
+    b = R.rk_step_unified(u, v, 1e-3, np.full((nx, ny), 2e-3), dx, dy, mask, nu_hyper_ratio=0.2)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_get_re_parameters_table():
+    from aerojax_amd.params import get_re_parameters as g
+    p = g(500.0, 512)
+    assert (p["C_s"], p["nu_hyper_ratio"], p["brinkman_eta"], p["dt_max"]) == (0.08, 0.0, 1e-4, 0.005)
+    assert g(1000.0, 512)["C_s"] == 0.08          # band bounds are inclusive
+    assert g(1000.1, 512)["nu_hyper_ratio"] == 0.01
+    p = g(5e4, 512)
+    assert (p["C_s"], p["nu_hyper_ratio"], p["brinkman_eta"]) == (0.12, 0.03, 1e-6)
+    p = g(5e5, 512)
+    assert (p["advection_scheme"], p["nu_h"], p["brinkman_eta"]) == ("rk3_high", 1e-5, 1e-7)
+    assert g(2e7, 512)["C_s"] == 0.20
+    p = g(5000.0, 256)                           # coarse-grid adjustment
+    assert np.isclose(p["C_s"], 0.12) and np.isclose(p["nu_hyper_ratio"], 0.015)
+    assert np.isclose(p["brinkman_eta"], 1.2e-4) and np.isclose(p["cfl_target"], 0.2)
