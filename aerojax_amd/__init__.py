@@ -15,6 +15,7 @@ from ._lib import AeroJaxError, AeroJaxLibraryError, load as load_library, versi
 from .advection_schemes import rk_step_unified
 from .operators import (constant_smagorinsky, divergence_nonperiodic, pressure_rhs, project,
                         project_collocated, smagorinsky_total)
+from .diagnostics import flow_diagnostics
 from .params import get_re_parameters
 from .pressure_solvers import (MultigridPlan, poisson_multigrid,
                                poisson_multigrid_to_tolerance)
@@ -23,7 +24,7 @@ from .step import CollocatedStepper, SimState, initial_state, step_collocated, s
 __all__ = [
     "AeroJaxError", "AeroJaxLibraryError", "load_library", "version", "rk_step_unified",
     "constant_smagorinsky", "divergence_nonperiodic", "pressure_rhs", "project",
-    "project_collocated", "smagorinsky_total", "get_re_parameters",
+    "project_collocated", "smagorinsky_total", "get_re_parameters", "flow_diagnostics",
     "MultigridPlan", "poisson_multigrid", "poisson_multigrid_to_tolerance",
     "CollocatedStepper", "SimState", "initial_state", "step_collocated", "step_pure",
 ]

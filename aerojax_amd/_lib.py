@@ -33,6 +33,7 @@ AJ_MG_OPT_GUESS_TEST = 2
 EXPORTS = (
     "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
     "aj_les_smagorinsky_total", "aj_divergence_rhs", "aj_project", "aj_project_collocated",
+    "aj_flow_diagnostics",
     "aj_mg_create", "aj_mg_destroy", "aj_mg_workspace_bytes",
     "aj_mg_num_levels", "aj_mg_set_option", "aj_poisson_mg", "aj_mg_get_stats",
     "aj_profile_begin", "aj_profile_end", "aj_profile_read",
@@ -59,6 +60,15 @@ class RkParams(ctypes.Structure):
                 ("dy", ctypes.c_double), ("U_inf", ctypes.c_double),
                 ("brinkman_eta", ctypes.c_double), ("nu_hyper_ratio", ctypes.c_double),
                 ("slip_walls", ctypes.c_int), ("fast_mode", ctypes.c_int)]
+
+
+class FlowDiag(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("pressure_residual", "rhs_norm",
+                                              "relative_residual", "div_rms", "l2_div",
+                                              "fluid_cells")]
+
+    def as_dict(self):
+        return {n: float(getattr(self, n)) for n, _ in self._fields_}
 
 
 class MgStats(ctypes.Structure):
@@ -101,6 +111,8 @@ _SIGS = {
     "aj_project": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
     "aj_project_collocated": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d,
                                    _vp]),
+    "aj_flow_diagnostics": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d,
+                                 ctypes.POINTER(FlowDiag), _vp]),
     "aj_mg_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _d, _d, _i]),
     "aj_mg_destroy": (_i, [_vp]),
     "aj_mg_workspace_bytes": (ctypes.c_size_t, [_vp]),

@@ -1,0 +1,121 @@
+// Flow diagnostics of the step, one fused reduction pass:
+//   * pressure residual of the viewer / debug path
+//     (viewer/simulation_controller.py:107-121, solver/pure_functions/
+//     step.py:229-235): ||lap_x(p) - rhs||_2 with rhs = div_nonperiodic(u, v)
+//     / dt and lap_x = laplacian_nonperiodic_x (solver/operators.py:67-72:
+//     edge-padded in x, periodic roll in y), relative to ||rhs||_2 + 1e-10;
+//   * divergence over the pure fluid (simulation_controller.py:186-194):
+//     div = np.gradient(u, dx, axis 0) + np.gradient(v, dy, axis 1)
+//     (central inside, one-sided first order on the edges), fluid = mask >
+//     0.99, div_rms = sqrt(sum (div fluid)^2 / (sum fluid + 1e-8)),
+//     l2_div = sqrt(sum (div fluid)^2 dx dy).
+// FP64 partial sums per block, fixed-order final sum.
+#include "aj_common.h"
+
+namespace aj {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag(const T* __restrict__ u, const T* __restrict__ v,
+                                              const T* __restrict__ p, const T* __restrict__ mask,
+                                              int nx, int ny, double dx, double dy, double dt,
+                                              double* __restrict__ part) {
+  __shared__ double sh[16];
+  const long n = (long)nx * ny;
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  double res2 = 0.0, rhs2 = 0.0, div2 = 0.0, fluid = 0.0;
+  if (c < n) {
+    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+    // rhs = divergence_nonperiodic(u, v) / dt: zero on the border
+    double rhs = 0.0;
+    if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1)
+      rhs = ((double)u[c + ny] - (double)u[c - ny]) / (2.0 * dx) +
+            ((double)v[c + 1] - (double)v[c - 1]) / (2.0 * dy);
+    rhs /= dt;
+    const double pc = p[c];
+    const double pW = p[i > 0 ? c - ny : c], pE = p[i < nx - 1 ? c + ny : c];
+    const double pS = p[j > 0 ? c - 1 : c + ny - 1], pN = p[j < ny - 1 ? c + 1 : c - ny + 1];
+    const double lap = (pE - 2.0 * pc + pW) / (dx * dx) + (pS + pN - 2.0 * pc) / (dy * dy);
+    res2 = (lap - rhs) * (lap - rhs);
+    rhs2 = rhs * rhs;
+    // np.gradient divergence
+    double gx, gy;
+    if (i == 0) gx = ((double)u[c + ny] - (double)u[c]) / dx;
+    else if (i == nx - 1) gx = ((double)u[c] - (double)u[c - ny]) / dx;
+    else gx = ((double)u[c + ny] - (double)u[c - ny]) / (2.0 * dx);
+    if (j == 0) gy = ((double)v[c + 1] - (double)v[c]) / dy;
+    else if (j == ny - 1) gy = ((double)v[c] - (double)v[c - 1]) / dy;
+    else gy = ((double)v[c + 1] - (double)v[c - 1]) / (2.0 * dy);
+    if ((double)mask[c] > 0.99) {
+      div2 = (gx + gy) * (gx + gy);
+      fluid = 1.0;
+    }
+  }
+  const double a = block_sum(res2, sh);
+  const double b = block_sum(rhs2, sh);
+  const double d = block_sum(div2, sh);
+  const double f = block_sum(fluid, sh);
+  if (threadIdx.x == 0) {
+    double* o = part + 4 * (long)blockIdx.x;
+    o[0] = a;
+    o[1] = b;
+    o[2] = d;
+    o[3] = f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ part, int np,
+                                                    double dx, double dy,
+                                                    aj_flow_diag* __restrict__ out) {
+  __shared__ double sh[16];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < np; k += 256)
+    for (int q = 0; q < 4; ++q) acc[q] += part[4 * (long)k + q];
+  double t[4];
+  for (int q = 0; q < 4; ++q) t[q] = block_sum(acc[q], sh);
+  if (threadIdx.x == 0) {
+    out->pressure_residual = sqrt(t[0]);
+    out->rhs_norm = sqrt(t[1]);
+    out->relative_residual = sqrt(t[0]) / (sqrt(t[1]) + 1e-10);
+    out->div_rms = sqrt(t[2] / (t[3] + 1e-8));
+    out->l2_div = sqrt(t[2] * dx * dy);
+    out->fluid_cells = t[3];
+  }
+}
+
+}  // namespace aj
+
+using namespace aj;
+
+extern "C" int aj_flow_diagnostics(int dtype, const void* u, const void* v, const void* p,
+                                   const void* mask, int nx, int ny, double dx, double dy,
+                                   double dt, aj_flow_diag* out, void* stream) {
+  if (!u || !v || !p || !mask || !out) return fail(AJ_ERR_INVALID, "aj_flow_diagnostics: null pointer");
+  if (nx < 3 || ny < 3) return fail(AJ_ERR_INVALID, "aj_flow_diagnostics: grid must be at least 3x3");
+  hipStream_t st = (hipStream_t)stream;
+  const long n = (long)nx * ny;
+  const int nb = (int)((n + 255) / 256);
+  void* buf = nullptr;
+  AJ_HIP(hipMallocAsync(&buf, (size_t)nb * 4 * sizeof(double) + sizeof(aj_flow_diag), st));
+  double* part = (double*)buf;
+  aj_flow_diag* dres = (aj_flow_diag*)(part + 4 * (long)nb);
+  if (dtype == AJ_F32)
+    hipLaunchKernelGGL(k_diag<float>, dim3(nb), dim3(256), 0, st, (const float*)u,
+                       (const float*)v, (const float*)p, (const float*)mask, nx, ny, dx, dy, dt,
+                       part);
+  else if (dtype == AJ_F64)
+    hipLaunchKernelGGL(k_diag<double>, dim3(nb), dim3(256), 0, st, (const double*)u,
+                       (const double*)v, (const double*)p, (const double*)mask, nx, ny, dx, dy,
+                       dt, part);
+  else {
+    (void)hipFreeAsync(buf, st);
+    return fail(AJ_ERR_INVALID, "aj_flow_diagnostics: bad dtype");
+  }
+  AJ_LAUNCH_CHECK("k_diag");
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, (const double*)part, nb, dx, dy,
+                     dres);
+  AJ_LAUNCH_CHECK("k_diag_final");
+  AJ_HIP(hipMemcpyAsync(out, dres, sizeof(aj_flow_diag), hipMemcpyDeviceToHost, st));
+  AJ_HIP(hipStreamSynchronize(st));
+  AJ_HIP(hipFreeAsync(buf, st));
+  return AJ_OK;
+}

@@ -170,6 +170,23 @@ int aj_mg_num_levels(const aj_mg_plan* plan);
 enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1, AJ_MG_OPT_GUESS_TEST = 2 };
 int aj_mg_set_option(aj_mg_plan* plan, int option, int value);
 
+/* ---- Flow diagnostics --------------------------------------------------
+ * Replaces the viewer's and step_pure's debug residual/divergence metrics
+ * (viewer/simulation_controller.py:107-121 and 186-194,
+ * solver/pure_functions/step.py:229-235): pressure residual
+ * ||laplacian_nonperiodic_x(p) - div_nonperiodic(u, v)/dt||_2 (edge pad in
+ * x, periodic in y, solver/operators.py:67-72) and its ratio to ||rhs||_2 +
+ * 1e-10; divergence of np.gradient over mask > 0.99 as RMS and as
+ * sqrt(sum div^2 dx dy).  One fused FP64 reduction; synchronises `stream`
+ * and fills *out. */
+typedef struct aj_flow_diag {
+  double pressure_residual, rhs_norm, relative_residual;
+  double div_rms, l2_div, fluid_cells;
+} aj_flow_diag;
+int aj_flow_diagnostics(int dtype, const void* u, const void* v, const void* p,
+                        const void* mask, int nx, int ny, double dx, double dy, double dt,
+                        aj_flow_diag* out, void* stream);
+
 #define AJ_MG_MAX_NORMS 64
 typedef struct aj_mg_stats {
   int cycles;      /* V-cycles executed                                  */

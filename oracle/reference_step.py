@@ -607,6 +607,31 @@ def step_collocated(u, v, mask, sdf, dx, dy, nu, U_inf, dt, use_les=False,
     return un, vn, p
 
 
+def laplacian_nonperiodic_x(f, dx, dy):
+    """laplacian_nonperiodic_x (solver/operators.py:67-72): edge padding in
+    x, periodic roll in y."""
+    fp = np.concatenate([f[:1], f, f[-1:]], axis=0)
+    f_xx = (fp[2:] - 2 * f + fp[:-2]) / (dx * dx)
+    f_yy = (np.roll(f, 1, axis=1) + np.roll(f, -1, axis=1) - 2 * f) / (dy * dy)
+    return f_xx + f_yy
+
+
+def flow_diagnostics(u, v, p, mask, dx, dy, dt):
+    """Viewer / debug metrics (viewer/simulation_controller.py:107-121,
+    186-194): pressure residual of lap_x(p) against div_nonperiodic(u, v)/dt,
+    and the np.gradient divergence over mask > 0.99."""
+    u, v, p, mask = (np.asarray(a, dtype=np.float64) for a in (u, v, p, mask))
+    rhs = divergence_nonperiodic(u, v, dx, dy) / dt
+    res = np.linalg.norm(laplacian_nonperiodic_x(p, dx, dy) - rhs)
+    rn = np.linalg.norm(rhs)
+    div = np.gradient(u, dx, axis=0) + np.gradient(v, dy, axis=1)
+    fluid = mask > 0.99
+    df = div * fluid
+    return {"pressure_residual": res, "rhs_norm": rn, "relative_residual": res / (rn + 1e-10),
+            "div_rms": np.sqrt(np.sum(df ** 2) / (np.sum(fluid) + 1e-8)),
+            "l2_div": np.sqrt(np.sum(df ** 2) * dx * dy), "fluid_cells": float(np.sum(fluid))}
+
+
 def rel_l2(a, b):
     """Relative L2 error ||a - b|| / ||b|| in float64 (0 when both vanish)."""
     a = np.asarray(a, dtype=np.float64)

@@ -373,3 +373,20 @@ def test_textbook_graph_replays_match_eager(gpu):
         assert stats["eager"]["converged"]
     for pl in plans.values():
         pl.close()
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_flow_diagnostics_parity(gpu, dtype):
+    """Viewer / debug metrics after one reference step of the 512x96 airfoil."""
+    import aerojax_amd as aj
+    c = case(512, 96, 4.0, 0.75)
+    dx, dy, dt = c["dx"], c["dy"], 1e-3
+    un, vn, p = R.step_pure(c["u"], c["v"], c["mask"], c["sdf"], dx, dy, 5e-4, 1.0, dt,
+                            v_cycles=3)
+    ref = R.flow_diagnostics(un, vn, p, c["mask"], dx, dy, dt)
+    got = aj.flow_diagnostics(dev(un, dtype), dev(vn, dtype), dev(p, dtype),
+                              dev(c["mask"], dtype), dx, dy, dt)
+    tol = 1e-5 if dtype == torch.float32 else 1e-12
+    for k, r in ref.items():
+        print(f"diag {k}: {got[k]:.6e} vs {r:.6e}", flush=True)
+        assert math.isclose(got[k], r, rel_tol=tol, abs_tol=1e-300), k

@@ -449,3 +449,38 @@ def test_get_re_parameters_table():
     p = g(5000.0, 256)                           # coarse-grid adjustment
     assert np.isclose(p["C_s"], 0.12) and np.isclose(p["nu_hyper_ratio"], 0.015)
     assert np.isclose(p["brinkman_eta"], 1.2e-4) and np.isclose(p["cfl_target"], 0.2)
+
+
+def test_laplacian_nonperiodic_x_closed_forms():
+    """cos(2 pi j / ny) is periodic in y: the rolled second difference is
+    exactly -4 sin^2(pi/ny) cos(...)/dy^2; a quadratic in x has the exact
+    second difference inside and the edge-padded one-sided value on the
+    first and last rows."""
+    nx, ny, dx, dy = 9, 16, 0.5, 0.25
+    j = np.arange(ny)[None, :]
+    f = np.cos(2 * np.pi * j / ny) * np.ones((nx, 1))
+    lap = R.laplacian_nonperiodic_x(f, dx, dy)
+    assert np.allclose(lap, -4 * np.sin(np.pi / ny) ** 2 / dy ** 2 * f, atol=1e-12)
+    i = np.arange(nx)[:, None] * np.ones((1, ny))
+    g = (i * dx) ** 2
+    lg = R.laplacian_nonperiodic_x(g, dx, dy)
+    assert np.allclose(lg[1:-1], 2.0)
+    assert np.allclose(lg[0], (g[1] - g[0]) / dx ** 2)
+    assert np.allclose(lg[-1], (g[-2] - g[-1]) / dx ** 2)
+
+
+def test_flow_diagnostics_divergence_free_field():
+    """u = sin(x) cos(y), v = -cos(x) sin(y) is divergence-free: the np.gradient
+    divergence is second-order small inside; a solid region is excluded."""
+    nx, ny = 64, 48
+    dx, dy = 2 * np.pi / nx, np.pi / ny
+    X = np.arange(nx)[:, None] * dx
+    Y = np.arange(ny)[None, :] * dy
+    u = np.sin(X) * np.cos(Y)
+    v = -np.cos(X) * np.sin(Y)
+    mask = np.ones((nx, ny))
+    mask[:, :2] = 0.5
+    d = R.flow_diagnostics(u, v, np.zeros((nx, ny)), mask, dx, dy, 1e-2)
+    assert d["fluid_cells"] == nx * (ny - 2)
+    assert d["div_rms"] < 0.05          # boundary one-sided differences dominate
+    assert np.isclose(d["relative_residual"], 1.0, rtol=1e-9)  # p = 0
