@@ -15,6 +15,7 @@ from ._lib import AeroJaxError, AeroJaxLibraryError, load as load_library, versi
 from .advection_schemes import rk_step_unified
 from .operators import (constant_smagorinsky, divergence_nonperiodic, pressure_rhs, project,
                         project_collocated, smagorinsky_total)
+from .batch import BatchedStepper
 from .diagnostics import flow_diagnostics
 from .params import get_re_parameters
 from .pressure_solvers import (MultigridPlan, poisson_multigrid,
@@ -26,5 +27,5 @@ __all__ = [
     "constant_smagorinsky", "divergence_nonperiodic", "pressure_rhs", "project",
     "project_collocated", "smagorinsky_total", "get_re_parameters", "flow_diagnostics",
     "MultigridPlan", "poisson_multigrid", "poisson_multigrid_to_tolerance",
-    "CollocatedStepper", "SimState", "initial_state", "step_collocated", "step_pure",
+    "BatchedStepper", "CollocatedStepper", "SimState", "initial_state", "step_collocated", "step_pure",
 ]

@@ -33,7 +33,9 @@ AJ_MG_OPT_GUESS_TEST = 2
 EXPORTS = (
     "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
     "aj_les_smagorinsky_total", "aj_divergence_rhs", "aj_project", "aj_project_collocated",
-    "aj_flow_diagnostics",
+    "aj_flow_diagnostics", "aj_batch_create", "aj_batch_destroy", "aj_batch_cells",
+    "aj_batch_num_instances", "aj_batch_offsets", "aj_batch_set_viscosity",
+    "aj_batch_set_option", "aj_batch_step", "aj_batch_get_stats",
     "aj_mg_create", "aj_mg_destroy", "aj_mg_workspace_bytes",
     "aj_mg_num_levels", "aj_mg_set_option", "aj_poisson_mg", "aj_mg_get_stats",
     "aj_profile_begin", "aj_profile_end", "aj_profile_read",
@@ -69,6 +71,12 @@ class FlowDiag(ctypes.Structure):
 
     def as_dict(self):
         return {n: float(getattr(self, n)) for n, _ in self._fields_}
+
+
+class BatchStepParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("U_inf", ctypes.c_double),
+                ("brinkman_eta", ctypes.c_double), ("tolerance", ctypes.c_double),
+                ("v_cycles", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 class MgStats(ctypes.Structure):
@@ -113,6 +121,16 @@ _SIGS = {
                                    _vp]),
     "aj_flow_diagnostics": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d,
                                  ctypes.POINTER(FlowDiag), _vp]),
+    "aj_batch_create": (_i, [ctypes.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _i]),
+    "aj_batch_destroy": (_i, [_vp]),
+    "aj_batch_cells": (ctypes.c_long, [_vp]),
+    "aj_batch_num_instances": (_i, [_vp]),
+    "aj_batch_offsets": (_i, [_vp, _vp]),
+    "aj_batch_set_viscosity": (_i, [_vp, _vp]),
+    "aj_batch_set_option": (_i, [_vp, _i, _i]),
+    "aj_batch_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                           ctypes.POINTER(BatchStepParams), _vp]),
+    "aj_batch_get_stats": (_i, [_vp, _vp, _vp]),
     "aj_mg_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _d, _d, _i]),
     "aj_mg_destroy": (_i, [_vp]),
     "aj_mg_workspace_bytes": (ctypes.c_size_t, [_vp]),

@@ -1,0 +1,116 @@
+"""Many independent simulations stepped together (``aj_batch_*``).
+
+Each instance is its own von Karman channel (grid, obstacle, viscosity);
+one step applies step_pure's composition (RK3 predictor, rhs, reference
+multigrid with ``v_cycles`` cycles, projection; solver/pure_functions/
+step.py:49-467) to every instance in shared, graph-captured launches.
+Instances are grouped by grid shape; fields live in one flat device buffer
+per quantity, instance k at ``offsets[k]``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _arrays as A
+from . import _lib
+
+
+class BatchedStepper:
+    """``groups``: list of ``(nx, ny, count, dx, dy)``; ``masks`` / ``sdfs``
+    / ``nus``: one entry per instance, group by group; ``u0`` / ``v0``:
+    per-instance initial fields."""
+
+    def __init__(self, groups, masks, sdfs, nus, u0, v0, dt, *, U_inf=1.0, brinkman_eta=0.005,
+                 v_cycles=3, tolerance=1e-6, levels=4, dtype=torch.float32, device=None):
+        A.require_gpu()
+        if dtype not in (torch.float32, torch.float64):
+            raise TypeError(f"unsupported dtype {dtype} (float32 or float64)")
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
+            else torch.device(device)
+        self.dtype = dtype
+        self.groups = [tuple(g) for g in groups]
+        ng = len(self.groups)
+        arr = lambda k, t: (t * ng)(*[t(g[k]) for g in self.groups])
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            code = _lib.AJ_F32 if dtype == torch.float32 else _lib.AJ_F64
+            rc = _lib.load().aj_batch_create(ctypes.byref(h), code,
+                                             ng, arr(0, ctypes.c_int), arr(1, ctypes.c_int),
+                                             arr(2, ctypes.c_int), arr(3, ctypes.c_double),
+                                             arr(4, ctypes.c_double), int(levels))
+        _lib.check(rc, "aj_batch_create")
+        self.handle = h
+        lib = _lib.load()
+        self.n = int(lib.aj_batch_num_instances(h))
+        self.cells = int(lib.aj_batch_cells(h))
+        off = np.zeros(self.n, dtype=np.int64)
+        _lib.check(lib.aj_batch_offsets(h, off.ctypes.data), "aj_batch_offsets")
+        self.offsets = off
+        self.shapes = [(g[0], g[1]) for g in self.groups for _ in range(g[2])]
+        for name, lst in (("masks", masks), ("sdfs", sdfs), ("nus", nus), ("u0", u0), ("v0", v0)):
+            if lst is not None and len(lst) != self.n:
+                raise ValueError(f"{name}: {len(lst)} entries for {self.n} instances")
+        nu = np.ascontiguousarray(np.asarray(nus, dtype=np.float64))
+        _lib.check(lib.aj_batch_set_viscosity(h, nu.ctypes.data), "aj_batch_set_viscosity")
+        self.mask = self._pack(masks)
+        self.sdf = self._pack(sdfs) if sdfs is not None else None
+        self.u = self._pack(u0)
+        self.v = self._pack(v0)
+        self.u_next = torch.empty_like(self.u)
+        self.v_next = torch.empty_like(self.v)
+        self.p = torch.zeros_like(self.u)
+        self.prm = _lib.BatchStepParams(float(dt), float(U_inf), float(brinkman_eta),
+                                        float(tolerance), int(v_cycles), 0)
+
+    def _pack(self, fields):
+        buf = torch.empty(self.cells, dtype=self.dtype, device=self.device)
+        for k, f in enumerate(fields):
+            nx, ny = self.shapes[k]
+            t = A.as_device(f, self.dtype, self.device, "field")
+            if tuple(t.shape) != (nx, ny):
+                raise ValueError(f"instance {k}: shape {tuple(t.shape)} != {(nx, ny)}")
+            buf[self.offsets[k]:self.offsets[k] + nx * ny].copy_(t.reshape(-1))
+        return buf
+
+    def view(self, buf, k):
+        """Instance k's (nx, ny) view of a flat batch buffer."""
+        nx, ny = self.shapes[k]
+        return buf[self.offsets[k]:self.offsets[k] + nx * ny].view(nx, ny)
+
+    def set_graph(self, on: bool):
+        _lib.check(_lib.load().aj_batch_set_option(self.handle, _lib.AJ_MG_OPT_GRAPH, int(on)),
+                   "aj_batch_set_option")
+
+    def step(self):
+        """Advance every instance by one step (u, v swapped with the outputs;
+        ``p`` holds the pressures)."""
+        with torch.cuda.device(self.device):
+            rc = _lib.load().aj_batch_step(self.handle, A.ptr(self.u), A.ptr(self.v),
+                                           A.ptr(self.mask), A.ptr(self.sdf), A.ptr(self.u_next),
+                                           A.ptr(self.v_next), A.ptr(self.p),
+                                           ctypes.byref(self.prm), A.stream_handle(self.device))
+        _lib.check(rc, "aj_batch_step")
+        self.u, self.u_next = self.u_next, self.u
+        self.v, self.v_next = self.v_next, self.v
+
+    def stats(self):
+        s = (_lib.MgStats * self.n)()
+        with torch.cuda.device(self.device):
+            rc = _lib.load().aj_batch_get_stats(self.handle, ctypes.cast(s, ctypes.c_void_p),
+                                                A.stream_handle(self.device))
+        _lib.check(rc, "aj_batch_get_stats")
+        return [x.as_dict() for x in s]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().aj_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

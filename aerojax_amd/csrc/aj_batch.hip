@@ -1,0 +1,753 @@
+// Batched stepping of many independent simulations (each with its own grid
+// shape, spacing, geometry and viscosity) in shared launches.
+//
+// Instances are grouped by grid shape; every field of the batch is one
+// buffer in which the instances follow each other group by group (instance
+// k occupies [off_k, off_k + nx_g * ny_g)).  Each kernel launch carries a
+// BatchWork descriptor per group (shape, spacing, first instance, element
+// base, work items per instance) and runs as a persistent grid: workgroup w
+// processes work items w, w + gridDim.x, ... and maps each to (group,
+// instance, tile) by a scan of the <= kMaxGroups group starts.  Per-instance
+// scalars (viscosity) and the per-instance multigrid gates live in device
+// arrays indexed by the instance id.
+//
+// One batched step is step_pure's composition (solver/pure_functions/
+// step.py:49-467, as aj_rk_step + aj_divergence_rhs + aj_poisson_mg
+// (reference) + aj_project) for every instance: RK3 predictor tiles, rhs
+// rows, reference multigrid V-cycles level by level (all instances of a
+// level in one launch; the 10-sweep coarsest solve one workgroup per
+// instance) with a per-instance convergence gate, projection rows.  The
+// launch sequence of a step has no host synchronisation and is captured
+// once into a hipGraph per set of buffers.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "aj_mg_ref_kernels.h"
+#include "aj_step_kernels.h"
+
+namespace aj {
+
+constexpr int kMaxGroups = 8;
+constexpr int kBatchRkNTH = 512;
+template <typename T> struct BatchRkTile;  // RK tile: 16x64 (FP32) / 8x64 (FP64) keeps LDS < 64 KiB
+template <> struct BatchRkTile<float> { static constexpr int TX = 16, TY = 64; };
+template <> struct BatchRkTile<double> { static constexpr int TX = 8, TY = 64; };
+constexpr int kBatchJX = 16, kBatchJY = 64;  // Jacobi tile
+constexpr int kResChunks = 16;  // residual-norm partial sums per instance
+// persistent grid sizes, workgroups per CU: a few rounds of the resident
+// workgroups (LDS-limited for the tile kernels) so that one workgroup's
+// loads overlap another's compute
+constexpr int kWgRk = 12, kWgJacobi = 16, kWgRows = 32, kWgCoarse = 2;
+
+struct BatchWork {
+  int ng, total;
+  int wstart[kMaxGroups + 1];  // work-item prefix over groups
+  int per_inst[kMaxGroups];    // work items per instance
+  int first[kMaxGroups];       // first instance id of the group
+  int nx[kMaxGroups], ny[kMaxGroups];
+  int tiles_y[kMaxGroups];     // tile kernels: tiles along y
+  long base[kMaxGroups];       // element offset of the group's first instance
+  long cbase[kMaxGroups];      // coarse-level buffers: same for the next level
+  int cnx[kMaxGroups], cny[kMaxGroups];
+  double dx[kMaxGroups], dy[kMaxGroups];
+};
+
+struct BatchItem {
+  int g, inst, sub;
+  long off, coff;
+};
+
+__device__ __forceinline__ BatchItem batch_item(const BatchWork& w, int item) {
+  int g = 0;
+#pragma unroll
+  for (int k = 1; k < kMaxGroups; ++k)
+    if (k < w.ng && item >= w.wstart[k]) g = k;
+  const int local = item - w.wstart[g];
+  const int k = local / w.per_inst[g];
+  BatchItem it;
+  it.g = g;
+  it.inst = w.first[g] + k;
+  it.sub = local - k * w.per_inst[g];
+  it.off = w.base[g] + (long)k * w.nx[g] * w.ny[g];
+  it.coff = w.cbase[g] + (long)k * w.cnx[g] * w.cny[g];
+  return it;
+}
+
+// ---------------------------------------------------------------- step kernels
+
+template <typename T>
+struct BatchStepArgs {
+  const T *u, *v, *mask, *sdf;
+  T *us, *vs, *rhs, *p, *uo, *vo;
+  const double* nu;  // [instances]
+  double dt, U_inf, inv_eta;
+};
+
+template <typename T, int TX, int TY, int NTH>
+__global__ __launch_bounds__(NTH) void kb_rk(BatchWork w, BatchStepArgs<T> s) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g;
+    RkArgs<T> a;
+    a.u = s.u + it.off;
+    a.v = s.v + it.off;
+    a.mask = s.mask + it.off;
+    a.sdf = s.sdf ? s.sdf + it.off : nullptr;
+    a.nu_sgs = nullptr;
+    a.nu_field = nullptr;
+    a.u_out = s.us + it.off;
+    a.v_out = s.vs + it.off;
+    a.nx = w.nx[g];
+    a.ny = w.ny[g];
+    a.dt = T(s.dt);
+    a.nu = T(s.nu[it.inst]);
+    a.inv_dx = T(1.0 / w.dx[g]);
+    a.inv_dy = T(1.0 / w.dy[g]);
+    a.inv_dx2 = T(1.0 / (w.dx[g] * w.dx[g]));
+    a.inv_dy2 = T(1.0 / (w.dy[g] * w.dy[g]));
+    a.U_inf = T(s.U_inf);
+    a.inv_eta = T(s.inv_eta);
+    a.sdf_scale = T(1.0 / (0.1 * w.dx[g]));
+    a.hyper = T(0);
+    a.slip = 1;
+    const int ty = it.sub % w.tiles_y[g], tx = it.sub / w.tiles_y[g];
+    rk_tile<T, TX, TY, false, false, NTH>(a, tx * TX, ty * TY, sm);
+    __syncthreads();  // LDS reused by the next item
+  }
+}
+
+// row items: kRowsPerItem rows x 256 columns, one cell per thread and row
+// (the rows' loads are independent, so they are in flight together)
+constexpr int kRowsPerItem = 4;
+
+// rhs = div * mask / dt
+template <typename T>
+__global__ __launch_bounds__(256) void kb_div(BatchWork w, BatchStepArgs<T> s) {
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g, nx = w.nx[g], ny = w.ny[g];
+    const int cpr = (ny + 255) / 256;
+    const int rb = it.sub / cpr, j = (it.sub - rb * cpr) * 256 + threadIdx.x;
+    if (j >= ny) continue;
+#pragma unroll
+    for (int q = 0; q < kRowsPerItem; ++q) {
+      const int i = rb * kRowsPerItem + q;
+      if (i < nx)
+        div_rhs_cell(s.us + it.off, s.vs + it.off, s.mask + it.off, s.rhs + it.off, nx, ny, i, j,
+                     T(1.0 / (2.0 * w.dx[g])), T(1.0 / (2.0 * w.dy[g])), T(1.0 / s.dt));
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_project(BatchWork w, BatchStepArgs<T> s) {
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g, nx = w.nx[g], ny = w.ny[g];
+    const int cpr = (ny + 255) / 256;
+    const int rb = it.sub / cpr, j = (it.sub - rb * cpr) * 256 + threadIdx.x;
+    if (j >= ny) continue;
+#pragma unroll
+    for (int q = 0; q < kRowsPerItem; ++q) {
+      const int i = rb * kRowsPerItem + q;
+      if (i < nx)
+        project_cell(s.us + it.off, s.vs + it.off, s.p + it.off, s.mask + it.off, s.uo + it.off,
+                     s.vo + it.off, nx, ny, i, j, T(s.dt), T(1.0 / (2.0 * w.dx[g])),
+                     T(1.0 / (2.0 * w.dy[g])), T(0.1 / w.dx[g]), T(s.U_inf));
+    }
+  }
+}
+
+// ---------------------------------------------------------------- multigrid
+
+template <typename T>
+struct BatchMgArgs {
+  const T* p_in;  // null -> zero guess
+  const T* e_c;   // coarse correction (level + 1 buffers), or null
+  const T* b;
+  T* p_out;
+  int bzl, clamp;
+  aj_mg_stats* stats;  // [instances]
+};
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(256) void kb_jacobi(BatchWork w, BatchMgArgs<T> m) {
+  __shared__ JacobiLds<T, TX, TY, 2> L;
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g;
+    RefJacobiArgs<T> a;
+    a.p_in = m.p_in ? m.p_in + it.off : nullptr;
+    a.e_c = m.e_c ? m.e_c + it.coff : nullptr;
+    a.ncx = w.cnx[g];
+    a.ncy = w.cny[g];
+    a.b = m.b + it.off;
+    a.b_zero_last = m.bzl;
+    a.p_out = m.p_out + it.off;
+    a.nx = w.nx[g];
+    a.ny = w.ny[g];
+    // fine-grid spacing on every level (multigrid_solver.py:158-171)
+    a.ax = T(1) / (T(w.dx[g]) * T(w.dx[g]));
+    a.ay = T(1) / (T(w.dy[g]) * T(w.dy[g]));
+    a.inv_diag = T(1) / (T(2) * (a.ax + a.ay));
+    a.clamp_last = m.clamp;
+    a.gate = m.stats + it.inst;
+    const int ty = it.sub % w.tiles_y[g], tx = it.sub / w.tiles_y[g];
+    ref_jacobi_tile<T, TX, TY, 2>(a, tx * TX, ty * TY, L);
+    __syncthreads();
+  }
+}
+
+// residual + restriction, rows of 256 coarse cells
+template <typename T>
+__global__ __launch_bounds__(256) void kb_resrestrict(BatchWork w, BatchMgArgs<T> m, T* bc) {
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g;
+    if (gated(m.stats + it.inst)) continue;
+    const int ncx = w.cnx[g], ncy = w.cny[g];
+    const int cpr = (ncy + 255) / 256;
+    const int rb = it.sub / cpr, J = (it.sub - rb * cpr) * 256 + threadIdx.x;
+    if (J >= ncy) continue;
+    const T ax = T(1) / (T(w.dx[g]) * T(w.dx[g])), ay = T(1) / (T(w.dy[g]) * T(w.dy[g]));
+    const int degenerate = (w.nx[g] / 2 < 2 || w.ny[g] / 2 < 2) ? 1 : 0;
+#pragma unroll
+    for (int q = 0; q < kRowsPerItem; ++q) {
+      const int I = rb * kRowsPerItem + q;
+      if (I < ncx)
+        ref_resrestrict_cell(m.p_out + it.off, m.b + it.off, m.bzl, w.nx[g], w.ny[g], ax, ay,
+                             bc + it.coff, ncy, degenerate, I, J);
+    }
+  }
+}
+
+// coarsest level: 10 sweeps from zero, one workgroup per instance
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void kb_coarse(BatchWork w, BatchMgArgs<T> m) {
+  extern __shared__ unsigned char smem_raw[];
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g;
+    if (gated(m.stats + it.inst)) continue;
+    const T ax = T(1) / (T(w.dx[g]) * T(w.dx[g])), ay = T(1) / (T(w.dy[g]) * T(w.dy[g]));
+    ref_coarse_wg(m.b + it.off, m.p_out + it.off, w.nx[g], w.ny[g], ax, ay,
+                  T(1) / (T(2) * (ax + ay)), 10, reinterpret_cast<T*>(smem_raw));
+    __syncthreads();
+  }
+}
+
+// sum(r^2) partials: kResChunks items per instance
+template <typename T>
+__global__ __launch_bounds__(256) void kb_resnorm(BatchWork w, const T* __restrict__ p,
+                                                  const T* __restrict__ b,
+                                                  double* __restrict__ partial) {
+  __shared__ double sh[16];
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g, nx = w.nx[g], ny = w.ny[g];
+    const long n = (long)nx * ny;
+    const T ax = T(1) / (T(w.dx[g]) * T(w.dx[g])), ay = T(1) / (T(w.dy[g]) * T(w.dy[g]));
+    double acc = 0.0;
+    for (long c = (long)it.sub * 256 + threadIdx.x; c < n; c += (long)kResChunks * 256) {
+      const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+      const double r = (double)ref_residual(p + it.off, b + it.off, 1, nx, ny, ax, ay, i, j);
+      acc += r * r;
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[(long)it.inst * kResChunks + it.sub] = t;
+  }
+}
+
+// per instance: sum the partials, record the norm, set the gate
+// (multigrid_solver.py:222-234)
+__global__ __launch_bounds__(64) void kb_resnorm_final(const double* __restrict__ partial,
+                                                       int ninst, const double* __restrict__ dxdy,
+                                                       const double* __restrict__ tol2,
+                                                       aj_mg_stats* __restrict__ stats) {
+  const int inst = blockIdx.x * 64 + threadIdx.x;
+  if (inst >= ninst) return;
+  double tot = 0.0;
+  for (int k = 0; k < kResChunks; ++k) tot += partial[(long)inst * kResChunks + k];
+  aj_mg_stats* st = stats + inst;
+  const double nsq = tot * dxdy[inst];
+  const int executed = st->converged == 0;
+  const int k = st->outer;
+  if (k < AJ_MG_MAX_NORMS) st->norms[k] = nsq;
+  st->outer = k + 1;
+  if (executed) st->cycles += 1;
+  st->norm = nsq;
+  if (!isfinite(nsq)) st->nonfinite = 1;
+  st->converged = (nsq < tol2[inst]) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(64) void kb_mg_begin(aj_mg_stats* stats, int ninst) {
+  const int inst = blockIdx.x;
+  if (inst >= ninst) return;
+  aj_mg_stats* st = stats + inst;
+  if (threadIdx.x == 0) {
+    st->cycles = 0;
+    st->outer = 0;
+    st->converged = 0;
+    st->nonfinite = 0;
+    st->norm = 0.0;
+  }
+  st->norms[threadIdx.x] = 0.0;  // 64 threads = AJ_MG_MAX_NORMS
+}
+
+}  // namespace aj
+
+using namespace aj;
+
+// ---------------------------------------------------------------- host side
+
+struct aj_batch {
+  int dtype = AJ_F32;
+  int ng = 0, ninst = 0, levels = 4, ml = 0;
+  std::vector<int> gnx, gny, gcount, gfirst;
+  std::vector<double> gdx, gdy;
+  std::vector<std::vector<int>> lnx, lny;  // [group][level]
+  std::vector<long> level_len;             // elements of one field at each level
+  std::vector<std::vector<long>> lbase;    // [level][group] element base
+  long cells = 0;
+  void* ws = nullptr;
+  std::vector<void*> B, Y, E;  // per level >= 1 (index level); level 0 uses step buffers
+  void *us = nullptr, *vs = nullptr, *rhs = nullptr;
+  double* partial = nullptr;
+  double* dxdy = nullptr;
+  double* tol2 = nullptr;
+  double* nu = nullptr;
+  aj_mg_stats* stats = nullptr;
+  double last_tol = -1.0;
+  int use_graph = 1;
+  // two cached step graphs (steppers alternate two sets of buffers)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  const void* key[2][8] = {};
+  double key_prm[2][5] = {};
+  int victim = 0;
+  int device = 0, cus = 256;
+};
+
+namespace {
+
+// kind: 0 = tiles tx x ty, 1 = row items (this level), 2 = row items of
+// the next coarser level, 3 = one item per instance, 4 = kResChunks per
+// instance
+BatchWork make_work(const aj_batch* b, int level, int kind, int tx = 16, int ty = 64) {
+  BatchWork w;
+  std::memset(&w, 0, sizeof(w));
+  int tot = 0, ng = 0;
+  for (int g = 0; g < b->ng; ++g) {
+    const int nx = b->lnx[g][level], ny = b->lny[g][level];
+    int per = 0;
+    w.tiles_y[ng] = (int)cdiv(ny, ty);
+    switch (kind) {
+      case 0: per = (int)(cdiv(nx, tx) * cdiv(ny, ty)); break;
+      case 1: per = (int)cdiv(nx, kRowsPerItem) * (int)cdiv(ny, 256); break;
+      case 2:
+        per = (int)cdiv(b->lnx[g][level + 1], kRowsPerItem) * (int)cdiv(b->lny[g][level + 1], 256);
+        break;
+      case 3: per = 1; break;
+      default: per = kResChunks; break;
+    }
+    w.wstart[ng] = tot;
+    w.per_inst[ng] = per;
+    w.first[ng] = b->gfirst[g];
+    w.nx[ng] = nx;
+    w.ny[ng] = ny;
+    w.base[ng] = b->lbase[level][g];
+    if (level + 1 <= b->ml) {
+      w.cnx[ng] = b->lnx[g][level + 1];
+      w.cny[ng] = b->lny[g][level + 1];
+      w.cbase[ng] = b->lbase[level + 1][g];
+    }
+    w.dx[ng] = b->gdx[g];
+    w.dy[ng] = b->gdy[g];
+    tot += per * b->gcount[g];
+    ++ng;
+  }
+  w.ng = ng;
+  w.wstart[ng] = tot;
+  w.total = tot;
+  return w;
+}
+
+unsigned persistent_grid(const aj_batch* b, int total, int per_cu) {
+  const long cap = (long)b->cus * per_cu;
+  return (unsigned)std::max<long>(1, std::min<long>(total, cap));
+}
+
+template <typename T>
+int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, int bzl,
+                 hipStream_t st) {
+  BatchMgArgs<T> m;
+  m.stats = b->stats;
+  m.bzl = bzl;
+  if (lev >= b->ml) {
+    // coarsest: 10 sweeps from zero; one workgroup per instance when it fits
+    BatchWork w = make_work(b, lev, 3);
+    long nmax = 0;
+    for (int g = 0; g < b->ng; ++g) nmax = std::max<long>(nmax, (long)b->lnx[g][lev] * b->lny[g][lev]);
+    if (nmax <= kCoarseThreads * kCoarsePerThread && nmax * 2 * (long)sizeof(T) <= 65536) {
+      m.p_in = nullptr;
+      m.e_c = nullptr;
+      m.b = bb;
+      m.p_out = p_dst;
+      m.clamp = 0;
+      hipLaunchKernelGGL(kb_coarse<T>, dim3(persistent_grid(b, w.total, kWgCoarse)), dim3(kCoarseThreads),
+                         (size_t)(2 * nmax * sizeof(T)), st, w, m);
+      AJ_LAUNCH_CHECK("kb_coarse");
+      return AJ_OK;
+    }
+    // larger coarsest grids: 5 fused pairs of Jacobi sweeps, ping-pong
+    BatchWork wt = make_work(b, lev, 0, kBatchJX, kBatchJY);
+    T* other = (T*)b->Y[lev];
+    const T* src = nullptr;
+    for (int k = 0; k < 5; ++k) {
+      T* dst = (k % 2 == 0) ? p_dst : other;
+      m.p_in = src;
+      m.e_c = nullptr;
+      m.b = bb;
+      m.p_out = dst;
+      m.clamp = 0;
+      hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), dim3(persistent_grid(b, wt.total, kWgJacobi)),
+                         dim3(256), 0, st, wt, m);
+      AJ_LAUNCH_CHECK("kb_jacobi");
+      src = dst;
+    }
+    return AJ_OK;
+  }
+  T* Yl = (T*)b->Y[lev];
+  BatchWork wt = make_work(b, lev, 0, kBatchJX, kBatchJY);
+  m.p_in = p_src;
+  m.e_c = nullptr;
+  m.b = bb;
+  m.p_out = Yl;
+  m.clamp = 1;
+  hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), dim3(persistent_grid(b, wt.total, kWgJacobi)),
+                     dim3(256), 0, st, wt, m);
+  AJ_LAUNCH_CHECK("kb_jacobi");
+  T* Bc = (T*)b->B[lev + 1];
+  BatchWork wr = make_work(b, lev, 2);
+  BatchMgArgs<T> r = m;
+  r.p_out = Yl;
+  hipLaunchKernelGGL(kb_resrestrict<T>, dim3(persistent_grid(b, wr.total, kWgRows)), dim3(256), 0, st,
+                     wr, r, Bc);
+  AJ_LAUNCH_CHECK("kb_resrestrict");
+  T* Ec = (T*)b->E[lev + 1];
+  if (int rc = batch_vcycle<T>(b, lev + 1, nullptr, Ec, Bc, 0, st)) return rc;
+  m.p_in = Yl;
+  m.e_c = Ec;
+  m.p_out = p_dst;
+  m.clamp = 1;
+  hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), dim3(persistent_grid(b, wt.total, kWgJacobi)),
+                     dim3(256), 0, st, wt, m);
+  AJ_LAUNCH_CHECK("kb_jacobi");
+  return AJ_OK;
+}
+
+template <typename T>
+int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void* u,
+                        const void* v, const void* mask, const void* sdf, void* u_out,
+                        void* v_out, void* p_out, hipStream_t st) {
+  BatchStepArgs<T> s;
+  s.u = (const T*)u;
+  s.v = (const T*)v;
+  s.mask = (const T*)mask;
+  s.sdf = (const T*)sdf;
+  s.us = (T*)b->us;
+  s.vs = (T*)b->vs;
+  s.rhs = (T*)b->rhs;
+  s.p = (T*)p_out;
+  s.uo = (T*)u_out;
+  s.vo = (T*)v_out;
+  s.nu = b->nu;
+  s.dt = prm->dt;
+  s.U_inf = prm->U_inf;
+  s.inv_eta = 1.0 / prm->brinkman_eta;
+  // RK3 predictor (the predictor BCs of step.py:182-191 are its own last step)
+  {
+    constexpr int TX = BatchRkTile<T>::TX, TY = BatchRkTile<T>::TY, H = 3;
+    BatchWork w = make_work(b, 0, 0, TX, TY);
+    const size_t smem = (size_t)(TX + 2 * H) * (TY + 2 * H) * 8 * sizeof(T);
+    static_assert((TX + 2 * H) * (TY + 2 * H) * 8 * sizeof(T) <= 65536, "RK tile LDS");
+    ProfScope prof(AJ_PROF_RK, (double)b->cells * sizeof(T) * 6, st);
+    hipLaunchKernelGGL((kb_rk<T, TX, TY, kBatchRkNTH>), dim3(persistent_grid(b, w.total, kWgRk)),
+                       dim3(kBatchRkNTH), smem, st, w, s);
+    AJ_LAUNCH_CHECK("kb_rk");
+  }
+  BatchWork wrow = make_work(b, 0, 1);
+  {
+    ProfScope prof(AJ_PROF_RHS, (double)b->cells * sizeof(T) * 4, st);
+    hipLaunchKernelGGL(kb_div<T>, dim3(persistent_grid(b, wrow.total, kWgRows)), dim3(256), 0, st,
+                       wrow, s);
+    AJ_LAUNCH_CHECK("kb_div");
+  }
+  // reference multigrid, v_cycles cycles with the per-instance gate
+  hipLaunchKernelGGL(kb_mg_begin, dim3(b->ninst), dim3(64), 0, st, b->stats, b->ninst);
+  AJ_LAUNCH_CHECK("kb_mg_begin");
+  BatchWork wn = make_work(b, 0, 4);
+  for (int c = 0; c < prm->v_cycles; ++c) {
+    const T* src = (c == 0) ? nullptr : (const T*)p_out;
+    if (int rc = batch_vcycle<T>(b, 0, src, (T*)p_out, (const T*)b->rhs, 1, st)) return rc;
+    hipLaunchKernelGGL(kb_resnorm<T>, dim3(persistent_grid(b, wn.total, kWgRows)), dim3(256), 0, st, wn,
+                       (const T*)p_out, (const T*)b->rhs, b->partial);
+    AJ_LAUNCH_CHECK("kb_resnorm");
+    hipLaunchKernelGGL(kb_resnorm_final, dim3(cdiv(b->ninst, 64)), dim3(64), 0, st,
+                       (const double*)b->partial, b->ninst, (const double*)b->dxdy,
+                       (const double*)b->tol2, b->stats);
+    AJ_LAUNCH_CHECK("kb_resnorm_final");
+  }
+  {
+    ProfScope prof(AJ_PROF_PROJECT, (double)b->cells * sizeof(T) * 6, st);
+    hipLaunchKernelGGL(kb_project<T>, dim3(persistent_grid(b, wrow.total, kWgRows)), dim3(256), 0, st,
+                       wrow, s);
+    AJ_LAUNCH_CHECK("kb_project");
+  }
+  return AJ_OK;
+}
+
+}  // namespace
+
+extern "C" int aj_batch_create(aj_batch** out, int dtype, int ngroups, const int* nx,
+                               const int* ny, const int* count, const double* dx,
+                               const double* dy, int levels) {
+  if (!out || !nx || !ny || !count || !dx || !dy) return fail(AJ_ERR_INVALID, "aj_batch_create: null pointer");
+  *out = nullptr;
+  if (dtype != AJ_F32 && dtype != AJ_F64) return fail(AJ_ERR_INVALID, "aj_batch_create: bad dtype");
+  if (ngroups < 1 || ngroups > kMaxGroups)
+    return fail(AJ_ERR_INVALID, "aj_batch_create: 1..8 groups of equal grids");
+  auto* b = new aj_batch();
+  b->dtype = dtype;
+  b->ng = ngroups;
+  b->levels = levels;
+  int ml_all = -1;
+  for (int g = 0; g < ngroups; ++g) {
+    if (nx[g] < 4 || ny[g] < 4 || count[g] < 1 || !(dx[g] > 0) || !(dy[g] > 0)) {
+      delete b;
+      return fail(AJ_ERR_INVALID, "aj_batch_create: bad group");
+    }
+    // reference hierarchy (multigrid_solver.py:96-106)
+    int ml = 1, tx = nx[g], ty = ny[g];
+    while (tx % 2 == 0 && ty % 2 == 0 && ml < levels) {
+      tx /= 2;
+      ty /= 2;
+      ++ml;
+    }
+    char msg[256];
+    std::snprintf(msg, sizeof(msg),
+                  "Grid dimensions (%d, %d) not suitable for multigrid with %d levels. Grid must "
+                  "be divisible by 2^%d",
+                  nx[g], ny[g], levels, levels);
+    if (ml < 2) {
+      delete b;
+      return fail(AJ_ERR_GRID, msg);
+    }
+    if (ml_all >= 0 && ml != ml_all) {
+      delete b;
+      return fail(AJ_ERR_INVALID, "aj_batch_create: all groups need the same multigrid depth");
+    }
+    ml_all = ml;
+    std::vector<int> sx{nx[g]}, sy{ny[g]};
+    for (int l = 0; l < ml; ++l) {
+      int cx = sx.back() / 2, cy = sy.back() / 2;
+      if (cx < 2 || cy < 2) cx = cy = 1;
+      sx.push_back(cx);
+      sy.push_back(cy);
+    }
+    b->lnx.push_back(sx);
+    b->lny.push_back(sy);
+    b->gnx.push_back(nx[g]);
+    b->gny.push_back(ny[g]);
+    b->gcount.push_back(count[g]);
+    b->gdx.push_back(dx[g]);
+    b->gdy.push_back(dy[g]);
+    b->gfirst.push_back(b->ninst);
+    b->ninst += count[g];
+  }
+  b->ml = ml_all;
+  const int L = b->ml;
+  b->lbase.assign(L + 1, std::vector<long>(ngroups, 0));
+  b->level_len.assign(L + 1, 0);
+  for (int l = 0; l <= L; ++l) {
+    long off = 0;
+    for (int g = 0; g < ngroups; ++g) {
+      b->lbase[l][g] = off;
+      off += (long)b->lnx[g][l] * b->lny[g][l] * b->gcount[g];
+    }
+    b->level_len[l] = off;
+  }
+  b->cells = b->level_len[0];
+  const size_t es = dtype == AJ_F64 ? 8 : 4;
+  size_t bytes = 0;
+  auto take = [&](size_t n) {
+    const size_t o = bytes;
+    bytes += (n + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t o_us = take(b->cells * es), o_vs = take(b->cells * es), o_rhs = take(b->cells * es);
+  std::vector<size_t> oB(L + 1), oY(L + 1), oE(L + 1);
+  for (int l = 0; l <= L; ++l) {
+    oB[l] = take(b->level_len[l] * es);
+    oY[l] = take(b->level_len[l] * es);
+    oE[l] = take(b->level_len[l] * es);
+  }
+  const size_t o_part = take((size_t)b->ninst * kResChunks * 8);
+  const size_t o_dxdy = take((size_t)b->ninst * 8), o_tol2 = take((size_t)b->ninst * 8);
+  const size_t o_nu = take((size_t)b->ninst * 8);
+  const size_t o_st = take((size_t)b->ninst * sizeof(aj_mg_stats));
+  hipError_t e = hipGetDevice(&b->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  if (e == hipSuccess) e = hipMalloc(&b->ws, bytes);
+  if (e != hipSuccess) {
+    delete b;
+    return hip_fail(e, "aj_batch_create");
+  }
+  char* base = (char*)b->ws;
+  b->us = base + o_us;
+  b->vs = base + o_vs;
+  b->rhs = base + o_rhs;
+  b->B.assign(L + 1, nullptr);
+  b->Y.assign(L + 1, nullptr);
+  b->E.assign(L + 1, nullptr);
+  for (int l = 0; l <= L; ++l) {
+    b->B[l] = base + oB[l];
+    b->Y[l] = base + oY[l];
+    b->E[l] = base + oE[l];
+  }
+  b->partial = (double*)(base + o_part);
+  b->dxdy = (double*)(base + o_dxdy);
+  b->tol2 = (double*)(base + o_tol2);
+  b->nu = (double*)(base + o_nu);
+  b->stats = (aj_mg_stats*)(base + o_st);
+  std::vector<double> dd(b->ninst);
+  for (int g = 0; g < ngroups; ++g)
+    for (int k = 0; k < b->gcount[g]; ++k) {
+      const double ddx = dtype == AJ_F64 ? dx[g] : (double)(float)dx[g];
+      const double ddy = dtype == AJ_F64 ? dy[g] : (double)(float)dy[g];
+      dd[b->gfirst[g] + k] = ddx * ddy;
+    }
+  e = hipMemcpy(b->dxdy, dd.data(), dd.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(b->stats, 0, (size_t)b->ninst * sizeof(aj_mg_stats));
+  if (e != hipSuccess) {
+    (void)hipFree(b->ws);
+    delete b;
+    return hip_fail(e, "aj_batch_create");
+  }
+  *out = b;
+  return AJ_OK;
+}
+
+extern "C" int aj_batch_destroy(aj_batch* b) {
+  if (!b) return AJ_OK;
+  for (int k = 0; k < 2; ++k) {
+    if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+  }
+  if (b->ws) (void)hipFree(b->ws);
+  delete b;
+  return AJ_OK;
+}
+
+extern "C" long aj_batch_cells(const aj_batch* b) { return b ? b->cells : 0; }
+extern "C" int aj_batch_num_instances(const aj_batch* b) { return b ? b->ninst : 0; }
+
+extern "C" int aj_batch_offsets(const aj_batch* b, long* offsets) {
+  if (!b || !offsets) return fail(AJ_ERR_INVALID, "aj_batch_offsets: null pointer");
+  for (int g = 0; g < b->ng; ++g)
+    for (int k = 0; k < b->gcount[g]; ++k)
+      offsets[b->gfirst[g] + k] = b->lbase[0][g] + (long)k * b->gnx[g] * b->gny[g];
+  return AJ_OK;
+}
+
+extern "C" int aj_batch_set_viscosity(aj_batch* b, const double* nu_host) {
+  if (!b || !nu_host) return fail(AJ_ERR_INVALID, "aj_batch_set_viscosity: null pointer");
+  AJ_HIP(hipMemcpy(b->nu, nu_host, (size_t)b->ninst * 8, hipMemcpyHostToDevice));
+  return AJ_OK;
+}
+
+extern "C" int aj_batch_set_option(aj_batch* b, int option, int value) {
+  if (!b) return fail(AJ_ERR_INVALID, "aj_batch_set_option: null batch");
+  if (option != AJ_MG_OPT_GRAPH) return fail(AJ_ERR_INVALID, "aj_batch_set_option: unknown option");
+  b->use_graph = value ? 1 : 0;
+  return AJ_OK;
+}
+
+extern "C" int aj_batch_step(aj_batch* b, const void* u, const void* v, const void* mask,
+                             const void* sdf, void* u_out, void* v_out, void* p_out,
+                             const aj_batch_step_params* prm, void* stream) {
+  if (!b || !u || !v || !mask || !u_out || !v_out || !p_out || !prm)
+    return fail(AJ_ERR_INVALID, "aj_batch_step: null pointer");
+  if (u_out == u || v_out == v) return fail(AJ_ERR_INVALID, "aj_batch_step: outputs must not alias inputs");
+  if (!(prm->brinkman_eta > 0) || prm->v_cycles < 1)
+    return fail(AJ_ERR_INVALID, "aj_batch_step: brinkman_eta > 0 and v_cycles >= 1 required");
+  hipStream_t st = (hipStream_t)stream;
+  if (prm->tolerance != b->last_tol) {
+    // tolerance**2 in the working precision (multigrid_solver.py:229-231)
+    std::vector<double> t2(b->ninst);
+    const double tt = b->dtype == AJ_F64 ? prm->tolerance * prm->tolerance
+                                         : (double)((float)prm->tolerance * (float)prm->tolerance);
+    for (auto& x : t2) x = tt;
+    AJ_HIP(hipMemcpyAsync(b->tol2, t2.data(), t2.size() * 8, hipMemcpyHostToDevice, st));
+    AJ_HIP(hipStreamSynchronize(st));
+    b->last_tol = prm->tolerance;
+  }
+  auto launches = [&](hipStream_t s) {
+    return b->dtype == AJ_F32
+               ? batch_step_launches<float>(b, prm, u, v, mask, sdf, u_out, v_out, p_out, s)
+               : batch_step_launches<double>(b, prm, u, v, mask, sdf, u_out, v_out, p_out, s);
+  };
+  if (!b->use_graph || prof_on()) return launches(st);
+  const void* key[8] = {u, v, mask, sdf, u_out, v_out, p_out, nullptr};
+  const double kp[5] = {prm->dt, prm->U_inf, prm->brinkman_eta, (double)prm->v_cycles,
+                        prm->tolerance};
+  int slot = -1;
+  for (int k = 0; k < 2; ++k)
+    if (b->exec[k] && !std::memcmp(key, b->key[k], sizeof(key)) &&
+        !std::memcmp(kp, b->key_prm[k], sizeof(kp)))
+      slot = k;
+  if (slot < 0) {
+    slot = b->victim;
+    b->victim ^= 1;
+    if (b->exec[slot]) {
+      (void)hipGraphExecDestroy(b->exec[slot]);
+      (void)hipGraphDestroy(b->graph[slot]);
+      b->exec[slot] = nullptr;
+      b->graph[slot] = nullptr;
+    }
+    hipStream_t cs;
+    AJ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int rc = AJ_OK;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      rc = launches(cs);
+      hipGraph_t g = nullptr;
+      e = hipStreamEndCapture(cs, &g);
+      if (e == hipSuccess && rc == AJ_OK) {
+        e = hipGraphInstantiate(&b->exec[slot], g, nullptr, nullptr, 0);
+        b->graph[slot] = g;
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc != AJ_OK) return rc;
+    if (e != hipSuccess) return hip_fail(e, "capture of the batched step");
+    std::memcpy(b->key[slot], key, sizeof(key));
+    std::memcpy(b->key_prm[slot], kp, sizeof(kp));
+  }
+  AJ_HIP(hipGraphLaunch(b->exec[slot], st));
+  return AJ_OK;
+}
+
+extern "C" int aj_batch_get_stats(aj_batch* b, aj_mg_stats* host, void* stream) {
+  if (!b || !host) return fail(AJ_ERR_INVALID, "aj_batch_get_stats: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  AJ_HIP(hipMemcpyAsync(host, b->stats, (size_t)b->ninst * sizeof(aj_mg_stats),
+                        hipMemcpyDeviceToHost, st));
+  AJ_HIP(hipStreamSynchronize(st));
+  return AJ_OK;
+}

@@ -12,6 +12,7 @@
 // device and sets the `converged` gate that every later kernel of the solve
 // reads first, so the whole solve is enqueued without a host round trip.
 #include "aj_mg.h"
+#include "aj_mg_ref_kernels.h"
 
 #include <cmath>
 #include <cstring>
@@ -20,116 +21,12 @@ namespace aj {
 
 constexpr int kNPartial = 1024;
 
-template <typename T>
-struct RefJacobiArgs {
-  const T* p_in;  // null -> zero initial guess
-  const T* e_c;   // null -> no prolongation; else coarse correction
-  int ncx, ncy;
-  const T* b;
-  int b_zero_last;  // level 0: b[-1,:] = 0
-  T* p_out;
-  int nx, ny;
-  T ax, ay, inv_diag;
-  int clamp_last;
-  const aj_mg_stats* gate;
-};
-
-__device__ __forceinline__ bool gated(const aj_mg_stats* g) {
-  return g != nullptr && *(volatile const int*)&g->converged != 0;
-}
-
-// prolong (multigrid_solver.py:134-156) evaluated at fine cell (i, j)
-template <typename T>
-__device__ __forceinline__ T prolong_ref(const T* __restrict__ c, int ncx, int ncy, int i, int j) {
-  const int nxf = 2 * ncx, nyf = 2 * ncy;
-  if (i == nxf - 1 || j == nyf - 1) return T(0);
-  const int I = i >> 1, J = j >> 1;
-  const bool io = i & 1, jo = j & 1;
-  const T c00 = c[(long)I * ncy + J];
-  if (!io && !jo) return c00;
-  if (io && !jo) return T(0.5) * (c00 + c[(long)(I + 1) * ncy + J]);
-  if (!io && jo) return T(0.5) * (c00 + c[(long)I * ncy + J + 1]);
-  return T(0.25) * (((c00 + c[(long)(I + 1) * ncy + J]) + c[(long)I * ncy + J + 1]) +
-                    c[(long)(I + 1) * ncy + J + 1]);
-}
-
-// NS Jacobi sweeps (smooth, multigrid_solver.py:158-171) on one tile.
 template <typename T, int TX, int TY, int NS>
 __global__ __launch_bounds__(256) void k_ref_jacobi(RefJacobiArgs<T> a) {
-  constexpr int RH = TX + 2 * NS, RW = TY + 2 * NS, RN = RH * RW;
-  __shared__ T s_p[2][RN];
-  __shared__ T s_b[RN];
-  const int nx = a.nx, ny = a.ny;
-  const int gx0 = blockIdx.y * TX - NS, gy0 = blockIdx.x * TY - NS;
-  const int tid = threadIdx.x;
-  {
-    // all loads of a thread in flight together (clamped, always in bounds)
-    constexpr int NIT = (RN + 255) / 256;
-    T rp[NIT], rb[NIT];
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = min(tid + it * 256, RN - 1);
-      const int lx = idx / RW, ly = idx - lx * RW;
-      const int gx = clampi(gx0 + lx, 0, nx - 1), gy = clampi(gy0 + ly, 0, ny - 1);
-      const long g = (long)gx * ny + gy;
-      T pv = a.p_in ? a.p_in[g] : T(0);
-      if (a.e_c) pv = pv + prolong_ref(a.e_c, a.ncx, a.ncy, gx, gy);
-      rp[it] = pv;
-      rb[it] = (a.b_zero_last && gx == nx - 1) ? T(0) : a.b[g];
-    }
-    if (gated(a.gate)) return;
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = tid + it * 256;
-      if (idx < RN) {
-        s_p[0][idx] = rp[it];
-        s_b[idx] = rb[it];
-      }
-    }
-  }
-  __syncthreads();
-  int cur = 0;
-#pragma unroll
-  for (int s = 1; s <= NS; ++s) {
-    const int m = NS - s;
-    const int W = TY + 2 * m, Hh = TX + 2 * m;
-    const T* src = s_p[cur];
-    T* dst = s_p[cur ^ 1];
-    for (int idx = tid; idx < W * Hh; idx += 256) {
-      const int lx = NS - m + idx / W, ly = NS - m + idx % W;
-      const int gx = gx0 + lx, gy = gy0 + ly;
-      if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-      const int l = lx * RW + ly;
-      const T pW = src[gx > 0 ? l - RW : l], pE = src[gx < nx - 1 ? l + RW : l];
-      const T pS = src[gy > 0 ? l - 1 : l], pN = src[gy < ny - 1 ? l + 1 : l];
-      dst[l] = (a.ax * (pE + pW) + a.ay * (pN + pS) - s_b[l]) * a.inv_diag;
-    }
-    __syncthreads();
-    cur ^= 1;
-  }
-  for (int idx = tid; idx < TX * TY; idx += 256) {
-    const int lx = NS + idx / TY, ly = NS + idx % TY;
-    const int gx = gx0 + lx, gy = gy0 + ly;
-    if (gx >= nx || gy >= ny) continue;
-    const T val = (a.clamp_last && gx == nx - 1) ? T(0) : s_p[cur][lx * RW + ly];
-    a.p_out[(long)gx * ny + gy] = val;
-  }
+  __shared__ JacobiLds<T, TX, TY, NS> L;
+  ref_jacobi_tile<T, TX, TY, NS>(a, blockIdx.y * TX, blockIdx.x * TY, L);
 }
 
-// b - apply_laplacian(p) at (i, j) (multigrid_solver.py:173-183, 198)
-template <typename T>
-__device__ __forceinline__ T ref_residual(const T* __restrict__ p, const T* __restrict__ b,
-                                          int bzl, int nx, int ny, T ax, T ay, int i, int j) {
-  const long g = (long)i * ny + j;
-  const T pc = p[g];
-  const T pW = i > 0 ? p[g - ny] : pc, pE = i < nx - 1 ? p[g + ny] : pc;
-  const T pS = j > 0 ? p[g - 1] : pc, pN = j < ny - 1 ? p[g + 1] : pc;
-  const T lap = ax * (pE + pW - T(2) * pc) + ay * (pN + pS - T(2) * pc);
-  const T bv = (bzl && i == nx - 1) ? T(0) : b[g];
-  return bv - lap;
-}
-
-// residual + restrict (multigrid_solver.py:108-132, 198-199)
 template <typename T>
 __global__ __launch_bounds__(256) void k_ref_resrestrict(const T* __restrict__ p,
                                                          const T* __restrict__ b, int bzl, int nx,
@@ -140,21 +37,8 @@ __global__ __launch_bounds__(256) void k_ref_resrestrict(const T* __restrict__ p
   const int J = blockIdx.x * blockDim.x + threadIdx.x;
   const int I = blockIdx.y;
   if (J >= ncy || I >= ncx) return;
-  if (degenerate) {  // coarse side < 2: fine[:1, :1]
-    bc[0] = ref_residual(p, b, bzl, nx, ny, ax, ay, 0, 0);
-    return;
-  }
-  const int i = 2 * I, j = 2 * J;
-  const T r00 = ref_residual(p, b, bzl, nx, ny, ax, ay, i, j);
-  const T r10 = ref_residual(p, b, bzl, nx, ny, ax, ay, i + 1, j);
-  const T r01 = ref_residual(p, b, bzl, nx, ny, ax, ay, i, j + 1);
-  const T r11 = ref_residual(p, b, bzl, nx, ny, ax, ay, i + 1, j + 1);
-  bc[(long)I * ncy + J] = T(0.25) * (((r00 + r10) + r01) + r11);
+  ref_resrestrict_cell(p, b, bzl, nx, ny, ax, ay, bc, ncy, degenerate, I, J);
 }
-
-// coarsest level: NSW Jacobi sweeps from zero in one workgroup
-constexpr int kCoarseThreads = 1024;
-constexpr int kCoarsePerThread = 8;
 
 template <typename T>
 __global__ __launch_bounds__(kCoarseThreads) void k_ref_coarse(const T* __restrict__ b,
@@ -163,39 +47,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_ref_coarse(const T* __restri
                                                                const aj_mg_stats* gate) {
   if (gated(gate)) return;
   extern __shared__ unsigned char smem_raw[];
-  T* s0 = reinterpret_cast<T*>(smem_raw);
-  const int n = nx * ny;
-  T* s1 = s0 + n;
-  T breg[kCoarsePerThread];
-#pragma unroll
-  for (int k = 0; k < kCoarsePerThread; ++k) {
-    const int c = threadIdx.x + k * kCoarseThreads;
-    breg[k] = c < n ? b[c] : T(0);
-    if (c < n) s0[c] = T(0);
-  }
-  __syncthreads();
-  T* src = s0;
-  T* dst = s1;
-  for (int s = 0; s < nsw; ++s) {
-#pragma unroll
-    for (int k = 0; k < kCoarsePerThread; ++k) {
-      const int c = threadIdx.x + k * kCoarseThreads;
-      if (c < n) {
-        const int i = c / ny, j = c - i * ny;
-        const T pc = src[c];
-        const T pW = i > 0 ? src[c - ny] : pc, pE = i < nx - 1 ? src[c + ny] : pc;
-        const T pS = j > 0 ? src[c - 1] : pc, pN = j < ny - 1 ? src[c + 1] : pc;
-        dst[c] = (ax * (pE + pW) + ay * (pN + pS) - breg[k]) * inv_diag;
-      }
-    }
-    __syncthreads();
-    T* t = src; src = dst; dst = t;
-  }
-#pragma unroll
-  for (int k = 0; k < kCoarsePerThread; ++k) {
-    const int c = threadIdx.x + k * kCoarseThreads;
-    if (c < n) out[c] = src[c];
-  }
+  ref_coarse_wg(b, out, nx, ny, ax, ay, inv_diag, nsw, reinterpret_cast<T*>(smem_raw));
 }
 
 // sum(r^2) partials (multigrid_solver.py:229-230), FP64 accumulation

@@ -8,267 +8,17 @@
 // HBM sees u, v, chi-source, mask read once and u*, v* written once.
 // Neighbour reads clamp the *global* index at the domain edge, which is the
 // reference's jnp.pad(mode='edge') applied to every stage field.
-#include "aj_common.h"
+#include "aj_step_kernels.h"
 
 #include <algorithm>
 
 namespace aj {
 
-// --------------------------------------------------------------------------
-// RK predictor (advection_schemes/rk3_simple_new.py:175-358)
-// --------------------------------------------------------------------------
-
-template <typename T>
-struct RkArgs {
-  const T* u;
-  const T* v;
-  const T* mask;
-  const T* sdf;     // may be null -> chi = 1 - mask
-  const T* nu_sgs;  // may be null
-  const T* nu_field;  // may be null: per-cell viscosity replacing nu
-  T* u_out;
-  T* v_out;
-  int nx, ny;
-  T dt, nu, inv_dx, inv_dy, inv_dx2, inv_dy2, U_inf, inv_eta, sdf_scale;
-  T hyper;  // nu_hyper_ratio * dx * dy (the biharmonic coefficient is nu * hyper)
-  int slip;
-};
-
-// One stencil evaluation of rhs = -upwind(f; u, v) + nu_tot * lap(f)
-// [- nu_h * lap(lap f)] for f = u and f = v at LDS cell l.
-template <typename T, bool HYPER>
-__device__ __forceinline__ void rk_rhs(const T* fu, const T* fv, const T* lu, const T* lv,
-                                       int l, int lW, int lE, int lS, int lN, T nu_tot,
-                                       T nu_h, const RkArgs<T>& a, T& ru, T& rv) {
-  const T uc = fu[l], vc = fv[l];
-  const T uW = fu[lW], uE = fu[lE], uS = fu[lS], uN = fu[lN];
-  const T vW = fv[lW], vE = fv[lE], vS = fv[lS], vN = fv[lN];
-  // upwind_advection (rk3_simple_new.py:287-298): vel > 0 -> backward diff
-  T ax_u = (uc > T(0)) ? uc * (uc - uW) * a.inv_dx : uc * (uE - uc) * a.inv_dx;
-  T ay_u = (vc > T(0)) ? vc * (uc - uS) * a.inv_dy : vc * (uN - uc) * a.inv_dy;
-  T ax_v = (uc > T(0)) ? uc * (vc - vW) * a.inv_dx : uc * (vE - vc) * a.inv_dx;
-  T ay_v = (vc > T(0)) ? vc * (vc - vS) * a.inv_dy : vc * (vN - vc) * a.inv_dy;
-  // laplacian (rk3_simple_new.py:215-219)
-  T lap_u = (uE + uW - T(2) * uc) * a.inv_dx2 + (uN + uS - T(2) * uc) * a.inv_dy2;
-  T lap_v = (vE + vW - T(2) * vc) * a.inv_dx2 + (vN + vS - T(2) * vc) * a.inv_dy2;
-  ru = -(ax_u + ay_u) + nu_tot * lap_u;
-  rv = -(ax_v + ay_v) + nu_tot * lap_v;
-  if constexpr (HYPER) {
-    const T bu = (lu[lE] + lu[lW] - T(2) * lu[l]) * a.inv_dx2 +
-                 (lu[lN] + lu[lS] - T(2) * lu[l]) * a.inv_dy2;
-    const T bv = (lv[lE] + lv[lW] - T(2) * lv[l]) * a.inv_dx2 +
-                 (lv[lN] + lv[lS] - T(2) * lv[l]) * a.inv_dy2;
-    ru = ru - nu_h * bu;
-    rv = rv - nu_h * bv;
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ T masked(T val, T m) {
-  return (m > T(0.5)) ? val : T(0);
-}
-
-// Fused RK kernel.  FAST = RK2 (2 stages), else RK3.  HYPER adds the
-// biharmonic term (per-stage stencil radius 2).
 template <typename T, int TX, int TY, bool FAST, bool HYPER, int NTH>
 __global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
-  constexpr int S = FAST ? 2 : 3;
-  constexpr int R1 = HYPER ? 2 : 1;  // per-stage stencil radius
-  constexpr int H = S * R1;          // total halo
-  constexpr int RH = TX + 2 * H;     // region rows (x)
-  constexpr int RW = TY + 2 * H;     // region cols (y, contiguous)
-  constexpr int RN = RH * RW;
-
   extern __shared__ unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
-  T* s_u = sm;              // u (margin H)
-  T* s_v = s_u + RN;
-  T* s_iD = s_v + RN;       // 1/(1 + dt chi/eta) (margin H - R1)
-  T* s_nt = s_iD + RN;      // nu + nu_sgs        (margin H - R1)
-  T* s_au = s_nt + RN;      // stage buffers
-  T* s_av = s_au + RN;
-  T* s_bu = s_av + RN;
-  T* s_bv = s_bu + RN;      // RK3: stage-2 output; RK2: k1
-  T* s_lu = s_bv + RN;      // HYPER only: laplacians of the stage input
-  T* s_lv = s_lu + RN;
-  T* s_nh = s_lv + RN;      // HYPER only: biharmonic coefficient nu * hyper
-
-  const int nx = a.nx, ny = a.ny;
-  const int y0 = blockIdx.x * TY;
-  const int x0 = blockIdx.y * TX;
-  const int gx0 = x0 - H, gy0 = y0 - H;  // global index of LDS (0, 0)
-  const int tid = threadIdx.x;
-  const bool has_nsgs = a.nu_sgs != nullptr;
-  const bool has_nuf = a.nu_field != nullptr;
-  const bool has_sdf = a.sdf != nullptr;
-
-  // mask of this thread's tile cells (final stage), loaded up front
-  constexpr int NT = (TX * TY + NTH - 1) / NTH;
-  T rmask[NT];
-#pragma unroll
-  for (int it = 0; it < NT; ++it) {
-    const int idx = min(tid + it * NTH, TX * TY - 1);
-    const int gx = clampi(x0 + idx / TY, 0, nx - 1), gy = clampi(y0 + idx % TY, 0, ny - 1);
-    rmask[it] = a.mask[(long)gx * ny + gy];
-  }
-  // ---- load u, v on margin H; chi factor and nu_tot on margin H - R1 ----
-  // All loads of a thread are issued before any is consumed (addresses are
-  // clamped into the domain, so every load is in bounds; slots outside the
-  // domain hold edge copies and are never read by the stencils).
-  {
-    constexpr int NIT = (RN + NTH - 1) / NTH;
-    T ru[NIT], rv[NIT], rc[NIT], rn[NIT], rf[NIT];
-    const T* csrc = has_sdf ? a.sdf : a.mask;
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = min(tid + it * NTH, RN - 1);
-      const int lx = idx / RW, ly = idx - lx * RW;
-      const long g = (long)clampi(gx0 + lx, 0, nx - 1) * ny + clampi(gy0 + ly, 0, ny - 1);
-      ru[it] = a.u[g];
-      rv[it] = a.v[g];
-      rc[it] = csrc[g];
-      rn[it] = has_nsgs ? a.nu_sgs[g] : T(0);
-      rf[it] = has_nuf ? a.nu_field[g] : a.nu;
-    }
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = tid + it * NTH;
-      if (idx >= RN) break;
-      s_u[idx] = ru[it];
-      s_v[idx] = rv[it];
-      // sigmoid(-sdf / (0.1 dx)) (rk3_simple_new.py:198), else 1 - mask
-      const T chi = has_sdf ? T(1) / (T(1) + exp(rc[it] * a.sdf_scale)) : T(1) - rc[it];
-      s_iD[idx] = T(1) / (T(1) + a.dt * chi * a.inv_eta);
-      // nu_total = nu (+ nu_sgs) with nu a scalar or a field
-      // (rk3_simple_new.py:304-309)
-      s_nt[idx] = has_nsgs ? rf[it] + rn[it] : rf[it];
-      if constexpr (HYPER) s_nh[idx] = rf[it] * a.hyper;
-    }
-  }
-  __syncthreads();
-
-  const T* fu = s_u;
-  const T* fv = s_v;
-#pragma unroll
-  for (int s = 1; s <= S; ++s) {
-    const int m = (S - s) * R1;  // output margin of this stage
-    if constexpr (HYPER) {
-      // laplacian of the stage input on margin m + 1
-      const int mm = m + 1;
-      const int W = TY + 2 * mm, Hh = TX + 2 * mm;
-      for (int idx = tid; idx < W * Hh; idx += NTH) {
-        const int lx = H - mm + idx / W, ly = H - mm + idx % W;
-        const int gx = gx0 + lx, gy = gy0 + ly;
-        if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-        const int l = lx * RW + ly;
-        const int lW = gx > 0 ? l - RW : l, lE = gx < nx - 1 ? l + RW : l;
-        const int lS = gy > 0 ? l - 1 : l, lN = gy < ny - 1 ? l + 1 : l;
-        s_lu[l] = (fu[lE] + fu[lW] - T(2) * fu[l]) * a.inv_dx2 +
-                  (fu[lN] + fu[lS] - T(2) * fu[l]) * a.inv_dy2;
-        s_lv[l] = (fv[lE] + fv[lW] - T(2) * fv[l]) * a.inv_dx2 +
-                  (fv[lN] + fv[lS] - T(2) * fv[l]) * a.inv_dy2;
-      }
-      __syncthreads();
-    }
-    T* ou = (s == 1) ? s_au : s_bu;
-    T* ov = (s == 1) ? s_av : s_bv;
-    // stage value at LDS cell l (RK3: rk3_simple_new.py:315-331; RK2: :339-347)
-    auto stage = [&](int l, int gx, int gy, T& nu_, T& nv_) {
-      const int lW = gx > 0 ? l - RW : l, lE = gx < nx - 1 ? l + RW : l;
-      const int lS = gy > 0 ? l - 1 : l, lN = gy < ny - 1 ? l + 1 : l;
-      T ku, kv;
-      rk_rhs<T, HYPER>(fu, fv, s_lu, s_lv, l, lW, lE, lS, lN, s_nt[l], HYPER ? s_nh[l] : T(0), a,
-                       ku, kv);
-      const T u0 = s_u[l], v0 = s_v[l], iD = s_iD[l];
-      if constexpr (FAST) {
-        if (s == 1) {
-          nu_ = (u0 + a.dt * ku) * iD;
-          nv_ = (v0 + a.dt * kv) * iD;
-          // k1 is needed again at the final stage (margin 0 within margin R1)
-          s_bu[l] = ku;
-          s_bv[l] = kv;
-        } else {
-          const T k1u = s_bu[l], k1v = s_bv[l];
-          nu_ = (u0 + T(0.5) * a.dt * (k1u + ku)) * iD;
-          nv_ = (v0 + T(0.5) * a.dt * (k1v + kv)) * iD;
-        }
-      } else {
-        if (s == 1) {
-          nu_ = (u0 + a.dt * ku) * iD;
-          nv_ = (v0 + a.dt * kv) * iD;
-        } else {
-          const T us = u0 + a.dt * ku, vs = v0 + a.dt * kv;
-          if (s == 2) {
-            nu_ = (T(0.75) * u0 + T(0.25) * us) * iD;
-            nv_ = (T(0.75) * v0 + T(0.25) * vs) * iD;
-          } else {
-            nu_ = (T(1.0 / 3.0) * u0 + T(2.0 / 3.0) * us) * iD;
-            nv_ = (T(1.0 / 3.0) * v0 + T(2.0 / 3.0) * vs) * iD;
-          }
-        }
-      }
-    };
-    if (s < S) {
-      const int W = TY + 2 * m, Hh = TX + 2 * m;
-      for (int idx = tid; idx < W * Hh; idx += NTH) {
-        const int lx = H - m + idx / W, ly = H - m + idx % W;
-        const int gx = gx0 + lx, gy = gy0 + ly;
-        if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-        const int l = lx * RW + ly;
-        T nu_, nv_;
-        stage(l, gx, gy, nu_, nv_);
-        ou[l] = nu_;
-        ov[l] = nv_;
-      }
-      __syncthreads();
-      fu = ou;
-      fv = ov;
-      continue;
-    }
-    // ---- final stage on the tile: hard mask, then BCs
-    // (rk3_simple_new.py:332-336, 264-279).  Boundary cells copy the masked
-    // value of an interior source; each source writes its own targets so
-    // every boundary cell is written exactly once.
-#pragma unroll
-    for (int it = 0; it < NT; ++it) {
-      const int idx = tid + it * NTH;
-      if (idx >= TX * TY) break;
-      const int lx = H + idx / TY, ly = H + idx % TY;
-      const int gx = gx0 + lx, gy = gy0 + ly;
-      if (gx >= nx || gy >= ny) continue;
-      const long g = (long)gx * ny + gy;
-      if (gx == 0) {
-        const bool wall = (gy == 0 || gy == ny - 1);
-        a.u_out[g] = (wall && !a.slip) ? T(0) : a.U_inf;
-        a.v_out[g] = T(0);
-        continue;
-      }
-      const bool interior = gx < nx - 1 && gy > 0 && gy < ny - 1;
-      if (!interior) continue;
-      T nu_, nv_;
-      stage(lx * RW + ly, gx, gy, nu_, nv_);
-      const T m0 = rmask[it];
-      const T fu_ = masked(nu_, m0), fv_ = masked(nv_, m0);
-      a.u_out[g] = fu_;
-      a.v_out[g] = fv_;
-      const T uw = a.slip ? fu_ : T(0);
-      const bool out_row = (gx == nx - 2);
-      if (out_row) {
-        a.u_out[g + ny] = fu_;
-        a.v_out[g + ny] = fv_;
-      }
-      if (gy == 1) {
-        a.u_out[g - 1] = uw;
-        a.v_out[g - 1] = T(0);
-        if (out_row) { a.u_out[g + ny - 1] = uw; a.v_out[g + ny - 1] = T(0); }
-      }
-      if (gy == ny - 2) {
-        a.u_out[g + 1] = uw;
-        a.v_out[g + 1] = T(0);
-        if (out_row) { a.u_out[g + ny + 1] = uw; a.v_out[g + ny + 1] = T(0); }
-      }
-    }
-  }
+  rk_tile<T, TX, TY, FAST, HYPER, NTH>(a, blockIdx.y * TX, blockIdx.x * TY,
+                                       reinterpret_cast<T*>(smem_raw));
 }
 
 template <typename T, int TX, int TY, bool FAST, bool HYPER>
@@ -368,13 +118,7 @@ __global__ __launch_bounds__(256) void k_div_rhs(const T* __restrict__ us, const
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= ny) return;
-  const long g = (long)i * ny + j;
-  T r = T(0);
-  if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
-    const T d = (us[g + ny] - us[g - ny]) * inv_2dx + (vs[g + 1] - vs[g - 1]) * inv_2dy;
-    r = mask ? d * mask[g] * inv_dt : d * inv_dt;
-  }
-  rhs[g] = r;
+  div_rhs_cell(us, vs, mask, rhs, nx, ny, i, j, inv_2dx, inv_2dy, inv_dt);
 }
 
 // --------------------------------------------------------------------------
@@ -390,48 +134,7 @@ __global__ __launch_bounds__(256) void k_project(const T* __restrict__ us, const
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= ny) return;
-  const long g = (long)i * ny + j;
-  if (i == 0) {
-    // inlet: u = U_inf (walls copy it), v = 0, then the final mask
-    const T m0 = mask[g];
-    uo[g] = masked(U_inf, m0);
-    vo[g] = T(0);
-    return;
-  }
-  if (!(i < nx - 1 && j > 0 && j < ny - 1)) return;  // written by a source
-  const T gx0 = (p[g + ny] - p[g - ny]) * inv_2dx;
-  const T gy0 = (p[g + 1] - p[g - 1]) * inv_2dy;
-  const T mx = (mask[g + ny] - mask[g - ny]) * inv_2dx;
-  const T my = (mask[g + 1] - mask[g - 1]) * inv_2dy;
-  const T gm = sqrt(mx * mx + my * my);
-  T gx = gx0, gy = gy0;
-  if (gm > iface_thr) {
-    const T nxv = mx / (gm + T(1e-8));
-    const T nyv = my / (gm + T(1e-8));
-    const T dpn = gx0 * nxv + gy0 * nyv;
-    gx = gx0 - dpn * nxv;
-    gy = gy0 - dpn * nyv;
-  }
-  const T m0 = mask[g];
-  const T un = masked(us[g] - dt * gx, m0);
-  const T vn = masked(vs[g] - dt * gy, m0);
-  uo[g] = un;  // second masking is idempotent on the source cell
-  vo[g] = vn;
-  const bool out_row = (i == nx - 2);
-  if (out_row) {
-    uo[g + ny] = masked(un, mask[g + ny]);
-    vo[g + ny] = masked(vn, mask[g + ny]);
-  }
-  if (j == 1) {
-    uo[g - 1] = masked(un, mask[g - 1]);
-    vo[g - 1] = T(0);
-    if (out_row) { uo[g + ny - 1] = masked(un, mask[g + ny - 1]); vo[g + ny - 1] = T(0); }
-  }
-  if (j == ny - 2) {
-    uo[g + 1] = masked(un, mask[g + 1]);
-    vo[g + 1] = T(0);
-    if (out_row) { uo[g + ny + 1] = masked(un, mask[g + ny + 1]); vo[g + ny + 1] = T(0); }
-  }
+  project_cell(us, vs, p, mask, uo, vo, nx, ny, i, j, dt, inv_2dx, inv_2dy, iface_thr, U_inf);
 }
 
 static dim3 row_grid(int nx, int ny) { return dim3(cdiv(ny, 256), nx); }

@@ -75,3 +75,30 @@ def warm_start_case(seed=0, nx=2048, ny=512, noise_sigma=1e-3):
     return dict(name=f"warmstart_{nx}x{ny}_seed{seed}", desc=desc, grid=grid,
                 sdf=np.nan_to_num(geo.sdf, nan=0.0), mask=geo.mask, u=u, v=v,
                 nu=G.viscosity(1.0, geo.L_char, Re), Re=Re, dt=2.5e-4, U_inf=1.0, tol=1e-6)
+
+
+def batched_sweep_case(n=256, nx=512, ny=128, seed=0, noise_sigma=1e-3):
+    """``n`` independent 512x128 channels (4x1, dx = dy = 1/128), NACA 0012
+    per instance with AoA ~ U[-15, 15] deg, chord ~ U[0.3, 0.6] (leading edge
+    at x = 1, mid-height), Re ~ U[500, 5000], mask eps 0.05, seeded inflow
+    noise; dt = 1e-3 (advective CFL 0.128), reference multigrid with 3
+    V-cycles per step."""
+    rng = np.random.default_rng(seed)
+    grid = G.GridParams(nx, ny, 4.0, 1.0)
+    masks, sdfs, nus, us, vs, desc = [], [], [], [], [], []
+    for k in range(n):
+        aoa = rng.uniform(-15.0, 15.0)
+        chord = rng.uniform(0.3, 0.6)
+        Re = rng.uniform(500.0, 5000.0)
+        geo = G.naca_geometry(grid, "0012", chord=chord, aoa_deg=aoa, x_le=1.0,
+                              y_le=grid.ly / 2, eps=0.05)
+        u, v = G.von_karman_ic(nx, ny, 1.0, noise_sigma, seed * 100003 + k)
+        masks.append(geo.mask)
+        sdfs.append(np.nan_to_num(geo.sdf, nan=0.0))
+        nus.append(G.viscosity(1.0, geo.L_char, Re))
+        us.append(u)
+        vs.append(v)
+        desc.append((aoa, chord, Re))
+    return dict(name=f"batched_{n}x{nx}x{ny}_seed{seed}", grid=grid, masks=masks, sdfs=sdfs,
+                nus=nus, u=us, v=vs, params=desc, dt=1e-3, U_inf=1.0, v_cycles=3,
+                brinkman_eta=0.005)

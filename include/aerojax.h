@@ -216,6 +216,44 @@ int aj_poisson_mg(aj_mg_plan* plan, const void* rhs, const void* p0,
  * (synchronises `stream`). */
 int aj_mg_get_stats(aj_mg_plan* plan, aj_mg_stats* out, void* stream);
 
+/* ---- Batched stepping ---------------------------------------------------
+ * Many independent simulations of the reference step (step_pure's
+ * composition: aj_rk_step + aj_divergence_rhs + reference aj_poisson_mg +
+ * aj_project, solver/pure_functions/step.py:49-467) advance together.
+ * Instances are grouped by grid shape (1..8 groups; group g has count[g]
+ * grids of nx[g] x ny[g] with spacing dx[g], dy[g]); every field is one
+ * buffer of aj_batch_cells() elements in which instance k starts at the
+ * offset aj_batch_offsets() reports (groups in order, instances in order).
+ * Each launch runs persistent workgroups over per-group descriptors; the
+ * multigrid gate (sum r^2 dx dy < tol^2) is per instance; the step's launch
+ * sequence is captured once into a hipGraph per set of buffers
+ * (AJ_MG_OPT_GRAPH via aj_batch_set_option).  All groups need the same
+ * multigrid depth.  No counterpart in the reference (which steps one grid
+ * per call); the per-instance arithmetic is that of the single-grid entry
+ * points. */
+typedef struct aj_batch aj_batch;
+typedef struct aj_batch_step_params {
+  double dt, U_inf, brinkman_eta, tolerance;
+  int v_cycles;
+  int reserved;
+} aj_batch_step_params;
+int aj_batch_create(aj_batch** batch, int dtype, int ngroups, const int* nx, const int* ny,
+                    const int* count, const double* dx, const double* dy, int levels);
+int aj_batch_destroy(aj_batch* batch);
+long aj_batch_cells(const aj_batch* batch);
+int aj_batch_num_instances(const aj_batch* batch);
+int aj_batch_offsets(const aj_batch* batch, long* offsets);
+/* per-instance kinematic viscosity (host array of aj_batch_num_instances) */
+int aj_batch_set_viscosity(aj_batch* batch, const double* nu);
+int aj_batch_set_option(aj_batch* batch, int option, int value);
+/* u, v, mask, sdf (may be NULL: chi = 1 - mask) in; u_out, v_out, p_out out */
+int aj_batch_step(aj_batch* batch, const void* u, const void* v, const void* mask,
+                  const void* sdf, void* u_out, void* v_out, void* p_out,
+                  const aj_batch_step_params* prm, void* stream);
+/* per-instance multigrid statistics of the last step (synchronises) */
+int aj_batch_get_stats(aj_batch* batch, aj_mg_stats* stats, void* stream);
+
+
 /* ---- Live kernel timing -------------------------------------------------
  * No counterpart in the reference (its profiling is wall-clock around the
  * jitted step, viewer/simulation_controller.py:383-408).  Between
