@@ -46,6 +46,7 @@ struct aj_mg_plan {
   int last_outer = 4;               // textbook: iterations the previous solve needed
   int use_graph = 1;                // textbook: replay a captured outer iteration
   int guess_test = 1;               // textbook: keep p0 only if ||r(p0)|| <= ||b||
+  int relative = 0;                 // textbook: tolerance relative to ||b||
   void* graph[2] = {nullptr, nullptr};       // hipGraph_t of an even / odd outer iteration
   void* graph_exec[2] = {nullptr, nullptr};  // hipGraphExec_t
   double* dtol = nullptr;           // textbook: tolerance of the current solve

@@ -329,6 +329,7 @@ extern "C" int aj_mg_set_option(aj_mg_plan* p, int option, int value) {
     case AJ_MG_OPT_ADAPTIVE: p->adaptive = value ? 1 : 0; return AJ_OK;
     case AJ_MG_OPT_GRAPH: p->use_graph = value ? 1 : 0; return AJ_OK;
     case AJ_MG_OPT_GUESS_TEST: p->guess_test = value ? 1 : 0; return AJ_OK;
+    case AJ_MG_OPT_RELATIVE: p->relative = value ? 1 : 0; return AJ_OK;
     default: return fail(AJ_ERR_INVALID, "aj_mg_set_option: unknown option");
   }
 }

@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_tb_resid_guess(const double* __restrict
 // (test = 0: keep any finite guess)
 __global__ void k_tb_guess_final(const double* __restrict__ part_g,
                                  const double* __restrict__ part_z, int np, double dxdy,
-                                 int test, const double* __restrict__ dtol, aj_mg_stats* st,
+                                 int test, int rel, double* __restrict__ dtol, aj_mg_stats* st,
                                  int* state) {
   __shared__ double sh[16];
   double ag = 0.0, az = 0.0;
@@ -371,6 +371,7 @@ __global__ void k_tb_guess_final(const double* __restrict__ part_g,
     st->norm = nrm;
     // a rejected (e.g. NaN) guess does not make the solve non-finite
     if (!isfinite(nrm)) st->nonfinite = 1;
+    if (rel) dtol[0] *= nz;  // relative to ||b||
     st->converged = (nrm <= dtol[0]) ? 1 : 0;
   }
 }
@@ -390,8 +391,10 @@ __global__ __launch_bounds__(256) void k_tb_fallback(double* __restrict__ P,
   r32[c] = (i < nx - 1) ? (float)b[c] : 0.0f;
 }
 
+// flip = 0: initial residual (with `rel`, the tolerance becomes relative
+// to it); flip = 1: after an outer iteration
 __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, double dxdy,
-                                const double* __restrict__ dtol, int flip, int cycles_added,
+                                double* __restrict__ dtol, int flip, int cycles_added, int rel,
                                 aj_mg_stats* st, int* state) {
   __shared__ double sh[16];
   const int skipped = st->converged;  // this outer iteration was skipped
@@ -412,6 +415,7 @@ __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, doub
     }
     st->norm = nrm;
     if (!isfinite(nrm)) st->nonfinite = 1;
+    if (rel) dtol[0] *= nrm;
     st->converged = (nrm <= dtol[0]) ? 1 : 0;
   }
 }
@@ -684,7 +688,7 @@ static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
                      p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_resid");
   hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
-                     p->dx * p->dy, (const double*)p->dtol, 1, kCyclesPerOuter, p->stats, p->state);
+                     p->dx * p->dy, p->dtol, 1, kCyclesPerOuter, 0, p->stats, p->state);
   AJ_LAUNCH_CHECK("k_tb_norm_final");
   return AJ_OK;
 }
@@ -738,7 +742,7 @@ static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int ma
     AJ_LAUNCH_CHECK("k_tb_resid_guess");
     hipLaunchKernelGGL(k_tb_guess_final, dim3(1), dim3(256), 0, st, (const double*)p->partial,
                        (const double*)p->partial2, nb, p->dx * p->dy, p->guess_test,
-                       (const double*)p->dtol, p->stats, p->state);
+                       p->relative, p->dtol, p->stats, p->state);
     AJ_LAUNCH_CHECK("k_tb_guess_final");
     hipLaunchKernelGGL(k_tb_fallback<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p->p64[0],
                        (const T*)p->bcopy, p->tB[0], (const int*)p->state, nx, ny);
@@ -749,7 +753,7 @@ static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int ma
                        p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
     AJ_LAUNCH_CHECK("k_tb_resid");
     hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
-                       p->dx * p->dy, (const double*)p->dtol, 0, 0, p->stats, p->state);
+                       p->dx * p->dy, p->dtol, 0, 0, p->relative, p->stats, p->state);
     AJ_LAUNCH_CHECK("k_tb_norm_final");
   }
   const int max_outer = (max_cycles + kCyclesPerOuter - 1) / kCyclesPerOuter;

@@ -51,14 +51,18 @@ class MultigridPlan:
         return int(_lib.load().aj_mg_workspace_bytes(self.handle))
 
     def set_option(self, adaptive: bool | None = None, graph: bool | None = None,
-                   guess_test: bool | None = None):
+                   guess_test: bool | None = None, relative: bool | None = None):
         """Textbook plans: ``adaptive`` reads convergence between outer
         iterations (host sync, default) instead of enqueueing all
         ``max_cycles``; ``graph`` replays a captured outer iteration
         (default) instead of launching its kernels one by one;
         ``guess_test`` keeps an initial guess only if its residual is not
-        above the zero guess's (default) instead of keeping any finite one."""
+        above the zero guess's (default) instead of keeping any finite one;
+        ``relative`` makes the tolerance relative to ||b||."""
         lib = _lib.load()
+        if relative is not None:
+            _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_RELATIVE, int(relative)),
+                       "aj_mg_set_option")
         if guess_test is not None:
             _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_GUESS_TEST,
                                             int(guess_test)), "aj_mg_set_option")

@@ -166,8 +166,12 @@ int aj_mg_num_levels(const aj_mg_plan* plan);
  *   AJ_MG_OPT_GUESS_TEST = 1 (default): an initial guess p0 is kept only if
  *     its FP64 residual norm is not above the zero guess's (||b||), else
  *     the solve restarts from zero; 0: a finite guess is always kept (used
- *     to measure what a guess is worth). */
-enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1, AJ_MG_OPT_GUESS_TEST = 2 };
+ *     to measure what a guess is worth).
+ *   AJ_MG_OPT_RELATIVE = 0 (default): `tol` of aj_poisson_mg is absolute
+ *     (||b - A p|| <= tol); 1: relative to the right-hand side
+ *     (||b - A p|| <= tol ||b||). */
+enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1, AJ_MG_OPT_GUESS_TEST = 2,
+       AJ_MG_OPT_RELATIVE = 3 };
 int aj_mg_set_option(aj_mg_plan* plan, int option, int value);
 
 /* ---- Flow diagnostics --------------------------------------------------

@@ -450,3 +450,22 @@ def test_batched_graph_replays_match_eager(gpu):
         e.step()
         torch.cuda.synchronize()
     assert torch.equal(a.u, e.u) and torch.equal(a.v, e.v) and torch.equal(a.p, e.p)
+
+
+def test_textbook_relative_tolerance(gpu):
+    """AJ_MG_OPT_RELATIVE: ||b - A p|| <= tol ||b||, with and without a guess."""
+    import aerojax_amd as aj
+    nx, ny = 512, 128
+    dx, dy = 4.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(8)
+    b = dev(rng.normal(size=(nx, ny)) * 300, torch.float32)
+    pl = aj.MultigridPlan(nx, ny, dx, dy, torch.float32, "textbook", 3)
+    pl.set_option(relative=True)
+    p = torch.empty_like(b)
+    for p0 in (None, torch.zeros_like(b)):
+        pl.solve(b, p, p0, 60, 1e-8)
+        s = pl.stats()
+        print("relative", s["norms"], flush=True)
+        assert s["converged"] and s["norm"] <= 1e-8 * s["norms"][0]
+        assert s["norms"][-2] > 1e-8 * s["norms"][0]  # the relative bound decided the stop
+    pl.close()
