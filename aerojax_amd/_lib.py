@@ -34,7 +34,8 @@ AJ_MG_OPT_RELATIVE = 3
 EXPORTS = (
     "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
     "aj_les_smagorinsky_total", "aj_divergence_rhs", "aj_project", "aj_project_collocated",
-    "aj_flow_diagnostics", "aj_batch_create", "aj_batch_destroy", "aj_batch_cells",
+    "aj_flow_diagnostics", "aj_tune_enable", "aj_tune_reset", "aj_tune_report",
+    "aj_batch_create", "aj_batch_destroy", "aj_batch_cells",
     "aj_batch_num_instances", "aj_batch_offsets", "aj_batch_set_viscosity",
     "aj_batch_set_option", "aj_batch_step", "aj_batch_get_stats",
     "aj_mg_create", "aj_mg_destroy", "aj_mg_workspace_bytes",
@@ -122,6 +123,9 @@ _SIGS = {
                                    _vp]),
     "aj_flow_diagnostics": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d,
                                  ctypes.POINTER(FlowDiag), _vp]),
+    "aj_tune_enable": (_i, [_i]),
+    "aj_tune_reset": (_i, []),
+    "aj_tune_report": (_i, [ctypes.c_char_p, _i]),
     "aj_batch_create": (_i, [ctypes.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _i]),
     "aj_batch_destroy": (_i, [_vp]),
     "aj_batch_cells": (ctypes.c_long, [_vp]),
@@ -221,3 +225,11 @@ _NN_SIGS = {
 _SIGS.update(_NN_SIGS)
 EXPORTS = EXPORTS + tuple(_NN_SIGS)
 PROF_TAGS = PROF_TAGS + ("nn_conv", "nn_dw")
+
+
+def tune_report() -> str:
+    """Autotuner decisions of this process, one "key -> pick [ms ...]" line
+    per tuned launch."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    load().aj_tune_report(buf, len(buf))
+    return buf.value.decode()

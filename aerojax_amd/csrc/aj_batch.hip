@@ -35,10 +35,7 @@ template <> struct BatchRkTile<float> { static constexpr int TX = 16, TY = 64; }
 template <> struct BatchRkTile<double> { static constexpr int TX = 8, TY = 64; };
 constexpr int kBatchJX = 16, kBatchJY = 64;  // Jacobi tile
 constexpr int kResChunks = 16;  // residual-norm partial sums per instance
-// persistent grid sizes, workgroups per CU: a few rounds of the resident
-// workgroups (LDS-limited for the tile kernels) so that one workgroup's
-// loads overlap another's compute
-constexpr int kWgRk = 12, kWgJacobi = 16, kWgRows = 32, kWgCoarse = 2;
+constexpr int kWgCoarse = 2;    // coarsest solve: 1024-thread workgroups per CU
 
 struct BatchWork {
   int ng, total;
@@ -322,6 +319,7 @@ struct aj_batch {
   aj_mg_stats* stats = nullptr;
   double last_tol = -1.0;
   int use_graph = 1;
+  int tuned = 0;
   // two cached step graphs (steppers alternate two sets of buffers)
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -380,6 +378,21 @@ unsigned persistent_grid(const aj_batch* b, int total, int per_cu) {
   return (unsigned)std::max<long>(1, std::min<long>(total, cap));
 }
 
+// persistent grid size, tuned per kernel and work size among
+// kWgCands workgroups per CU (outputs never feed the same launch)
+constexpr int kWgCands[4] = {4, 8, 16, 32};
+
+template <typename F>
+int batch_launch(const aj_batch* b, const char* name, int total, F&& launch, hipStream_t st) {
+  char key[96];
+  std::snprintf(key, sizeof(key), "batch/%s/%d", name, total);
+  auto L = [&](int c, hipStream_t s) -> int {
+    return launch(dim3(persistent_grid(b, total, kWgCands[c])), s);
+  };
+  int pick = 0;
+  return tune_pick(key, 4, L, st, &pick);
+}
+
 template <typename T>
 int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, int bzl,
                  hipStream_t st) {
@@ -413,9 +426,12 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
       m.b = bb;
       m.p_out = dst;
       m.clamp = 0;
-      hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), dim3(persistent_grid(b, wt.total, kWgJacobi)),
-                         dim3(256), 0, st, wt, m);
-      AJ_LAUNCH_CHECK("kb_jacobi");
+      if (int rc = batch_launch(b, "jacobi", wt.total, [&](dim3 g, hipStream_t s) -> int {
+            hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), g, dim3(256), 0, s, wt, m);
+            AJ_LAUNCH_CHECK("kb_jacobi");
+            return AJ_OK;
+          }, st))
+        return rc;
       src = dst;
     }
     return AJ_OK;
@@ -427,26 +443,29 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
   m.b = bb;
   m.p_out = Yl;
   m.clamp = 1;
-  hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), dim3(persistent_grid(b, wt.total, kWgJacobi)),
-                     dim3(256), 0, st, wt, m);
-  AJ_LAUNCH_CHECK("kb_jacobi");
+  auto jacobi = [&](dim3 g, hipStream_t s) -> int {
+    hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), g, dim3(256), 0, s, wt, m);
+    AJ_LAUNCH_CHECK("kb_jacobi");
+    return AJ_OK;
+  };
+  if (int rc = batch_launch(b, "jacobi", wt.total, jacobi, st)) return rc;
   T* Bc = (T*)b->B[lev + 1];
   BatchWork wr = make_work(b, lev, 2);
   BatchMgArgs<T> r = m;
   r.p_out = Yl;
-  hipLaunchKernelGGL(kb_resrestrict<T>, dim3(persistent_grid(b, wr.total, kWgRows)), dim3(256), 0, st,
-                     wr, r, Bc);
-  AJ_LAUNCH_CHECK("kb_resrestrict");
+  if (int rc = batch_launch(b, "resrestrict", wr.total, [&](dim3 g, hipStream_t s) -> int {
+        hipLaunchKernelGGL(kb_resrestrict<T>, g, dim3(256), 0, s, wr, r, Bc);
+        AJ_LAUNCH_CHECK("kb_resrestrict");
+        return AJ_OK;
+      }, st))
+    return rc;
   T* Ec = (T*)b->E[lev + 1];
   if (int rc = batch_vcycle<T>(b, lev + 1, nullptr, Ec, Bc, 0, st)) return rc;
   m.p_in = Yl;
   m.e_c = Ec;
   m.p_out = p_dst;
   m.clamp = 1;
-  hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), dim3(persistent_grid(b, wt.total, kWgJacobi)),
-                     dim3(256), 0, st, wt, m);
-  AJ_LAUNCH_CHECK("kb_jacobi");
-  return AJ_OK;
+  return batch_launch(b, "jacobi_post", wt.total, jacobi, st);
 }
 
 template <typename T>
@@ -475,16 +494,22 @@ int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void
     const size_t smem = (size_t)(TX + 2 * H) * (TY + 2 * H) * 8 * sizeof(T);
     static_assert((TX + 2 * H) * (TY + 2 * H) * 8 * sizeof(T) <= 65536, "RK tile LDS");
     ProfScope prof(AJ_PROF_RK, (double)b->cells * sizeof(T) * 6, st);
-    hipLaunchKernelGGL((kb_rk<T, TX, TY, kBatchRkNTH>), dim3(persistent_grid(b, w.total, kWgRk)),
-                       dim3(kBatchRkNTH), smem, st, w, s);
-    AJ_LAUNCH_CHECK("kb_rk");
+    if (int rc = batch_launch(b, "rk", w.total, [&](dim3 g, hipStream_t t) -> int {
+          hipLaunchKernelGGL((kb_rk<T, TX, TY, kBatchRkNTH>), g, dim3(kBatchRkNTH), smem, t, w, s);
+          AJ_LAUNCH_CHECK("kb_rk");
+          return AJ_OK;
+        }, st))
+      return rc;
   }
   BatchWork wrow = make_work(b, 0, 1);
   {
     ProfScope prof(AJ_PROF_RHS, (double)b->cells * sizeof(T) * 4, st);
-    hipLaunchKernelGGL(kb_div<T>, dim3(persistent_grid(b, wrow.total, kWgRows)), dim3(256), 0, st,
-                       wrow, s);
-    AJ_LAUNCH_CHECK("kb_div");
+    if (int rc = batch_launch(b, "div", wrow.total, [&](dim3 g, hipStream_t t) -> int {
+          hipLaunchKernelGGL(kb_div<T>, g, dim3(256), 0, t, wrow, s);
+          AJ_LAUNCH_CHECK("kb_div");
+          return AJ_OK;
+        }, st))
+      return rc;
   }
   // reference multigrid, v_cycles cycles with the per-instance gate
   hipLaunchKernelGGL(kb_mg_begin, dim3(b->ninst), dim3(64), 0, st, b->stats, b->ninst);
@@ -493,9 +518,13 @@ int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void
   for (int c = 0; c < prm->v_cycles; ++c) {
     const T* src = (c == 0) ? nullptr : (const T*)p_out;
     if (int rc = batch_vcycle<T>(b, 0, src, (T*)p_out, (const T*)b->rhs, 1, st)) return rc;
-    hipLaunchKernelGGL(kb_resnorm<T>, dim3(persistent_grid(b, wn.total, kWgRows)), dim3(256), 0, st, wn,
-                       (const T*)p_out, (const T*)b->rhs, b->partial);
-    AJ_LAUNCH_CHECK("kb_resnorm");
+    if (int rc = batch_launch(b, "resnorm", wn.total, [&](dim3 g, hipStream_t t) -> int {
+          hipLaunchKernelGGL(kb_resnorm<T>, g, dim3(256), 0, t, wn, (const T*)p_out,
+                             (const T*)b->rhs, b->partial);
+          AJ_LAUNCH_CHECK("kb_resnorm");
+          return AJ_OK;
+        }, st))
+      return rc;
     hipLaunchKernelGGL(kb_resnorm_final, dim3(cdiv(b->ninst, 64)), dim3(64), 0, st,
                        (const double*)b->partial, b->ninst, (const double*)b->dxdy,
                        (const double*)b->tol2, b->stats);
@@ -503,9 +532,12 @@ int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void
   }
   {
     ProfScope prof(AJ_PROF_PROJECT, (double)b->cells * sizeof(T) * 6, st);
-    hipLaunchKernelGGL(kb_project<T>, dim3(persistent_grid(b, wrow.total, kWgRows)), dim3(256), 0, st,
-                       wrow, s);
-    AJ_LAUNCH_CHECK("kb_project");
+    if (int rc = batch_launch(b, "project", wrow.total, [&](dim3 g, hipStream_t t) -> int {
+          hipLaunchKernelGGL(kb_project<T>, g, dim3(256), 0, t, wrow, s);
+          AJ_LAUNCH_CHECK("kb_project");
+          return AJ_OK;
+        }, st))
+      return rc;
   }
   return AJ_OK;
 }
@@ -700,7 +732,13 @@ extern "C" int aj_batch_step(aj_batch* b, const void* u, const void* v, const vo
                ? batch_step_launches<float>(b, prm, u, v, mask, sdf, u_out, v_out, p_out, s)
                : batch_step_launches<double>(b, prm, u, v, mask, sdf, u_out, v_out, p_out, s);
   };
-  if (!b->use_graph || prof_on()) return launches(st);
+  // the first step runs eagerly so that every launch shape is tuned before
+  // the step is captured
+  if (!b->use_graph || prof_on() || !b->tuned) {
+    const int rc = launches(st);
+    b->tuned = 1;
+    return rc;
+  }
   const void* key[8] = {u, v, mask, sdf, u_out, v_out, p_out, nullptr};
   const double kp[5] = {prm->dt, prm->U_inf, prm->brinkman_eta, (double)prm->v_cycles,
                         prm->tolerance};

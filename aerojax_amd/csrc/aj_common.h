@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <functional>
 #include <string>
 
 #include "../../include/aerojax.h"
@@ -50,6 +51,13 @@ struct ProfScope {
     if (slot >= 0) prof_end_launch(slot, bytes, st);
   }
 };
+
+// ---- launch-shape autotuner (aj_tune.hip) --------------------------------
+// launch(c, stream) launches candidate c (returns an AJ status).  The first
+// call for `key` times every candidate and keeps the fastest; later calls
+// launch the kept candidate.  *pick receives the candidate used.
+int tune_pick(const std::string& key, int ncand,
+              const std::function<int(int, hipStream_t)>& launch, hipStream_t st, int* pick);
 
 // ---- device helpers -----------------------------------------------------
 __device__ __forceinline__ int clampi(int x, int lo, int hi) {

@@ -452,9 +452,6 @@ __global__ void k_tb_begin(aj_mg_stats* st, int* state, double* dtol, double tol
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-constexpr int kPX = 32, kPY = 64;  // pre tile
-constexpr int kQX = 32, kQY = 64;  // post tile
-constexpr int kTileThreads = 1024;
 
 static TbLevel make_level(const aj_mg_plan* p, int l) {
   TbLevel L;
@@ -481,6 +478,8 @@ struct Carve {
     return r;
   }
 };
+
+static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st);
 
 int tb_create(aj_mg_plan* p) {
   // hierarchy: levels = 0 coarsens while n*m > kTbCoarseCells; levels = L > 0
@@ -605,6 +604,17 @@ int tb_create(aj_mg_plan* p) {
   AJ_HIP(hipMemcpy(p->coarse_coef, Coef.data(), Coef.size() * 8, hipMemcpyHostToDevice));
   AJ_HIP(hipMemcpy(p->coarse_dinv, Dinv.data(), Dinv.size() * 8, hipMemcpyHostToDevice));
   AJ_HIP(hipHostMalloc((void**)&p->host_stats, sizeof(aj_mg_stats), hipHostMallocDefault));
+  // pick the smoother tiles of every level now (one cold and one warm
+  // V-cycle on a zero right-hand side, gate open), so that the captured
+  // outer iteration is built from the tuned launches
+  if (p->ml > 0) {
+    AJ_HIP(hipMemset(p->stats, 0, sizeof(aj_mg_stats)));
+    AJ_HIP(hipMemset(p->tB[0], 0, (size_t)p->nx * p->ny * sizeof(float)));
+    AJ_HIP(hipMemset(p->tE[0], 0, (size_t)p->nx * p->ny * sizeof(float)));
+    if (int rc = tb_vcycle(p, 0, false, nullptr)) return rc;
+    if (int rc = tb_vcycle(p, 0, true, nullptr)) return rc;
+    AJ_HIP(hipStreamSynchronize(nullptr));
+  }
   return AJ_OK;
 }
 
@@ -640,31 +650,71 @@ static int tb_coarse(aj_mg_plan* p, hipStream_t st) {
 
 // one FP32 V-cycle for A_lev e = tB[lev]; result in tE[lev]; warm = use
 // tE[lev] as the start (the exact coarsest solve ignores it)
+template <int TX, int TY, int NTH>
+static int tb_launch_pre(const TbLevel& Lv, const float* pin, const float* b, float* Y,
+                         float* Bc, const aj_mg_stats* gate, hipStream_t st) {
+  hipLaunchKernelGGL((k_tb_pre<TX, TY, NTH>), dim3(cdiv(Lv.m, TY), cdiv(Lv.n, TX)), dim3(NTH), 0,
+                     st, Lv, pin, b, Y, Bc, gate);
+  AJ_LAUNCH_CHECK("k_tb_pre");
+  return AJ_OK;
+}
+
+template <int TX, int TY, int NTH>
+static int tb_launch_post(const TbLevel& Lv, const float* Y, const float* Ec, int nc, int mc,
+                          float gamc, const float* b, float* E, const aj_mg_stats* gate,
+                          hipStream_t st) {
+  hipLaunchKernelGGL((k_tb_post<TX, TY, NTH>), dim3(cdiv(Lv.m, TY), cdiv(Lv.n, TX)), dim3(NTH), 0,
+                     st, Lv, Y, Ec, nc, mc, gamc, b, E, gate);
+  AJ_LAUNCH_CHECK("k_tb_post");
+  return AJ_OK;
+}
+
+// smoother tile candidates (rows x columns, threads): tuned per level shape
+constexpr int kTbTileCands = 4;
+
 static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
   if (lev == p->ml) return tb_coarse(p, st);
   const TbLevel Lv = make_level(p, lev);
   const int n = Lv.n, m = Lv.m;
-  dim3 gpre(cdiv(m, kPY), cdiv(n, kPX));
+  const aj_mg_stats* gate = p->stats;
+  char key[64];
   {
-  // p_in (warm), b read; Y and the quarter-size coarse rhs written
-  ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, (double)n * m * (warm ? 13 : 9), st);
-  hipLaunchKernelGGL((k_tb_pre<kPX, kPY, kTileThreads>), gpre, dim3(kTileThreads), 0, st, Lv,
-                     warm ? (const float*)p->tE[lev] : (const float*)nullptr,
-                     (const float*)p->tB[lev], p->tY[lev], p->tB[lev + 1],
-                     (const aj_mg_stats*)p->stats);
-  AJ_LAUNCH_CHECK("k_tb_pre");
+    // p_in (warm), b read; Y and the quarter-size coarse rhs written
+    const float* pin = warm ? (const float*)p->tE[lev] : nullptr;
+    const float* b = p->tB[lev];
+    float *Y = p->tY[lev], *Bc = p->tB[lev + 1];
+    auto launch = [&](int c, hipStream_t s) -> int {
+      switch (c) {
+        case 1: return tb_launch_pre<16, 64, 512>(Lv, pin, b, Y, Bc, gate, s);
+        case 2: return tb_launch_pre<32, 32, 512>(Lv, pin, b, Y, Bc, gate, s);
+        case 3: return tb_launch_pre<16, 32, 256>(Lv, pin, b, Y, Bc, gate, s);
+        default: return tb_launch_pre<32, 64, 1024>(Lv, pin, b, Y, Bc, gate, s);
+      }
+    };
+    std::snprintf(key, sizeof(key), "tb_pre/%dx%d", n, m);
+    ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, (double)n * m * (warm ? 13 : 9), st);
+    int pick = 0;
+    if (int rc = tune_pick(key, kTbTileCands, launch, st, &pick)) return rc;
   }
   if (int rc = tb_vcycle(p, lev + 1, false, st)) return rc;
-  dim3 gpost(cdiv(m, kQY), cdiv(n, kQX));
   {
-  // Y, quarter-size coarse correction, b read; E written
-  ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, (double)n * m * 13, st);
-  hipLaunchKernelGGL((k_tb_post<kQX, kQY, kTileThreads>), gpost, dim3(kTileThreads), 0, st, Lv,
-                     (const float*)p->tY[lev],
-                     (const float*)p->tE[lev + 1], p->lnx[lev + 1], p->lny[lev + 1],
-                     (float)outlet_gamma(lev + 1), (const float*)p->tB[lev], p->tE[lev],
-                     (const aj_mg_stats*)p->stats);
-  AJ_LAUNCH_CHECK("k_tb_post");
+    // Y, quarter-size coarse correction, b read; E written
+    const float *Y = p->tY[lev], *Ec = p->tE[lev + 1], *b = p->tB[lev];
+    float* E = p->tE[lev];
+    const int nc = p->lnx[lev + 1], mc = p->lny[lev + 1];
+    const float gamc = (float)outlet_gamma(lev + 1);
+    auto launch = [&](int c, hipStream_t s) -> int {
+      switch (c) {
+        case 1: return tb_launch_post<16, 64, 512>(Lv, Y, Ec, nc, mc, gamc, b, E, gate, s);
+        case 2: return tb_launch_post<32, 32, 512>(Lv, Y, Ec, nc, mc, gamc, b, E, gate, s);
+        case 3: return tb_launch_post<16, 32, 256>(Lv, Y, Ec, nc, mc, gamc, b, E, gate, s);
+        default: return tb_launch_post<32, 64, 1024>(Lv, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      }
+    };
+    std::snprintf(key, sizeof(key), "tb_post/%dx%d", n, m);
+    ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, (double)n * m * 13, st);
+    int pick = 0;
+    if (int rc = tune_pick(key, kTbTileCands, launch, st, &pick)) return rc;
   }
   return AJ_OK;
 }

@@ -12,6 +12,8 @@
 // +-1, Adam.
 #include "aj_nn_kernels.h"
 
+#include <cstdio>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -226,12 +228,20 @@ static int conv_launch_t(const ConvArgs& c, int nt, hipStream_t st) {
   return AJ_OK;
 }
 
+// tile candidates 8x32 and 4x16 pixels, tuned per layer shape; launches
+// that accumulate into their output (input gradients of a concatenated
+// input) reuse the choice of the plain launch of the same shape
 int conv_launch(const ConvArgs& c, hipStream_t st) {
   if (c.ci % 32 || c.co % 16) return fail(AJ_ERR_INVALID, "conv: ci % 32 or co % 16 != 0");
   const int nt = (c.co % 64 == 0) ? 64 : (c.co % 32 == 0 ? 32 : 16);
-  const long big = (long)c.B * ((c.H + 7) / 8) * ((c.W + 31) / 32) * (c.co / nt);
-  if (big >= 512 && c.W >= 32) return conv_launch_t<8, 32>(c, nt, st);
-  return conv_launch_t<4, 16>(c, nt, st);
+  auto launch = [&](int k, hipStream_t s) -> int {
+    return k == 1 ? conv_launch_t<4, 16>(c, nt, s) : conv_launch_t<8, 32>(c, nt, s);
+  };
+  char key[96];
+  std::snprintf(key, sizeof(key), "conv/%dx%dx%d/%d->%d/%s", c.B, c.H, c.W, c.ci, c.co,
+                c.outf ? "f32" : "bf16");
+  int pick = 0;
+  return tune_pick(key, c.accum ? 1 : 2, launch, st, &pick);
 }
 
 // forward conv + GELU: out (post) and pre both stored

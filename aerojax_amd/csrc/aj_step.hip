@@ -11,6 +11,7 @@
 #include "aj_step_kernels.h"
 
 #include <algorithm>
+#include <cstdio>
 
 namespace aj {
 
@@ -67,17 +68,40 @@ static int rk_step_t(const void* u, const void* v, const void* mask, const void*
   a.slip = p->slip_walls ? 1 : 0;
   const bool hyper = p->nu_hyper_ratio > 0.0;
   const bool fast = p->fast_mode != 0;
-  if constexpr (sizeof(T) == 8) {
-    if (!fast && !hyper) return launch_rk<T, 8, 64, false, false>(a, st);
-    if (fast && !hyper) return launch_rk<T, 8, 64, true, false>(a, st);
-    if (!fast && hyper) return launch_rk<T, 4, 32, false, true>(a, st);
-    return launch_rk<T, 4, 32, true, true>(a, st);
-  } else {
-    if (!fast && !hyper) return launch_rk<T, 16, 64, false, false>(a, st);
-    if (fast && !hyper) return launch_rk<T, 16, 64, true, false>(a, st);
-    if (!fast && hyper) return launch_rk<T, 8, 64, false, true>(a, st);
-    return launch_rk<T, 8, 64, true, true>(a, st);
-  }
+  // tile candidates (rows x columns), tuned per grid shape and variant
+  auto launch = [&](int c, hipStream_t s) -> int {
+    if constexpr (sizeof(T) == 8) {
+      if (hyper) return fast ? launch_rk<T, 4, 32, true, true>(a, s)
+                             : launch_rk<T, 4, 32, false, true>(a, s);
+      switch (c) {
+        case 1: return fast ? launch_rk<T, 4, 64, true, false>(a, s)
+                            : launch_rk<T, 4, 64, false, false>(a, s);
+        case 2: return fast ? launch_rk<T, 16, 64, true, false>(a, s)
+                            : launch_rk<T, 16, 64, false, false>(a, s);
+        default: return fast ? launch_rk<T, 8, 64, true, false>(a, s)
+                             : launch_rk<T, 8, 64, false, false>(a, s);
+      }
+    } else {
+      if (hyper) return fast ? launch_rk<T, 8, 64, true, true>(a, s)
+                             : launch_rk<T, 8, 64, false, true>(a, s);
+      switch (c) {
+        case 1: return fast ? launch_rk<T, 8, 64, true, false>(a, s)
+                            : launch_rk<T, 8, 64, false, false>(a, s);
+        case 2: return fast ? launch_rk<T, 32, 64, true, false>(a, s)
+                            : launch_rk<T, 32, 64, false, false>(a, s);
+        case 3: return fast ? launch_rk<T, 8, 128, true, false>(a, s)
+                            : launch_rk<T, 8, 128, false, false>(a, s);
+        default: return fast ? launch_rk<T, 16, 64, true, false>(a, s)
+                             : launch_rk<T, 16, 64, false, false>(a, s);
+      }
+    }
+  };
+  const int ncand = hyper ? 1 : (sizeof(T) == 8 ? 3 : 4);
+  char key[96];
+  std::snprintf(key, sizeof(key), "rk/%s/%s%s/%dx%d", sizeof(T) == 8 ? "f64" : "f32",
+                fast ? "rk2" : "rk3", hyper ? "+hyper" : "", nx, ny);
+  int pick = 0;
+  return tune_pick(key, ncand, launch, st, &pick);
 }
 
 // --------------------------------------------------------------------------

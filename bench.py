@@ -282,6 +282,7 @@ def main():
     d = hbm[dom]
     log("kernel profile: " + ", ".join(f"{k} {v['avg_us']:.1f}us x{v['count']}"
                                          for k, v in prof.items()))
+    log("autotuner:\n" + _lib.tune_report())
     mfma = None
     if "nn_conv" in prof:
         c = prof["nn_conv"]

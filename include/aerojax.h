@@ -174,6 +174,17 @@ enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1, AJ_MG_OPT_GUESS_TEST = 2,
        AJ_MG_OPT_RELATIVE = 3 };
 int aj_mg_set_option(aj_mg_plan* plan, int option, int value);
 
+/* ---- Launch-shape autotuner ----------------------------------------------
+ * Kernels with several compiled tile shapes / grid sizes (RK predictor,
+ * textbook smoother tiles, batched persistent grids, network convolutions)
+ * time every candidate on their first launch for a problem shape and keep
+ * the fastest for the process.  No counterpart in the reference.  Tuning is
+ * on by default (environment AJ_TUNE=0 or aj_tune_enable(0): first
+ * candidate everywhere); aj_tune_report writes "key -> pick [ms ...]" lines. */
+int aj_tune_enable(int enabled);
+int aj_tune_reset(void);
+int aj_tune_report(char* buf, int len);
+
 /* ---- Flow diagnostics --------------------------------------------------
  * Replaces the viewer's and step_pure's debug residual/divergence metrics
  * (viewer/simulation_controller.py:107-121 and 186-194,

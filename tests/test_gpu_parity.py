@@ -469,3 +469,35 @@ def test_textbook_relative_tolerance(gpu):
         assert s["converged"] and s["norm"] <= 1e-8 * s["norms"][0]
         assert s["norms"][-2] > 1e-8 * s["norms"][0]  # the relative bound decided the stop
     pl.close()
+
+
+def test_autotuned_launches_are_bit_identical(gpu):
+    """Tile shapes change which workgroup computes a cell, not the arithmetic:
+    the RK step and a textbook solve give identical bits with tuning off
+    (first candidate everywhere) and on; the tuner records its decisions."""
+    import aerojax_amd as aj
+    from aerojax_amd import _lib
+    c = case(512, 128, 4.0, 1.0, kind="cylinder")
+    dx, dy = c["dx"], c["dy"]
+    rng = np.random.default_rng(9)
+    b = dev(rng.normal(size=(512, 128)) * 100, torch.float32)
+    res = []
+    for on in (0, 1):
+        _lib.load().aj_tune_enable(on)
+        _lib.load().aj_tune_reset()
+        us, vs = aj.rk_step_unified(dev(c["u"], torch.float32), dev(c["v"], torch.float32), 1e-3,
+                                    5e-4, dx, dy, dev(c["mask"], torch.float32),
+                                    sdf=dev(c["sdf"], torch.float32), brinkman_eta=0.005)
+        pl = aj.MultigridPlan(512, 128, dx, dy, torch.float32, "textbook", 3)
+        p = torch.empty_like(b)
+        pl.solve(b, p, None, 40, 1e-6)
+        torch.cuda.synchronize()
+        res.append((us.clone(), vs.clone(), p.clone()))
+        pl.close()
+        if on:
+            rep = _lib.tune_report()
+            print(rep, flush=True)
+            assert "rk/f32/rk3/512x128" in rep and "tb_pre/512x128" in rep
+    _lib.load().aj_tune_enable(1)
+    for a, bb in zip(res[0], res[1]):
+        assert torch.equal(a, bb)
