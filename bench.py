@@ -239,13 +239,14 @@ def main():
     sps_nn = None
     if not args.no_nn:
         from aerojax_amd.nn import WarmStartNet
-        from aerojax_amd.warmstart import generate_pairs, train
+        from aerojax_amd.warmstart import generate_pairs, normalise_targets, train
         net = WarmStartNet(base_channels=32, levels=4, seed=0, r=args.nn_r, device=dev)
         t0 = time.perf_counter()
         X, T = generate_pairs(net, args.nn_pairs, range(1000, 100000), nx, ny,
                               steps_per_seed=40, stride=2, skip=2)
         t_gen = time.perf_counter() - t0
         log(f"generated {X.shape[0]} pairs at {X.shape[1]}x{X.shape[2]} in {t_gen:.1f}s")
+        pscale = normalise_targets(net, T)
         t0 = time.perf_counter()
         hist = train(net, X, T, epochs=args.nn_epochs, batch=16, lr=5e-4, log=log)
         t_train = time.perf_counter() - t0
@@ -259,6 +260,7 @@ def main():
                    "guess_over_zero_residual": ratio, "train_pairs": args.nn_pairs,
                    "train_epochs": args.nn_epochs, "train_loss": hist, "gen_s": round(t_gen, 1),
                    "train_s": round(t_train, 1), "net_resolution": [nx >> args.nn_r, ny >> args.nn_r],
+                   "target_scale": pscale,
                    "params": net.num_params}
 
     # ---- live per-kernel timing (HIP event pairs on the launching stream)
