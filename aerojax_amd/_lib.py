@@ -37,9 +37,11 @@ EXPORTS = (
     "aj_flow_diagnostics", "aj_tune_enable", "aj_tune_reset", "aj_tune_report",
     "aj_batch_create", "aj_batch_destroy", "aj_batch_cells",
     "aj_batch_num_instances", "aj_batch_offsets", "aj_batch_set_viscosity",
-    "aj_batch_set_option", "aj_batch_step", "aj_batch_get_stats",
-    "aj_mg_create", "aj_mg_destroy", "aj_mg_workspace_bytes",
-    "aj_mg_num_levels", "aj_mg_set_option", "aj_poisson_mg", "aj_mg_get_stats",
+    "aj_batch_set_option", "aj_batch_step", "aj_batch_predict", "aj_batch_solve",
+    "aj_batch_project", "aj_batch_get_stats",
+    "aj_mg_create", "aj_mg_create_batched", "aj_mg_batch_count", "aj_mg_destroy",
+    "aj_mg_workspace_bytes", "aj_mg_num_levels", "aj_mg_set_option", "aj_poisson_mg",
+    "aj_mg_get_stats", "aj_mg_get_stats_batched",
     "aj_profile_begin", "aj_profile_end", "aj_profile_read",
 )
 
@@ -78,7 +80,8 @@ class FlowDiag(ctypes.Structure):
 class BatchStepParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("U_inf", ctypes.c_double),
                 ("brinkman_eta", ctypes.c_double), ("tolerance", ctypes.c_double),
-                ("v_cycles", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("v_cycles", ctypes.c_int), ("solver", ctypes.c_int),
+                ("smagorinsky_cs", ctypes.c_double)]
 
 
 class MgStats(ctypes.Structure):
@@ -135,8 +138,15 @@ _SIGS = {
     "aj_batch_set_option": (_i, [_vp, _i, _i]),
     "aj_batch_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                            ctypes.POINTER(BatchStepParams), _vp]),
+    "aj_batch_predict": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BatchStepParams),
+                              _vp]),
+    "aj_batch_solve": (_i, [_vp, _vp, _vp, _vp, ctypes.POINTER(BatchStepParams), _vp]),
+    "aj_batch_project": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BatchStepParams), _vp]),
     "aj_batch_get_stats": (_i, [_vp, _vp, _vp]),
     "aj_mg_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _d, _d, _i]),
+    "aj_mg_create_batched": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _i, _i]),
+    "aj_mg_batch_count": (_i, [_vp]),
+    "aj_mg_get_stats_batched": (_i, [_vp, _vp, _vp]),
     "aj_mg_destroy": (_i, [_vp]),
     "aj_mg_workspace_bytes": (ctypes.c_size_t, [_vp]),
     "aj_mg_num_levels": (_i, [_vp]),
@@ -221,10 +231,12 @@ _NN_SIGS = {
                                _vp, _vp, _vp]),
     "aj_nn_warm_start": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_float, ctypes.c_float,
                               _vp]),
+    "aj_nn_warm_start_batched": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, ctypes.c_float,
+                                      ctypes.c_float, _vp]),
 }
 _SIGS.update(_NN_SIGS)
 EXPORTS = EXPORTS + tuple(_NN_SIGS)
-PROF_TAGS = PROF_TAGS + ("nn_conv", "nn_dw")
+PROF_TAGS = PROF_TAGS + ("nn_conv", "nn_dw", "les")
 
 
 def tune_report() -> str:

@@ -6,6 +6,13 @@ multigrid with ``v_cycles`` cycles, projection; solver/pure_functions/
 step.py:49-467) to every instance in shared, graph-captured launches.
 Instances are grouped by grid shape; fields live in one flat device buffer
 per quantity, instance k at ``offsets[k]``.
+
+``solver="textbook"`` solves every pressure to ``tolerance`` instead (one
+batched textbook plan per group, at most ``max_cycles`` V-cycles);
+``smagorinsky_cs`` > 0 feeds the constant Smagorinsky eddy viscosity to the
+RK step; ``warm_start`` (a ``WarmStartNet``) gives every instance a learned
+initial guess (one inference per group), kept per instance only if its
+residual is not above the zero guess's.
 """
 from __future__ import annotations
 
@@ -24,10 +31,17 @@ class BatchedStepper:
     per-instance initial fields."""
 
     def __init__(self, groups, masks, sdfs, nus, u0, v0, dt, *, U_inf=1.0, brinkman_eta=0.005,
-                 v_cycles=3, tolerance=1e-6, levels=4, dtype=torch.float32, device=None):
+                 v_cycles=3, tolerance=1e-6, levels=4, dtype=torch.float32, device=None,
+                 solver="reference", max_cycles=40, smagorinsky_cs=0.0, warm_start=None):
         A.require_gpu()
         if dtype not in (torch.float32, torch.float64):
             raise TypeError(f"unsupported dtype {dtype} (float32 or float64)")
+        if solver not in ("reference", "textbook"):
+            raise ValueError(f"solver={solver!r}: 'reference' or 'textbook'")
+        if warm_start is not None and (solver != "textbook" or dtype != torch.float32):
+            raise ValueError("warm_start needs solver='textbook' and float32 fields")
+        self.solver = solver
+        self.warm_start = warm_start
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
             else torch.device(device)
         self.dtype = dtype
@@ -62,8 +76,16 @@ class BatchedStepper:
         self.u_next = torch.empty_like(self.u)
         self.v_next = torch.empty_like(self.v)
         self.p = torch.zeros_like(self.u)
+        tb = solver == "textbook"
         self.prm = _lib.BatchStepParams(float(dt), float(U_inf), float(brinkman_eta),
-                                        float(tolerance), int(v_cycles), 0)
+                                        float(tolerance), int(max_cycles if tb else v_cycles),
+                                        _lib.AJ_MG_TEXTBOOK if tb else _lib.AJ_MG_REFERENCE,
+                                        float(smagorinsky_cs))
+        if warm_start is not None:
+            if self.sdf is None:
+                raise ValueError("warm_start needs the instances' sdf")
+            self.rhs = torch.empty_like(self.u)
+            self.p0 = torch.empty_like(self.u)
 
     def _pack(self, fields):
         buf = torch.empty(self.cells, dtype=self.dtype, device=self.device)
@@ -87,12 +109,32 @@ class BatchedStepper:
     def step(self):
         """Advance every instance by one step (u, v swapped with the outputs;
         ``p`` holds the pressures)."""
+        lib = _lib.load()
+        prm = ctypes.byref(self.prm)
         with torch.cuda.device(self.device):
-            rc = _lib.load().aj_batch_step(self.handle, A.ptr(self.u), A.ptr(self.v),
-                                           A.ptr(self.mask), A.ptr(self.sdf), A.ptr(self.u_next),
-                                           A.ptr(self.v_next), A.ptr(self.p),
-                                           ctypes.byref(self.prm), A.stream_handle(self.device))
-        _lib.check(rc, "aj_batch_step")
+            st = A.stream_handle(self.device)
+            if self.warm_start is None:
+                rc = lib.aj_batch_step(self.handle, A.ptr(self.u), A.ptr(self.v),
+                                       A.ptr(self.mask), A.ptr(self.sdf), A.ptr(self.u_next),
+                                       A.ptr(self.v_next), A.ptr(self.p), prm, st)
+                _lib.check(rc, "aj_batch_step")
+            else:
+                rc = lib.aj_batch_predict(self.handle, A.ptr(self.u), A.ptr(self.v),
+                                          A.ptr(self.mask), A.ptr(self.sdf), A.ptr(self.rhs),
+                                          prm, st)
+                _lib.check(rc, "aj_batch_predict")
+                k = 0
+                for (nx, ny, cnt, _dx, _dy) in self.groups:
+                    o, n = int(self.offsets[k]), nx * ny * cnt
+                    self.warm_start.warm_start_batched(self.rhs[o:o + n], self.sdf[o:o + n],
+                                                       self.p0[o:o + n], cnt, nx, ny)
+                    k += cnt
+                rc = lib.aj_batch_solve(self.handle, A.ptr(self.rhs), A.ptr(self.p0),
+                                        A.ptr(self.p), prm, st)
+                _lib.check(rc, "aj_batch_solve")
+                rc = lib.aj_batch_project(self.handle, A.ptr(self.p), A.ptr(self.mask),
+                                          A.ptr(self.u_next), A.ptr(self.v_next), prm, st)
+                _lib.check(rc, "aj_batch_project")
         self.u, self.u_next = self.u_next, self.u
         self.v, self.v_next = self.v_next, self.v
 
@@ -102,7 +144,7 @@ class BatchedStepper:
             rc = _lib.load().aj_batch_get_stats(self.handle, ctypes.cast(s, ctypes.c_void_p),
                                                 A.stream_handle(self.device))
         _lib.check(rc, "aj_batch_get_stats")
-        return [x.as_dict() for x in s]
+        return [x.as_dict(textbook=self.solver == "textbook") for x in s]
 
     def close(self):
         if getattr(self, "handle", None):

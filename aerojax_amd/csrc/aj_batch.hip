@@ -13,16 +13,23 @@
 //
 // One batched step is step_pure's composition (solver/pure_functions/
 // step.py:49-467, as aj_rk_step + aj_divergence_rhs + aj_poisson_mg
-// (reference) + aj_project) for every instance: RK3 predictor tiles, rhs
-// rows, reference multigrid V-cycles level by level (all instances of a
-// level in one launch; the 10-sweep coarsest solve one workgroup per
-// instance) with a per-instance convergence gate, projection rows.  The
-// launch sequence of a step has no host synchronisation and is captured
-// once into a hipGraph per set of buffers.
+// (reference) + aj_project) for every instance: [Smagorinsky rows], RK3
+// predictor tiles, rhs rows, reference multigrid V-cycles level by level
+// (all instances of a level in one launch; the 10-sweep coarsest solve one
+// workgroup per instance) with a per-instance convergence gate, projection
+// rows.  The launch sequence of a step has no host synchronisation and is
+// captured once into a hipGraph per set of buffers.
+//
+// With the textbook solver the pressure of every instance is solved to the
+// tolerance instead: one batched textbook plan per group (its instances in
+// blockIdx.z, each with its own gate, guess test and live buffer), all
+// groups' outer iterations enqueued together and checked with one host
+// synchronisation per round.
 #include <cstdio>
 #include <cstring>
 #include <vector>
 
+#include "aj_mg.h"
 #include "aj_mg_ref_kernels.h"
 #include "aj_step_kernels.h"
 
@@ -77,8 +84,9 @@ template <typename T>
 struct BatchStepArgs {
   const T *u, *v, *mask, *sdf;
   T *us, *vs, *rhs, *p, *uo, *vo;
+  T* nsgs;           // Smagorinsky nu_sgs, or null (no LES)
   const double* nu;  // [instances]
-  double dt, U_inf, inv_eta;
+  double dt, U_inf, inv_eta, cs;
 };
 
 template <typename T, int TX, int TY, int NTH>
@@ -93,7 +101,7 @@ __global__ __launch_bounds__(NTH) void kb_rk(BatchWork w, BatchStepArgs<T> s) {
     a.v = s.v + it.off;
     a.mask = s.mask + it.off;
     a.sdf = s.sdf ? s.sdf + it.off : nullptr;
-    a.nu_sgs = nullptr;
+    a.nu_sgs = s.nsgs ? s.nsgs + it.off : nullptr;
     a.nu_field = nullptr;
     a.u_out = s.us + it.off;
     a.v_out = s.vs + it.off;
@@ -119,6 +127,28 @@ __global__ __launch_bounds__(NTH) void kb_rk(BatchWork w, BatchStepArgs<T> s) {
 // row items: kRowsPerItem rows x 256 columns, one cell per thread and row
 // (the rows' loads are independent, so they are in flight together)
 constexpr int kRowsPerItem = 4;
+
+// nu_sgs = constant Smagorinsky with delta = sqrt(dx dy) (step.py:111-122)
+template <typename T>
+__global__ __launch_bounds__(256) void kb_les(BatchWork w, BatchStepArgs<T> s) {
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g, nx = w.nx[g], ny = w.ny[g];
+    const int cpr = (ny + 255) / 256;
+    const int rb = it.sub / cpr, j = (it.sub - rb * cpr) * 256 + threadIdx.x;
+    if (j >= ny) continue;
+    const double cd = s.cs * sqrt(w.dx[g] * w.dy[g]);
+    const T coef = T(cd * cd);
+#pragma unroll
+    for (int q = 0; q < kRowsPerItem; ++q) {
+      const int i = rb * kRowsPerItem + q;
+      if (i < nx)
+        s.nsgs[it.off + (long)i * ny + j] =
+            les_cell(s.u + it.off, s.v + it.off, nx, ny, i, j, T(1.0 / (2.0 * w.dx[g])),
+                     T(1.0 / (2.0 * w.dy[g])), coef);
+    }
+  }
+}
 
 // rhs = div * mask / dt
 template <typename T>
@@ -311,7 +341,9 @@ struct aj_batch {
   long cells = 0;
   void* ws = nullptr;
   std::vector<void*> B, Y, E;  // per level >= 1 (index level); level 0 uses step buffers
-  void *us = nullptr, *vs = nullptr, *rhs = nullptr;
+  void *us = nullptr, *vs = nullptr, *rhs = nullptr, *nsgs = nullptr;
+  std::vector<aj_mg_plan*> tb;  // textbook solver: one batched plan per group (created on first use)
+  int last_solver = AJ_MG_REFERENCE;
   double* partial = nullptr;
   double* dxdy = nullptr;
   double* tol2 = nullptr;
@@ -324,7 +356,7 @@ struct aj_batch {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   const void* key[2][8] = {};
-  double key_prm[2][5] = {};
+  double key_prm[2][7] = {};
   int victim = 0;
   int device = 0, cus = 256;
 };
@@ -469,9 +501,9 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
 }
 
 template <typename T>
-int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void* u,
-                        const void* v, const void* mask, const void* sdf, void* u_out,
-                        void* v_out, void* p_out, hipStream_t st) {
+BatchStepArgs<T> step_args(aj_batch* b, const aj_batch_step_params* prm, const void* u,
+                           const void* v, const void* mask, const void* sdf, void* rhs, void* p,
+                           void* u_out, void* v_out) {
   BatchStepArgs<T> s;
   s.u = (const T*)u;
   s.v = (const T*)v;
@@ -479,21 +511,39 @@ int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void
   s.sdf = (const T*)sdf;
   s.us = (T*)b->us;
   s.vs = (T*)b->vs;
-  s.rhs = (T*)b->rhs;
-  s.p = (T*)p_out;
+  s.rhs = (T*)rhs;
+  s.p = (T*)p;
   s.uo = (T*)u_out;
   s.vo = (T*)v_out;
+  s.nsgs = prm->smagorinsky_cs > 0 ? (T*)b->nsgs : nullptr;
   s.nu = b->nu;
   s.dt = prm->dt;
   s.U_inf = prm->U_inf;
   s.inv_eta = 1.0 / prm->brinkman_eta;
+  s.cs = prm->smagorinsky_cs;
+  return s;
+}
+
+// [LES] + RK3 predictor (u*, v* in the batch's buffers) + rhs
+template <typename T>
+int predict_launches(aj_batch* b, const BatchStepArgs<T>& s, hipStream_t st) {
+  BatchWork wrow = make_work(b, 0, 1);
+  if (s.nsgs) {
+    ProfScope prof(AJ_PROF_LES, (double)b->cells * sizeof(T) * 3, st);
+    if (int rc = batch_launch(b, "les", wrow.total, [&](dim3 g, hipStream_t t) -> int {
+          hipLaunchKernelGGL(kb_les<T>, g, dim3(256), 0, t, wrow, s);
+          AJ_LAUNCH_CHECK("kb_les");
+          return AJ_OK;
+        }, st))
+      return rc;
+  }
   // RK3 predictor (the predictor BCs of step.py:182-191 are its own last step)
   {
     constexpr int TX = BatchRkTile<T>::TX, TY = BatchRkTile<T>::TY, H = 3;
     BatchWork w = make_work(b, 0, 0, TX, TY);
     const size_t smem = (size_t)(TX + 2 * H) * (TY + 2 * H) * 8 * sizeof(T);
     static_assert((TX + 2 * H) * (TY + 2 * H) * 8 * sizeof(T) <= 65536, "RK tile LDS");
-    ProfScope prof(AJ_PROF_RK, (double)b->cells * sizeof(T) * 6, st);
+    ProfScope prof(AJ_PROF_RK, (double)b->cells * sizeof(T) * (s.nsgs ? 7 : 6), st);
     if (int rc = batch_launch(b, "rk", w.total, [&](dim3 g, hipStream_t t) -> int {
           hipLaunchKernelGGL((kb_rk<T, TX, TY, kBatchRkNTH>), g, dim3(kBatchRkNTH), smem, t, w, s);
           AJ_LAUNCH_CHECK("kb_rk");
@@ -501,26 +551,27 @@ int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void
         }, st))
       return rc;
   }
-  BatchWork wrow = make_work(b, 0, 1);
-  {
-    ProfScope prof(AJ_PROF_RHS, (double)b->cells * sizeof(T) * 4, st);
-    if (int rc = batch_launch(b, "div", wrow.total, [&](dim3 g, hipStream_t t) -> int {
-          hipLaunchKernelGGL(kb_div<T>, g, dim3(256), 0, t, wrow, s);
-          AJ_LAUNCH_CHECK("kb_div");
-          return AJ_OK;
-        }, st))
-      return rc;
-  }
-  // reference multigrid, v_cycles cycles with the per-instance gate
+  ProfScope prof(AJ_PROF_RHS, (double)b->cells * sizeof(T) * 4, st);
+  return batch_launch(b, "div", wrow.total, [&](dim3 g, hipStream_t t) -> int {
+    hipLaunchKernelGGL(kb_div<T>, g, dim3(256), 0, t, wrow, s);
+    AJ_LAUNCH_CHECK("kb_div");
+    return AJ_OK;
+  }, st);
+}
+
+// reference multigrid, v_cycles cycles with the per-instance gate
+template <typename T>
+int refmg_launches(aj_batch* b, const aj_batch_step_params* prm, const T* rhs, T* p_out,
+                   hipStream_t st) {
   hipLaunchKernelGGL(kb_mg_begin, dim3(b->ninst), dim3(64), 0, st, b->stats, b->ninst);
   AJ_LAUNCH_CHECK("kb_mg_begin");
   BatchWork wn = make_work(b, 0, 4);
   for (int c = 0; c < prm->v_cycles; ++c) {
-    const T* src = (c == 0) ? nullptr : (const T*)p_out;
-    if (int rc = batch_vcycle<T>(b, 0, src, (T*)p_out, (const T*)b->rhs, 1, st)) return rc;
+    const T* src = (c == 0) ? nullptr : p_out;
+    if (int rc = batch_vcycle<T>(b, 0, src, p_out, rhs, 1, st)) return rc;
     if (int rc = batch_launch(b, "resnorm", wn.total, [&](dim3 g, hipStream_t t) -> int {
-          hipLaunchKernelGGL(kb_resnorm<T>, g, dim3(256), 0, t, wn, (const T*)p_out,
-                             (const T*)b->rhs, b->partial);
+          hipLaunchKernelGGL(kb_resnorm<T>, g, dim3(256), 0, t, wn, (const T*)p_out, rhs,
+                             b->partial);
           AJ_LAUNCH_CHECK("kb_resnorm");
           return AJ_OK;
         }, st))
@@ -530,14 +581,79 @@ int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void
                        (const double*)b->tol2, b->stats);
     AJ_LAUNCH_CHECK("kb_resnorm_final");
   }
-  {
-    ProfScope prof(AJ_PROF_PROJECT, (double)b->cells * sizeof(T) * 6, st);
-    if (int rc = batch_launch(b, "project", wrow.total, [&](dim3 g, hipStream_t t) -> int {
-          hipLaunchKernelGGL(kb_project<T>, g, dim3(256), 0, t, wrow, s);
-          AJ_LAUNCH_CHECK("kb_project");
-          return AJ_OK;
-        }, st))
-      return rc;
+  return AJ_OK;
+}
+
+template <typename T>
+int project_launches(aj_batch* b, const BatchStepArgs<T>& s, hipStream_t st) {
+  BatchWork wrow = make_work(b, 0, 1);
+  ProfScope prof(AJ_PROF_PROJECT, (double)b->cells * sizeof(T) * 6, st);
+  return batch_launch(b, "project", wrow.total, [&](dim3 g, hipStream_t t) -> int {
+    hipLaunchKernelGGL(kb_project<T>, g, dim3(256), 0, t, wrow, s);
+    AJ_LAUNCH_CHECK("kb_project");
+    return AJ_OK;
+  }, st);
+}
+
+template <typename T>
+int batch_step_launches(aj_batch* b, const aj_batch_step_params* prm, const void* u,
+                        const void* v, const void* mask, const void* sdf, void* u_out,
+                        void* v_out, void* p_out, hipStream_t st) {
+  const BatchStepArgs<T> s = step_args<T>(b, prm, u, v, mask, sdf, b->rhs, p_out, u_out, v_out);
+  if (int rc = predict_launches<T>(b, s, st)) return rc;
+  if (int rc = refmg_launches<T>(b, prm, (const T*)b->rhs, (T*)p_out, st)) return rc;
+  return project_launches<T>(b, s, st);
+}
+
+// tolerance**2 in the working precision (multigrid_solver.py:229-231)
+int set_tolerance(aj_batch* b, double tol, hipStream_t st) {
+  if (tol == b->last_tol) return AJ_OK;
+  std::vector<double> t2(b->ninst);
+  const double tt = b->dtype == AJ_F64 ? tol * tol : (double)((float)tol * (float)tol);
+  for (auto& x : t2) x = tt;
+  AJ_HIP(hipMemcpyAsync(b->tol2, t2.data(), t2.size() * 8, hipMemcpyHostToDevice, st));
+  AJ_HIP(hipStreamSynchronize(st));
+  b->last_tol = tol;
+  return AJ_OK;
+}
+
+// textbook solve of every group to prm->tolerance; p0 may be null
+int textbook_solve(aj_batch* b, const void* rhs, const void* p0, void* p_out,
+                   const aj_batch_step_params* prm, hipStream_t st) {
+  const size_t es = b->dtype == AJ_F64 ? 8 : 4;
+  if (b->tb.empty()) {
+    b->tb.assign(b->ng, nullptr);
+    for (int g = 0; g < b->ng; ++g)
+      if (int rc = aj_mg_create_batched(&b->tb[g], b->dtype, b->gnx[g], b->gny[g], b->gdx[g],
+                                        b->gdy[g], 0, b->gcount[g]))
+        return rc;
+  }
+  std::vector<const void*> r(b->ng), g0(b->ng);
+  std::vector<void*> o(b->ng);
+  for (int g = 0; g < b->ng; ++g) {
+    const size_t off = (size_t)b->lbase[0][g] * es;
+    r[g] = (const char*)rhs + off;
+    g0[g] = p0 ? (const char*)p0 + off : nullptr;
+    o[g] = (char*)p_out + off;
+  }
+  b->last_solver = AJ_MG_TEXTBOOK;
+  return tb_solve_many(b->tb.data(), b->ng, r.data(), g0.data(), o.data(), prm->v_cycles,
+                       prm->tolerance, st);
+}
+
+int check_params(const aj_batch_step_params* prm, const char* who) {
+  char msg[160];
+  if (!(prm->brinkman_eta > 0) || prm->v_cycles < 1) {
+    std::snprintf(msg, sizeof(msg), "%s: brinkman_eta > 0 and v_cycles >= 1 required", who);
+    return fail(AJ_ERR_INVALID, msg);
+  }
+  if (prm->solver != AJ_MG_REFERENCE && prm->solver != AJ_MG_TEXTBOOK) {
+    std::snprintf(msg, sizeof(msg), "%s: solver must be AJ_MG_REFERENCE or AJ_MG_TEXTBOOK", who);
+    return fail(AJ_ERR_INVALID, msg);
+  }
+  if (!(prm->smagorinsky_cs >= 0)) {
+    std::snprintf(msg, sizeof(msg), "%s: smagorinsky_cs must be >= 0", who);
+    return fail(AJ_ERR_INVALID, msg);
   }
   return AJ_OK;
 }
@@ -621,6 +737,7 @@ extern "C" int aj_batch_create(aj_batch** out, int dtype, int ngroups, const int
     return o;
   };
   const size_t o_us = take(b->cells * es), o_vs = take(b->cells * es), o_rhs = take(b->cells * es);
+  const size_t o_nsgs = take(b->cells * es);
   std::vector<size_t> oB(L + 1), oY(L + 1), oE(L + 1);
   for (int l = 0; l <= L; ++l) {
     oB[l] = take(b->level_len[l] * es);
@@ -642,6 +759,7 @@ extern "C" int aj_batch_create(aj_batch** out, int dtype, int ngroups, const int
   b->us = base + o_us;
   b->vs = base + o_vs;
   b->rhs = base + o_rhs;
+  b->nsgs = base + o_nsgs;
   b->B.assign(L + 1, nullptr);
   b->Y.assign(L + 1, nullptr);
   b->E.assign(L + 1, nullptr);
@@ -679,6 +797,8 @@ extern "C" int aj_batch_destroy(aj_batch* b) {
     if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
     if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
   }
+  for (aj_mg_plan* p : b->tb)
+    if (p) (void)aj_mg_destroy(p);
   if (b->ws) (void)hipFree(b->ws);
   delete b;
   return AJ_OK;
@@ -714,19 +834,17 @@ extern "C" int aj_batch_step(aj_batch* b, const void* u, const void* v, const vo
   if (!b || !u || !v || !mask || !u_out || !v_out || !p_out || !prm)
     return fail(AJ_ERR_INVALID, "aj_batch_step: null pointer");
   if (u_out == u || v_out == v) return fail(AJ_ERR_INVALID, "aj_batch_step: outputs must not alias inputs");
-  if (!(prm->brinkman_eta > 0) || prm->v_cycles < 1)
-    return fail(AJ_ERR_INVALID, "aj_batch_step: brinkman_eta > 0 and v_cycles >= 1 required");
+  if (int rc = check_params(prm, "aj_batch_step")) return rc;
   hipStream_t st = (hipStream_t)stream;
-  if (prm->tolerance != b->last_tol) {
-    // tolerance**2 in the working precision (multigrid_solver.py:229-231)
-    std::vector<double> t2(b->ninst);
-    const double tt = b->dtype == AJ_F64 ? prm->tolerance * prm->tolerance
-                                         : (double)((float)prm->tolerance * (float)prm->tolerance);
-    for (auto& x : t2) x = tt;
-    AJ_HIP(hipMemcpyAsync(b->tol2, t2.data(), t2.size() * 8, hipMemcpyHostToDevice, st));
-    AJ_HIP(hipStreamSynchronize(st));
-    b->last_tol = prm->tolerance;
+  if (prm->solver == AJ_MG_TEXTBOOK) {
+    // predictor and projection around the to-tolerance solve (its outer
+    // iterations replay the plans' captured graphs)
+    if (int rc = aj_batch_predict(b, u, v, mask, sdf, b->rhs, prm, stream)) return rc;
+    if (int rc = aj_batch_solve(b, b->rhs, nullptr, p_out, prm, stream)) return rc;
+    return aj_batch_project(b, p_out, mask, u_out, v_out, prm, stream);
   }
+  if (int rc = set_tolerance(b, prm->tolerance, st)) return rc;
+  b->last_solver = AJ_MG_REFERENCE;
   auto launches = [&](hipStream_t s) {
     return b->dtype == AJ_F32
                ? batch_step_launches<float>(b, prm, u, v, mask, sdf, u_out, v_out, p_out, s)
@@ -740,8 +858,8 @@ extern "C" int aj_batch_step(aj_batch* b, const void* u, const void* v, const vo
     return rc;
   }
   const void* key[8] = {u, v, mask, sdf, u_out, v_out, p_out, nullptr};
-  const double kp[5] = {prm->dt, prm->U_inf, prm->brinkman_eta, (double)prm->v_cycles,
-                        prm->tolerance};
+  const double kp[7] = {prm->dt,        prm->U_inf,         prm->brinkman_eta, (double)prm->v_cycles,
+                        prm->tolerance, prm->smagorinsky_cs, (double)prm->solver};
   int slot = -1;
   for (int k = 0; k < 2; ++k)
     if (b->exec[k] && !std::memcmp(key, b->key[k], sizeof(key)) &&
@@ -781,11 +899,62 @@ extern "C" int aj_batch_step(aj_batch* b, const void* u, const void* v, const vo
   return AJ_OK;
 }
 
+extern "C" int aj_batch_predict(aj_batch* b, const void* u, const void* v, const void* mask,
+                                const void* sdf, void* rhs, const aj_batch_step_params* prm,
+                                void* stream) {
+  if (!b || !u || !v || !mask || !rhs || !prm)
+    return fail(AJ_ERR_INVALID, "aj_batch_predict: null pointer");
+  if (int rc = check_params(prm, "aj_batch_predict")) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (b->dtype == AJ_F32)
+    return predict_launches<float>(
+        b, step_args<float>(b, prm, u, v, mask, sdf, rhs, nullptr, nullptr, nullptr), st);
+  return predict_launches<double>(
+      b, step_args<double>(b, prm, u, v, mask, sdf, rhs, nullptr, nullptr, nullptr), st);
+}
+
+extern "C" int aj_batch_solve(aj_batch* b, const void* rhs, const void* p0, void* p_out,
+                              const aj_batch_step_params* prm, void* stream) {
+  if (!b || !rhs || !p_out || !prm) return fail(AJ_ERR_INVALID, "aj_batch_solve: null pointer");
+  if (rhs == p_out || (p0 && p0 == p_out))
+    return fail(AJ_ERR_INVALID, "aj_batch_solve: p_out must not alias rhs or p0");
+  if (int rc = check_params(prm, "aj_batch_solve")) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (prm->solver == AJ_MG_TEXTBOOK) return textbook_solve(b, rhs, p0, p_out, prm, st);
+  if (p0) return fail(AJ_ERR_INVALID, "aj_batch_solve: the reference solver starts from zero");
+  if (int rc = set_tolerance(b, prm->tolerance, st)) return rc;
+  b->last_solver = AJ_MG_REFERENCE;
+  if (b->dtype == AJ_F32) return refmg_launches<float>(b, prm, (const float*)rhs, (float*)p_out, st);
+  return refmg_launches<double>(b, prm, (const double*)rhs, (double*)p_out, st);
+}
+
+extern "C" int aj_batch_project(aj_batch* b, const void* p, const void* mask, void* u_out,
+                                void* v_out, const aj_batch_step_params* prm, void* stream) {
+  if (!b || !p || !mask || !u_out || !v_out || !prm)
+    return fail(AJ_ERR_INVALID, "aj_batch_project: null pointer");
+  if (int rc = check_params(prm, "aj_batch_project")) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (b->dtype == AJ_F32)
+    return project_launches<float>(
+        b, step_args<float>(b, prm, nullptr, nullptr, mask, nullptr, nullptr, (void*)p, u_out, v_out),
+        st);
+  return project_launches<double>(
+      b, step_args<double>(b, prm, nullptr, nullptr, mask, nullptr, nullptr, (void*)p, u_out, v_out),
+      st);
+}
+
 extern "C" int aj_batch_get_stats(aj_batch* b, aj_mg_stats* host, void* stream) {
   if (!b || !host) return fail(AJ_ERR_INVALID, "aj_batch_get_stats: null pointer");
   hipStream_t st = (hipStream_t)stream;
-  AJ_HIP(hipMemcpyAsync(host, b->stats, (size_t)b->ninst * sizeof(aj_mg_stats),
-                        hipMemcpyDeviceToHost, st));
+  if (b->last_solver == AJ_MG_TEXTBOOK && !b->tb.empty()) {
+    // instances are numbered group by group, as the plans hold them
+    for (int g = 0; g < b->ng; ++g)
+      AJ_HIP(hipMemcpyAsync(host + b->gfirst[g], b->tb[g]->stats,
+                            (size_t)b->gcount[g] * sizeof(aj_mg_stats), hipMemcpyDeviceToHost, st));
+  } else {
+    AJ_HIP(hipMemcpyAsync(host, b->stats, (size_t)b->ninst * sizeof(aj_mg_stats),
+                          hipMemcpyDeviceToHost, st));
+  }
   AJ_HIP(hipStreamSynchronize(st));
   return AJ_OK;
 }

@@ -10,6 +10,7 @@ struct aj_mg_plan {
   int variant = AJ_MG_REFERENCE;
   int nx = 0, ny = 0;
   int levels = 4;     // requested levels (textbook: 0 = automatic)
+  int batch = 1;      // textbook: independent instances of the grid, stored one after the other
   int ml = 0;         // reference: recursion depth (coarsest level index)
   double dx = 0, dy = 0;
   std::vector<int> lnx, lny;  // level shapes, level 0 = fine grid
@@ -57,4 +58,10 @@ namespace aj {
 int tb_create(aj_mg_plan* p);
 int tb_solve(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out, int max_cycles,
              double tol, hipStream_t st);
+// one to-tolerance solve of several textbook plans on one stream (p0 may be
+// null, or hold null entries), with one host synchronisation per round of
+// convergence checks for all of them
+int tb_solve_many(aj_mg_plan* const* plans, int count, const void* const* rhs,
+                  const void* const* p0, void* const* p_out, int max_cycles, double tol,
+                  hipStream_t st);
 }  // namespace aj

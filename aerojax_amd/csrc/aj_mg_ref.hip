@@ -272,8 +272,8 @@ static int ref_create(aj_mg_plan* p) {
 
 using namespace aj;
 
-extern "C" int aj_mg_create(aj_mg_plan** out, int dtype, int variant, int nx, int ny, double dx,
-                            double dy, int levels) {
+static int mg_create(aj_mg_plan** out, int dtype, int variant, int nx, int ny, double dx,
+                     double dy, int levels, int count) {
   if (!out) return fail(AJ_ERR_INVALID, "aj_mg_create: null plan pointer");
   *out = nullptr;
   if (dtype != AJ_F32 && dtype != AJ_F64) return fail(AJ_ERR_INVALID, "aj_mg_create: bad dtype");
@@ -290,6 +290,7 @@ extern "C" int aj_mg_create(aj_mg_plan** out, int dtype, int variant, int nx, in
   p->dx = dx;
   p->dy = dy;
   p->levels = levels;
+  p->batch = count;
   if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
   const int rc = (variant == AJ_MG_REFERENCE) ? ref_create(p) : tb_create(p);
   if (rc != AJ_OK) {
@@ -300,6 +301,21 @@ extern "C" int aj_mg_create(aj_mg_plan** out, int dtype, int variant, int nx, in
   *out = p;
   return AJ_OK;
 }
+
+extern "C" int aj_mg_create(aj_mg_plan** out, int dtype, int variant, int nx, int ny, double dx,
+                            double dy, int levels) {
+  return mg_create(out, dtype, variant, nx, ny, dx, dy, levels, 1);
+}
+
+extern "C" int aj_mg_create_batched(aj_mg_plan** out, int dtype, int nx, int ny, double dx,
+                                    double dy, int levels, int count) {
+  if (out) *out = nullptr;
+  if (count < 1 || count > 65535)
+    return fail(AJ_ERR_INVALID, "aj_mg_create_batched: count must be in 1..65535");
+  return mg_create(out, dtype, AJ_MG_TEXTBOOK, nx, ny, dx, dy, levels, count);
+}
+
+extern "C" int aj_mg_batch_count(const aj_mg_plan* p) { return p ? p->batch : 0; }
 
 extern "C" int aj_mg_destroy(aj_mg_plan* p) {
   if (!p) return AJ_OK;
@@ -355,6 +371,14 @@ extern "C" int aj_mg_get_stats(aj_mg_plan* p, aj_mg_stats* out, void* stream) {
   if (!p || !out) return fail(AJ_ERR_INVALID, "aj_mg_get_stats: null pointer");
   hipStream_t st = (hipStream_t)stream;
   AJ_HIP(hipMemcpyAsync(out, p->stats, sizeof(aj_mg_stats), hipMemcpyDeviceToHost, st));
+  AJ_HIP(hipStreamSynchronize(st));
+  return AJ_OK;
+}
+
+extern "C" int aj_mg_get_stats_batched(aj_mg_plan* p, aj_mg_stats* out, void* stream) {
+  if (!p || !out) return fail(AJ_ERR_INVALID, "aj_mg_get_stats_batched: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  AJ_HIP(hipMemcpyAsync(out, p->stats, sizeof(aj_mg_stats) * p->batch, hipMemcpyDeviceToHost, st));
   AJ_HIP(hipStreamSynchronize(st));
   return AJ_OK;
 }

@@ -23,6 +23,12 @@
 //   resid: p64' = p64 + e (on the fly), r = b - A p64' in FP64, writes
 //          r32 and p64', FP64 partial sums of r^2
 //   final: norm = sqrt(sum r^2 dx dy), convergence gate, buffer flip
+//
+// A plan may hold `batch` independent instances of the same grid, stored one
+// after the other (instance z at z * cells of every buffer): every kernel
+// takes the instance from blockIdx.z (the per-instance reductions from
+// blockIdx.x), and each instance has its own convergence gate, guess test,
+// tolerance and live FP64 buffer.
 #include "aj_mg.h"
 
 #include <cmath>
@@ -141,8 +147,16 @@ __global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restri
   __shared__ float s_p[RN];
   __shared__ float s_b[RN];
   __shared__ float s_r[TX * TY];
-  const int gated = gate->converged;
   const int n = L.n, m = L.m;
+  {
+    const long z = blockIdx.z;
+    gate += z;
+    if (p_in) p_in += z * n * m;
+    b += z * n * m;
+    Y += z * n * m;
+    Bc += z * (n >> 1) * (m >> 1);
+  }
+  const int gated = gate->converged;
   const int gx0 = blockIdx.y * TX - H, gy0 = blockIdx.x * TY - H;
   const int tid = threadIdx.x;
   {
@@ -203,8 +217,16 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
   constexpr int H = 4, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
   __shared__ float s_p[RN];
   __shared__ float s_b[RN];
-  const int gated = gate->converged;
   const int n = L.n, m = L.m;
+  {
+    const long z = blockIdx.z;
+    gate += z;
+    Yin += z * n * m;
+    Ec += z * nc * mc;
+    b += z * n * m;
+    E += z * n * m;
+  }
+  const int gated = gate->converged;
   const int gx0 = blockIdx.y * TX - H, gy0 = blockIdx.x * TY - H;
   const int tid = threadIdx.x;
   {
@@ -253,8 +275,9 @@ __global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, const
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= (long)nx * ny) return;
   const int i = (int)(c / ny);
-  P[c] = (p0 && i < nx - 1) ? (double)p0[c] : 0.0;
-  bcopy[c] = rhs[c];
+  const long q = (long)blockIdx.z * nx * ny + c;
+  P[q] = (p0 && i < nx - 1) ? (double)p0[q] : 0.0;
+  bcopy[q] = rhs[q];
 }
 
 // p' = P[cur] + e (e may be null), r = b - A p' in FP64 on rows < nx-1.
@@ -278,8 +301,18 @@ __global__ __launch_bounds__(256) void k_tb_resid(const double* __restrict__ src
                                                   double* __restrict__ partial, int nx, int ny,
                                                   double ax, double ay, const aj_mg_stats* gate) {
   __shared__ double sh[16];
-  const int gated = gate->converged;
   const long n = (long)nx * ny;
+  {
+    const long z = blockIdx.z;
+    gate += z;
+    src += z * n;
+    if (dst) dst += z * n;
+    if (e) e += z * n;
+    b += z * n;
+    r32 += z * n;
+    partial += z * gridDim.x;
+  }
+  const int gated = gate->converged;
   const long blk = xcd_block(blockIdx.x, gridDim.x);
   const long c = blk * 256 + threadIdx.x;
   double r = 0.0, pc = 0.0;
@@ -316,6 +349,14 @@ __global__ __launch_bounds__(256) void k_tb_resid_guess(const double* __restrict
                                                         double ax, double ay) {
   __shared__ double sh[16];
   const long n = (long)nx * ny;
+  {
+    const long z = blockIdx.z;
+    P += z * n;
+    b += z * n;
+    r32 += z * n;
+    part_g += z * gridDim.x;
+    part_z += z * gridDim.x;
+  }
   double ag = 0.0, az = 0.0;
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c < n) {
@@ -351,6 +392,14 @@ __global__ void k_tb_guess_final(const double* __restrict__ part_g,
                                  int test, int rel, double* __restrict__ dtol, aj_mg_stats* st,
                                  int* state) {
   __shared__ double sh[16];
+  {
+    const int z = blockIdx.x;  // one workgroup per instance
+    part_g += (long)z * np;
+    part_z += (long)z * np;
+    dtol += z;
+    st += z;
+    state += 2 * z;
+  }
   double ag = 0.0, az = 0.0;
 #pragma unroll 8
   for (int k = threadIdx.x; k < np; k += 256) {
@@ -383,12 +432,14 @@ __global__ __launch_bounds__(256) void k_tb_fallback(double* __restrict__ P,
                                                      float* __restrict__ r32,
                                                      const int* __restrict__ state, int nx,
                                                      int ny) {
-  if (state[1] == 0) return;
+  const long z = blockIdx.z;
+  if (state[2 * z + 1] == 0) return;
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= (long)nx * ny) return;
   const int i = (int)(c / ny);
-  P[c] = 0.0;
-  r32[c] = (i < nx - 1) ? (float)b[c] : 0.0f;
+  const long q = z * nx * ny + c;
+  P[q] = 0.0;
+  r32[q] = (i < nx - 1) ? (float)b[q] : 0.0f;
 }
 
 // flip = 0: initial residual (with `rel`, the tolerance becomes relative
@@ -397,6 +448,13 @@ __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, doub
                                 double* __restrict__ dtol, int flip, int cycles_added, int rel,
                                 aj_mg_stats* st, int* state) {
   __shared__ double sh[16];
+  {
+    const int z = blockIdx.x;  // one workgroup per instance
+    partial += (long)z * np;
+    dtol += z;
+    st += z;
+    state += 2 * z;
+  }
   const int skipped = st->converged;  // this outer iteration was skipped
   double acc = 0.0;
 #pragma unroll 8
@@ -427,12 +485,19 @@ __global__ __launch_bounds__(256) void k_tb_out(const double* __restrict__ P0,
                                                 long n) {
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= n) return;
-  const int s = state[0];
-  const double a0 = P0[c], a1 = P1[c];  // both loads in flight with the state
-  out[c] = (T)(s ? a1 : a0);
+  const long z = blockIdx.z, q = z * n + c;
+  const int s = state[2 * z];
+  const double a0 = P0[q], a1 = P1[q];  // both loads in flight with the state
+  out[q] = (T)(s ? a1 : a0);
 }
 
 __global__ void k_tb_begin(aj_mg_stats* st, int* state, double* dtol, double tol) {
+  {
+    const int z = blockIdx.x;  // one workgroup per instance
+    st += z;
+    state += 2 * z;
+    dtol += z;
+  }
   if (threadIdx.x == 0) {
     dtol[0] = tol;
     st->cycles = 0;
@@ -568,32 +633,33 @@ int tb_create(aj_mg_plan* p) {
   p->ml = L;
 
   const long n0 = (long)p->nx * p->ny;
+  const long nb = p->batch;  // instances: every per-instance buffer holds nb copies
   for (int pass = 0; pass < 2; ++pass) {
     Carve cv{pass == 0 ? nullptr : (char*)p->ws};
     p->tB.assign(L + 1, nullptr);
     p->tY.assign(L + 1, nullptr);
     p->tE.assign(L + 1, nullptr);
     for (int l = 0; l <= L; ++l) {
-      const size_t n = (size_t)p->lnx[l] * p->lny[l];
+      const size_t n = (size_t)p->lnx[l] * p->lny[l] * nb;
       p->tB[l] = (float*)cv.take(n * 4);
       p->tY[l] = (float*)cv.take(n * 4);
       p->tE[l] = (float*)cv.take(n * 4);
     }
-    p->p64[0] = (double*)cv.take(n0 * 8);
-    p->p64[1] = (double*)cv.take(n0 * 8);
+    p->p64[0] = (double*)cv.take(n0 * nb * 8);
+    p->p64[1] = (double*)cv.take(n0 * nb * 8);
     p->coarse_C = (double*)cv.take(Cm.size() * 8);
     p->coarse_Cf = (float*)cv.take(Cf.size() * 4);
     p->coarse_coef = (double*)cv.take(Coef.size() * 8);
     p->coarse_dinv = (double*)cv.take(Dinv.size() * 8);
-    p->coarse_W = (double*)cv.take((size_t)mL * rows * 8);
-    p->coarse_X = (double*)cv.take((size_t)mL * rows * 8);
-    p->npartial = (int)cdiv(n0, 256);
-    p->partial = (double*)cv.take((size_t)p->npartial * 8);
-    p->partial2 = (double*)cv.take((size_t)p->npartial * 8);
-    p->stats = (aj_mg_stats*)cv.take(sizeof(aj_mg_stats));
-    p->state = (int*)cv.take(64);
-    p->dtol = (double*)cv.take(64);
-    p->bcopy = cv.take(n0 * (p->dtype == AJ_F64 ? 8 : 4));
+    p->coarse_W = (double*)cv.take((size_t)mL * rows * nb * 8);
+    p->coarse_X = (double*)cv.take((size_t)mL * rows * nb * 8);
+    p->npartial = (int)cdiv(n0, 256);  // per instance
+    p->partial = (double*)cv.take((size_t)p->npartial * nb * 8);
+    p->partial2 = (double*)cv.take((size_t)p->npartial * nb * 8);
+    p->stats = (aj_mg_stats*)cv.take(sizeof(aj_mg_stats) * nb);
+    p->state = (int*)cv.take(2 * sizeof(int) * nb);
+    p->dtol = (double*)cv.take(sizeof(double) * nb);
+    p->bcopy = cv.take(n0 * nb * (p->dtype == AJ_F64 ? 8 : 4));
     if (pass == 0) {
       p->ws_bytes = cv.off;
       AJ_HIP(hipMalloc(&p->ws, p->ws_bytes));
@@ -603,14 +669,14 @@ int tb_create(aj_mg_plan* p) {
   AJ_HIP(hipMemcpy(p->coarse_Cf, Cf.data(), Cf.size() * 4, hipMemcpyHostToDevice));
   AJ_HIP(hipMemcpy(p->coarse_coef, Coef.data(), Coef.size() * 8, hipMemcpyHostToDevice));
   AJ_HIP(hipMemcpy(p->coarse_dinv, Dinv.data(), Dinv.size() * 8, hipMemcpyHostToDevice));
-  AJ_HIP(hipHostMalloc((void**)&p->host_stats, sizeof(aj_mg_stats), hipHostMallocDefault));
+  AJ_HIP(hipHostMalloc((void**)&p->host_stats, sizeof(aj_mg_stats) * nb, hipHostMallocDefault));
   // pick the smoother tiles of every level now (one cold and one warm
   // V-cycle on a zero right-hand side, gate open), so that the captured
   // outer iteration is built from the tuned launches
   if (p->ml > 0) {
-    AJ_HIP(hipMemset(p->stats, 0, sizeof(aj_mg_stats)));
-    AJ_HIP(hipMemset(p->tB[0], 0, (size_t)p->nx * p->ny * sizeof(float)));
-    AJ_HIP(hipMemset(p->tE[0], 0, (size_t)p->nx * p->ny * sizeof(float)));
+    AJ_HIP(hipMemset(p->stats, 0, sizeof(aj_mg_stats) * nb));
+    AJ_HIP(hipMemset(p->tB[0], 0, (size_t)n0 * nb * sizeof(float)));
+    AJ_HIP(hipMemset(p->tE[0], 0, (size_t)n0 * nb * sizeof(float)));
     if (int rc = tb_vcycle(p, 0, false, nullptr)) return rc;
     if (int rc = tb_vcycle(p, 0, true, nullptr)) return rc;
     AJ_HIP(hipStreamSynchronize(nullptr));
@@ -634,15 +700,16 @@ static int tb_coarse(aj_mg_plan* p, hipStream_t st) {
   c.X = p->coarse_X;
   const aj_mg_stats* gate = p->stats;
   // b read, e written (the per-mode FP64 intermediates stay in L2)
-  ProfScope prof(AJ_PROF_MG_COARSE, (double)c.n * c.m * 8, st);
-  hipLaunchKernelGGL(k_tb_cfwd, dim3(cdiv(c.rows, kCT), cdiv(c.m, kCT)), dim3(256), 0, st, c,
+  ProfScope prof(AJ_PROF_MG_COARSE, (double)c.n * c.m * 8 * p->batch, st);
+  const unsigned nb = (unsigned)p->batch;
+  hipLaunchKernelGGL(k_tb_cfwd, dim3(cdiv(c.rows, kCT), cdiv(c.m, kCT), nb), dim3(256), 0, st, c,
                      (const float*)p->tB[L], gate);
   AJ_LAUNCH_CHECK("k_tb_cfwd");
   const int nt = c.rows <= 128 ? 64 : (c.rows <= 256 ? 128 : (c.rows <= 512 ? 256 : 512));
-  hipLaunchKernelGGL(k_tb_cpcr, dim3(c.m), dim3(nt), (size_t)2 * c.rows * sizeof(double), st, c,
-                     gate);
+  hipLaunchKernelGGL(k_tb_cpcr, dim3(c.m, 1, nb), dim3(nt), (size_t)2 * c.rows * sizeof(double), st,
+                     c, gate);
   AJ_LAUNCH_CHECK("k_tb_cpcr");
-  hipLaunchKernelGGL(k_tb_cinv, dim3(cdiv(c.n, kCT), cdiv(c.m, kCT)), dim3(256), 0, st, c,
+  hipLaunchKernelGGL(k_tb_cinv, dim3(cdiv(c.n, kCT), cdiv(c.m, kCT), nb), dim3(256), 0, st, c,
                      p->tE[L], gate);
   AJ_LAUNCH_CHECK("k_tb_cinv");
   return AJ_OK;
@@ -651,20 +718,20 @@ static int tb_coarse(aj_mg_plan* p, hipStream_t st) {
 // one FP32 V-cycle for A_lev e = tB[lev]; result in tE[lev]; warm = use
 // tE[lev] as the start (the exact coarsest solve ignores it)
 template <int TX, int TY, int NTH>
-static int tb_launch_pre(const TbLevel& Lv, const float* pin, const float* b, float* Y,
+static int tb_launch_pre(const TbLevel& Lv, int nb, const float* pin, const float* b, float* Y,
                          float* Bc, const aj_mg_stats* gate, hipStream_t st) {
-  hipLaunchKernelGGL((k_tb_pre<TX, TY, NTH>), dim3(cdiv(Lv.m, TY), cdiv(Lv.n, TX)), dim3(NTH), 0,
-                     st, Lv, pin, b, Y, Bc, gate);
+  hipLaunchKernelGGL((k_tb_pre<TX, TY, NTH>), dim3(cdiv(Lv.m, TY), cdiv(Lv.n, TX), nb), dim3(NTH),
+                     0, st, Lv, pin, b, Y, Bc, gate);
   AJ_LAUNCH_CHECK("k_tb_pre");
   return AJ_OK;
 }
 
 template <int TX, int TY, int NTH>
-static int tb_launch_post(const TbLevel& Lv, const float* Y, const float* Ec, int nc, int mc,
-                          float gamc, const float* b, float* E, const aj_mg_stats* gate,
+static int tb_launch_post(const TbLevel& Lv, int nb, const float* Y, const float* Ec, int nc,
+                          int mc, float gamc, const float* b, float* E, const aj_mg_stats* gate,
                           hipStream_t st) {
-  hipLaunchKernelGGL((k_tb_post<TX, TY, NTH>), dim3(cdiv(Lv.m, TY), cdiv(Lv.n, TX)), dim3(NTH), 0,
-                     st, Lv, Y, Ec, nc, mc, gamc, b, E, gate);
+  hipLaunchKernelGGL((k_tb_post<TX, TY, NTH>), dim3(cdiv(Lv.m, TY), cdiv(Lv.n, TX), nb),
+                     dim3(NTH), 0, st, Lv, Y, Ec, nc, mc, gamc, b, E, gate);
   AJ_LAUNCH_CHECK("k_tb_post");
   return AJ_OK;
 }
@@ -675,7 +742,7 @@ constexpr int kTbTileCands = 4;
 static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
   if (lev == p->ml) return tb_coarse(p, st);
   const TbLevel Lv = make_level(p, lev);
-  const int n = Lv.n, m = Lv.m;
+  const int n = Lv.n, m = Lv.m, nb = p->batch;
   const aj_mg_stats* gate = p->stats;
   char key[64];
   {
@@ -685,14 +752,15 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
     float *Y = p->tY[lev], *Bc = p->tB[lev + 1];
     auto launch = [&](int c, hipStream_t s) -> int {
       switch (c) {
-        case 1: return tb_launch_pre<16, 64, 512>(Lv, pin, b, Y, Bc, gate, s);
-        case 2: return tb_launch_pre<32, 32, 512>(Lv, pin, b, Y, Bc, gate, s);
-        case 3: return tb_launch_pre<16, 32, 256>(Lv, pin, b, Y, Bc, gate, s);
-        default: return tb_launch_pre<32, 64, 1024>(Lv, pin, b, Y, Bc, gate, s);
+        case 1: return tb_launch_pre<16, 64, 512>(Lv, nb, pin, b, Y, Bc, gate, s);
+        case 2: return tb_launch_pre<32, 32, 512>(Lv, nb, pin, b, Y, Bc, gate, s);
+        case 3: return tb_launch_pre<16, 32, 256>(Lv, nb, pin, b, Y, Bc, gate, s);
+        default: return tb_launch_pre<32, 64, 1024>(Lv, nb, pin, b, Y, Bc, gate, s);
       }
     };
-    std::snprintf(key, sizeof(key), "tb_pre/%dx%d", n, m);
-    ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, (double)n * m * (warm ? 13 : 9), st);
+    std::snprintf(key, sizeof(key), "tb_pre/%dx%dx%d", nb, n, m);
+    ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, (double)n * m * nb * (warm ? 13 : 9),
+                   st);
     int pick = 0;
     if (int rc = tune_pick(key, kTbTileCands, launch, st, &pick)) return rc;
   }
@@ -705,14 +773,14 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
     const float gamc = (float)outlet_gamma(lev + 1);
     auto launch = [&](int c, hipStream_t s) -> int {
       switch (c) {
-        case 1: return tb_launch_post<16, 64, 512>(Lv, Y, Ec, nc, mc, gamc, b, E, gate, s);
-        case 2: return tb_launch_post<32, 32, 512>(Lv, Y, Ec, nc, mc, gamc, b, E, gate, s);
-        case 3: return tb_launch_post<16, 32, 256>(Lv, Y, Ec, nc, mc, gamc, b, E, gate, s);
-        default: return tb_launch_post<32, 64, 1024>(Lv, Y, Ec, nc, mc, gamc, b, E, gate, s);
+        case 1: return tb_launch_post<16, 64, 512>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+        case 2: return tb_launch_post<32, 32, 512>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+        case 3: return tb_launch_post<16, 32, 256>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+        default: return tb_launch_post<32, 64, 1024>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
       }
     };
-    std::snprintf(key, sizeof(key), "tb_post/%dx%d", n, m);
-    ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, (double)n * m * 13, st);
+    std::snprintf(key, sizeof(key), "tb_post/%dx%dx%d", nb, n, m);
+    ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, (double)n * m * nb * 13, st);
     int pick = 0;
     if (int rc = tune_pick(key, kTbTileCands, launch, st, &pick)) return rc;
   }
@@ -726,18 +794,19 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
 // the parity of the enqueue count names the live buffer.
 template <typename T>
 static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
-  const int nx = p->nx, ny = p->ny;
+  const int nx = p->nx, ny = p->ny, B = p->batch;
   const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
   const int nb = p->npartial;
   for (int c = 0; c < kCyclesPerOuter; ++c)
     if (int rc = tb_vcycle(p, 0, c > 0, st)) return rc;
   // p64, e, b read; p64', r32 written
-  ProfScope prof(AJ_PROF_MG_RESID, (double)nx * ny * (8 + 4 + sizeof(T) + 8 + 4), st);
-  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, (const double*)p->p64[parity],
-                     p->p64[parity ^ 1], (const float*)p->tE[0], (const T*)p->bcopy, p->tB[0],
-                     p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
+  ProfScope prof(AJ_PROF_MG_RESID, (double)nx * ny * B * (8 + 4 + sizeof(T) + 8 + 4), st);
+  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb, 1, B), dim3(256), 0, st,
+                     (const double*)p->p64[parity], p->p64[parity ^ 1], (const float*)p->tE[0],
+                     (const T*)p->bcopy, p->tB[0], p->partial, nx, ny, ax, ay,
+                     (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_resid");
-  hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
+  hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
                      p->dx * p->dy, p->dtol, 1, kCyclesPerOuter, 0, p->stats, p->state);
   AJ_LAUNCH_CHECK("k_tb_norm_final");
   return AJ_OK;
@@ -772,74 +841,136 @@ static int tb_outer_iteration(aj_mg_plan* p, int parity, hipStream_t st) {
   return AJ_OK;
 }
 
+// state reset, FP64 copies of the guess and the rhs, initial residual (and
+// the guess test)
 template <typename T>
-static int tb_solve_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int max_cycles,
-                      double tol, hipStream_t st) {
-  const int nx = p->nx, ny = p->ny;
+static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStream_t st) {
+  const int nx = p->nx, ny = p->ny, B = p->batch;
   const long n = (long)nx * ny;
   const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
   const int nb = p->npartial;
-  hipLaunchKernelGGL(k_tb_begin, dim3(1), dim3(64), 0, st, p->stats, p->state, p->dtol, tol);
+  hipLaunchKernelGGL(k_tb_begin, dim3(B), dim3(64), 0, st, p->stats, p->state, p->dtol, tol);
   AJ_LAUNCH_CHECK("k_tb_begin");
-  hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p0, rhs, p->p64[0],
-                     (T*)p->bcopy, nx, ny);
+  hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st, p0, rhs,
+                     p->p64[0], (T*)p->bcopy, nx, ny);
   AJ_LAUNCH_CHECK("k_tb_init");
   // outer iteration 0: residual of the initial guess
   if (p0) {
     // a guess is kept only if its residual is not above the zero guess's
-    hipLaunchKernelGGL(k_tb_resid_guess<T>, dim3(nb), dim3(256), 0, st, (const double*)p->p64[0],
-                       (const T*)p->bcopy, p->tB[0], p->partial, p->partial2, nx, ny, ax, ay);
+    hipLaunchKernelGGL(k_tb_resid_guess<T>, dim3(nb, 1, B), dim3(256), 0, st,
+                       (const double*)p->p64[0], (const T*)p->bcopy, p->tB[0], p->partial,
+                       p->partial2, nx, ny, ax, ay);
     AJ_LAUNCH_CHECK("k_tb_resid_guess");
-    hipLaunchKernelGGL(k_tb_guess_final, dim3(1), dim3(256), 0, st, (const double*)p->partial,
+    hipLaunchKernelGGL(k_tb_guess_final, dim3(B), dim3(256), 0, st, (const double*)p->partial,
                        (const double*)p->partial2, nb, p->dx * p->dy, p->guess_test,
                        p->relative, p->dtol, p->stats, p->state);
     AJ_LAUNCH_CHECK("k_tb_guess_final");
-    hipLaunchKernelGGL(k_tb_fallback<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, p->p64[0],
+    hipLaunchKernelGGL(k_tb_fallback<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st, p->p64[0],
                        (const T*)p->bcopy, p->tB[0], (const int*)p->state, nx, ny);
     AJ_LAUNCH_CHECK("k_tb_fallback");
   } else {
-    hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb), dim3(256), 0, st, (const double*)p->p64[0],
+    hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb, 1, B), dim3(256), 0, st, (const double*)p->p64[0],
                        p->p64[1], (const float*)nullptr, (const T*)p->bcopy, p->tB[0],
                        p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
     AJ_LAUNCH_CHECK("k_tb_resid");
-    hipLaunchKernelGGL(k_tb_norm_final, dim3(1), dim3(256), 0, st, (const double*)p->partial, nb,
+    hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
                        p->dx * p->dy, p->dtol, 0, 0, p->relative, p->stats, p->state);
     AJ_LAUNCH_CHECK("k_tb_norm_final");
   }
-  const int max_outer = (max_cycles + kCyclesPerOuter - 1) / kCyclesPerOuter;
-  if (!p->adaptive) {
-    // fully stream-ordered: every iteration is enqueued, those after
-    // convergence exit at their first instruction
-    for (int k = 0; k < max_outer; ++k)
-      if (int rc = tb_outer_iteration<T>(p, k & 1, st)) return rc;
-  } else {
-    // enqueue the number of iterations the previous solve needed, then read
-    // the device state once and top up one iteration at a time
-    int done = std::min(std::max(p->last_outer, 1), max_outer);
-    for (int k = 0; k < done; ++k)
-      if (int rc = tb_outer_iteration<T>(p, k & 1, st)) return rc;
-    for (;;) {
-      AJ_HIP(hipMemcpyAsync(p->host_stats, p->stats, sizeof(aj_mg_stats), hipMemcpyDeviceToHost, st));
-      AJ_HIP(hipStreamSynchronize(st));
-      if (p->host_stats->converged || p->host_stats->nonfinite || done >= max_outer) break;
-      if (int rc = tb_outer_iteration<T>(p, done & 1, st)) return rc;
-      ++done;
-    }
-    p->last_outer = std::max(p->host_stats->outer, 1);
-  }
-  hipLaunchKernelGGL(k_tb_out<T>, dim3(cdiv(n, 256)), dim3(256), 0, st, (const double*)p->p64[0],
-                     (const double*)p->p64[1], (const int*)p->state, p_out, n);
+  return AJ_OK;
+}
+
+template <typename T>
+static int tb_finish(aj_mg_plan* p, T* p_out, hipStream_t st) {
+  const long n = (long)p->nx * p->ny;
+  hipLaunchKernelGGL(k_tb_out<T>, dim3(cdiv(n, 256), 1, p->batch), dim3(256), 0, st,
+                     (const double*)p->p64[0], (const double*)p->p64[1], (const int*)p->state,
+                     p_out, n);
   AJ_LAUNCH_CHECK("k_tb_out");
   return AJ_OK;
 }
 
+// host copy of the stats read: every instance converged or non-finite
+static bool tb_settled(const aj_mg_plan* p) {
+  for (int z = 0; z < p->batch; ++z) {
+    const aj_mg_stats& s = p->host_stats[z];
+    if (!s.converged && !s.nonfinite) return false;
+  }
+  return true;
+}
+
+static int tb_max_outer(const aj_mg_plan* p) {
+  int o = 1;
+  for (int z = 0; z < p->batch; ++z) o = std::max(o, p->host_stats[z].outer);
+  return o;
+}
+
+template <typename T>
+static int tb_solve_many_t(aj_mg_plan* const* plans, int count, const void* const* rhs,
+                           const void* const* p0, void* const* p_out, int max_cycles, double tol,
+                           hipStream_t st) {
+  const int max_outer = (max_cycles + kCyclesPerOuter - 1) / kCyclesPerOuter;
+  std::vector<int> done(count, 0), open(count, 0);
+  for (int q = 0; q < count; ++q)
+    if (int rc = tb_start<T>(plans[q], (const T*)rhs[q], p0 ? (const T*)p0[q] : nullptr, tol, st))
+      return rc;
+  // non-adaptive plans: every iteration is enqueued (fully stream-ordered;
+  // those after convergence exit at their first instruction).  Adaptive
+  // plans: as many iterations as their previous solve needed, then the
+  // device state of every open plan is read with one synchronisation and
+  // each unsettled plan is topped up by one iteration.
+  for (int q = 0; q < count; ++q) {
+    aj_mg_plan* p = plans[q];
+    done[q] = p->adaptive ? std::min(std::max(p->last_outer, 1), max_outer) : max_outer;
+    open[q] = p->adaptive;
+    for (int k = 0; k < done[q]; ++k)
+      if (int rc = tb_outer_iteration<T>(p, k & 1, st)) return rc;
+  }
+  for (;;) {
+    bool any = false;
+    for (int q = 0; q < count; ++q)
+      if (open[q]) {
+        AJ_HIP(hipMemcpyAsync(plans[q]->host_stats, plans[q]->stats,
+                              sizeof(aj_mg_stats) * plans[q]->batch, hipMemcpyDeviceToHost, st));
+        any = true;
+      }
+    if (!any) break;
+    AJ_HIP(hipStreamSynchronize(st));
+    for (int q = 0; q < count; ++q) {
+      if (!open[q]) continue;
+      aj_mg_plan* p = plans[q];
+      if (tb_settled(p) || done[q] >= max_outer) {
+        open[q] = 0;
+        p->last_outer = tb_max_outer(p);
+        continue;
+      }
+      if (int rc = tb_outer_iteration<T>(p, done[q] & 1, st)) return rc;
+      ++done[q];
+    }
+  }
+  for (int q = 0; q < count; ++q)
+    if (int rc = tb_finish<T>(plans[q], (T*)p_out[q], st)) return rc;
+  return AJ_OK;
+}
+
+int tb_solve_many(aj_mg_plan* const* plans, int count, const void* const* rhs,
+                  const void* const* p0, void* const* p_out, int max_cycles, double tol,
+                  hipStream_t st) {
+  if (count < 1) return AJ_OK;
+  for (int q = 1; q < count; ++q)
+    if (plans[q]->dtype != plans[0]->dtype)
+      return fail(AJ_ERR_INVALID, "textbook multigrid: plans of one solve need one dtype");
+  if (plans[0]->dtype == AJ_F32)
+    return tb_solve_many_t<float>(plans, count, rhs, p0, p_out, max_cycles, tol, st);
+  return tb_solve_many_t<double>(plans, count, rhs, p0, p_out, max_cycles, tol, st);
+}
+
 int tb_solve(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out, int max_cycles,
              double tol, hipStream_t st) {
-  if (p->dtype == AJ_F32)
-    return tb_solve_t<float>(p, (const float*)rhs, (const float*)p0, (float*)p_out, max_cycles,
-                             tol, st);
-  return tb_solve_t<double>(p, (const double*)rhs, (const double*)p0, (double*)p_out, max_cycles,
-                            tol, st);
+  const void* r[1] = {rhs};
+  const void* g[1] = {p0};
+  void* o[1] = {p_out};
+  return tb_solve_many(&p, 1, r, p0 ? g : nullptr, o, max_cycles, tol, st);
 }
 
 }  // namespace aj

@@ -17,6 +17,7 @@
 //               T_k (precomputed in FP64 at plan creation)
 //   k_tb_cinv : e[i][j] = sum_k C[k][j] X[k][i] in FP32 (an orthogonal
 //               transform: FP32 rounding stays at FP32 level)
+// Batched plans: instance blockIdx.z (W, X, b and e offset per instance).
 // Included by aj_mg_tb.hip.
 #pragma once
 
@@ -43,6 +44,12 @@ __global__ __launch_bounds__(256) void k_tb_cfwd(TbCoarse c, const float* __rest
   __shared__ double sc[kCT][kCJC + 1];
   const int i0 = blockIdx.x * kCT, k0 = blockIdx.y * kCT;
   const int t = threadIdx.x, r = t % kCT, kk = t / kCT;
+  {
+    const long z = blockIdx.z;
+    gate += z;
+    b += z * c.n * c.m;
+    c.W += z * c.m * c.rows;
+  }
   const int gated = gate->converged;
   double acc = 0.0;
   for (int j0 = 0; j0 < c.m; j0 += kCJC) {
@@ -79,6 +86,12 @@ __global__ __launch_bounds__(512) void k_tb_cpcr(TbCoarse c, const aj_mg_stats* 
   const int ia = threadIdx.x, ib = threadIdx.x + nt;
   const bool ona = ia < rows, onb = ib < rows;
   const int ja = ona ? ia : 0, jb = onb ? ib : 0;
+  {
+    const long z = blockIdx.z;
+    gate += z;
+    c.W += z * m * rows;
+    c.X += z * m * rows;
+  }
   const int gated = gate->converged;
   double al[2][kCMaxSteps], ga[2][kCMaxSteps];
 #pragma unroll
@@ -127,6 +140,12 @@ __global__ __launch_bounds__(256) void k_tb_cinv(TbCoarse c, float* __restrict__
   __shared__ float sq[kCJC][kCT + 1];
   const int i0 = blockIdx.x * kCT, j0 = blockIdx.y * kCT;
   const int t = threadIdx.x, jj = t % kCT, r = t / kCT;
+  {
+    const long z = blockIdx.z;
+    gate += z;
+    c.X += z * c.m * c.rows;
+    e += z * c.n * c.m;
+  }
   const int gated = gate->converged;
   float acc = 0.0f;
   for (int k0 = 0; k0 < c.m; k0 += kCJC) {

@@ -153,15 +153,13 @@ X* carve(char*& base, size_t& off, size_t count, bool alloc) {
   return r;
 }
 
+// activation buffers for a (B, H, W) input, carved from one allocation that
+// only grows (callers alternating between input shapes reuse it)
 int ensure_act(aj_nn* n, int B, int H, int W) {
   Act& a = n->act;
   if (a.B == B && a.H == H && a.W == W && a.mem) return AJ_OK;
   if (H % (1 << (n->L - 1)) || W % (1 << (n->L - 1)))
     return fail(AJ_ERR_INVALID, "network input H, W must be divisible by 2^(levels-1)");
-  if (a.mem) {
-    (void)hipFree(a.mem);
-    a.mem = nullptr;
-  }
   const int L = n->L;
   for (int pass = 0; pass < 2; ++pass) {
     const bool al = pass == 1;
@@ -204,9 +202,12 @@ int ensure_act(aj_nn* n, int B, int H, int W) {
     a.dout = carve<float>(base, off, P(0), al);
     a.dpre = carve<bf16_t>(base, off, P(0) * 4 * n->c0, al);
     a.loss = carve<float>(base, off, 64, al);
-    if (!al) {
-      a.bytes = off;
+    if (!al && (!a.mem || off > a.bytes)) {
+      if (a.mem) (void)hipFree(a.mem);
+      a.mem = nullptr;
+      a.bytes = 0;
       AJ_HIP(hipMalloc(&a.mem, off));
+      a.bytes = off;
     }
   }
   a.B = B;
@@ -425,24 +426,30 @@ __global__ void k_restrict2(const float* f, int n, int m, float* c) {
   c[e] = 0.25f * ((f[q] + f[q + m]) + (f[q + 1] + f[q + m + 1]));
 }
 
+// per-instance sum of squares: instance blockIdx.y, n values each
 __global__ void k_sumsq(const float* f, long n, double* out) {
   __shared__ double sh[16];
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  f += (long)blockIdx.y * n;
   double v = 0.0;
   if (i < n) v = (double)f[i] * f[i];
   const double t = block_sum(v, sh);
-  if (threadIdx.x == 0) atomicAdd(out, t);
+  if (threadIdx.x == 0) atomicAdd(out + blockIdx.y, t);
 }
 
-__global__ void k_zero_d(double* p) { *p = 0.0; }
+__global__ void k_zero_d(double* p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
 
-// x[p] = (rhs/s, tanh(sdf/sdf_ref)), s = sqrt(mean rhs^2) + 1e-30
-__global__ void k_normalise(const float* rhs, const float* sdf, long n, const double* sumsq,
-                            float sdf_ref, float* x, float* scale_out) {
+// x[p] = (rhs/s, tanh(sdf/sdf_ref)), s = sqrt(mean rhs^2) + 1e-30 per
+// instance (ni values each, n in total)
+__global__ void k_normalise(const float* rhs, const float* sdf, long n, long ni,
+                            const double* sumsq, float sdf_ref, float* x, float* scale_out) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  const float s = (float)(sqrt(*sumsq / (double)n) + 1e-30);
-  if (i == 0 && scale_out) *scale_out = s;
+  if (i == 0 && scale_out) *scale_out = (float)(sqrt(sumsq[0] / (double)ni) + 1e-30);
   if (i >= n) return;
+  const float s = (float)(sqrt(sumsq[i / ni] / (double)ni) + 1e-30);
   x[2 * i] = rhs[i] / s;
   x[2 * i + 1] = tanhf(sdf[i] / sdf_ref);
 }
@@ -455,10 +462,11 @@ __global__ void k_scale_target(const float* p, long n, const double* sumsq, floa
   t[i] = p[i] / (s * pscale);
 }
 
-__global__ void k_unscale(const float* out, long n, const double* sumsq, float pscale, float* p) {
+__global__ void k_unscale(const float* out, long n, long ni, const double* sumsq, float pscale,
+                          float* p) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const float s = (float)(sqrt(*sumsq / (double)n) + 1e-30);
+  const float s = (float)(sqrt(sumsq[i / ni] / (double)ni) + 1e-30);
   p[i] = out[i] * s * pscale;
 }
 
@@ -467,8 +475,10 @@ __global__ void k_unscale(const float* out, long n, const double* sumsq, float p
 // outlet the ghost is gam * last (linear through the fine Dirichlet point)
 __global__ void k_prolong_values(const float* e, int nc, int mc, float gamc, float* f, int nf,
                                  int mf, int fine_is_level0) {
-  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  long q = (long)blockIdx.x * 256 + threadIdx.x;
   if (q >= (long)nf * mf) return;
+  e += (long)blockIdx.y * nc * mc;  // instance blockIdx.y
+  f += (long)blockIdx.y * nf * mf;
   const int i = (int)(q / mf), j = (int)(q - (long)i * mf);
   if (fine_is_level0 && i == nf - 1) { f[q] = 0.f; return; }
   const int I = min(i >> 1, nc - 1), J = min(j >> 1, mc - 1);
@@ -673,9 +683,10 @@ extern "C" int aj_nn_train_step(aj_nn* n, const float* x, const float* t, const 
 
 namespace {
 
-// scratch of the sample/warm-start pipeline: 8 fine-size float slots
-int ensure_ws(aj_nn* n, long nf) {
-  const size_t need = (size_t)(8 * nf + 64) * 4;
+// scratch of the sample/warm-start pipeline: 8 fine-size float slots and
+// the per-instance sums of squares
+int ensure_ws(aj_nn* n, long nf, int B = 1) {
+  const size_t need = (size_t)(8 * nf + 64) * 4 + (size_t)B * 8;
   if (n->ws_bytes >= need) return AJ_OK;
   if (n->ws_mem) (void)hipFree(n->ws_mem);
   n->ws_mem = nullptr;
@@ -726,10 +737,11 @@ extern "C" int aj_nn_make_sample(aj_nn* n, const float* rhs, const float* sdf, c
   if (int rc = restrict_r(sdf, nx, ny, r, S, tmp, st)) return rc;
   if (p && t_out)
     if (int rc = restrict_r(p, nx, ny, r, Pc, tmp, st)) return rc;
-  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, st, sumsq);
+  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, st, sumsq, 1);
   hipLaunchKernelGGL(k_sumsq, dim3(blocks(nc)), dim3(256), 0, st, (const float*)R, nc, sumsq);
   hipLaunchKernelGGL(k_normalise, dim3(blocks(nc)), dim3(256), 0, st, (const float*)R,
-                     (const float*)S, nc, (const double*)sumsq, sdf_ref, x_out, (float*)nullptr);
+                     (const float*)S, nc, nc, (const double*)sumsq, sdf_ref, x_out,
+                     (float*)nullptr);
   if (p && t_out)
     hipLaunchKernelGGL(k_scale_target, dim3(blocks(nc)), dim3(256), 0, st, (const float*)Pc, nc,
                        (const double*)sumsq, pscale, t_out);
@@ -738,40 +750,46 @@ extern "C" int aj_nn_make_sample(aj_nn* n, const float* rhs, const float* sdf, c
 }
 
 // warm start: p0 (fine, nx x ny) = P^r( s*pscale * net(rhs_r/s, tanh(sdf_r/sdf_ref)) )
-// with the multigrid's cell-centred bilinear prolongation (outlet row 0)
-extern "C" int aj_nn_warm_start(aj_nn* n, const float* rhs, const float* sdf, float* p0, int nx,
-                                int ny, int r, float sdf_ref, float pscale, void* stream) {
-  if (!n || !rhs || !sdf || !p0 || r < 0) return fail(AJ_ERR_INVALID, "aj_nn_warm_start: bad args");
+// with the multigrid's cell-centred bilinear prolongation (outlet row 0);
+// B instances stored one after the other, each normalised by its own s.
+// Stacked instances restrict as one (B nx) x ny array: nx is divisible by
+// 2^r, so no 2x2 average straddles two instances.
+extern "C" int aj_nn_warm_start_batched(aj_nn* n, const float* rhs, const float* sdf, float* p0,
+                                        int B, int nx, int ny, int r, float sdf_ref, float pscale,
+                                        void* stream) {
+  if (!n || !rhs || !sdf || !p0 || r < 0 || B < 1 || B > 65535)
+    return fail(AJ_ERR_INVALID, "aj_nn_warm_start: bad args");
   if ((nx >> r) << r != nx || (ny >> r) << r != ny)
     return fail(AJ_ERR_INVALID, "aj_nn_warm_start: grid not divisible by 2^r");
   hipStream_t st = (hipStream_t)stream;
   const int H = nx >> r, W = ny >> r;
-  const long nf = (long)nx * ny, nc = (long)H * W;
-  if (int rc = ensure_ws(n, nf)) return rc;
+  const long ni = (long)H * W, nf = (long)B * nx * ny, nc = (long)B * ni;
+  if (int rc = ensure_ws(n, nf, B)) return rc;
   float* w = (float*)n->ws_mem;
   float *R = w, *S = w + nf, *X = w + 2 * nf, *A = w + 4 * nf, *Bf = w + 5 * nf, *tmp = w + 6 * nf;
-  double* sumsq = (double*)(w + 8 * nf);
-  if (int rc = restrict_r(rhs, nx, ny, r, R, tmp, st)) return rc;
-  if (int rc = restrict_r(sdf, nx, ny, r, S, tmp, st)) return rc;
-  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, st, sumsq);
-  hipLaunchKernelGGL(k_sumsq, dim3(blocks(nc)), dim3(256), 0, st, (const float*)R, nc, sumsq);
+  double* sumsq = (double*)(w + 8 * nf + 64);
+  if (int rc = restrict_r(rhs, B * nx, ny, r, R, tmp, st)) return rc;
+  if (int rc = restrict_r(sdf, B * nx, ny, r, S, tmp, st)) return rc;
+  hipLaunchKernelGGL(k_zero_d, dim3(blocks(B)), dim3(256), 0, st, sumsq, B);
+  hipLaunchKernelGGL(k_sumsq, dim3(blocks(ni), B), dim3(256), 0, st, (const float*)R, ni, sumsq);
   hipLaunchKernelGGL(k_normalise, dim3(blocks(nc)), dim3(256), 0, st, (const float*)R,
-                     (const float*)S, nc, (const double*)sumsq, sdf_ref, X, (float*)nullptr);
+                     (const float*)S, nc, ni, (const double*)sumsq, sdf_ref, X, (float*)nullptr);
   AJ_LAUNCH_CHECK("k_normalise");
-  if (int rc = ensure_act(n, 1, H, W)) return rc;
+  if (int rc = ensure_act(n, B, H, W)) return rc;
   hipLaunchKernelGGL(k_gather, dim3(blocks(nc)), dim3(256), 0, st, (const float*)X,
-                     (const float*)nullptr, (const int*)nullptr, 1, nc, n->act.X0, (float*)nullptr);
+                     (const float*)nullptr, (const int*)nullptr, B, ni, n->act.X0,
+                     (float*)nullptr);
   AJ_LAUNCH_CHECK("k_gather");
-  if (int rc = forward(n, 1, H, W, st)) return rc;
+  if (int rc = forward(n, B, H, W, st)) return rc;
   float* cur = (r == 0) ? p0 : A;
   hipLaunchKernelGGL(k_unscale, dim3(blocks(nc)), dim3(256), 0, st, (const float*)n->act.out, nc,
-                     (const double*)sumsq, pscale, cur);
+                     ni, (const double*)sumsq, pscale, cur);
   AJ_LAUNCH_CHECK("k_unscale");
   int cn = H, cm = W;
   for (int l = r - 1; l >= 0; --l) {
     const int fn = cn * 2, fm = cm * 2;
     float* dst = (l == 0) ? p0 : (cur == A ? Bf : A);
-    hipLaunchKernelGGL(k_prolong_values, dim3(blocks((long)fn * fm)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_prolong_values, dim3(blocks((long)fn * fm), B), dim3(256), 0, st,
                        (const float*)cur, cn, cm, (float)gamma_level(l + 1), dst, fn, fm,
                        l == 0 ? 1 : 0);
     AJ_LAUNCH_CHECK("k_prolong_values");
@@ -780,4 +798,9 @@ extern "C" int aj_nn_warm_start(aj_nn* n, const float* rhs, const float* sdf, fl
     cm = fm;
   }
   return AJ_OK;
+}
+
+extern "C" int aj_nn_warm_start(aj_nn* n, const float* rhs, const float* sdf, float* p0, int nx,
+                                int ny, int r, float sdf_ref, float pscale, void* stream) {
+  return aj_nn_warm_start_batched(n, rhs, sdf, p0, 1, nx, ny, r, sdf_ref, pscale, stream);
 }

@@ -115,17 +115,8 @@ __global__ __launch_bounds__(256) void k_les(const T* __restrict__ u, const T* _
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= ny) return;
-  // periodic jnp.roll neighbours (solver/operators.py:11-14, 35-38)
-  const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
-  const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
   const long c = (long)i * ny;
-  const T du_dx = (u[(long)ip * ny + j] - u[(long)im * ny + j]) * inv_2dx;
-  const T du_dy = (u[c + jp] - u[c + jm]) * inv_2dy;
-  const T dv_dx = (v[(long)ip * ny + j] - v[(long)im * ny + j]) * inv_2dx;
-  const T dv_dy = (v[c + jp] - v[c + jm]) * inv_2dy;
-  const T sxy = T(0.5) * (du_dy + dv_dx);
-  const T smag = sqrt(T(2) * (du_dx * du_dx + dv_dy * dv_dy + T(2) * sxy * sxy) + T(1e-10));
-  const T s = coef * smag;
+  const T s = les_cell(u, v, nx, ny, i, j, inv_2dx, inv_2dy, coef);
   nsgs[c + j] = s;
   if (ntot) ntot[c + j] = nu + s;  // nu_total = nu + nu_sgs (step_handlers.py:76)
 }

@@ -8,6 +8,27 @@
 namespace aj {
 
 // --------------------------------------------------------------------------
+// Smagorinsky eddy viscosity at (i, j) (solver/les_models.py:22-54,
+// 179-200): (Cs delta)^2 sqrt(2 (Sxx^2 + Syy^2 + 2 Sxy^2) + 1e-10) with
+// periodic jnp.roll neighbours (solver/operators.py:11-14, 35-38);
+// coef = (Cs delta)^2
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T les_cell(const T* __restrict__ u, const T* __restrict__ v, int nx,
+                                      int ny, int i, int j, T inv_2dx, T inv_2dy, T coef) {
+  const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
+  const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
+  const long c = (long)i * ny;
+  const T du_dx = (u[(long)ip * ny + j] - u[(long)im * ny + j]) * inv_2dx;
+  const T du_dy = (u[c + jp] - u[c + jm]) * inv_2dy;
+  const T dv_dx = (v[(long)ip * ny + j] - v[(long)im * ny + j]) * inv_2dx;
+  const T dv_dy = (v[c + jp] - v[c + jm]) * inv_2dy;
+  const T sxy = T(0.5) * (du_dy + dv_dx);
+  const T smag = sqrt(T(2) * (du_dx * du_dx + dv_dy * dv_dy + T(2) * sxy * sxy) + T(1e-10));
+  return coef * smag;
+}
+
+// --------------------------------------------------------------------------
 // RK predictor (advection_schemes/rk3_simple_new.py:175-358)
 // --------------------------------------------------------------------------
 

@@ -108,6 +108,18 @@ class WarmStartNet:
         _lib.check(rc, "aj_nn_warm_start")
         return p0
 
+    def warm_start_batched(self, rhs, sdf, p0_out, count, nx, ny):
+        """Initial guesses of ``count`` instances of one (nx, ny) grid stored
+        one after the other in the flat tensors rhs / sdf / p0_out (one
+        inference of batch ``count``)."""
+        with torch.cuda.device(self.device):
+            rc = _lib.load().aj_nn_warm_start_batched(self.handle, rhs.data_ptr(), sdf.data_ptr(),
+                                                      p0_out.data_ptr(), int(count), int(nx),
+                                                      int(ny), self.r, self.sdf_ref, self.pscale,
+                                                      self._s())
+        _lib.check(rc, "aj_nn_warm_start_batched")
+        return p0_out
+
     def __call__(self, rhs, sdf):
         return self.warm_start(rhs, sdf)
 

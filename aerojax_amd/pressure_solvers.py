@@ -24,15 +24,18 @@ _plans_lock = threading.Lock()
 
 
 class MultigridPlan:
-    """Owns one ``aj_mg_plan`` (level hierarchy workspace) on one device."""
+    """Owns one ``aj_mg_plan`` (level hierarchy workspace) on one device.
+    ``count`` > 1 (textbook only): that many independent instances of the
+    grid, stored one after the other in rhs / p0 / p_out, solved together."""
 
     def __init__(self, nx, ny, dx, dy, dtype=torch.float32, variant="reference", levels=4,
-                 device=None):
+                 device=None, count=1):
         A.require_gpu()
         self.nx, self.ny, self.dx, self.dy = int(nx), int(ny), float(dx), float(dy)
         self.dtype = dtype
         self.variant = variant
         self.levels = int(levels)
+        self.count = int(count)
         self.device = torch.device("cuda" if device is None else device)
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -41,8 +44,14 @@ class MultigridPlan:
         h = ctypes.c_void_p()
         lib = _lib.load()
         with torch.cuda.device(self.device):
-            rc = lib.aj_mg_create(ctypes.byref(h), code, var, self.nx, self.ny, self.dx, self.dy,
-                                  self.levels)
+            if self.count != 1:
+                if variant != "textbook":
+                    raise ValueError("batched plans (count > 1) are textbook plans")
+                rc = lib.aj_mg_create_batched(ctypes.byref(h), code, self.nx, self.ny, self.dx,
+                                              self.dy, self.levels, self.count)
+            else:
+                rc = lib.aj_mg_create(ctypes.byref(h), code, var, self.nx, self.ny, self.dx,
+                                      self.dy, self.levels)
         _lib.check(rc, "aj_mg_create")
         self.handle = h
 
@@ -92,6 +101,15 @@ class MultigridPlan:
                                               A.stream_handle(self.device))
         _lib.check(rc, "aj_mg_get_stats")
         return s.as_dict(textbook=self.variant != "reference")
+
+    def stats_all(self) -> list:
+        """Statistics of every instance of the last solve."""
+        s = (_lib.MgStats * self.count)()
+        with torch.cuda.device(self.device):
+            rc = _lib.load().aj_mg_get_stats_batched(self.handle, ctypes.cast(s, ctypes.c_void_p),
+                                                      A.stream_handle(self.device))
+        _lib.check(rc, "aj_mg_get_stats_batched")
+        return [x.as_dict(textbook=self.variant != "reference") for x in s]
 
     def close(self):
         if getattr(self, "handle", None):

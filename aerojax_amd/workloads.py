@@ -102,3 +102,33 @@ def batched_sweep_case(n=256, nx=512, ny=128, seed=0, noise_sigma=1e-3):
     return dict(name=f"batched_{n}x{nx}x{ny}_seed{seed}", grid=grid, masks=masks, sdfs=sdfs,
                 nus=nus, u=us, v=vs, params=desc, dt=1e-3, U_inf=1.0, v_cycles=3,
                 brinkman_eta=0.005)
+
+
+C5_SIZES = ((256, 64), (512, 128), (1024, 256), (2048, 512))
+
+
+def uneven_batch_case(per_size=16, sizes=C5_SIZES, seed=0, noise_sigma=1e-3, cs=0.17):
+    """``per_size`` channels (4x1, dx = dy = 1/ny) of every size in
+    ``sizes``, each with its own obstacle and Reynolds number drawn as in
+    ``warm_start_case`` (cylinder or NACA 4-digit, Re log-uniform in
+    [500, 5e4]); Smagorinsky LES with Cs = ``cs``; one dt = 2.5e-4 for all
+    (advective CFL 0.128 on the finest grid, less on the coarser ones); every
+    pressure solved to ||div|| <= 1e-6.  Returns the ``BatchedStepper``
+    groups and per-instance lists, group by group."""
+    groups, masks, sdfs, nus, us, vs, desc = [], [], [], [], [], [], []
+    k = 0
+    for (nx, ny) in sizes:
+        grid = G.GridParams(nx, ny, 4.0, 1.0)
+        groups.append((nx, ny, per_size, grid.dx, grid.dy))
+        for _ in range(per_size):
+            c = warm_start_case(seed * 100003 + k, nx, ny, noise_sigma)
+            masks.append(c["mask"])
+            sdfs.append(c["sdf"])
+            nus.append(c["nu"])
+            us.append(c["u"])
+            vs.append(c["v"])
+            desc.append((nx, ny, c["desc"], c["Re"]))
+            k += 1
+    name = "uneven_" + "+".join(f"{per_size}x{nx}x{ny}" for nx, ny in sizes) + f"_seed{seed}"
+    return dict(name=name, groups=groups, masks=masks, sdfs=sdfs, nus=nus, u=us, v=vs,
+                params=desc, dt=2.5e-4, U_inf=1.0, cs=cs, tol=1e-6, brinkman_eta=0.005)

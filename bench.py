@@ -120,6 +120,121 @@ def run_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def train_net(args, dev, nx=2048, ny=512):
+    """Warm-start network trained on solver-generated pairs at nx x ny."""
+    import torch
+    from aerojax_amd.nn import WarmStartNet
+    from aerojax_amd.warmstart import generate_pairs, normalise_targets, train
+    net = WarmStartNet(base_channels=32, levels=4, seed=0, r=args.nn_r, device=dev)
+    t0 = time.perf_counter()
+    X, T = generate_pairs(net, args.nn_pairs, range(1000, 100000), nx, ny,
+                          steps_per_seed=40, stride=2, skip=2)
+    t_gen = time.perf_counter() - t0
+    log(f"generated {X.shape[0]} pairs at {X.shape[1]}x{X.shape[2]} in {t_gen:.1f}s")
+    pscale = normalise_targets(net, T)
+    t0 = time.perf_counter()
+    hist = train(net, X, T, epochs=args.nn_epochs, batch=16, lr=5e-4, log=log)
+    t_train = time.perf_counter() - t0
+    del X, T
+    torch.cuda.empty_cache()
+    info = {"train_pairs": args.nn_pairs, "train_epochs": args.nn_epochs, "train_loss": hist,
+            "gen_s": round(t_gen, 1), "train_s": round(t_train, 1), "target_scale": pscale,
+            "params": net.num_params}
+    return net, info
+
+
+def run_uneven(args):
+    """Batched uneven + learned: 16 instances of each of 256x64, 512x128,
+    1024x256 and 2048x512 (obstacle and Re drawn per instance), Smagorinsky
+    LES, every pressure solved to ||div|| <= 1e-6 by the batched textbook
+    multigrid; plain arm (zero guesses) and NN arm (one network inference
+    per group each step).  Prints one JSON line."""
+    import torch
+
+    import aerojax_amd as aj
+    from aerojax_amd import _lib, workloads
+
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    w = workloads.uneven_batch_case(per_size=args.per_size, seed=args.seed)
+    n = len(w["masks"])
+    cells = sum(nx * ny * c for nx, ny, c, _, _ in w["groups"])
+    log(f"workload {w['name']}: {n} instances, {cells} cells, built in "
+        f"{time.perf_counter() - t0:.1f}s")
+
+    def arm(net):
+        b = aj.BatchedStepper(w["groups"], w["masks"], w["sdfs"], w["nus"], w["u"], w["v"],
+                              w["dt"], solver="textbook", tolerance=w["tol"], max_cycles=60,
+                              smagorinsky_cs=w["cs"], brinkman_eta=w["brinkman_eta"],
+                              warm_start=net, device=dev)
+        for _ in range(args.warmup):
+            b.step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            b.step()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        st = b.stats()
+        assert all(s["converged"] and s["norm"] <= w["tol"] for s in st), \
+            [(k, s["norm"]) for k, s in enumerate(st) if not s["converged"]]
+        return b, el, st
+
+    bp, el_plain, st_plain = arm(None)
+    sps_plain = args.steps / el_plain
+    cyc = {}
+    k = 0
+    for nx, ny, c, _, _ in w["groups"]:
+        cyc[f"{nx}x{ny}"] = sorted({s["cycles"] for s in st_plain[k:k + c]})
+        k += c
+    log(f"plain: {sps_plain * n:.0f} instance-steps/s ({1e3 / sps_plain:.2f} ms per batched "
+        f"step), V-cycles per group {cyc}")
+    _lib.profile_begin(50000)
+    for _ in range(3):
+        bp.step()
+    torch.cuda.synchronize()
+    _lib.profile_end()
+    prof = _lib.profile_read()
+    log("kernel profile (plain): " + ", ".join(f"{k} {v['avg_us']:.1f}us x{v['count']} "
+                                                 f"{v['gbps']:.0f} GB/s" for k, v in prof.items()))
+    bp.close()
+    del bp
+    torch.cuda.empty_cache()
+    nn_info, sps_nn, el_nn = None, None, None
+    if not args.no_nn:
+        net, nn_info = train_net(args, dev)
+        bn, el_nn, st_nn = arm(net)
+        sps_nn = args.steps / el_nn
+        used = [bool(s["used_guess"]) for s in st_nn]
+        ratio = [s["guess_norm"] / s["zero_norm"] for s in st_nn if s["zero_norm"] > 0]
+        nn_info.update(instance_steps_per_s=sps_nn * n, guesses_used=sum(used),
+                       guess_over_zero_residual_median=float(sorted(ratio)[len(ratio) // 2])
+                       if ratio else None)
+        log(f"NN: {sps_nn * n:.0f} instance-steps/s, guesses kept {sum(used)}/{n}")
+        bn.close()
+    hbm = {k: v for k, v in prof.items() if not k.startswith("nn_") and k != "mg_coarse"}
+    dom = max(hbm, key=lambda k: hbm[k]["ms"])
+    d = hbm[dom]
+    sps = sps_nn if sps_nn is not None else sps_plain
+    el = el_nn if el_nn is not None else el_plain
+    line = {"metric": f"instance-steps/s, {n} uneven instances "
+                      "({256x64,512x128,1024x256,2048x512}) + LES to ||div||<=1e-6",
+            "value": sps * n, "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": w["name"], "instances": n, "cells": cells,
+                       "arm": "nn_warm_start" if sps_nn is not None else "plain_mg",
+                       "plain_instance_steps_per_s": sps_plain * n,
+                       "plain_cell_updates_per_s": sps_plain * cells,
+                       "plain_vcycles_per_group": cyc, "nn": nn_info},
+            "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gbps"], "peak": 8000.0,
+                         "unit": "GB/s", "frac": d["gbps"] / 8000.0, "traffic": None},
+            "kernels": {k: {"avg_us": round(v["avg_us"], 2), "gbps": round(v["gbps"], 1)}
+                        for k, v in prof.items()}}
+    print(json.dumps(line), flush=True)
+
+
 def run_large(args):
     """4096x1024 cylinder at Re 1e5 with Smagorinsky LES (Cs 0.17), textbook
     multigrid in FP32 inside FP64 defect correction to a relative residual
@@ -184,11 +299,15 @@ def main():
     ap.add_argument("--nn-pairs", type=int, default=2000)
     ap.add_argument("--nn-epochs", type=int, default=3)
     ap.add_argument("--nn-r", type=int, default=2, help="network runs on the grid restricted r times")
-    ap.add_argument("--workload", choices=["warmstart", "batched", "large"], default="warmstart",
+    ap.add_argument("--workload", choices=["warmstart", "batched", "large", "uneven"],
+                    default="warmstart",
                     help="warmstart: 2048x512 to ||div|| <= 1e-6, NN vs plain (default); "
                          "batched: 256 x 512x128 reference steps in shared launches; "
-                         "large: 4096x1024 + LES to relative residual 1e-8, FP32 vs FP64")
+                         "large: 4096x1024 + LES to relative residual 1e-8, FP32 vs FP64; "
+                         "uneven: 16 each of 256x64..2048x512 + LES to ||div|| <= 1e-6, "
+                         "NN vs plain")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--per-size", type=int, default=16, help="uneven: instances per grid size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 5 times (reference traffic for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -209,6 +328,10 @@ def main():
     if args.workload == "large":
         torch.cuda.set_device(0)
         run_large(args)
+        return
+    if args.workload == "uneven":
+        torch.cuda.set_device(0)
+        run_uneven(args)
         return
     if args.copy_calibration:
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -238,30 +361,14 @@ def main():
     nn_info = None
     sps_nn = None
     if not args.no_nn:
-        from aerojax_amd.nn import WarmStartNet
-        from aerojax_amd.warmstart import generate_pairs, normalise_targets, train
-        net = WarmStartNet(base_channels=32, levels=4, seed=0, r=args.nn_r, device=dev)
-        t0 = time.perf_counter()
-        X, T = generate_pairs(net, args.nn_pairs, range(1000, 100000), nx, ny,
-                              steps_per_seed=40, stride=2, skip=2)
-        t_gen = time.perf_counter() - t0
-        log(f"generated {X.shape[0]} pairs at {X.shape[1]}x{X.shape[2]} in {t_gen:.1f}s")
-        pscale = normalise_targets(net, T)
-        t0 = time.perf_counter()
-        hist = train(net, X, T, epochs=args.nn_epochs, batch=16, lr=5e-4, log=log)
-        t_train = time.perf_counter() - t0
-        del X, T
-        torch.cuda.empty_cache()
+        net, tinfo = train_net(args, dev, nx, ny)
         sps_nn, el_nn, stp_nn, _ = run_arm(aj, case, dev, args.steps, args.warmup, warm_start=net)
         cyc_nn, used, ratio = cycles_per_step(aj, stp_nn, 5)
         log(f"NN warm start: {sps_nn:.1f} steps/s, V-cycles/step {cyc_nn}, guess used {used}, "
             f"guess/zero residual {['%.2f' % r for r in ratio]}")
         nn_info = {"steps_per_s": sps_nn, "vcycles_per_step": cyc_nn, "guess_used": used,
-                   "guess_over_zero_residual": ratio, "train_pairs": args.nn_pairs,
-                   "train_epochs": args.nn_epochs, "train_loss": hist, "gen_s": round(t_gen, 1),
-                   "train_s": round(t_train, 1), "net_resolution": [nx >> args.nn_r, ny >> args.nn_r],
-                   "target_scale": pscale,
-                   "params": net.num_params}
+                   "guess_over_zero_residual": ratio,
+                   "net_resolution": [nx >> args.nn_r, ny >> args.nn_r], **tinfo}
 
     # ---- live per-kernel timing (HIP event pairs on the launching stream)
     stp = stp_nn if sps_nn is not None else stp_plain

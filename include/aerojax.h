@@ -228,8 +228,20 @@ typedef struct aj_mg_stats {
 int aj_poisson_mg(aj_mg_plan* plan, const void* rhs, const void* p0,
                   void* p_out, int max_cycles, double tolerance, void* stream);
 /* Copy the statistics of the last aj_poisson_mg call on this plan
- * (synchronises `stream`). */
+ * (synchronises `stream`; instance 0 of a batched plan). */
 int aj_mg_get_stats(aj_mg_plan* plan, aj_mg_stats* out, void* stream);
+/* Batched TEXTBOOK plan: `count` independent instances of one nx x ny grid
+ * (no counterpart in the reference, which solves one grid per call).
+ * aj_poisson_mg then takes rhs, p0 and p_out holding the instances one after
+ * the other (instance k at k * nx * ny); every instance is solved to the
+ * tolerance with its own convergence gate, guess test and statistics, all in
+ * shared launches (instances in the grid's third dimension). */
+int aj_mg_create_batched(aj_mg_plan** plan, int dtype, int nx, int ny, double dx, double dy,
+                         int levels, int count);
+int aj_mg_batch_count(const aj_mg_plan* plan);
+/* statistics of every instance of the last solve (`count` entries;
+ * synchronises `stream`) */
+int aj_mg_get_stats_batched(aj_mg_plan* plan, aj_mg_stats* out, void* stream);
 
 /* ---- Batched stepping ---------------------------------------------------
  * Many independent simulations of the reference step (step_pure's
@@ -243,14 +255,23 @@ int aj_mg_get_stats(aj_mg_plan* plan, aj_mg_stats* out, void* stream);
  * multigrid gate (sum r^2 dx dy < tol^2) is per instance; the step's launch
  * sequence is captured once into a hipGraph per set of buffers
  * (AJ_MG_OPT_GRAPH via aj_batch_set_option).  All groups need the same
- * multigrid depth.  No counterpart in the reference (which steps one grid
- * per call); the per-instance arithmetic is that of the single-grid entry
- * points. */
+ * reference multigrid depth.  No counterpart in the reference (which steps
+ * one grid per call); the per-instance arithmetic is that of the
+ * single-grid entry points.
+ *
+ * solver = AJ_MG_TEXTBOOK solves every instance's pressure to `tolerance`
+ * (||b - A p|| in FP64, at most v_cycles V-cycles) with one batched textbook
+ * plan per group (aj_mg_create_batched, automatic hierarchy); the solve
+ * synchronises the stream once per round of convergence checks, so such a
+ * step is not captured as a whole.  smagorinsky_cs > 0 computes the constant
+ * Smagorinsky nu_sgs (delta = sqrt(dx dy)) of every instance and feeds it to
+ * the RK step (rk_step_unified's nu_sgs, rk3_simple_new.py:304). */
 typedef struct aj_batch aj_batch;
 typedef struct aj_batch_step_params {
   double dt, U_inf, brinkman_eta, tolerance;
-  int v_cycles;
-  int reserved;
+  int v_cycles;           /* REFERENCE: V-cycles per step; TEXTBOOK: V-cycle cap */
+  int solver;             /* AJ_MG_REFERENCE (0) or AJ_MG_TEXTBOOK                */
+  double smagorinsky_cs;  /* 0: no LES                                             */
 } aj_batch_step_params;
 int aj_batch_create(aj_batch** batch, int dtype, int ngroups, const int* nx, const int* ny,
                     const int* count, const double* dx, const double* dy, int levels);
@@ -265,6 +286,19 @@ int aj_batch_set_option(aj_batch* batch, int option, int value);
 int aj_batch_step(aj_batch* batch, const void* u, const void* v, const void* mask,
                   const void* sdf, void* u_out, void* v_out, void* p_out,
                   const aj_batch_step_params* prm, void* stream);
+/* The step in three calls, for callers that act between them (a learned
+ * initial guess): aj_batch_predict runs [LES +] RK3 and writes rhs =
+ * div(u*, v*) * mask / dt (u*, v* stay in the batch); aj_batch_solve solves
+ * every instance's pressure (REFERENCE: p0 must be NULL; TEXTBOOK: p0 NULL or
+ * one guess per instance, each kept only if its residual is not above the
+ * zero guess's); aj_batch_project applies the projection to the u*, v* of the
+ * last aj_batch_predict. */
+int aj_batch_predict(aj_batch* batch, const void* u, const void* v, const void* mask,
+                     const void* sdf, void* rhs, const aj_batch_step_params* prm, void* stream);
+int aj_batch_solve(aj_batch* batch, const void* rhs, const void* p0, void* p_out,
+                   const aj_batch_step_params* prm, void* stream);
+int aj_batch_project(aj_batch* batch, const void* p, const void* mask, void* u_out, void* v_out,
+                     const aj_batch_step_params* prm, void* stream);
 /* per-instance multigrid statistics of the last step (synchronises) */
 int aj_batch_get_stats(aj_batch* batch, aj_mg_stats* stats, void* stream);
 
@@ -291,7 +325,8 @@ enum {
   AJ_PROF_NN_CONV = 9,     /* network 3x3 conv (forward / input grad);  */
                            /* "bytes" holds FLOPs for this tag          */
   AJ_PROF_NN_DW = 10,      /* network weight gradient (FLOPs)           */
-  AJ_PROF_NUM_TAGS = 11
+  AJ_PROF_LES = 11,        /* Smagorinsky nu_sgs (batched step)         */
+  AJ_PROF_NUM_TAGS = 12
 };
 int aj_profile_begin(int max_records);
 int aj_profile_end(void);
@@ -340,6 +375,10 @@ int aj_nn_make_sample(aj_nn* net, const float* rhs, const float* sdf, const floa
  * cell-centred bilinear one (outlet row 0).  Stream-ordered, no sync. */
 int aj_nn_warm_start(aj_nn* net, const float* rhs, const float* sdf, float* p0, int nx, int ny,
                      int r, float sdf_ref, float pscale, void* stream);
+/* The same for B instances of one grid stored one after the other (each
+ * normalised by its own rhs RMS), in one network inference of batch B. */
+int aj_nn_warm_start_batched(aj_nn* net, const float* rhs, const float* sdf, float* p0, int B,
+                             int nx, int ny, int r, float sdf_ref, float pscale, void* stream);
 
 #ifdef __cplusplus
 }

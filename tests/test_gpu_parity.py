@@ -497,7 +497,7 @@ def test_autotuned_launches_are_bit_identical(gpu):
         if on:
             rep = _lib.tune_report()
             print(rep, flush=True)
-            assert "rk/f32/rk3/512x128" in rep and "tb_pre/512x128" in rep
+            assert "rk/f32/rk3/512x128" in rep and "tb_pre/1x512x128" in rep
     _lib.load().aj_tune_enable(1)
     for a, bb in zip(res[0], res[1]):
         assert torch.equal(a, bb)
