@@ -33,7 +33,8 @@ AJ_MG_OPT_RELATIVE = 3
 # Every symbol the header declares; tests check the library exports them.
 EXPORTS = (
     "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
-    "aj_les_smagorinsky_total", "aj_divergence_rhs", "aj_project", "aj_project_collocated",
+    "aj_les_smagorinsky_total", "aj_divergence_rhs", "aj_divergence_periodic", "aj_project",
+    "aj_project_collocated",
     "aj_flow_diagnostics", "aj_tune_enable", "aj_tune_reset", "aj_tune_report",
     "aj_batch_create", "aj_batch_destroy", "aj_batch_cells",
     "aj_batch_num_instances", "aj_batch_offsets", "aj_batch_set_viscosity",
@@ -71,7 +72,7 @@ class RkParams(ctypes.Structure):
 class FlowDiag(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("pressure_residual", "rhs_norm",
                                               "relative_residual", "div_rms", "l2_div",
-                                              "fluid_cells")]
+                                              "fluid_cells", "max_speed", "cfl")]
 
     def as_dict(self):
         return {n: float(getattr(self, n)) for n, _ in self._fields_}
@@ -121,6 +122,7 @@ _SIGS = {
     "aj_les_smagorinsky": (_i, [_i, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
     "aj_les_smagorinsky_total": (_i, [_i, _vp, _vp, _d, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
     "aj_divergence_rhs": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _vp]),
+    "aj_divergence_periodic": (_i, [_i, _vp, _vp, _vp, _i, _i, _d, _d, _vp]),
     "aj_project": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
     "aj_project_collocated": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d,
                                    _vp]),

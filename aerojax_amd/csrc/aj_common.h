@@ -81,4 +81,20 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return t;  // valid in threads 0..63
 }
 
+// Block-wide maximum of non-negative doubles (same shape rules).
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  double t = 0.0;
+  if (threadIdx.x < 64) {
+    t = (lane < nw) ? sh[lane] : 0.0;
+    for (int off = 32; off > 0; off >>= 1) t = fmax(t, __shfl_xor(t, off, 64));
+  }
+  __syncthreads();
+  return t;  // valid in threads 0..63
+}
+
 }  // namespace aj

@@ -8,8 +8,10 @@
 //     div = np.gradient(u, dx, axis 0) + np.gradient(v, dy, axis 1)
 //     (central inside, one-sided first order on the edges), fluid = mask >
 //     0.99, div_rms = sqrt(sum (div fluid)^2 / (sum fluid + 1e-8)),
-//     l2_div = sqrt(sum (div fluid)^2 dx dy).
-// FP64 partial sums per block, fixed-order final sum.
+//     l2_div = sqrt(sum (div fluid)^2 dx dy);
+//   * check_cfl (advection_schemes/utils.py:14-20): max_speed =
+//     max sqrt(u^2 + v^2), cfl = max(max_speed dt / dx, max_speed dt / dy).
+// FP64 partial sums (and maxima) per block, fixed-order final reduction.
 #include "aj_common.h"
 
 namespace aj {
@@ -22,7 +24,7 @@ __global__ __launch_bounds__(256) void k_diag(const T* __restrict__ u, const T* 
   __shared__ double sh[16];
   const long n = (long)nx * ny;
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
-  double res2 = 0.0, rhs2 = 0.0, div2 = 0.0, fluid = 0.0;
+  double res2 = 0.0, rhs2 = 0.0, div2 = 0.0, fluid = 0.0, speed = 0.0;
   if (c < n) {
     const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
     // rhs = divergence_nonperiodic(u, v) / dt: zero on the border
@@ -49,30 +51,39 @@ __global__ __launch_bounds__(256) void k_diag(const T* __restrict__ u, const T* 
       div2 = (gx + gy) * (gx + gy);
       fluid = 1.0;
     }
+    const double uc = u[c], vc = v[c];
+    speed = sqrt(uc * uc + vc * vc);
   }
   const double a = block_sum(res2, sh);
   const double b = block_sum(rhs2, sh);
   const double d = block_sum(div2, sh);
   const double f = block_sum(fluid, sh);
+  const double s = block_max(speed, sh);
   if (threadIdx.x == 0) {
-    double* o = part + 4 * (long)blockIdx.x;
+    double* o = part + 5 * (long)blockIdx.x;
     o[0] = a;
     o[1] = b;
     o[2] = d;
     o[3] = f;
+    o[4] = s;
   }
 }
 
 __global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ part, int np,
-                                                    double dx, double dy,
+                                                    double dx, double dy, double dt,
                                                     aj_flow_diag* __restrict__ out) {
   __shared__ double sh[16];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k = threadIdx.x; k < np; k += 256)
-    for (int q = 0; q < 4; ++q) acc[q] += part[4 * (long)k + q];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0}, smax = 0.0;
+  for (int k = threadIdx.x; k < np; k += 256) {
+    for (int q = 0; q < 4; ++q) acc[q] += part[5 * (long)k + q];
+    smax = fmax(smax, part[5 * (long)k + 4]);
+  }
   double t[4];
   for (int q = 0; q < 4; ++q) t[q] = block_sum(acc[q], sh);
+  const double vmax = block_max(smax, sh);
   if (threadIdx.x == 0) {
+    out->max_speed = vmax;
+    out->cfl = fmax(vmax * dt / dx, vmax * dt / dy);
     out->pressure_residual = sqrt(t[0]);
     out->rhs_norm = sqrt(t[1]);
     out->relative_residual = sqrt(t[0]) / (sqrt(t[1]) + 1e-10);
@@ -95,9 +106,9 @@ extern "C" int aj_flow_diagnostics(int dtype, const void* u, const void* v, cons
   const long n = (long)nx * ny;
   const int nb = (int)((n + 255) / 256);
   void* buf = nullptr;
-  AJ_HIP(hipMallocAsync(&buf, (size_t)nb * 4 * sizeof(double) + sizeof(aj_flow_diag), st));
+  AJ_HIP(hipMallocAsync(&buf, (size_t)nb * 5 * sizeof(double) + sizeof(aj_flow_diag), st));
   double* part = (double*)buf;
-  aj_flow_diag* dres = (aj_flow_diag*)(part + 4 * (long)nb);
+  aj_flow_diag* dres = (aj_flow_diag*)(part + 5 * (long)nb);
   if (dtype == AJ_F32)
     hipLaunchKernelGGL(k_diag<float>, dim3(nb), dim3(256), 0, st, (const float*)u,
                        (const float*)v, (const float*)p, (const float*)mask, nx, ny, dx, dy, dt,
@@ -111,7 +122,7 @@ extern "C" int aj_flow_diagnostics(int dtype, const void* u, const void* v, cons
     return fail(AJ_ERR_INVALID, "aj_flow_diagnostics: bad dtype");
   }
   AJ_LAUNCH_CHECK("k_diag");
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, (const double*)part, nb, dx, dy,
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, (const double*)part, nb, dx, dy, dt,
                      dres);
   AJ_LAUNCH_CHECK("k_diag_final");
   AJ_HIP(hipMemcpyAsync(out, dres, sizeof(aj_flow_diag), hipMemcpyDeviceToHost, st));

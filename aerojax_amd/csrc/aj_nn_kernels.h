@@ -54,6 +54,8 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 // and reused across the wave's (PH/4)(PW/16) pixel sub-tiles.  LDS rows are
 // 32 bf16 + 8 pad (80 B): the 16 lanes of a quarter-wave then hit 16
 // distinct 4-bank groups, so the 16-byte fragment reads are conflict-free.
+// The next slice's global loads are issued before the current slice's
+// MFMAs, so their latency overlaps the arithmetic.
 // ---------------------------------------------------------------------------
 struct ConvArgs {
   const bf16_t* in;
@@ -80,6 +82,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   constexpr int WR = PH / 4;         // pixel rows per wave
   constexpr int SX = PW / 16;        // 16-pixel sub-tiles per row
   constexpr int MS = WR * SX, NS = NT / 16;
+  constexpr int NXL = (RP * 4 + 255) / 256;      // 16-byte input loads per thread and slice
+  constexpr int NWL = (9 * NT * 4 + 255) / 256;  // 16-byte weight loads per thread and slice
   __shared__ __attribute__((aligned(16))) bf16_t sx[RP * kLdsRow];
   __shared__ __attribute__((aligned(16))) bf16_t sw[9 * NT * kLdsRow];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -92,32 +96,65 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   const int h0 = th * PH, w0 = tw * PW;
   const int co0 = blockIdx.y * NT;
   const int kq = (lane >> 4) * 8;
+  const int ns = a.ci / 32;
   floatx4 acc[MS][NS];
 #pragma unroll
   for (int m = 0; m < MS; ++m)
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[m][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   const bf16_t* img = a.in + (long)b * a.H * a.W * a.ldi;
-  for (int c0 = 0; c0 < a.ci; c0 += 32) {
-    // input patch with a one-pixel halo, zero outside the image
-    for (int e = tid; e < RP * 4; e += 256) {
-      const int px = e >> 2, q = e & 3;
-      const int rr = px / RW, cc = px - rr * RW;
-      const int hh = h0 + rr - 1, ww = w0 + cc - 1;
+  uint4 rx[NXL], rw[NWL];
+  // global -> registers: the input patch (one-pixel halo, zero outside the
+  // image) and the slice's weights, row (tap, co) holding 32 channels
+  auto fetch = [&](int c0) {
+#pragma unroll
+    for (int k = 0; k < NXL; ++k) {
+      const int e = tid + k * 256;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
-        v = *reinterpret_cast<const uint4*>(img + ((long)hh * a.W + ww) * a.ldi + c0 + q * 8);
-      *reinterpret_cast<uint4*>(sx + px * kLdsRow + q * 8) = v;
+      if (e < RP * 4) {
+        const int px = e >> 2, q = e & 3;
+        const int rr = px / RW, cc = px - rr * RW;
+        const int hh = h0 + rr - 1, ww = w0 + cc - 1;
+        if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
+          v = *reinterpret_cast<const uint4*>(img + ((long)hh * a.W + ww) * a.ldi + c0 + q * 8);
+      }
+      rx[k] = v;
     }
-    // weights of the slice: row (tap, co) holds 32 channels
-    for (int e = tid; e < 9 * NT * 4; e += 256) {
-      const int q = e & 3, r = e >> 2;
-      const int col = r / 9, tap = r - col * 9;
-      const uint4 v = *reinterpret_cast<const uint4*>(a.w + ((long)(co0 + col) * 9 + tap) * a.ci +
-                                                      c0 + q * 8);
-      *reinterpret_cast<uint4*>(sw + (tap * NT + col) * kLdsRow + q * 8) = v;
+#pragma unroll
+    for (int k = 0; k < NWL; ++k) {
+      const int e = tid + k * 256;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (e < 9 * NT * 4) {
+        const int q = e & 3, r = e >> 2;
+        const int col = r / 9, tap = r - col * 9;
+        v = *reinterpret_cast<const uint4*>(a.w + ((long)(co0 + col) * 9 + tap) * a.ci + c0 + q * 8);
+      }
+      rw[k] = v;
     }
+  };
+  // registers -> LDS
+  auto stash = [&]() {
+#pragma unroll
+    for (int k = 0; k < NXL; ++k) {
+      const int e = tid + k * 256;
+      if (e < RP * 4) *reinterpret_cast<uint4*>(sx + (e >> 2) * kLdsRow + (e & 3) * 8) = rx[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NWL; ++k) {
+      const int e = tid + k * 256;
+      if (e < 9 * NT * 4) {
+        const int q = e & 3, r = e >> 2;
+        const int col = r / 9, tap = r - col * 9;
+        *reinterpret_cast<uint4*>(sw + (tap * NT + col) * kLdsRow + q * 8) = rw[k];
+      }
+    }
+  };
+  fetch(0);
+  for (int s = 0; s < ns; ++s) {
+    if (s > 0) __syncthreads();  // every wave is done with the previous slice
+    stash();
     __syncthreads();
+    if (s + 1 < ns) fetch((s + 1) * 32);
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
       const int dy = tap / 3, dx = tap - (tap / 3) * 3;
@@ -136,7 +173,6 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
           acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bf[j], acc[m][j], 0, 0, 0);
       }
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int j = 0; j < NS; ++j) {

@@ -287,6 +287,42 @@ extern "C" int aj_divergence_rhs(int dtype, const void* us, const void* vs, cons
   return AJ_OK;
 }
 
+// periodic divergence grad_x(u) + grad_y(v) with jnp.roll wrap
+// (solver/operators.py:11-14, 35-38, 76-78)
+template <typename T>
+__global__ __launch_bounds__(256) void k_div_periodic(const T* __restrict__ u,
+                                                      const T* __restrict__ v, T* __restrict__ div,
+                                                      int nx, int ny, T inv_2dx, T inv_2dy) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= ny) return;
+  const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
+  const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
+  const long c = (long)i * ny;
+  div[c + j] = (u[(long)ip * ny + j] - u[(long)im * ny + j]) * inv_2dx +
+               (v[c + jp] - v[c + jm]) * inv_2dy;
+}
+
+extern "C" int aj_divergence_periodic(int dtype, const void* u, const void* v, void* div, int nx,
+                                      int ny, double dx, double dy, void* stream) {
+  if (!u || !v || !div) return fail(AJ_ERR_INVALID, "aj_divergence_periodic: null pointer");
+  if (int rc = check_grid(nx, ny)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == AJ_F32) {
+    hipLaunchKernelGGL(k_div_periodic<float>, row_grid(nx, ny), dim3(256), 0, st, (const float*)u,
+                       (const float*)v, (float*)div, nx, ny, float(1.0 / (2.0 * dx)),
+                       float(1.0 / (2.0 * dy)));
+  } else if (dtype == AJ_F64) {
+    hipLaunchKernelGGL(k_div_periodic<double>, row_grid(nx, ny), dim3(256), 0, st,
+                       (const double*)u, (const double*)v, (double*)div, nx, ny,
+                       1.0 / (2.0 * dx), 1.0 / (2.0 * dy));
+  } else {
+    return fail(AJ_ERR_INVALID, "aj_divergence_periodic: bad dtype");
+  }
+  AJ_LAUNCH_CHECK("k_div_periodic");
+  return AJ_OK;
+}
+
 extern "C" int aj_project(int dtype, const void* us, const void* vs, const void* p,
                           const void* mask, void* uo, void* vo, int nx, int ny, double dt,
                           double dx, double dy, double U_inf, void* stream) {

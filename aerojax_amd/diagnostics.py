@@ -2,8 +2,9 @@
 
 Mirrors the debug/viewer metrics of the reference:
 ``pressure_residual`` / ``relative_residual`` (viewer/simulation_controller.py
-:107-121, solver/pure_functions/step.py:229-235) and ``div_rms`` / ``l2_div``
-over the pure fluid (simulation_controller.py:186-194).  The multigrid's own
+:107-121, solver/pure_functions/step.py:229-235), ``div_rms`` / ``l2_div``
+over the pure fluid (simulation_controller.py:186-194) and ``max_speed`` /
+``cfl`` of check_cfl (advection_schemes/utils.py:14-20).  The multigrid's own
 residual norm is ``MultigridPlan.stats()["norm"]``.
 """
 from __future__ import annotations
@@ -18,7 +19,8 @@ from . import _lib
 
 def flow_diagnostics(u, v, p, mask, dx: float, dy: float, dt: float) -> dict:
     """Dict with pressure_residual, rhs_norm, relative_residual, div_rms,
-    l2_div, fluid_cells (host floats; synchronises the stream)."""
+    l2_div, fluid_cells, max_speed, cfl (host floats; synchronises the
+    stream)."""
     A.require_gpu()
     dt_ = A.torch_dtype_of(u)
     ut = A.as_device(u, dt_, name="u")

@@ -37,6 +37,19 @@ def divergence_nonperiodic(u, v, dx: float, dy: float, *, out=None):
     return A.out_like(o, want_np)
 
 
+def divergence(u, v, dx: float, dy: float, *, out=None):
+    """Periodic divergence (solver/operators.py:76-78)."""
+    want_np, ut, (vt,) = _prep(u, v, names=("u", "v"))
+    nx, ny = ut.shape
+    o = torch.empty_like(ut) if out is None else out
+    with torch.cuda.device(ut.device):
+        rc = _lib.load().aj_divergence_periodic(A.dtype_code(ut), A.ptr(ut), A.ptr(vt), A.ptr(o),
+                                                nx, ny, float(dx), float(dy),
+                                                A.stream_handle(ut.device))
+    _lib.check(rc, "aj_divergence_periodic")
+    return A.out_like(o, want_np)
+
+
 def pressure_rhs(u_star, v_star, mask, dt: float, dx: float, dy: float, *, out=None):
     """rhs = divergence_nonperiodic(u*, v*) * mask / dt (step.py:194-199);
     ``mask=None`` gives div / dt (step_handlers.py:92-98)."""

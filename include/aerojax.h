@@ -102,6 +102,11 @@ int aj_les_smagorinsky_total(int dtype, const void* u, const void* v, double nu,
 int aj_divergence_rhs(int dtype, const void* u_star, const void* v_star,
                       const void* mask, void* rhs, int nx, int ny, double dx,
                       double dy, double dt, void* stream);
+/* Replaces divergence (solver/operators.py:76-78): periodic central
+ * differences, the field the reference's training-data writer stores
+ * (neural_operators/training_utils.py:66-69). */
+int aj_divergence_periodic(int dtype, const void* u, const void* v, void* div, int nx, int ny,
+                           double dx, double dy, void* stream);
 
 /* ---- Projection ---------------------------------------------------------
  * Replaces the projection tail of step_pure (step.py:257-264, 305-349,
@@ -139,9 +144,11 @@ int aj_project_collocated(int dtype, const void* u_star, const void* v_star,
  *   `levels` as in the reference (default 4).
  * AJ_MG_TEXTBOOK: this project's to-tolerance solver for the consistent
  *   outlet-Dirichlet operator: FP32 V(2,2) cycles (red-black Gauss-Seidel,
- *   cell-centred transfers, dense coarsest solve) inside an FP64 defect
- *   correction; stops when sqrt(sum_{i<nx-1} r^2 dx dy) <= tolerance with
- *   r = b - A p in FP64.  `levels` is ignored (coarsens to <= 64 cells).
+ *   cell-centred transfers, exact FP64 coarsest solve by a cosine transform
+ *   along y and cyclic reduction along x) inside an FP64 defect correction;
+ *   stops when sqrt(sum_{i<nx-1} r^2 dx dy) <= tolerance with r = b - A p in
+ *   FP64.  `levels` = 0 coarsens while a level has more than 65536 cells or
+ *   more than 1024 rows; L > 0 asks for exactly L levels.
  */
 typedef struct aj_mg_plan aj_mg_plan;
 enum { AJ_MG_REFERENCE = 0, AJ_MG_TEXTBOOK = 1 };
@@ -192,11 +199,14 @@ int aj_tune_report(char* buf, int len);
  * ||laplacian_nonperiodic_x(p) - div_nonperiodic(u, v)/dt||_2 (edge pad in
  * x, periodic in y, solver/operators.py:67-72) and its ratio to ||rhs||_2 +
  * 1e-10; divergence of np.gradient over mask > 0.99 as RMS and as
- * sqrt(sum div^2 dx dy).  One fused FP64 reduction; synchronises `stream`
- * and fills *out. */
+ * sqrt(sum div^2 dx dy); the largest speed max sqrt(u^2 + v^2) and the CFL
+ * number max(speed dt/dx, speed dt/dy) of check_cfl
+ * (advection_schemes/utils.py:14-20).  One fused FP64 reduction;
+ * synchronises `stream` and fills *out. */
 typedef struct aj_flow_diag {
   double pressure_residual, rhs_norm, relative_residual;
   double div_rms, l2_div, fluid_cells;
+  double max_speed, cfl;
 } aj_flow_diag;
 int aj_flow_diagnostics(int dtype, const void* u, const void* v, const void* p,
                         const void* mask, int nx, int ny, double dx, double dy, double dt,

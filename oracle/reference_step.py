@@ -616,10 +616,17 @@ def laplacian_nonperiodic_x(f, dx, dy):
     return f_xx + f_yy
 
 
+def check_cfl(u, v, dt, dx, dy):
+    """check_cfl (advection_schemes/utils.py:14-20): the largest speed and
+    max(speed dt / dx, speed dt / dy).  Returns (max_speed, cfl)."""
+    vmax = float(np.max(np.sqrt(np.asarray(u, np.float64) ** 2 + np.asarray(v, np.float64) ** 2)))
+    return vmax, max(vmax * dt / dx, vmax * dt / dy)
+
+
 def flow_diagnostics(u, v, p, mask, dx, dy, dt):
     """Viewer / debug metrics (viewer/simulation_controller.py:107-121,
     186-194): pressure residual of lap_x(p) against div_nonperiodic(u, v)/dt,
-    and the np.gradient divergence over mask > 0.99."""
+    the np.gradient divergence over mask > 0.99, and check_cfl."""
     u, v, p, mask = (np.asarray(a, dtype=np.float64) for a in (u, v, p, mask))
     rhs = divergence_nonperiodic(u, v, dx, dy) / dt
     res = np.linalg.norm(laplacian_nonperiodic_x(p, dx, dy) - rhs)
@@ -629,7 +636,8 @@ def flow_diagnostics(u, v, p, mask, dx, dy, dt):
     df = div * fluid
     return {"pressure_residual": res, "rhs_norm": rn, "relative_residual": res / (rn + 1e-10),
             "div_rms": np.sqrt(np.sum(df ** 2) / (np.sum(fluid) + 1e-8)),
-            "l2_div": np.sqrt(np.sum(df ** 2) * dx * dy), "fluid_cells": float(np.sum(fluid))}
+            "l2_div": np.sqrt(np.sum(df ** 2) * dx * dy), "fluid_cells": float(np.sum(fluid)),
+            **dict(zip(("max_speed", "cfl"), check_cfl(u, v, dt, dx, dy)))}
 
 
 def rel_l2(a, b):

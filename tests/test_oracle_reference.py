@@ -484,3 +484,16 @@ def test_flow_diagnostics_divergence_free_field():
     assert d["fluid_cells"] == nx * (ny - 2)
     assert d["div_rms"] < 0.05          # boundary one-sided differences dominate
     assert np.isclose(d["relative_residual"], 1.0, rtol=1e-9)  # p = 0
+
+
+def test_check_cfl_closed_form():
+    """Largest speed of a 7-24-25 triangle at one cell, CFL against the
+    smaller spacing (check_cfl, advection_schemes/utils.py:14-20)."""
+    u = np.full((16, 8), 0.3)
+    v = np.full((16, 8), 0.4)
+    u[5, 3], v[5, 3] = -7.0, 24.0
+    vmax, cfl = R.check_cfl(u, v, 1e-3, 0.1, 0.05)
+    assert vmax == 25.0
+    assert np.isclose(cfl, 25.0 * 1e-3 / 0.05, rtol=1e-15)
+    d = R.flow_diagnostics(u, v, np.zeros_like(u), np.ones_like(u), 0.1, 0.05, 1e-3)
+    assert d["max_speed"] == 25.0 and np.isclose(d["cfl"], 0.5)
