@@ -35,7 +35,8 @@ EXPORTS = (
     "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
     "aj_les_smagorinsky_total", "aj_divergence_rhs", "aj_divergence_periodic", "aj_project",
     "aj_project_collocated",
-    "aj_flow_diagnostics", "aj_tune_enable", "aj_tune_reset", "aj_tune_report",
+    "aj_flow_diagnostics", "aj_geometry", "aj_batch_geometry", "aj_tune_enable", "aj_tune_reset",
+    "aj_tune_report",
     "aj_batch_create", "aj_batch_destroy", "aj_batch_cells",
     "aj_batch_num_instances", "aj_batch_offsets", "aj_batch_set_viscosity",
     "aj_batch_set_option", "aj_batch_step", "aj_batch_predict", "aj_batch_solve",
@@ -76,6 +77,17 @@ class FlowDiag(ctypes.Structure):
 
     def as_dict(self):
         return {n: float(getattr(self, n)) for n, _ in self._fields_}
+
+
+AJ_OBSTACLE_CYLINDER = 0
+AJ_OBSTACLE_NACA = 1
+
+
+class Obstacle(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("reserved", ctypes.c_int), ("x0", ctypes.c_double),
+                ("y0", ctypes.c_double), ("radius", ctypes.c_double), ("chord", ctypes.c_double),
+                ("aoa_deg", ctypes.c_double), ("m", ctypes.c_double), ("p", ctypes.c_double),
+                ("t", ctypes.c_double), ("eps", ctypes.c_double)]
 
 
 class BatchStepParams(ctypes.Structure):
@@ -128,6 +140,8 @@ _SIGS = {
                                    _vp]),
     "aj_flow_diagnostics": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d,
                                  ctypes.POINTER(FlowDiag), _vp]),
+    "aj_geometry": (_i, [_i, ctypes.POINTER(Obstacle), _i, _i, _d, _d, _vp, _vp, _vp]),
+    "aj_batch_geometry": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "aj_tune_enable": (_i, [_i]),
     "aj_tune_reset": (_i, []),
     "aj_tune_report": (_i, [ctypes.c_char_p, _i]),

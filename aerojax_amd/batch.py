@@ -28,11 +28,14 @@ from . import _lib
 class BatchedStepper:
     """``groups``: list of ``(nx, ny, count, dx, dy)``; ``masks`` / ``sdfs``
     / ``nus``: one entry per instance, group by group; ``u0`` / ``v0``:
-    per-instance initial fields."""
+    per-instance initial fields.  ``obstacles`` (one ``aj_obstacle`` per
+    instance, see ``geometry.naca_obstacle`` / ``cylinder_obstacle``) builds
+    the masks and SDFs on the GPU instead (pass ``masks=sdfs=None``)."""
 
     def __init__(self, groups, masks, sdfs, nus, u0, v0, dt, *, U_inf=1.0, brinkman_eta=0.005,
                  v_cycles=3, tolerance=1e-6, levels=4, dtype=torch.float32, device=None,
-                 solver="reference", max_cycles=40, smagorinsky_cs=0.0, warm_start=None):
+                 solver="reference", max_cycles=40, smagorinsky_cs=0.0, warm_start=None,
+                 obstacles=None):
         A.require_gpu()
         if dtype not in (torch.float32, torch.float64):
             raise TypeError(f"unsupported dtype {dtype} (float32 or float64)")
@@ -64,13 +67,25 @@ class BatchedStepper:
         _lib.check(lib.aj_batch_offsets(h, off.ctypes.data), "aj_batch_offsets")
         self.offsets = off
         self.shapes = [(g[0], g[1]) for g in self.groups for _ in range(g[2])]
-        for name, lst in (("masks", masks), ("sdfs", sdfs), ("nus", nus), ("u0", u0), ("v0", v0)):
+        for name, lst in (("masks", masks), ("sdfs", sdfs), ("nus", nus), ("u0", u0), ("v0", v0),
+                          ("obstacles", obstacles)):
             if lst is not None and len(lst) != self.n:
                 raise ValueError(f"{name}: {len(lst)} entries for {self.n} instances")
         nu = np.ascontiguousarray(np.asarray(nus, dtype=np.float64))
         _lib.check(lib.aj_batch_set_viscosity(h, nu.ctypes.data), "aj_batch_set_viscosity")
-        self.mask = self._pack(masks)
-        self.sdf = self._pack(sdfs) if sdfs is not None else None
+        if obstacles is not None:
+            if masks is not None or sdfs is not None:
+                raise ValueError("give either obstacles or masks / sdfs")
+            self.mask = torch.empty(self.cells, dtype=self.dtype, device=self.device)
+            self.sdf = torch.empty_like(self.mask)
+            arr = (_lib.Obstacle * self.n)(*obstacles)
+            with torch.cuda.device(self.device):
+                rc = lib.aj_batch_geometry(h, ctypes.cast(arr, ctypes.c_void_p), A.ptr(self.sdf),
+                                           A.ptr(self.mask), A.stream_handle(self.device))
+            _lib.check(rc, "aj_batch_geometry")
+        else:
+            self.mask = self._pack(masks)
+            self.sdf = self._pack(sdfs) if sdfs is not None else None
         self.u = self._pack(u0)
         self.v = self._pack(v0)
         self.u_next = torch.empty_like(self.u)

@@ -29,6 +29,7 @@
 #include <cstring>
 #include <vector>
 
+#include "aj_geometry_kernels.h"
 #include "aj_mg.h"
 #include "aj_mg_ref_kernels.h"
 #include "aj_step_kernels.h"
@@ -146,6 +147,37 @@ __global__ __launch_bounds__(256) void kb_les(BatchWork w, BatchStepArgs<T> s) {
         s.nsgs[it.off + (long)i * ny + j] =
             les_cell(s.u + it.off, s.v + it.off, nx, ny, i, j, T(1.0 / (2.0 * w.dx[g])),
                      T(1.0 / (2.0 * w.dy[g])), coef);
+    }
+  }
+}
+
+// obstacle geometry per instance (rotation precomputed on the host)
+struct ObstacleDev {
+  aj_obstacle o;
+  double ca, sa;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_geometry(BatchWork w, const ObstacleDev* __restrict__ obs,
+                                                   T* __restrict__ sdf, T* __restrict__ mask) {
+  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g, nx = w.nx[g], ny = w.ny[g];
+    const int cpr = (ny + 255) / 256;
+    const int rb = it.sub / cpr, j = (it.sub - rb * cpr) * 256 + threadIdx.x;
+    if (j >= ny) continue;
+    const ObstacleDev od = obs[it.inst];
+    const double lx = nx * w.dx[g], ly = ny * w.dy[g];
+    const double Y = linspace_at(j, ny, ly);
+#pragma unroll
+    for (int q = 0; q < kRowsPerItem; ++q) {
+      const int i = rb * kRowsPerItem + q;
+      if (i < nx) {
+        const double s = obstacle_sdf(od.o, od.ca, od.sa, linspace_at(i, nx, lx), Y);
+        const long c = it.off + (long)i * ny + j;
+        if (sdf) sdf[c] = (T)s;
+        if (mask) mask[c] = (T)obstacle_mask(s, od.o.eps);
+      }
     }
   }
 }
@@ -348,6 +380,7 @@ struct aj_batch {
   double* dxdy = nullptr;
   double* tol2 = nullptr;
   double* nu = nullptr;
+  void* obs = nullptr;  // ObstacleDev[instances]
   aj_mg_stats* stats = nullptr;
   double last_tol = -1.0;
   int use_graph = 1;
@@ -747,6 +780,7 @@ extern "C" int aj_batch_create(aj_batch** out, int dtype, int ngroups, const int
   const size_t o_part = take((size_t)b->ninst * kResChunks * 8);
   const size_t o_dxdy = take((size_t)b->ninst * 8), o_tol2 = take((size_t)b->ninst * 8);
   const size_t o_nu = take((size_t)b->ninst * 8);
+  const size_t o_obs = take((size_t)b->ninst * sizeof(ObstacleDev));
   const size_t o_st = take((size_t)b->ninst * sizeof(aj_mg_stats));
   hipError_t e = hipGetDevice(&b->device);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device);
@@ -772,6 +806,7 @@ extern "C" int aj_batch_create(aj_batch** out, int dtype, int ngroups, const int
   b->dxdy = (double*)(base + o_dxdy);
   b->tol2 = (double*)(base + o_tol2);
   b->nu = (double*)(base + o_nu);
+  b->obs = base + o_obs;
   b->stats = (aj_mg_stats*)(base + o_st);
   std::vector<double> dd(b->ninst);
   for (int g = 0; g < ngroups; ++g)
@@ -941,6 +976,36 @@ extern "C" int aj_batch_project(aj_batch* b, const void* p, const void* mask, vo
   return project_launches<double>(
       b, step_args<double>(b, prm, nullptr, nullptr, mask, nullptr, nullptr, (void*)p, u_out, v_out),
       st);
+}
+
+namespace aj {
+void obstacle_rotation(const aj_obstacle& o, double* ca, double* sa);
+int check_obstacle(const aj_obstacle& o, const char* who);
+}  // namespace aj
+
+extern "C" int aj_batch_geometry(aj_batch* b, const aj_obstacle* obstacles, void* sdf, void* mask,
+                                 void* stream) {
+  if (!b || !obstacles || (!sdf && !mask)) return fail(AJ_ERR_INVALID, "aj_batch_geometry: null pointer");
+  std::vector<ObstacleDev> h(b->ninst);
+  for (int k = 0; k < b->ninst; ++k) {
+    if (int rc = check_obstacle(obstacles[k], "aj_batch_geometry")) return rc;
+    h[k].o = obstacles[k];
+    obstacle_rotation(obstacles[k], &h[k].ca, &h[k].sa);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  AJ_HIP(hipMemcpyAsync(b->obs, h.data(), h.size() * sizeof(ObstacleDev), hipMemcpyHostToDevice, st));
+  BatchWork w = make_work(b, 0, 1);
+  const dim3 grid(persistent_grid(b, w.total, 8));
+  if (b->dtype == AJ_F32)
+    hipLaunchKernelGGL(kb_geometry<float>, grid, dim3(256), 0, st, w, (const ObstacleDev*)b->obs,
+                       (float*)sdf, (float*)mask);
+  else
+    hipLaunchKernelGGL(kb_geometry<double>, grid, dim3(256), 0, st, w, (const ObstacleDev*)b->obs,
+                       (double*)sdf, (double*)mask);
+  AJ_LAUNCH_CHECK("kb_geometry");
+  // the host array must outlive the copy
+  AJ_HIP(hipStreamSynchronize(st));
+  return AJ_OK;
 }
 
 extern "C" int aj_batch_get_stats(aj_batch* b, aj_mg_stats* host, void* stream) {

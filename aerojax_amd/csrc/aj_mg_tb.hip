@@ -544,7 +544,7 @@ struct Carve {
   }
 };
 
-static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st);
+static int tb_cycles(aj_mg_plan* p, hipStream_t st);
 
 int tb_create(aj_mg_plan* p) {
   // hierarchy: levels = 0 coarsens while n*m > kTbCoarseCells; levels = L > 0
@@ -677,8 +677,7 @@ int tb_create(aj_mg_plan* p) {
     AJ_HIP(hipMemset(p->stats, 0, sizeof(aj_mg_stats) * nb));
     AJ_HIP(hipMemset(p->tB[0], 0, (size_t)n0 * nb * sizeof(float)));
     AJ_HIP(hipMemset(p->tE[0], 0, (size_t)n0 * nb * sizeof(float)));
-    if (int rc = tb_vcycle(p, 0, false, nullptr)) return rc;
-    if (int rc = tb_vcycle(p, 0, true, nullptr)) return rc;
+    if (int rc = tb_cycles(p, nullptr)) return rc;
     AJ_HIP(hipStreamSynchronize(nullptr));
   }
   return AJ_OK;
@@ -739,53 +738,74 @@ static int tb_launch_post(const TbLevel& Lv, int nb, const float* Y, const float
 // smoother tile candidates (rows x columns, threads): tuned per level shape
 constexpr int kTbTileCands = 4;
 
-static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
-  if (lev == p->ml) return tb_coarse(p, st);
+// pre-smoothing on level lev (lev < coarsest): pin (null = zero start) ->
+// Y, coarse rhs tB[lev + 1]
+static int tb_pre(aj_mg_plan* p, int lev, const float* pin, float* Y, hipStream_t st) {
   const TbLevel Lv = make_level(p, lev);
   const int n = Lv.n, m = Lv.m, nb = p->batch;
   const aj_mg_stats* gate = p->stats;
+  const float* b = p->tB[lev];
+  float* Bc = p->tB[lev + 1];
+  auto launch = [&](int c, hipStream_t s) -> int {
+    switch (c) {
+      case 1: return tb_launch_pre<16, 64, 512>(Lv, nb, pin, b, Y, Bc, gate, s);
+      case 2: return tb_launch_pre<32, 32, 512>(Lv, nb, pin, b, Y, Bc, gate, s);
+      case 3: return tb_launch_pre<16, 32, 256>(Lv, nb, pin, b, Y, Bc, gate, s);
+      default: return tb_launch_pre<32, 64, 1024>(Lv, nb, pin, b, Y, Bc, gate, s);
+    }
+  };
   char key[64];
-  {
-    // p_in (warm), b read; Y and the quarter-size coarse rhs written
-    const float* pin = warm ? (const float*)p->tE[lev] : nullptr;
-    const float* b = p->tB[lev];
-    float *Y = p->tY[lev], *Bc = p->tB[lev + 1];
-    auto launch = [&](int c, hipStream_t s) -> int {
-      switch (c) {
-        case 1: return tb_launch_pre<16, 64, 512>(Lv, nb, pin, b, Y, Bc, gate, s);
-        case 2: return tb_launch_pre<32, 32, 512>(Lv, nb, pin, b, Y, Bc, gate, s);
-        case 3: return tb_launch_pre<16, 32, 256>(Lv, nb, pin, b, Y, Bc, gate, s);
-        default: return tb_launch_pre<32, 64, 1024>(Lv, nb, pin, b, Y, Bc, gate, s);
-      }
-    };
-    std::snprintf(key, sizeof(key), "tb_pre/%dx%dx%d", nb, n, m);
-    ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, (double)n * m * nb * (warm ? 13 : 9),
-                   st);
-    int pick = 0;
-    if (int rc = tune_pick(key, kTbTileCands, launch, st, &pick)) return rc;
-  }
+  std::snprintf(key, sizeof(key), "tb_pre/%dx%dx%d", nb, n, m);
+  // p_in (warm), b read; Y and the quarter-size coarse rhs written
+  ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE,
+                 (double)n * m * nb * (pin ? 13 : 9), st);
+  int pick = 0;
+  return tune_pick(key, kTbTileCands, launch, st, &pick);
+}
+
+// post-smoothing on level lev: Y + prolong(tE[lev + 1]) -> E
+static int tb_post(aj_mg_plan* p, int lev, const float* Y, float* E, hipStream_t st) {
+  const TbLevel Lv = make_level(p, lev);
+  const int n = Lv.n, m = Lv.m, nb = p->batch;
+  const aj_mg_stats* gate = p->stats;
+  const float *Ec = p->tE[lev + 1], *b = p->tB[lev];
+  const int nc = p->lnx[lev + 1], mc = p->lny[lev + 1];
+  const float gamc = (float)outlet_gamma(lev + 1);
+  auto launch = [&](int c, hipStream_t s) -> int {
+    switch (c) {
+      case 1: return tb_launch_post<16, 64, 512>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      case 2: return tb_launch_post<32, 32, 512>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      case 3: return tb_launch_post<16, 32, 256>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      default: return tb_launch_post<32, 64, 1024>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+    }
+  };
+  char key[64];
+  std::snprintf(key, sizeof(key), "tb_post/%dx%dx%d", nb, n, m);
+  // Y, quarter-size coarse correction, b read; E written
+  ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, (double)n * m * nb * 13, st);
+  int pick = 0;
+  return tune_pick(key, kTbTileCands, launch, st, &pick);
+}
+
+// one FP32 V-cycle for A_lev e = tB[lev]; result in tE[lev]; warm = start
+// from tE[lev] (the exact coarsest solve ignores it)
+static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
+  if (lev == p->ml) return tb_coarse(p, st);
+  if (int rc = tb_pre(p, lev, warm ? p->tE[lev] : nullptr, p->tY[lev], st)) return rc;
   if (int rc = tb_vcycle(p, lev + 1, false, st)) return rc;
-  {
-    // Y, quarter-size coarse correction, b read; E written
-    const float *Y = p->tY[lev], *Ec = p->tE[lev + 1], *b = p->tB[lev];
-    float* E = p->tE[lev];
-    const int nc = p->lnx[lev + 1], mc = p->lny[lev + 1];
-    const float gamc = (float)outlet_gamma(lev + 1);
-    auto launch = [&](int c, hipStream_t s) -> int {
-      switch (c) {
-        case 1: return tb_launch_post<16, 64, 512>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
-        case 2: return tb_launch_post<32, 32, 512>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
-        case 3: return tb_launch_post<16, 32, 256>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
-        default: return tb_launch_post<32, 64, 1024>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
-      }
-    };
-    std::snprintf(key, sizeof(key), "tb_post/%dx%dx%d", nb, n, m);
-    ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, (double)n * m * nb * 13, st);
-    int pick = 0;
-    if (int rc = tune_pick(key, kTbTileCands, launch, st, &pick)) return rc;
-  }
+  return tb_post(p, lev, p->tY[lev], p->tE[lev], st);
+}
+
+// the V-cycles of an outer iteration for A e = tB[0] from zero, result in
+// tE[0]; a single-level hierarchy is one exact solve
+static int tb_cycles(aj_mg_plan* p, hipStream_t st) {
+  if (p->ml == 0) return tb_coarse(p, st);
+  for (int c = 0; c < kCyclesPerOuter; ++c)
+    if (int rc = tb_vcycle(p, 0, c > 0, st)) return rc;
   return AJ_OK;
 }
+
+static int tb_cycles_per_outer(const aj_mg_plan* p) { return p->ml == 0 ? 1 : kCyclesPerOuter; }
 
 // launches of one defect-correction outer iteration (2 V-cycles, FP64
 // residual with the update p64 += e, norm and gate); only plan-owned
@@ -797,8 +817,7 @@ static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
   const int nx = p->nx, ny = p->ny, B = p->batch;
   const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
   const int nb = p->npartial;
-  for (int c = 0; c < kCyclesPerOuter; ++c)
-    if (int rc = tb_vcycle(p, 0, c > 0, st)) return rc;
+  if (int rc = tb_cycles(p, st)) return rc;
   // p64, e, b read; p64', r32 written
   ProfScope prof(AJ_PROF_MG_RESID, (double)nx * ny * B * (8 + 4 + sizeof(T) + 8 + 4), st);
   hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb, 1, B), dim3(256), 0, st,
@@ -807,7 +826,7 @@ static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
                      (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_resid");
   hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
-                     p->dx * p->dy, p->dtol, 1, kCyclesPerOuter, 0, p->stats, p->state);
+                     p->dx * p->dy, p->dtol, 1, tb_cycles_per_outer(p), 0, p->stats, p->state);
   AJ_LAUNCH_CHECK("k_tb_norm_final");
   return AJ_OK;
 }
@@ -909,8 +928,12 @@ template <typename T>
 static int tb_solve_many_t(aj_mg_plan* const* plans, int count, const void* const* rhs,
                            const void* const* p0, void* const* p_out, int max_cycles, double tol,
                            hipStream_t st) {
-  const int max_outer = (max_cycles + kCyclesPerOuter - 1) / kCyclesPerOuter;
-  std::vector<int> done(count, 0), open(count, 0);
+  // max_cycles bounds the V-cycles (one exact solve counts as one)
+  std::vector<int> done(count, 0), open(count, 0), max_outer(count, 0);
+  for (int q = 0; q < count; ++q) {
+    const int cpo = tb_cycles_per_outer(plans[q]);
+    max_outer[q] = (max_cycles + cpo - 1) / cpo;
+  }
   for (int q = 0; q < count; ++q)
     if (int rc = tb_start<T>(plans[q], (const T*)rhs[q], p0 ? (const T*)p0[q] : nullptr, tol, st))
       return rc;
@@ -921,7 +944,7 @@ static int tb_solve_many_t(aj_mg_plan* const* plans, int count, const void* cons
   // each unsettled plan is topped up by one iteration.
   for (int q = 0; q < count; ++q) {
     aj_mg_plan* p = plans[q];
-    done[q] = p->adaptive ? std::min(std::max(p->last_outer, 1), max_outer) : max_outer;
+    done[q] = p->adaptive ? std::min(std::max(p->last_outer, 1), max_outer[q]) : max_outer[q];
     open[q] = p->adaptive;
     for (int k = 0; k < done[q]; ++k)
       if (int rc = tb_outer_iteration<T>(p, k & 1, st)) return rc;
@@ -939,7 +962,7 @@ static int tb_solve_many_t(aj_mg_plan* const* plans, int count, const void* cons
     for (int q = 0; q < count; ++q) {
       if (!open[q]) continue;
       aj_mg_plan* p = plans[q];
-      if (tb_settled(p) || done[q] >= max_outer) {
+      if (tb_settled(p) || done[q] >= max_outer[q]) {
         open[q] = 0;
         p->last_outer = tb_max_outer(p);
         continue;

@@ -301,7 +301,7 @@ int conv_bwd(aj_nn* n, int li, const float* gpost, int ldg, const bf16_t* pre, c
   return AJ_OK;
 }
 
-int forward(aj_nn* n, int B, int H, int W, hipStream_t st) {
+int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true) {
   Act& a = n->act;
   const int L = n->L;
   for (int l = 0; l < L; ++l) {
@@ -340,6 +340,7 @@ int forward(aj_nn* n, int B, int H, int W, hipStream_t st) {
     if (int rc = conv_fwd(n, dec_b(n, l), a.DA[l], c, a.DB[l], c, a.DPB[l], B, Hl, Wl, st))
       return rc;
   }
+  if (!head) return AJ_OK;
   const long P0 = (long)B * H * W;
   hipLaunchKernelGGL(k_head, dim3(blocks(P0)), dim3(256), 0, st, (const bf16_t*)a.DB[0], n->c0,
                      n->c0, (const float*)(n->params + n->head_w),
@@ -495,6 +496,133 @@ __global__ void k_prolong_values(const float* e, int nc, int mc, float gamc, flo
 }
 
 double gamma_level(int l) { return 1.0 - 1.0 / (0.5 - std::ldexp(1.0, -l - 1)); }
+
+// ---- warm start in two launches on each side of the network ---------------
+// (the arithmetic of the separate restrict / sum-of-squares / normalise /
+// gather and head / unscale / prolong passes, operation for operation)
+
+// value of cell (I, J) of the grid restricted R times from f (row length m):
+// R nested 2x2 averages, each 0.25 ((a + b) + (c + d)) as k_restrict2
+template <int R>
+__device__ __forceinline__ float nested_avg(const float* f, int m, int I, int J) {
+  if constexpr (R == 0) {
+    return f[(long)I * m + J];
+  } else {
+    const float a = nested_avg<R - 1>(f, m, 2 * I, 2 * J);
+    const float b = nested_avg<R - 1>(f, m, 2 * I + 1, 2 * J);
+    const float c = nested_avg<R - 1>(f, m, 2 * I, 2 * J + 1);
+    const float d = nested_avg<R - 1>(f, m, 2 * I + 1, 2 * J + 1);
+    return 0.25f * ((a + b) + (c + d));
+  }
+}
+
+// per coarse cell of instance blockIdx.y: restricted rhs R, tanh(sdf_r /
+// sdf_ref) T, and the block's sum of R^2 (fixed-order partials)
+template <int RR>
+__global__ __launch_bounds__(256) void k_ws_prep(const float* __restrict__ rhs,
+                                                 const float* __restrict__ sdf, int nx, int ny,
+                                                 float sdf_ref, float* __restrict__ Rout,
+                                                 float* __restrict__ Tout,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int H = nx >> RR, W = ny >> RR;
+  const long ni = (long)H * W;
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  const long b = blockIdx.y;
+  double v = 0.0;
+  if (e < ni) {
+    const int I = (int)(e / W), J = (int)(e - (long)I * W);
+    const float r = nested_avg<RR>(rhs + b * nx * ny, ny, I, J);
+    const float sd = nested_avg<RR>(sdf + b * nx * ny, ny, I, J);
+    Rout[b * ni + e] = r;
+    Tout[b * ni + e] = tanhf(sd / sdf_ref);
+    v = (double)r * r;
+  }
+  const double t = block_sum(v, sh);
+  if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = t;
+}
+
+// s = sqrt(mean R^2) + 1e-30 of the instance (every block sums the
+// instance's partials in the same order), network input row (R / s, T, 0..)
+__global__ __launch_bounds__(256) void k_ws_input(const float* __restrict__ R,
+                                                  const float* __restrict__ T,
+                                                  const double* __restrict__ part, int nblk,
+                                                  long ni, bf16_t* __restrict__ X0,
+                                                  float* __restrict__ s_out) {
+  __shared__ double sh[16];
+  __shared__ float s_sh;
+  const long b = blockIdx.y;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += 256) acc += part[b * nblk + k];
+  const double tot = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    s_sh = (float)(sqrt(tot / (double)ni) + 1e-30);
+    if (blockIdx.x == 0) s_out[b] = s_sh;
+  }
+  __syncthreads();
+  const float s = s_sh;
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ni) return;
+  const long q = b * ni + e;
+  bf16_t* o = X0 + q * kCinPad;
+  o[0] = f2bf(R[q] / s);
+  o[1] = f2bf(T[q]);
+  for (int c = 2; c < kCinPad; ++c) o[c] = 0;
+}
+
+// head (1x1 conv C -> 1 + bias, as k_head) times s pscale (as k_unscale)
+__global__ __launch_bounds__(256) void k_ws_head(const bf16_t* x, int ldx, int C, const float* w,
+                                                 const float* bias, const float* __restrict__ s,
+                                                 long ni, float pscale, float* out, long P) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  float v = bias[0];
+  for (int c = 0; c < C; ++c) v += bf2f(x[p * ldx + c]) * w[c];
+  out[p] = v * s[p / ni] * pscale;
+}
+
+// value at (i, j) of the grid K prolongations finer than c (hc x wc), the
+// arithmetic of K k_prolong_values passes; the finest of the r levels is the
+// fine grid (outlet row 0); gam[l] = outlet ghost factor of level l
+template <int K>
+__device__ __forceinline__ float up_val(const float* c, int hc, int wc, int r, const float* gam,
+                                        int i, int j) {
+  if constexpr (K == 0) {
+    return c[(long)i * wc + j];
+  } else {
+    const int nc = hc << (K - 1), mc = wc << (K - 1), nf = 2 * nc;
+    const int l = r - K;  // level of the value computed here
+    if (l == 0 && i == nf - 1) return 0.f;
+    const int I = min(i >> 1, nc - 1), J = min(j >> 1, mc - 1);
+    int In = (i & 1) ? I + 1 : I - 1;
+    int Jn = (j & 1) ? J + 1 : J - 1;
+    float sx = 1.0f;
+    if (In < 0) In = I;
+    else if (In >= nc) { In = I; sx = gam[l + 1]; }
+    if (Jn < 0) Jn = J;
+    else if (Jn >= mc) Jn = J;
+    const float own = up_val<K - 1>(c, hc, wc, r, gam, I, J);
+    const float xn = up_val<K - 1>(c, hc, wc, r, gam, In, J) * sx;
+    const float yn = up_val<K - 1>(c, hc, wc, r, gam, I, Jn);
+    const float dn = up_val<K - 1>(c, hc, wc, r, gam, In, Jn) * sx;
+    return 0.5625f * own + 0.1875f * (xn + yn) + 0.0625f * dn;
+  }
+}
+
+struct Gam4 {
+  float g[4];
+};
+
+template <int RR>
+__global__ __launch_bounds__(256) void k_ws_prolong(const float* __restrict__ c, int H, int W,
+                                                    Gam4 gam, float* __restrict__ p0, int nx,
+                                                    int ny) {
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (long)nx * ny) return;
+  const long b = blockIdx.y;
+  const int i = (int)(q / ny), j = (int)(q - (long)i * ny);
+  p0[b * nx * ny + q] = up_val<RR>(c + b * H * W, H, W, RR, gam.g, i, j);
+}
 
 }  // namespace
 
@@ -768,6 +896,38 @@ extern "C" int aj_nn_warm_start_batched(aj_nn* n, const float* rhs, const float*
   float* w = (float*)n->ws_mem;
   float *R = w, *S = w + nf, *X = w + 2 * nf, *A = w + 4 * nf, *Bf = w + 5 * nf, *tmp = w + 6 * nf;
   double* sumsq = (double*)(w + 8 * nf + 64);
+  if (r >= 1 && r <= 3) {
+    // two launches before the network, two after
+    if (int rc = ensure_act(n, B, H, W)) return rc;
+    const int nblk = (int)blocks(ni);
+    double* part = (double*)(w + 6 * nf);  // B * nblk doubles (<= nf / 128 floats)
+    float* sv = (float*)(w + 7 * nf);      // per-instance s
+    const dim3 g2(nblk, B);
+    switch (r) {
+      case 1: hipLaunchKernelGGL(k_ws_prep<1>, g2, dim3(256), 0, st, rhs, sdf, nx, ny, sdf_ref, R, S, part); break;
+      case 2: hipLaunchKernelGGL(k_ws_prep<2>, g2, dim3(256), 0, st, rhs, sdf, nx, ny, sdf_ref, R, S, part); break;
+      default: hipLaunchKernelGGL(k_ws_prep<3>, g2, dim3(256), 0, st, rhs, sdf, nx, ny, sdf_ref, R, S, part); break;
+    }
+    AJ_LAUNCH_CHECK("k_ws_prep");
+    hipLaunchKernelGGL(k_ws_input, g2, dim3(256), 0, st, (const float*)R, (const float*)S,
+                       (const double*)part, nblk, ni, n->act.X0, sv);
+    AJ_LAUNCH_CHECK("k_ws_input");
+    if (int rc = forward(n, B, H, W, st, false)) return rc;
+    hipLaunchKernelGGL(k_ws_head, dim3(blocks(nc)), dim3(256), 0, st, (const bf16_t*)n->act.DB[0],
+                       n->c0, n->c0, (const float*)(n->params + n->head_w),
+                       (const float*)(n->params + n->head_b), (const float*)sv, ni, pscale, A, nc);
+    AJ_LAUNCH_CHECK("k_ws_head");
+    Gam4 gam{};
+    for (int l = 0; l <= r; ++l) gam.g[l] = (float)gamma_level(l);
+    const dim3 gp(blocks((long)nx * ny), B);
+    switch (r) {
+      case 1: hipLaunchKernelGGL(k_ws_prolong<1>, gp, dim3(256), 0, st, (const float*)A, H, W, gam, p0, nx, ny); break;
+      case 2: hipLaunchKernelGGL(k_ws_prolong<2>, gp, dim3(256), 0, st, (const float*)A, H, W, gam, p0, nx, ny); break;
+      default: hipLaunchKernelGGL(k_ws_prolong<3>, gp, dim3(256), 0, st, (const float*)A, H, W, gam, p0, nx, ny); break;
+    }
+    AJ_LAUNCH_CHECK("k_ws_prolong");
+    return AJ_OK;
+  }
   if (int rc = restrict_r(rhs, B * nx, ny, r, R, tmp, st)) return rc;
   if (int rc = restrict_r(sdf, B * nx, ny, r, S, tmp, st)) return rc;
   hipLaunchKernelGGL(k_zero_d, dim3(blocks(B)), dim3(256), 0, st, sumsq, B);

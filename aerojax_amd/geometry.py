@@ -148,3 +148,44 @@ def cylinder_geometry(grid: GridParams, cx, cy, r, eps=0.05) -> Geometry:
 def viscosity(U_inf: float, L_char: float, Re: float) -> float:
     """nu = U L / Re (FlowParams.resolve, solver/params.py:143-148)."""
     return U_inf * L_char / Re
+
+
+# ---------------------------------------------------------------------------
+# the same geometry evaluated on the GPU (aj_geometry / aj_batch_geometry)
+# ---------------------------------------------------------------------------
+
+def naca_obstacle(designation="0012", chord=0.5, aoa_deg=0.0, x_le=1.0, y_le=0.5, eps=0.05):
+    """``aj_obstacle`` of a NACA 4-digit section (leading edge at (x_le, y_le))."""
+    from . import _lib
+    m, p, t = parse_naca_4digit(designation)
+    return _lib.Obstacle(kind=_lib.AJ_OBSTACLE_NACA, x0=float(x_le), y0=float(y_le),
+                         chord=float(chord), aoa_deg=float(aoa_deg), m=m, p=p, t=t,
+                         eps=float(eps))
+
+
+def cylinder_obstacle(cx, cy, r, eps=0.05):
+    """``aj_obstacle`` of a cylinder of radius r centred at (cx, cy)."""
+    from . import _lib
+    return _lib.Obstacle(kind=_lib.AJ_OBSTACLE_CYLINDER, x0=float(cx), y0=float(cy),
+                         radius=float(r), eps=float(eps))
+
+
+def device_geometry(grid: GridParams, obstacle, dtype=None, device=None):
+    """(sdf, mask) of ``obstacle`` on ``grid`` computed on the GPU."""
+    import ctypes
+
+    import torch
+
+    from . import _arrays as A
+    from . import _lib
+    A.require_gpu()
+    dtype = torch.float32 if dtype is None else dtype
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    sdf = torch.empty((grid.nx, grid.ny), dtype=dtype, device=dev)
+    mask = torch.empty_like(sdf)
+    with torch.cuda.device(dev):
+        rc = _lib.load().aj_geometry(A.dtype_code(sdf), ctypes.byref(obstacle), grid.nx, grid.ny,
+                                     float(grid.lx), float(grid.ly), A.ptr(sdf), A.ptr(mask),
+                                     A.stream_handle(dev))
+    _lib.check(rc, "aj_geometry")
+    return sdf, mask

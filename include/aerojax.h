@@ -131,6 +131,28 @@ int aj_project_collocated(int dtype, const void* u_star, const void* v_star,
                           int nx, int ny, double dt, double dx, double dy, double U_in,
                           void* stream);
 
+/* ---- Obstacle geometry --------------------------------------------------
+ * Replaces the host-side mask generation (solver/simulation/mask_generator.py
+ * :60-75 and 145-160, obstacles/naca_airfoils.py:109-178): the SDF of a
+ * cylinder (sqrt((X-cx)^2 + (Y-cy)^2) - r) or of a NACA 4-digit section
+ * (naca_surface_distance: rotation by +AoA about the leading edge (x0, y0),
+ * surface / edge distances, inside test with 0.01 chord tolerance) and
+ * mask = 1 - sigmoid(-sdf/eps), on the reference's coordinates
+ * X = linspace(0, lx, nx), Y = linspace(0, ly, ny) (solver/params.py:54-56).
+ * FP64 arithmetic; sdf or mask may be NULL. */
+enum { AJ_OBSTACLE_CYLINDER = 0, AJ_OBSTACLE_NACA = 1 };
+typedef struct aj_obstacle {
+  int kind;                 /* AJ_OBSTACLE_*                                  */
+  int reserved;
+  double x0, y0;            /* cylinder centre / NACA leading edge             */
+  double radius;            /* cylinder                                       */
+  double chord, aoa_deg;    /* NACA                                           */
+  double m, p, t;           /* NACA 4-digit camber, its position, thickness   */
+  double eps;               /* mask smoothing width                           */
+} aj_obstacle;
+int aj_geometry(int dtype, const aj_obstacle* obstacle, int nx, int ny, double lx, double ly,
+                void* sdf, void* mask, void* stream);
+
 /* ---- Multigrid pressure Poisson ----------------------------------------
  * A plan owns the level hierarchy workspace (allocated once, hipMalloc)
  * for one grid shape and spacing.
@@ -311,6 +333,12 @@ int aj_batch_project(aj_batch* batch, const void* p, const void* mask, void* u_o
                      const aj_batch_step_params* prm, void* stream);
 /* per-instance multigrid statistics of the last step (synchronises) */
 int aj_batch_get_stats(aj_batch* batch, aj_mg_stats* stats, void* stream);
+/* SDF and mask of every instance (obstacles: host array of
+ * aj_batch_num_instances entries) into batch-layout buffers, as aj_geometry
+ * with lx = nx dx and ly = ny dy of the instance's group.  No counterpart in
+ * the reference (its masks are built on the host per simulation). */
+int aj_batch_geometry(aj_batch* batch, const aj_obstacle* obstacles, void* sdf, void* mask,
+                      void* stream);
 
 
 /* ---- Live kernel timing -------------------------------------------------

@@ -306,7 +306,8 @@ class TextbookMG:
                 break
             r32 = r64.astype(self.dtype)
             e = None
-            for _c in range(cycles_per_outer):
+            # a single-level hierarchy is one exact solve per outer iteration
+            for _c in range(1 if len(self.shapes) == 1 else cycles_per_outer):
                 e = self.vcycle(r32, 0, e)
                 cycles += 1
             p64 = p64 + e.astype(np.float64)
