@@ -216,6 +216,13 @@ int ensure_act(aj_nn* n, int B, int H, int W) {
   return AJ_OK;
 }
 
+// input mode of a forward convolution (ConvArgs::src and its operands)
+struct ConvSrc {
+  int mode = 0;
+  const bf16_t* in2 = nullptr;
+  int ld2 = 0, cu = 0;
+};
+
 // tile choice: 8x32-pixel patches where the layer has enough of them to
 // fill the chip, 4x16 otherwise; NT = 64 / 32 / 16 output channels
 template <int PH, int PW>
@@ -239,15 +246,15 @@ int conv_launch(const ConvArgs& c, hipStream_t st) {
     return k == 1 ? conv_launch_t<4, 16>(c, nt, s) : conv_launch_t<8, 32>(c, nt, s);
   };
   char key[96];
-  std::snprintf(key, sizeof(key), "conv/%dx%dx%d/%d->%d/%s", c.B, c.H, c.W, c.ci, c.co,
-                c.outf ? "f32" : "bf16");
+  std::snprintf(key, sizeof(key), "conv/%dx%dx%d/%d->%d/%s%s", c.B, c.H, c.W, c.ci, c.co,
+                c.outf ? "f32" : "bf16", c.src == 1 ? "/pool" : (c.src == 2 ? "/up" : ""));
   int pick = 0;
   return tune_pick(key, c.accum ? 1 : 2, launch, st, &pick);
 }
 
 // forward conv + GELU: out (post) and pre both stored
 int conv_fwd(aj_nn* n, int li, const bf16_t* in, int ldi, bf16_t* out, int ldo, bf16_t* pre,
-             int B, int H, int W, hipStream_t st) {
+             int B, int H, int W, hipStream_t st, const ConvSrc& src = ConvSrc()) {
   const Layer& Ly = n->layers[li];
   ConvArgs c{};
   c.in = in; c.ldi = ldi; c.ci = Ly.ci_pad;
@@ -255,6 +262,7 @@ int conv_fwd(aj_nn* n, int li, const bf16_t* in, int ldi, bf16_t* out, int ldo, 
   c.B = B; c.H = H; c.W = W;
   c.out = out; c.ldo = ldo; c.pre = pre; c.ldp = Ly.co;
   c.outf = nullptr; c.ldf = 0; c.accum = 0; c.act = 1;
+  c.src = src.mode; c.in2 = src.in2; c.ld2 = src.ld2; c.cu = src.cu;
   ProfScope prof(AJ_PROF_NN_CONV, 2.0 * 9 * Ly.ci * Ly.co * (double)B * H * W, st);
   return conv_launch(c, st);
 }
@@ -301,7 +309,11 @@ int conv_bwd(aj_nn* n, int li, const float* gpost, int ldg, const bf16_t* pre, c
   return AJ_OK;
 }
 
-int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true) {
+// fused = true (inference): pooling and upsampling are read by the
+// convolutions directly instead of being materialised (the backward pass
+// needs them materialised, so training uses fused = false)
+int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true,
+            bool fused = false) {
   Act& a = n->act;
   const int L = n->L;
   for (int l = 0; l < L; ++l) {
@@ -309,8 +321,15 @@ int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true) {
     const long P = (long)B * Hl * Wl;
     const bf16_t* in;
     int ldi;
+    ConvSrc src{};
     if (l == 0) { in = a.X0; ldi = kCinPad; }
-    else {
+    else if (fused) {
+      // 2x2 average of the skip of level l-1 (channels chan(l) .. of CAT[l-1])
+      const int cp = chan(n, l - 1);
+      in = a.CAT[l - 1] + chan(n, l);
+      ldi = chan(n, l) + cp;
+      src.mode = 1;
+    } else {
       // pool the skip of level l-1 (channels chan(l) .. of CAT[l-1])
       const int cp = chan(n, l - 1);
       const int cc = chan(n, l) + cp;
@@ -321,7 +340,7 @@ int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true) {
       in = a.POOL[l];
       ldi = cp;
     }
-    if (int rc = conv_fwd(n, enc_a(l), in, ldi, a.A[l], c, a.PA[l], B, Hl, Wl, st)) return rc;
+    if (int rc = conv_fwd(n, enc_a(l), in, ldi, a.A[l], c, a.PA[l], B, Hl, Wl, st, src)) return rc;
     bf16_t* ob;
     int ldo;
     if (l < L - 1) { ob = a.CAT[l] + chan(n, l + 1); ldo = chan(n, l + 1) + c; }
@@ -331,11 +350,20 @@ int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true) {
   for (int l = L - 2; l >= 0; --l) {
     const int Hl = H >> l, Wl = W >> l, c = chan(n, l), cu = chan(n, l + 1);
     const long P = (long)B * Hl * Wl;
-    const bf16_t* src = (l == L - 2) ? a.Ebot : a.DB[l + 1];
-    hipLaunchKernelGGL(k_up, dim3(blocks(P * cu)), dim3(256), 0, st, src, cu, a.CAT[l], cu + c,
-                       cu, B, Hl, Wl);
-    AJ_LAUNCH_CHECK("k_up");
-    if (int rc = conv_fwd(n, dec_a(n, l), a.CAT[l], cu + c, a.DA[l], c, a.DPA[l], B, Hl, Wl, st))
+    const bf16_t* lower = (l == L - 2) ? a.Ebot : a.DB[l + 1];
+    ConvSrc src{};
+    if (fused) {
+      src.mode = 2;  // channels < cu: nearest upsampling of `lower`
+      src.in2 = lower;
+      src.ld2 = cu;
+      src.cu = cu;
+    } else {
+      hipLaunchKernelGGL(k_up, dim3(blocks(P * cu)), dim3(256), 0, st, lower, cu, a.CAT[l],
+                         cu + c, cu, B, Hl, Wl);
+      AJ_LAUNCH_CHECK("k_up");
+    }
+    if (int rc = conv_fwd(n, dec_a(n, l), a.CAT[l], cu + c, a.DA[l], c, a.DPA[l], B, Hl, Wl, st,
+                          src))
       return rc;
     if (int rc = conv_fwd(n, dec_b(n, l), a.DA[l], c, a.DB[l], c, a.DPB[l], B, Hl, Wl, st))
       return rc;
@@ -912,7 +940,7 @@ extern "C" int aj_nn_warm_start_batched(aj_nn* n, const float* rhs, const float*
     hipLaunchKernelGGL(k_ws_input, g2, dim3(256), 0, st, (const float*)R, (const float*)S,
                        (const double*)part, nblk, ni, n->act.X0, sv);
     AJ_LAUNCH_CHECK("k_ws_input");
-    if (int rc = forward(n, B, H, W, st, false)) return rc;
+    if (int rc = forward(n, B, H, W, st, false, true)) return rc;
     hipLaunchKernelGGL(k_ws_head, dim3(blocks(nc)), dim3(256), 0, st, (const bf16_t*)n->act.DB[0],
                        n->c0, n->c0, (const float*)(n->params + n->head_w),
                        (const float*)(n->params + n->head_b), (const float*)sv, ni, pscale, A, nc);

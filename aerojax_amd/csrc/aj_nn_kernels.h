@@ -56,6 +56,12 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 // distinct 4-bank groups, so the 16-byte fragment reads are conflict-free.
 // The next slice's global loads are issued before the current slice's
 // MFMAs, so their latency overlaps the arithmetic.
+//
+// Input modes (inference, where the pooled / upsampled tensors are not
+// needed afterwards): src = 1 reads the input as the 2x2 average of `in`
+// (2H x 2W, the arithmetic of k_pool); src = 2 reads channels < cu as the
+// nearest upsampling of in2 (H/2 x W/2, stride ld2, as k_up) and channels
+// >= cu from `in` (the concatenation's skip part).
 // ---------------------------------------------------------------------------
 struct ConvArgs {
   const bf16_t* in;
@@ -71,7 +77,24 @@ struct ConvArgs {
   float* outf;          // fp32 output (input-gradient mode, may be null)
   int ldf, accum;
   int act;              // 1 = GELU
+  int src;              // 0 = in, 1 = 2x2 average of in, 2 = [up(in2), in]
+  const bf16_t* in2;    // src = 2: lower-resolution tensor
+  int ld2, cu;          // its channel stride, channels taken from it
 };
+
+// 2x2 average of four bf16x8 rows, channel by channel (k_pool's order)
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ uint4 pool4(uint4 a, uint4 b, uint4 c, uint4 d) {
+  const u16x8 x00 = __builtin_bit_cast(u16x8, a), x01 = __builtin_bit_cast(u16x8, b);
+  const u16x8 x10 = __builtin_bit_cast(u16x8, c), x11 = __builtin_bit_cast(u16x8, d);
+  u16x8 o;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float s = bf2f(x00[k]) + bf2f(x01[k]) + bf2f(x10[k]) + bf2f(x11[k]);
+    o[k] = f2bf(0.25f * s);
+  }
+  return __builtin_bit_cast(uint4, o);
+}
 
 constexpr int kLdsRow = 40;  // bf16 per LDS row (32 channels + pad)
 
@@ -103,6 +126,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[m][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   const bf16_t* img = a.in + (long)b * a.H * a.W * a.ldi;
+  const bf16_t* img4 = a.in + (long)b * 4 * a.H * a.W * a.ldi;  // src = 1: 2H x 2W input
+  const bf16_t* img2 = a.in2 ? a.in2 + (long)b * (a.H >> 1) * (a.W >> 1) * a.ld2 : nullptr;
   uint4 rx[NXL], rw[NWL];
   // global -> registers: the input patch (one-pixel halo, zero outside the
   // image) and the slice's weights, row (tap, co) holding 32 channels
@@ -115,8 +140,22 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         const int px = e >> 2, q = e & 3;
         const int rr = px / RW, cc = px - rr * RW;
         const int hh = h0 + rr - 1, ww = w0 + cc - 1;
-        if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
-          v = *reinterpret_cast<const uint4*>(img + ((long)hh * a.W + ww) * a.ldi + c0 + q * 8);
+        if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W) {
+          if (a.src == 1) {
+            const int W2 = 2 * a.W;
+            const bf16_t* f = img4 + ((long)(2 * hh) * W2 + 2 * ww) * a.ldi + c0 + q * 8;
+            const uint4 x00 = *reinterpret_cast<const uint4*>(f);
+            const uint4 x01 = *reinterpret_cast<const uint4*>(f + a.ldi);
+            const uint4 x10 = *reinterpret_cast<const uint4*>(f + (long)W2 * a.ldi);
+            const uint4 x11 = *reinterpret_cast<const uint4*>(f + (long)(W2 + 1) * a.ldi);
+            v = pool4(x00, x01, x10, x11);
+          } else if (a.src == 2 && c0 < a.cu) {
+            v = *reinterpret_cast<const uint4*>(
+                img2 + ((long)(hh >> 1) * (a.W >> 1) + (ww >> 1)) * a.ld2 + c0 + q * 8);
+          } else {
+            v = *reinterpret_cast<const uint4*>(img + ((long)hh * a.W + ww) * a.ldi + c0 + q * 8);
+          }
+        }
       }
       rx[k] = v;
     }
