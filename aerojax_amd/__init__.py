@@ -13,8 +13,9 @@ All arithmetic runs in hand-written HIP kernels in ``libaerojax_hip.so``.
 """
 from ._lib import AeroJaxError, AeroJaxLibraryError, load as load_library, version
 from .advection_schemes import rk_step_unified
-from .operators import (constant_smagorinsky, divergence, divergence_nonperiodic, pressure_rhs,
-                        project, project_collocated, smagorinsky_total)
+from .operators import (constant_smagorinsky, divergence, divergence_nonperiodic,
+                        dynamic_smagorinsky, pressure_rhs, project, project_collocated,
+                        smagorinsky_total)
 from .batch import BatchedStepper
 from .diagnostics import flow_diagnostics
 from .params import get_re_parameters
@@ -26,7 +27,7 @@ from .step import CollocatedStepper, SimState, initial_state, step_collocated, s
 
 __all__ = [
     "AeroJaxError", "AeroJaxLibraryError", "load_library", "version", "rk_step_unified",
-    "constant_smagorinsky", "divergence", "divergence_nonperiodic", "pressure_rhs", "project",
+    "constant_smagorinsky", "dynamic_smagorinsky", "divergence", "divergence_nonperiodic", "pressure_rhs", "project",
     "project_collocated", "smagorinsky_total", "get_re_parameters", "flow_diagnostics",
     "MultigridPlan", "poisson_multigrid", "poisson_multigrid_to_tolerance",
     "BatchedStepper", "CollocatedStepper", "SimState", "initial_state", "step_collocated", "step_pure",

@@ -33,7 +33,7 @@ AJ_MG_OPT_RELATIVE = 3
 # Every symbol the header declares; tests check the library exports them.
 EXPORTS = (
     "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
-    "aj_les_smagorinsky_total", "aj_divergence_rhs", "aj_divergence_periodic", "aj_project",
+    "aj_les_smagorinsky_total", "aj_les_dynamic_smagorinsky", "aj_divergence_rhs", "aj_divergence_periodic", "aj_project",
     "aj_project_collocated",
     "aj_flow_diagnostics", "aj_geometry", "aj_batch_geometry", "aj_tune_enable", "aj_tune_reset",
     "aj_tune_report",
@@ -133,6 +133,8 @@ _SIGS = {
                                  ctypes.POINTER(RkParams), _vp]),
     "aj_les_smagorinsky": (_i, [_i, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
     "aj_les_smagorinsky_total": (_i, [_i, _vp, _vp, _d, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),
+    "aj_les_dynamic_smagorinsky": (_i, [_i, _vp, _vp, _d, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d,
+                                        _vp]),
     "aj_divergence_rhs": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _vp]),
     "aj_divergence_periodic": (_i, [_i, _vp, _vp, _vp, _i, _i, _d, _d, _vp]),
     "aj_project": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),

@@ -236,20 +236,26 @@ static int conv_launch_t(const ConvArgs& c, int nt, hipStream_t st) {
   return AJ_OK;
 }
 
-// tile candidates 8x32 and 4x16 pixels, tuned per layer shape; launches
-// that accumulate into their output (input gradients of a concatenated
-// input) reuse the choice of the plain launch of the same shape
+// candidates: pixel tiles 8x32 and 4x16 with the widest output-channel
+// block NT that divides co, then 4x16 with narrower blocks (more workgroups
+// for layers with few pixel tiles); tuned per layer shape.  Launches that
+// accumulate into their output (input gradients of a concatenated input)
+// reuse the choice of the plain launch of the same shape.
 int conv_launch(const ConvArgs& c, hipStream_t st) {
   if (c.ci % 32 || c.co % 16) return fail(AJ_ERR_INVALID, "conv: ci % 32 or co % 16 != 0");
   const int nt = (c.co % 64 == 0) ? 64 : (c.co % 32 == 0 ? 32 : 16);
+  int cands[4][2] = {{0, nt}, {1, nt}, {1, 0}, {1, 0}};
+  int ncand = 2;
+  for (int w = nt / 2; w >= 16 && ncand < 4; w /= 2) cands[ncand++][1] = w;
   auto launch = [&](int k, hipStream_t s) -> int {
-    return k == 1 ? conv_launch_t<4, 16>(c, nt, s) : conv_launch_t<8, 32>(c, nt, s);
+    return cands[k][0] == 1 ? conv_launch_t<4, 16>(c, cands[k][1], s)
+                            : conv_launch_t<8, 32>(c, cands[k][1], s);
   };
   char key[96];
   std::snprintf(key, sizeof(key), "conv/%dx%dx%d/%d->%d/%s%s", c.B, c.H, c.W, c.ci, c.co,
                 c.outf ? "f32" : "bf16", c.src == 1 ? "/pool" : (c.src == 2 ? "/up" : ""));
   int pick = 0;
-  return tune_pick(key, c.accum ? 1 : 2, launch, st, &pick);
+  return tune_pick(key, c.accum ? 1 : ncand, launch, st, &pick);
 }
 
 // forward conv + GELU: out (post) and pre both stored

@@ -242,9 +242,12 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
 // Weight gradient: dW[co][ci][kh][kw] += sum_p g[p][co] * x[p + tap][ci]
 // (fp32, torch OIHW layout, atomics across pixel chunks).
 // A (16 co x 32 pixels) and B (32 pixels x 16 ci) are staged transposed in
-// LDS: sg[co][px], sx[ci][row][px] for 32 pixels of one image row and the
-// 3 x 34 input pixels their taps touch.  Wave wv owns output channels
-// co0 + 16 wv .. +15 and keeps 9 taps x 2 ci sub-tiles of accumulators.
+// LDS from 16-byte loads: sg[co][px], and sx[dw][ci][row][px], the 3 x 34
+// input pixels the taps of 32 pixels of one image row touch, stored once per
+// tap column offset dw so that every fragment is one aligned 16-byte read.
+// The next segment's loads are issued before the current segment's MFMAs.
+// Wave wv owns output channels co0 + 16 wv .. +15 and keeps 9 taps x 2 ci
+// sub-tiles of accumulators.
 // ---------------------------------------------------------------------------
 struct DwArgs {
   const bf16_t* g;   // dPre, NHWC, stride ldg, co channels
@@ -257,9 +260,12 @@ struct DwArgs {
 };
 
 __global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
-  __shared__ bf16_t sg[64][40];      // [co][px] (32 used, padded rows)
-  __shared__ bf16_t sx[32][3][48];   // [ci][dh][px + 1] (34 used)
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // sg[co][px]: 32 pixels of dPre per output channel (rows of 40: 16-byte
+  // aligned fragments); sx[dw][ci][dh][px]: the input row segment shifted by
+  // the tap's column offset dw, so every B fragment is one aligned read
+  __shared__ __attribute__((aligned(16))) bf16_t sg[64][40];
+  __shared__ __attribute__((aligned(16))) bf16_t sx[3][32][3][40];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int co0 = blockIdx.y * 64, ci0 = blockIdx.z * 32;
   const int nseg = (a.W + 31) / 32;
   const long nrows = (long)a.B * a.H;
@@ -268,44 +274,78 @@ __global__ __launch_bounds__(256) void k_conv_dw(DwArgs a) {
   for (int t = 0; t < 9; ++t) acc[t][0] = acc[t][1] = floatx4{0.f, 0.f, 0.f, 0.f};
   const long r0 = (long)blockIdx.x * a.rows_per_block;
   const long r1 = r0 + a.rows_per_block < nrows ? r0 + a.rows_per_block : nrows;
-  for (long row = r0; row < r1; ++row) {
+  const long nsegs = (r1 > r0) ? (r1 - r0) * nseg : 0;
+  // per segment and thread: one 8-channel dPre load, two 8-channel input loads
+  uint4 rg, rx0, rx1;
+  auto fetch = [&](long k) {
+    const long row = r0 + k / nseg;
+    const int w0 = (int)(k % nseg) * 32;
     const int b = (int)(row / a.H), h = (int)(row - (long)b * a.H);
-    for (int s = 0; s < nseg; ++s) {
-      const int w0 = s * 32;
-      __syncthreads();
-      // stage g: 32 px x 64 co (transposed)
-      for (int e = threadIdx.x; e < 32 * 64; e += 256) {
-        const int px = e >> 6, c = e & 63;
-        const int ww = w0 + px, co = co0 + c;
-        bf16_t v = 0;
-        if (ww < a.W && co < a.co) v = a.g[((long)(b * a.H + h) * a.W + ww) * a.ldg + co];
-        sg[c][px] = v;
-      }
-      // stage x: 3 rows x 34 px x 32 ci (transposed)
-      for (int e = threadIdx.x; e < 3 * 34 * 32; e += 256) {
-        const int c = e & 31, q = e >> 5;
+    {
+      const int px = tid >> 3, cg = (tid & 7) * 8;
+      const int ww = w0 + px, co = co0 + cg;
+      rg = make_uint4(0, 0, 0, 0);
+      if (ww < a.W && co < a.co)
+        rg = *reinterpret_cast<const uint4*>(a.g + ((long)(b * a.H + h) * a.W + ww) * a.ldg + co);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = tid + u * 256;  // 3 rows x 34 px x 4 channel groups
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (e < 3 * 34 * 4) {
+        const int cg = (e & 3) * 8, q = e >> 2;
         const int dh = q / 34, px = q - dh * 34;
-        const int hh = h + dh - 1, ww = w0 + px - 1, ci = ci0 + c;
-        bf16_t v = 0;
+        const int hh = h + dh - 1, ww = w0 + px - 1, ci = ci0 + cg;
         if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W && ci < a.ci)
-          v = a.x[((long)(b * a.H + hh) * a.W + ww) * a.ldx + ci];
-        sx[c][dh][px] = v;
+          v = *reinterpret_cast<const uint4*>(a.x + ((long)(b * a.H + hh) * a.W + ww) * a.ldx + ci);
       }
-      __syncthreads();
-      if (co0 + wv * 16 >= a.co) continue;
-      const int kq = (lane >> 4) * 8;
-      const bf16x8 A = *reinterpret_cast<const bf16x8*>(&sg[wv * 16 + (lane & 15)][kq]);
+      if (u == 0) rx0 = v; else rx1 = v;
+    }
+  };
+  auto stash = [&]() {
+    {
+      const int px = tid >> 3, cg = (tid & 7) * 8;
+      const bf16_t* s = reinterpret_cast<const bf16_t*>(&rg);
 #pragma unroll
-      for (int t = 0; t < 9; ++t) {
-        const int dh = t / 3, dw = t % 3;
+      for (int k = 0; k < 8; ++k) sg[cg + k][px] = s[k];
+    }
 #pragma unroll
-        for (int nsub = 0; nsub < 2; ++nsub) {
-          const bf16_t* src = &sx[nsub * 16 + (lane & 15)][dh][kq + dw];
-          bf16x8 Bm;
+    for (int u = 0; u < 2; ++u) {
+      const int e = tid + u * 256;
+      if (e < 3 * 34 * 4) {
+        const int cg = (e & 3) * 8, q = e >> 2;
+        const int dh = q / 34, px = q - dh * 34;  // px = column + 1 of the segment
+        const uint4 v = (u == 0) ? rx0 : rx1;
+        const bf16_t* s = reinterpret_cast<const bf16_t*>(&v);
+        // copy dw holds column p + dw - 1 at position p (p = px - dw)
 #pragma unroll
-          for (int k = 0; k < 8; ++k) Bm[k] = __builtin_bit_cast(__bf16, src[k]);
-          acc[t][nsub] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bm, acc[t][nsub], 0, 0, 0);
+        for (int dw = 0; dw < 3; ++dw) {
+          const int p = px - dw;
+          if (p >= 0 && p < 32)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sx[dw][cg + k][dh][p] = s[k];
         }
+      }
+    }
+  };
+  if (nsegs > 0) fetch(0);
+  const int kq = (lane >> 4) * 8;
+  const bool active = co0 + wv * 16 < a.co;
+  for (long k = 0; k < nsegs; ++k) {
+    if (k > 0) __syncthreads();  // the previous segment's fragments are read
+    stash();
+    __syncthreads();
+    if (k + 1 < nsegs) fetch(k + 1);
+    if (!active) continue;
+    const bf16x8 A = *reinterpret_cast<const bf16x8*>(&sg[wv * 16 + (lane & 15)][kq]);
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int dh = t / 3, dw = t % 3;
+#pragma unroll
+      for (int nsub = 0; nsub < 2; ++nsub) {
+        const bf16x8 Bm =
+            *reinterpret_cast<const bf16x8*>(&sx[dw][nsub * 16 + (lane & 15)][dh][kq]);
+        acc[t][nsub] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bm, acc[t][nsub], 0, 0, 0);
       }
     }
   }

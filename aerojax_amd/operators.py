@@ -117,6 +117,26 @@ def smagorinsky_total(u, v, nu: float, dx: float, dy: float, delta: float, C_s: 
     return A.out_like(s, want_np), A.out_like(t, want_np)
 
 
+def dynamic_smagorinsky(u, v, dx: float, dy: float, delta: float, alpha: float = 2.0, *,
+                        nu: float = None):
+    """``(nu_sgs, C_d)`` of the dynamic Smagorinsky model
+    (solver/les_models.py:78-176, collocated).  With ``nu`` given, returns
+    ``(nu_sgs, nu + nu_sgs, C_d)`` (the GUI step's viscosity) from the same
+    launches."""
+    want_np, ut, (vt,) = _prep(u, v, names=("u", "v"))
+    nx, ny = ut.shape
+    s, c = torch.empty_like(ut), torch.empty_like(ut)
+    t = torch.empty_like(ut) if nu is not None else None
+    with torch.cuda.device(ut.device):
+        rc = _lib.load().aj_les_dynamic_smagorinsky(
+            A.dtype_code(ut), A.ptr(ut), A.ptr(vt), float(nu or 0.0), A.ptr(s), A.ptr(t), A.ptr(c),
+            nx, ny, float(dx), float(dy), float(delta), float(alpha), A.stream_handle(ut.device))
+    _lib.check(rc, "aj_les_dynamic_smagorinsky")
+    if nu is None:
+        return A.out_like(s, want_np), A.out_like(c, want_np)
+    return A.out_like(s, want_np), A.out_like(t, want_np), A.out_like(c, want_np)
+
+
 def project_collocated(u_star, v_star, p, mask, dt: float, dx: float, dy: float, U_in: float, *,
                        out=None):
     """Pressure correction and BCs of the GUI step (step_handlers.py:131-176):

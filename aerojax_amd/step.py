@@ -22,8 +22,8 @@ import torch
 
 from . import _arrays as A
 from .advection_schemes import rk_step_unified
-from .operators import (constant_smagorinsky, pressure_rhs, project, project_collocated,
-                        smagorinsky_total)
+from .operators import (constant_smagorinsky, dynamic_smagorinsky, pressure_rhs, project,
+                        project_collocated, smagorinsky_total)
 from .pressure_solvers import get_plan
 
 
@@ -76,15 +76,18 @@ def step_pure(state: SimState, mask, sdf, dx: float, dy: float, nu: float, U_inf
         raise NotImplementedError("only flow_type='von_karman' is implemented")
     if adaptive_dt:
         raise NotImplementedError("adaptive_dt: the PID dt controller is not implemented")
-    if use_les and les_model != "smagorinsky":
-        raise NotImplementedError("only les_model='smagorinsky' is implemented")
+    if use_les and les_model not in ("smagorinsky", "dynamic_smagorinsky"):
+        raise NotImplementedError(f"les_model={les_model!r}")
     u, v, dt = state.u, state.v, state.dt
     mt = A.as_device(mask, u.dtype, u.device, "mask")
     st = A.as_device(sdf, u.dtype, u.device, "sdf")
     nu_sgs = None
     if use_les and les_coupling == "apply":
         delta = (dx * dy) ** 0.5
-        nu_sgs = constant_smagorinsky(u, v, dx, dy, delta, smagorinsky_constant)
+        if les_model == "dynamic_smagorinsky":
+            nu_sgs, _ = dynamic_smagorinsky(u, v, dx, dy, delta)
+        else:
+            nu_sgs = constant_smagorinsky(u, v, dx, dy, delta, smagorinsky_constant)
     us, vs = rk_step_unified(u, v, dt, nu, dx, dy, mt, sdf=st, U_inf=U_inf, nu_sgs=nu_sgs,
                              nu_hyper_ratio=0.0, slip_walls=True, fast_mode=fast_mode,
                              brinkman_eta=brinkman_eta, flow_type=flow_type)
@@ -144,8 +147,8 @@ def step_collocated(u, v, mask, dt: float, *, sdf, dx: float, dy: float, nu: flo
     """
     if flow_type != "von_karman":
         raise NotImplementedError("only flow_type='von_karman' is implemented")
-    if use_les and les_model == "dynamic_smagorinsky":
-        raise NotImplementedError("les_model='dynamic_smagorinsky' is not implemented")
+    if use_les and les_model not in ("smagorinsky", "dynamic_smagorinsky"):
+        raise NotImplementedError(f"les_model={les_model!r}")
     A.require_gpu()
     want_np = A.is_numpy(u)
     dt_ = A.torch_dtype_of(u)
@@ -156,7 +159,10 @@ def step_collocated(u, v, mask, dt: float, *, sdf, dx: float, dy: float, nu: flo
     nu_arg, nu_sgs = nu, None
     if use_les:
         delta = (dx * dy) ** 0.5
-        nu_sgs, nu_arg = smagorinsky_total(ut, vt, nu, dx, dy, delta, smagorinsky_constant)
+        if les_model == "dynamic_smagorinsky":
+            nu_sgs, nu_arg, _ = dynamic_smagorinsky(ut, vt, dx, dy, delta, nu=nu)
+        else:
+            nu_sgs, nu_arg = smagorinsky_total(ut, vt, nu, dx, dy, delta, smagorinsky_constant)
     us, vs = rk_step_unified(ut, vt, dt, nu_arg, dx, dy, mt, sdf=st, U_inf=U_inf, nu_sgs=nu_sgs,
                              nu_hyper_ratio=nu_hyper_ratio, slip_walls=slip_walls,
                              fast_mode=fast_mode, brinkman_eta=brinkman_eta, flow_type=flow_type)

@@ -94,6 +94,16 @@ int aj_les_smagorinsky_total(int dtype, const void* u, const void* v, double nu,
                              void* nu_sgs, void* nu_total, int nx, int ny, double dx,
                              double dy, double delta, double c_s, void* stream);
 
+/* Replaces dynamic_smagorinsky (solver/les_models.py:57-176), collocated:
+ * 5x5 periodic test filter, Leonard stress and M_ij with alpha (reference
+ * default 2), 3x3 interior least-squares averages, C_d = clip(LM / (MM +
+ * 1e-8), 0, 0.2) (0 on the border), nu_sgs = C_d delta^2 |S|.  nu_total (=
+ * nu + nu_sgs, the GUI step's viscosity) and c_d may be NULL.  Uses a
+ * stream-ordered scratch allocation of 14 fields. */
+int aj_les_dynamic_smagorinsky(int dtype, const void* u, const void* v, double nu, void* nu_sgs,
+                               void* nu_total, void* c_d, int nx, int ny, double dx, double dy,
+                               double delta, double alpha, void* stream);
+
 /* ---- Pressure right-hand side -----------------------------------------
  * Replaces divergence_nonperiodic (solver/operators.py:81-96) followed by
  * rhs = div*mask/dt (solver/pure_functions/step.py:194-199).  mask == NULL

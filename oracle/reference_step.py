@@ -192,6 +192,60 @@ def constant_smagorinsky(u, v, dx, dy, delta, C_s=0.1):
     return (dt.type(C_s) * dt.type(delta)) ** 2 * S
 
 
+def strain_rate(u, v, dx, dy):
+    """compute_strain_rate (solver/les_models.py:22-54), collocated:
+    (|S|, Sxx, Syy, Sxy) with periodic central differences."""
+    du_dx = grad_x_periodic(u, dx)
+    du_dy = grad_y_periodic(u, dy)
+    dv_dx = grad_x_periodic(v, dx)
+    dv_dy = grad_y_periodic(v, dy)
+    Sxx, Syy, Sxy = du_dx, dv_dy, 0.5 * (du_dy + dv_dx)
+    S = np.sqrt(2.0 * (Sxx ** 2 + Syy ** 2 + 2.0 * Sxy ** 2) + 1e-10)
+    return S, Sxx, Syy, Sxy
+
+
+def box_filter_2d(f, filter_ratio=2):
+    """box_filter_2d (solver/les_models.py:57-75): mean over the
+    (2r+1) x (2r+1) periodic neighbourhood, accumulated in the reference's
+    scan order (x offset outer, y offset inner, each -r..r) of
+    roll(roll(f, i, 0), j, 1)."""
+    acc = np.zeros_like(f)
+    for i in range(-filter_ratio, filter_ratio + 1):
+        for j in range(-filter_ratio, filter_ratio + 1):
+            acc = acc + np.roll(np.roll(f, i, axis=0), j, axis=1)
+    return acc / f.dtype.type((2 * filter_ratio + 1) ** 2)
+
+
+def _local_average(f):
+    """3x3 interior average, zero on the border (les_models.py:155-165)."""
+    out = np.zeros_like(f)
+    out[1:-1, 1:-1] = (f[1:-1, 1:-1] + f[2:, 1:-1] + f[:-2, 1:-1] + f[1:-1, 2:] + f[1:-1, :-2]
+                       + f[2:, 2:] + f[2:, :-2] + f[:-2, 2:] + f[:-2, :-2]) / f.dtype.type(9.0)
+    return out
+
+
+def dynamic_smagorinsky(u, v, dx, dy, delta, alpha=2.0):
+    """dynamic_smagorinsky (solver/les_models.py:78-176), collocated: Germano
+    identity with the 5x5 test filter, L_ij and M_ij, 3x3-averaged least
+    squares C_d = LM / (MM + 1e-8) clipped to [0, 0.2],
+    nu_sgs = C_d delta^2 |S|.  Returns (nu_sgs, C_d)."""
+    T = u.dtype.type
+    S, Sxx, Syy, Sxy = strain_rate(u, v, dx, dy)
+    ut, vt = box_filter_2d(u), box_filter_2d(v)
+    St, Sxxt, Syyt, Sxyt = strain_rate(ut, vt, dx, dy)
+    L11 = ut * ut - box_filter_2d(u * u)
+    L12 = ut * vt - box_filter_2d(u * v)
+    L22 = vt * vt - box_filter_2d(v * v)
+    a2 = T(alpha) ** 2
+    M11 = a2 * St * Sxxt - box_filter_2d(S * Sxx)
+    M12 = a2 * St * Sxyt - box_filter_2d(S * Sxy)
+    M22 = a2 * St * Syyt - box_filter_2d(S * Syy)
+    LM = _local_average(L11 * M11 + L12 * M12 + L22 * M22)
+    MM = _local_average(M11 * M11 + M12 * M12 + M22 * M22)
+    C = np.clip(LM / (MM + T(1e-8)), 0.0, 0.2).astype(u.dtype)
+    return C * (T(delta) * T(delta)) * S, C
+
+
 # ---------------------------------------------------------------------------
 # rk_step_unified (advection_schemes/rk3_simple_new.py:175-358)
 # ---------------------------------------------------------------------------
@@ -583,16 +637,19 @@ def project_collocated(u_star, v_star, p, mask, dt, dx, dy, U_in):
 def step_collocated(u, v, mask, sdf, dx, dy, nu, U_inf, dt, use_les=False,
                     smagorinsky_constant=0.17, nu_hyper_ratio=0.0, slip_walls=True,
                     fast_mode=False, brinkman_eta=0.01, v_cycles=5, solver_dt=None,
-                    U_in=None, levels=4, return_parts=False):
-    """_step_collocated (step_handlers.py:62-185), constant Smagorinsky,
-    multigrid pressure.  With LES the RK step gets nu + nu_sgs as ``nu`` and
-    nu_sgs as ``nu_sgs`` (:76-90); rhs = div / dt without the mask (:92-98).
-    Returns ``(u, v, p)``."""
+                    U_in=None, levels=4, return_parts=False, les_model="smagorinsky"):
+    """_step_collocated (step_handlers.py:62-185), constant or dynamic
+    Smagorinsky (:71-77), multigrid pressure.  With LES the RK step gets
+    nu + nu_sgs as ``nu`` and nu_sgs as ``nu_sgs`` (:76-90); rhs = div / dt
+    without the mask (:92-98).  Returns ``(u, v, p)``."""
     T = u.dtype.type
     nu_arg, nu_sgs = nu, None
     if use_les:
         delta = (dx * dy) ** 0.5
-        nu_sgs = constant_smagorinsky(u, v, dx, dy, delta, smagorinsky_constant)
+        if les_model == "dynamic_smagorinsky":
+            nu_sgs, _ = dynamic_smagorinsky(u, v, dx, dy, delta)
+        else:
+            nu_sgs = constant_smagorinsky(u, v, dx, dy, delta, smagorinsky_constant)
         nu_arg = T(nu) + nu_sgs
     us, vs = rk_step_unified(u, v, dt, nu_arg, dx, dy, mask, sdf=sdf, U_inf=U_inf,
                              nu_sgs=nu_sgs, nu_hyper_ratio=nu_hyper_ratio,

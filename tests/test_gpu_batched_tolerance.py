@@ -282,3 +282,42 @@ def test_batched_geometry_matches_per_instance(gpu):
         rs, rm = _oracle_geometry(nx, ny, nx * dx, ny * dy, ob)
         _check_geometry(b.view(b.sdf, k), b.view(b.mask, k), rs, rm, torch.float64, f"instance {k}")
     b.close()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_dynamic_smagorinsky_parity(gpu, dtype):
+    """dynamic_smagorinsky (nu_sgs, C_d, nu + nu_sgs) against the oracle on a
+    wake-like field, and the GUI step with les_model='dynamic_smagorinsky'."""
+    import aerojax_amd as aj
+    c = case(256, 64, 4.0, 1.0, kind="cylinder", seed=4)
+    dx, dy = c["dx"], c["dy"]
+    delta = (dx * dy) ** 0.5
+    f = {torch.float32: np.float32, torch.float64: np.float64}[dtype]
+    rnu, rC = R.dynamic_smagorinsky(c["u"], c["v"], dx, dy, delta)
+    rnu_d, rC_d = R.dynamic_smagorinsky(c["u"].astype(f), c["v"].astype(f), dx, dy, delta)
+    nu_sgs, nu_tot, C = aj.dynamic_smagorinsky(dev(c["u"], dtype), dev(c["v"], dtype), dx, dy,
+                                               delta, nu=1e-3)
+    # L_ij = u~u~ - (uu)~ cancels (|u| ~ 1 against a sub-filter variance
+    # ~1e-3), so FP32 rounding (6e-8) reaches ~1e-4 in C_d whatever the
+    # summation order: the FP32 restatement of the reference formula is itself
+    # ~7e-4 from FP64.  FP32 bar: 2e-3 against FP64; FP64 bar: 1e-12.
+    tol = 2e-3 if dtype == torch.float32 else 1e-12
+    for name, got, r64, rdt in (("nu_sgs", nu_sgs, rnu, rnu_d), ("C_d", C, rC, rC_d)):
+        e64, edt = R.rel_l2(host(got), r64), R.rel_l2(host(got), rdt)
+        print(f"dynamic smagorinsky {name} {dtype}: vs fp64 {e64:.2e}, vs same dtype {edt:.2e}",
+              flush=True)
+        assert e64 <= tol, (name, e64)
+    assert R.rel_l2(host(nu_tot), 1e-3 + rnu) <= tol
+    assert host(C).max() <= 0.2 and host(C).min() >= 0.0
+    # GUI step with the dynamic model
+    kw = dict(use_les=True, les_model="dynamic_smagorinsky", nu_hyper_ratio=0.0, fast_mode=False,
+              brinkman_eta=0.01, v_cycles=5)
+    ru, rv, rp = R.step_collocated(c["u"].astype(f), c["v"].astype(f), c["mask"].astype(f),
+                                   c["sdf"].astype(f), dx, dy, 1e-3, 1.0, 2e-4, **kw)
+    u, v, p = aj.step_collocated(dev(c["u"], dtype), dev(c["v"], dtype), dev(c["mask"], dtype),
+                                 2e-4, sdf=dev(c["sdf"], dtype), dx=dx, dy=dy, nu=1e-3, U_inf=1.0,
+                                 **kw)
+    for name, got, r in (("u", u, ru), ("v", v, rv), ("p", p, rp)):
+        e = R.rel_l2(host(got), r)
+        print(f"GUI step dynamic LES {dtype} {name}: {e:.2e}", flush=True)
+        assert e <= (1e-5 if dtype == torch.float32 else 1e-11), name

@@ -497,3 +497,81 @@ def test_check_cfl_closed_form():
     assert np.isclose(cfl, 25.0 * 1e-3 / 0.05, rtol=1e-15)
     d = R.flow_diagnostics(u, v, np.zeros_like(u), np.ones_like(u), 0.1, 0.05, 1e-3)
     assert d["max_speed"] == 25.0 and np.isclose(d["cfl"], 0.5)
+
+
+def _dyn_smag_loops(u, v, dx, dy, delta, alpha=2.0):
+    """dynamic_smagorinsky written out cell by cell with modular indices
+    (an independent restatement of les_models.py:78-176 for tiny grids)."""
+    nx, ny = u.shape
+    P = lambda f, i, j: f[i % nx][j % ny]  # noqa: E731
+
+    def strain(a, b):
+        S, xx, yy, xy = (np.zeros((nx, ny)) for _ in range(4))
+        for i in range(nx):
+            for j in range(ny):
+                ax = (P(a, i + 1, j) - P(a, i - 1, j)) / (2 * dx)
+                ay = (P(a, i, j + 1) - P(a, i, j - 1)) / (2 * dy)
+                bx = (P(b, i + 1, j) - P(b, i - 1, j)) / (2 * dx)
+                by = (P(b, i, j + 1) - P(b, i, j - 1)) / (2 * dy)
+                xx[i, j], yy[i, j], xy[i, j] = ax, by, 0.5 * (ay + bx)
+                S[i, j] = math.sqrt(2 * (ax * ax + by * by + 2 * xy[i, j] ** 2) + 1e-10)
+        return S, xx, yy, xy
+
+    def box(f):
+        out = np.zeros((nx, ny))
+        for i in range(nx):
+            for j in range(ny):
+                out[i, j] = sum(P(f, i + a, j + b) for a in range(-2, 3) for b in range(-2, 3)) / 25
+        return out
+
+    S, xx, yy, xy = strain(u, v)
+    ut, vt = box(u), box(v)
+    St, xxt, yyt, xyt = strain(ut, vt)
+    L = [ut * ut - box(u * u), ut * vt - box(u * v), vt * vt - box(v * v)]
+    M = [alpha ** 2 * St * xxt - box(S * xx), alpha ** 2 * St * xyt - box(S * xy),
+         alpha ** 2 * St * yyt - box(S * yy)]
+    lm = L[0] * M[0] + L[1] * M[1] + L[2] * M[2]
+    mm = M[0] * M[0] + M[1] * M[1] + M[2] * M[2]
+    C = np.zeros((nx, ny))
+    for i in range(1, nx - 1):
+        for j in range(1, ny - 1):
+            a = sum(lm[i + p, j + q] for p in (-1, 0, 1) for q in (-1, 0, 1)) / 9
+            b = sum(mm[i + p, j + q] for p in (-1, 0, 1) for q in (-1, 0, 1)) / 9
+            C[i, j] = min(max(a / (b + 1e-8), 0.0), 0.2)
+    return C * delta * delta * S, C
+
+
+def test_dynamic_smagorinsky_matches_cell_by_cell_restatement():
+    rng = np.random.default_rng(21)
+    nx, ny, dx, dy = 9, 7, 0.1, 0.07
+    X = np.arange(nx)[:, None] * 2 * np.pi / nx
+    Y = np.arange(ny)[None, :] * 2 * np.pi / ny
+    u = 1.0 + 0.3 * np.sin(X) * np.cos(2 * Y) + 0.05 * rng.normal(size=(nx, ny))
+    v = 0.2 * np.cos(2 * X) * np.sin(Y) + 0.05 * rng.normal(size=(nx, ny))
+    delta = (dx * dy) ** 0.5
+    nu, C = R.dynamic_smagorinsky(u, v, dx, dy, delta)
+    nu2, C2 = _dyn_smag_loops(u, v, dx, dy, delta)
+    # L and M are differences of nearly equal terms: compare in relative L2
+    assert R.rel_l2(C, C2) < 1e-12 and R.rel_l2(nu, nu2) < 1e-12
+    assert C.max() > 0 and C.max() <= 0.2 and C.min() >= 0.0
+    assert np.all(C[0] == 0) and np.all(C[-1] == 0) and np.all(C[:, 0] == 0) and np.all(C[:, -1] == 0)
+
+
+def test_box_filter_closed_forms():
+    """A constant is unchanged; a Fourier mode is scaled by the product of
+    the 1D factors (1/5) sum_{i=-2..2} cos(2 pi k i / n)."""
+    nx, ny, kx, ky = 16, 12, 3, 2
+    X = np.arange(nx)[:, None]
+    Y = np.arange(ny)[None, :]
+    assert np.allclose(R.box_filter_2d(np.full((nx, ny), 2.5)), 2.5, rtol=1e-15)
+    f = np.cos(2 * np.pi * kx * X / nx) * np.cos(2 * np.pi * ky * Y / ny)
+    gx = sum(np.cos(2 * np.pi * kx * i / nx) for i in range(-2, 3)) / 5
+    gy = sum(np.cos(2 * np.pi * ky * j / ny) for j in range(-2, 3)) / 5
+    assert np.allclose(R.box_filter_2d(f), gx * gy * f, atol=1e-14)
+
+
+def test_dynamic_smagorinsky_uniform_flow_is_zero():
+    u = np.full((10, 8), 1.3)
+    v = np.full((10, 8), -0.4)
+    nu, C = R.dynamic_smagorinsky(u, v, 0.1, 0.1, 0.1)
+    assert np.all(nu == 0) and np.all(C == 0)
