@@ -288,8 +288,12 @@ int conv_bwd(aj_nn* n, int li, const float* gpost, int ldg, const bf16_t* pre, c
     d.x = in; d.ldx = ldi; d.ci = Ly.ci;
     d.dw = n->grads + Ly.w_off;
     d.B = B; d.H = H; d.W = W;
+    // about 2048 workgroups: enough to fill the chip, few enough that the
+    // per-workgroup atomic adds of the 9 co x ci partial sums stay cheap
     const long rows = (long)B * H;
-    d.rows_per_block = (int)std::max<long>(1, rows / 256);
+    const long yz = (long)((Ly.co + 63) / 64) * ((Ly.ci + 31) / 32);
+    const long nrb = std::max<long>(1, std::min<long>(rows, 2048 / yz));
+    d.rows_per_block = (int)((rows + nrb - 1) / nrb);
     dim3 grid((unsigned)((rows + d.rows_per_block - 1) / d.rows_per_block), (Ly.co + 63) / 64,
               (Ly.ci + 31) / 32);
     ProfScope prof(AJ_PROF_NN_DW, 2.0 * 9 * Ly.ci * Ly.co * (double)P, st);

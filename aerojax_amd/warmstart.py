@@ -15,30 +15,39 @@ import numpy as np
 import torch
 
 from . import workloads
+from .geometry import device_geometry
 from .nn import WarmStartNet
 from .step import CollocatedStepper
 
 
 def generate_pairs(net: WarmStartNet, n_pairs: int, seeds, nx=2048, ny=512, steps_per_seed=40,
-                   stride=2, skip=2, log=None):
+                   stride=2, skip=2, log=None, noise_sigma=1e-3):
     """Advance one flow per seed with the plain solver and record every
-    ``stride``-th step after ``skip`` steps.  Returns device tensors
-    x [N, H, W, 2] and t [N, H, W] (float32)."""
+    ``stride``-th step after ``skip`` steps.  Each seed draws its obstacle and
+    Reynolds number as ``workloads.warm_start_case``; the geometry is built on
+    the GPU and the inflow noise is drawn there (torch generator seeded with
+    the seed).  Returns device tensors x [N, H, W, 2] and t [N, H, W]
+    (float32)."""
     dev = net.device
     H, W = nx >> net.r, ny >> net.r
     X = torch.empty((n_pairs, H, W, 2), dtype=torch.float32, device=dev)
     T = torch.empty((n_pairs, H, W), dtype=torch.float32, device=dev)
     k = 0
     t0 = time.perf_counter()
+    gen = torch.Generator(device=dev)
     for seed in seeds:
         if k >= n_pairs:
             break
-        case = workloads.warm_start_case(int(seed), nx, ny)
-        g = case["grid"]
-        stp = CollocatedStepper(nx, ny, g.dx, g.dy, case["mask"], case["sdf"], case["nu"],
-                                case["U_inf"], case["dt"], device=dev, mg_variant="textbook",
-                                tolerance=case["tol"], max_cycles=60)
-        stp.load(case["u"], case["v"])
+        g, ob, Re, nu = workloads.warm_start_params(int(seed), nx, ny)
+        sdf, mask = device_geometry(g, workloads.obstacle_struct(ob), torch.float32, dev)
+        sdf = torch.nan_to_num(sdf, nan=0.0)
+        gen.manual_seed(int(seed))
+        u = 1.0 + noise_sigma * torch.randn((nx, ny), generator=gen, device=dev)
+        v = noise_sigma * torch.randn((nx, ny), generator=gen, device=dev)
+        stp = CollocatedStepper(nx, ny, g.dx, g.dy, mask, sdf, nu, 1.0, 2.5e-4, device=dev,
+                                mg_variant="textbook", tolerance=1e-6, max_cycles=60)
+        stp.load(u, v)
+        case = {"desc": ob["desc"], "Re": Re}
         for s in range(steps_per_seed):
             stp.step()
             if s >= skip and (s - skip) % stride == 0 and k < n_pairs:

@@ -36,27 +36,56 @@ def large_cylinder(noise_sigma=1e-3, seed=0):
                 nu=G.viscosity(1.0, 0.2, 1e5), dt=1e-4, U_inf=1.0, les=True, cs=0.17)
 
 
-def random_obstacle(grid, rng, eps=0.05):
+def draw_obstacle(grid, rng):
     """Cylinder with radius in [0.05, 0.15] or NACA 4-digit with thickness
-    6-18 % and AoA in [-15, 15] deg (drawn from ``rng``)."""
+    6-18 % and AoA in [-15, 15] deg (drawn from ``rng``): a dict of the
+    parameters."""
     if rng.random() < 0.5:
         r = rng.uniform(0.05, 0.15)
         cx = rng.uniform(0.6, 1.2)
         cy = grid.ly / 2 + rng.uniform(-0.1, 0.1)
-        geo = G.cylinder_geometry(grid, cx, cy, r, eps=eps)
-        desc = f"cylinder r={r:.3f} at ({cx:.2f},{cy:.2f})"
-    else:
-        tpct = int(rng.integers(6, 19))
-        m = int(rng.integers(0, 7))
-        p = int(rng.integers(2, 7)) if m > 0 else 0
-        des = f"{m}{p}{tpct:02d}"
-        aoa = rng.uniform(-15.0, 15.0)
-        chord = rng.uniform(0.3, 0.6)
-        x_le = rng.uniform(0.5, 1.0)
-        geo = G.naca_geometry(grid, des, chord=chord, aoa_deg=aoa, x_le=x_le,
-                              y_le=grid.ly / 2, eps=eps)
-        desc = f"NACA {des} c={chord:.2f} aoa={aoa:.1f}"
-    return geo, desc
+        return dict(kind="cylinder", cx=cx, cy=cy, r=r, L=2.0 * r,
+                    desc=f"cylinder r={r:.3f} at ({cx:.2f},{cy:.2f})")
+    tpct = int(rng.integers(6, 19))
+    m = int(rng.integers(0, 7))
+    p = int(rng.integers(2, 7)) if m > 0 else 0
+    des = f"{m}{p}{tpct:02d}"
+    aoa = rng.uniform(-15.0, 15.0)
+    chord = rng.uniform(0.3, 0.6)
+    x_le = rng.uniform(0.5, 1.0)
+    return dict(kind="naca", designation=des, chord=chord, aoa=aoa, x_le=x_le, y_le=grid.ly / 2,
+                L=chord, desc=f"NACA {des} c={chord:.2f} aoa={aoa:.1f}")
+
+
+def obstacle_geometry(grid, ob, eps=0.05):
+    """Host SDF and mask of a drawn obstacle."""
+    if ob["kind"] == "cylinder":
+        return G.cylinder_geometry(grid, ob["cx"], ob["cy"], ob["r"], eps=eps)
+    return G.naca_geometry(grid, ob["designation"], chord=ob["chord"], aoa_deg=ob["aoa"],
+                           x_le=ob["x_le"], y_le=ob["y_le"], eps=eps)
+
+
+def obstacle_struct(ob, eps=0.05):
+    """The same obstacle as an ``aj_obstacle`` for the GPU geometry."""
+    if ob["kind"] == "cylinder":
+        return G.cylinder_obstacle(ob["cx"], ob["cy"], ob["r"], eps)
+    return G.naca_obstacle(ob["designation"], ob["chord"], ob["aoa"], ob["x_le"], ob["y_le"], eps)
+
+
+def random_obstacle(grid, rng, eps=0.05):
+    """Draw an obstacle (``draw_obstacle``) and build its host geometry."""
+    ob = draw_obstacle(grid, rng)
+    return obstacle_geometry(grid, ob, eps), ob["desc"]
+
+
+def warm_start_params(seed=0, nx=2048, ny=512):
+    """The draws of ``warm_start_case`` without building the fields:
+    (grid, obstacle dict, Re, nu)."""
+    rng = np.random.default_rng(seed)
+    grid = G.GridParams(nx, ny, 4.0, 1.0)
+    ob = draw_obstacle(grid, rng)
+    Re = math.exp(rng.uniform(math.log(500.0), math.log(5e4)))
+    return grid, ob, Re, G.viscosity(1.0, ob["L"], Re)
 
 
 def warm_start_case(seed=0, nx=2048, ny=512, noise_sigma=1e-3):

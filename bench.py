@@ -296,8 +296,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nn", action="store_true")
-    ap.add_argument("--nn-pairs", type=int, default=2000)
-    ap.add_argument("--nn-epochs", type=int, default=3)
+    ap.add_argument("--nn-pairs", type=int, default=20000)
+    ap.add_argument("--nn-epochs", type=int, default=1)
     ap.add_argument("--nn-r", type=int, default=2, help="network runs on the grid restricted r times")
     ap.add_argument("--workload", choices=["warmstart", "batched", "large", "uneven"],
                     default="warmstart",

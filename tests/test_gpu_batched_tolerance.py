@@ -321,3 +321,36 @@ def test_dynamic_smagorinsky_parity(gpu, dtype):
         e = R.rel_l2(host(got), r)
         print(f"GUI step dynamic LES {dtype} {name}: {e:.2e}", flush=True)
         assert e <= (1e-5 if dtype == torch.float32 else 1e-11), name
+
+
+def test_nonfinite_inputs_are_flagged_per_instance(gpu):
+    """A NaN in the right-hand side sets the non-finite flag (the reference
+    propagates NaN silently) and ends the to-tolerance solve; in a batched
+    plan only that instance is affected, the others converge as alone."""
+    import aerojax_amd as aj
+    nx, ny, B = 512, 128, 3
+    dx, dy = 4.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(31)
+    b = torch.stack([dev(rng.normal(size=(nx, ny)) * 100, torch.float32) for _ in range(B)])
+    b = b.contiguous()
+    bad = b.clone()
+    bad[1, 10, 20] = float("nan")
+    # reference multigrid: all cycles run, the flag is raised
+    _, st = aj.poisson_multigrid(bad[1].contiguous(), None, dx, dy, v_cycles=3, return_stats=True)
+    assert st["nonfinite"] and not st["converged"]
+    # textbook, single grid: the adaptive driver stops on the flag
+    _, st = aj.poisson_multigrid_to_tolerance(bad[1].contiguous(), dx, dy, tolerance=1e-6,
+                                              max_cycles=40, return_stats=True)
+    assert st["nonfinite"] and not st["converged"]
+    # batched: instances 0 and 2 converge and equal the clean solves
+    pb = aj.MultigridPlan(nx, ny, dx, dy, torch.float32, "textbook", 0, count=B)
+    out_bad, out_ok = torch.empty_like(b), torch.empty_like(b)
+    pb.solve(bad, out_bad, None, 40, 1e-6)
+    sb = pb.stats_all()
+    pb.solve(b, out_ok, None, 40, 1e-6)
+    so = pb.stats_all()
+    assert sb[1]["nonfinite"] and not sb[1]["converged"]
+    for k in (0, 2):
+        assert sb[k]["converged"] and not sb[k]["nonfinite"]
+        assert torch.equal(out_bad[k], out_ok[k]) and sb[k]["norms"] == so[k]["norms"]
+    pb.close()
