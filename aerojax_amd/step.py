@@ -184,12 +184,21 @@ def step_collocated(u, v, mask, dt: float, *, sdf, dx: float, dy: float, nu: flo
 
 class CollocatedStepper:
     """Repeated steps on preallocated device buffers (no allocation per
-    step): ping-pong velocity buffers, one multigrid plan."""
+    step): ping-pong velocity buffers, one multigrid plan.
+
+    With a ``warm_start`` network and ``overlap_guess=True`` (textbook
+    solver), the zero-guess solve is started at once on the current stream
+    while the network's guess and its residual test run on a second stream
+    (the test is the plan's: the guess is kept only if its residual is not
+    above the zero guess's).  A kept guess is then solved from to the same
+    tolerance and that solution is used; a rejected one costs nothing on the
+    critical path.  ``guess_stats`` holds the last test."""
 
     def __init__(self, nx, ny, dx, dy, mask, sdf, nu, U_inf, dt, *, dtype=torch.float32,
                  device="cuda", brinkman_eta=0.005, v_cycles=3, mg_variant="reference",
                  tolerance=1e-6, max_cycles=40, fast_mode=False, use_les=False,
-                 les_coupling="discard", smagorinsky_constant=0.1, warm_start=None):
+                 les_coupling="discard", smagorinsky_constant=0.1, warm_start=None,
+                 overlap_guess=False):
         A.require_gpu()
         self.nx, self.ny, self.dx, self.dy = nx, ny, float(dx), float(dy)
         self.nu, self.U_inf, self.dt = float(nu), float(U_inf), float(dt)
@@ -214,6 +223,16 @@ class CollocatedStepper:
         variant = "reference" if mg_variant == "reference" else "textbook"
         self.plan = get_plan(nx, ny, dx, dy, dtype, variant, 4 if variant == "reference" else 0,
                              dev)
+        self.overlap = bool(overlap_guess) and warm_start is not None and variant == "textbook"
+        self.guess_stats = None
+        if self.overlap:
+            from .pressure_solvers import MultigridPlan
+            # a private plan for the guess test: stream-ordered (no host sync)
+            self.guess_plan = MultigridPlan(nx, ny, dx, dy, dtype, "textbook", 0, dev)
+            self.guess_plan.set_option(adaptive=False)
+            self.side = torch.cuda.Stream(device=dev)  # default priority
+            self.p0, self.p_guess = e(), e()
+            self.ev_rhs, self.ev_guess = torch.cuda.Event(), torch.cuda.Event()
 
     def load(self, u, v):
         self.u[self.cur].copy_(A.as_device(u, self.u[0].dtype, self.device, "u"))
@@ -235,13 +254,34 @@ class CollocatedStepper:
                         U_inf=self.U_inf, nu_sgs=nsgs, fast_mode=self.fast_mode,
                         brinkman_eta=self.eta, out=(self.us, self.vs))
         pressure_rhs(self.us, self.vs, self.mask, self.dt, self.dx, self.dy, out=self.rhs)
-        if p0 is None and self.warm_start is not None:
-            p0 = self.warm_start(self.rhs, self.sdf)
-        if self.mg_variant == "reference":
-            self.plan.solve(self.rhs, self.p, p0, self.v_cycles, self.tolerance)
+        if p0 is None and self.overlap:
+            self._solve_overlapped()
         else:
-            self.plan.solve(self.rhs, self.p, p0, self.max_cycles, self.tolerance)
+            if p0 is None and self.warm_start is not None:
+                p0 = self.warm_start(self.rhs, self.sdf)
+            self._solve(p0)
         project(self.us, self.vs, self.p, self.mask, self.dt, self.dx, self.dy, self.U_inf,
                 out=(un, vn))
         self.cur = 1 - self.cur
         return un, vn
+
+    def _solve(self, p0):
+        if self.mg_variant == "reference":
+            self.plan.solve(self.rhs, self.p, p0, self.v_cycles, self.tolerance)
+        else:
+            self.plan.solve(self.rhs, self.p, p0, self.max_cycles, self.tolerance)
+
+    def _solve_overlapped(self):
+        main = torch.cuda.current_stream(self.device)
+        self.ev_rhs.record(main)
+        with torch.cuda.stream(self.side):
+            self.side.wait_event(self.ev_rhs)
+            self.warm_start.warm_start(self.rhs, self.sdf, self.p0)
+            # guess test only (no outer iteration): used_guess, guess / zero norms
+            self.guess_plan.solve(self.rhs, self.p_guess, self.p0, 0, self.tolerance)
+            self.ev_guess.record(self.side)
+        self._solve(None)  # zero guess, concurrently with the network
+        main.wait_event(self.ev_guess)
+        self.guess_stats = self.guess_plan.stats()
+        if self.guess_stats["used_guess"]:
+            self._solve(self.p0)

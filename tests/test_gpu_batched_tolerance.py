@@ -354,3 +354,48 @@ def test_nonfinite_inputs_are_flagged_per_instance(gpu):
         assert sb[k]["converged"] and not sb[k]["nonfinite"]
         assert torch.equal(out_bad[k], out_ok[k]) and sb[k]["norms"] == so[k]["norms"]
     pb.close()
+
+
+class _SolvedGuess:
+    """A warm start whose guess is the converged pressure of the rhs itself
+    (solved with a separate plan), so the guess test keeps it."""
+
+    def __init__(self, nx, ny, dx, dy):
+        import aerojax_amd as aj
+        self.plan = aj.MultigridPlan(nx, ny, dx, dy, torch.float32, "textbook", 0)
+
+    def warm_start(self, rhs, sdf, p0_out=None):
+        p0 = torch.empty_like(rhs) if p0_out is None else p0_out
+        self.plan.solve(rhs, p0, None, 60, 1e-7)
+        return p0
+
+    __call__ = warm_start
+
+
+@pytest.mark.parametrize("guess", ["network", "solved"])
+def test_overlapped_warm_start_equals_sequential(gpu, guess):
+    """The overlapped stepper (zero-guess solve concurrent with the guess and
+    its test on a second stream) gives the sequential stepper's bits, with a
+    rejected guess (untrained network) and with a kept one."""
+    import aerojax_amd as aj
+    from aerojax_amd.nn import WarmStartNet
+    nx, ny = 512, 128
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder", seed=7)
+    ws = WarmStartNet(base_channels=32, levels=4, seed=3) if guess == "network" else \
+        _SolvedGuess(nx, ny, c["dx"], c["dy"])
+    mk = lambda ov: aj.CollocatedStepper(nx, ny, c["dx"], c["dy"], c["mask"], c["sdf"], 1e-3,  # noqa
+                                         1.0, 1e-3, mg_variant="textbook", tolerance=1e-6,
+                                         max_cycles=60, warm_start=ws, overlap_guess=ov)
+    a, b = mk(False), mk(True)
+    a.load(c["u"], c["v"])
+    b.load(c["u"], c["v"])
+    for _ in range(3):
+        a.step()
+        b.step()
+        sa, sb = a.plan.stats(), b.plan.stats()
+        assert sa["converged"] and sb["converged"]
+        used = b.guess_stats["used_guess"]
+        assert used == (guess == "solved")
+    torch.cuda.synchronize()
+    assert torch.equal(a.state[0], b.state[0]) and torch.equal(a.state[1], b.state[1])
+    assert torch.equal(a.p, b.p)
