@@ -12,6 +12,7 @@
 // +-1, Adam.
 #include "aj_nn_kernels.h"
 
+#include <algorithm>
 #include <cstdio>
 
 #include <cmath>
@@ -63,6 +64,16 @@ struct aj_nn {
   // warm-start scratch (per fine-grid shape)
   void* ws_mem = nullptr;
   size_t ws_bytes = 0;
+  // captured warm starts, one per call signature (buffers, shape, scales):
+  // the first call with a signature runs eagerly (the launch tuner decides
+  // every shape), the second is captured, later ones replay the graph
+  struct WsGraph {
+    std::vector<uint64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<WsGraph> ws_graphs;
+  std::vector<std::vector<uint64_t>> ws_seen;
 };
 
 namespace {
@@ -764,6 +775,10 @@ extern "C" int aj_nn_create(aj_nn** out, int base_channels, int levels, int seed
 extern "C" int aj_nn_destroy(aj_nn* n) {
   if (!n) return AJ_OK;
   if (n->act.mem) (void)hipFree(n->act.mem);
+  for (auto& g : n->ws_graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
   if (n->ws_mem) (void)hipFree(n->ws_mem);
   if (n->wmem) (void)hipFree(n->wmem);
   delete n;
@@ -920,14 +935,9 @@ extern "C" int aj_nn_make_sample(aj_nn* n, const float* rhs, const float* sdf, c
 // B instances stored one after the other, each normalised by its own s.
 // Stacked instances restrict as one (B nx) x ny array: nx is divisible by
 // 2^r, so no 2x2 average straddles two instances.
-extern "C" int aj_nn_warm_start_batched(aj_nn* n, const float* rhs, const float* sdf, float* p0,
-                                        int B, int nx, int ny, int r, float sdf_ref, float pscale,
-                                        void* stream) {
-  if (!n || !rhs || !sdf || !p0 || r < 0 || B < 1 || B > 65535)
-    return fail(AJ_ERR_INVALID, "aj_nn_warm_start: bad args");
-  if ((nx >> r) << r != nx || (ny >> r) << r != ny)
-    return fail(AJ_ERR_INVALID, "aj_nn_warm_start: grid not divisible by 2^r");
-  hipStream_t st = (hipStream_t)stream;
+static int warm_start_launches(aj_nn* n, const float* rhs, const float* sdf, float* p0, int B,
+                               int nx, int ny, int r, float sdf_ref, float pscale,
+                               hipStream_t st) {
   const int H = nx >> r, W = ny >> r;
   const long ni = (long)H * W, nf = (long)B * nx * ny, nc = (long)B * ni;
   if (int rc = ensure_ws(n, nf, B)) return rc;
@@ -995,6 +1005,77 @@ extern "C" int aj_nn_warm_start_batched(aj_nn* n, const float* rhs, const float*
     cn = fn;
     cm = fm;
   }
+  return AJ_OK;
+}
+
+extern "C" int aj_nn_warm_start_batched(aj_nn* n, const float* rhs, const float* sdf, float* p0,
+                                        int B, int nx, int ny, int r, float sdf_ref, float pscale,
+                                        void* stream) {
+  if (!n || !rhs || !sdf || !p0 || r < 0 || B < 1 || B > 65535)
+    return fail(AJ_ERR_INVALID, "aj_nn_warm_start: bad args");
+  if ((nx >> r) << r != nx || (ny >> r) << r != ny)
+    return fail(AJ_ERR_INVALID, "aj_nn_warm_start: grid not divisible by 2^r");
+  hipStream_t st = (hipStream_t)stream;
+  // the buffers a replay uses are fixed once the scratch and activations
+  // exist for this shape; allocate them first (never during a capture)
+  const int H = nx >> r, W = ny >> r;
+  if (int rc = ensure_ws(n, (long)B * nx * ny, B)) return rc;
+  if (int rc = ensure_act(n, B, H, W)) return rc;
+  auto bits = [](float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (uint64_t)u;
+  };
+  const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)rhs, (uint64_t)(uintptr_t)sdf,
+                                     (uint64_t)(uintptr_t)p0, (uint64_t)B, (uint64_t)nx,
+                                     (uint64_t)ny, (uint64_t)r, bits(sdf_ref), bits(pscale),
+                                     (uint64_t)(uintptr_t)n->ws_mem,
+                                     (uint64_t)(uintptr_t)n->act.mem};
+  if (prof_on() || !(r >= 1 && r <= 3))
+    return warm_start_launches(n, rhs, sdf, p0, B, nx, ny, r, sdf_ref, pscale, st);
+  for (auto& g : n->ws_graphs)
+    if (g.key == key) {
+      AJ_HIP(hipGraphLaunch(g.exec, st));
+      return AJ_OK;
+    }
+  if (std::find(n->ws_seen.begin(), n->ws_seen.end(), key) == n->ws_seen.end()) {
+    // first call with this signature: eager (tunes the launch shapes)
+    if (n->ws_seen.size() >= 16) n->ws_seen.erase(n->ws_seen.begin());
+    n->ws_seen.push_back(key);
+    return warm_start_launches(n, rhs, sdf, p0, B, nx, ny, r, sdf_ref, pscale, st);
+  }
+  if (n->ws_graphs.size() >= 8) {  // keep the eight most recent signatures
+    auto& old = n->ws_graphs.front();
+    if (old.exec) (void)hipGraphExecDestroy(old.exec);
+    if (old.graph) (void)hipGraphDestroy(old.graph);
+    n->ws_graphs.erase(n->ws_graphs.begin());
+  }
+  aj_nn::WsGraph wg;
+  wg.key = key;
+  hipStream_t cs;
+  AJ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  int rc = AJ_OK;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = warm_start_launches(n, rhs, sdf, p0, B, nx, ny, r, sdf_ref, pscale, cs);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    if (e == hipSuccess && rc == AJ_OK) {
+      e = hipGraphInstantiate(&wg.exec, g, nullptr, nullptr, 0);
+      wg.graph = g;
+    } else if (g) {
+      (void)hipGraphDestroy(g);
+    }
+  }
+  (void)hipStreamDestroy(cs);
+  if (rc != AJ_OK || e != hipSuccess) {
+    if (wg.exec) (void)hipGraphExecDestroy(wg.exec);
+    if (wg.graph) (void)hipGraphDestroy(wg.graph);
+    if (rc != AJ_OK) return rc;
+    return hip_fail(e, "capture of the warm start");
+  }
+  n->ws_graphs.push_back(wg);
+  AJ_HIP(hipGraphLaunch(wg.exec, st));
   return AJ_OK;
 }
 

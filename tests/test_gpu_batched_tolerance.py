@@ -399,3 +399,30 @@ def test_overlapped_warm_start_equals_sequential(gpu, guess):
     torch.cuda.synchronize()
     assert torch.equal(a.state[0], b.state[0]) and torch.equal(a.state[1], b.state[1])
     assert torch.equal(a.p, b.p)
+
+
+def test_warm_start_graph_replays_match_eager(gpu):
+    """The warm start of a repeated call signature is captured once and
+    replayed: every replay (new inputs in the same buffers, eager work and
+    synchronisations in between) equals the eager launches (forced by the
+    live-timing mode, which bypasses graphs)."""
+    from aerojax_amd import _lib
+    from aerojax_amd.nn import WarmStartNet
+    nx, ny = 256, 64
+    net = WarmStartNet(base_channels=32, levels=4, seed=5)
+    rng = np.random.default_rng(12)
+    rhs = torch.empty((nx, ny), device="cuda")
+    sdf = dev(rng.normal(size=(nx, ny)), torch.float32)
+    p0 = torch.empty_like(rhs)
+    p1 = torch.empty_like(rhs)
+    for k in range(5):
+        rhs.copy_(dev(rng.normal(size=(nx, ny)) * 10 ** k, torch.float32))
+        got = net.warm_start(rhs, sdf, p0).clone()  # eager, captured, then replays
+        torch.cuda.synchronize()
+        _ = (rhs * 2).sum().item()  # unrelated eager work
+        _lib.profile_begin(1000)
+        ref = net.warm_start(rhs, sdf, p1)
+        torch.cuda.synchronize()
+        _lib.profile_end()
+        assert torch.equal(got, ref), k
+    net.close()

@@ -502,3 +502,34 @@ def test_autotuned_launches_are_bit_identical(gpu):
     _lib.load().aj_tune_enable(1)
     for a, bb in zip(res[0], res[1]):
         assert torch.equal(a, bb)
+
+
+def test_config_small_500_step_trajectory(gpu):
+    """The 512x96 airfoil configuration for its full 500 steps (reference
+    multigrid, 3 V-cycles): the FP64 GPU trajectory stays on the FP64 oracle's
+    (round-off level), the FP32 one is reported against it."""
+    import aerojax_amd as aj
+    from aerojax_amd import workloads
+    w = workloads.small_airfoil()
+    g = w["grid"]
+    u64, v64 = w["u"], w["v"]
+    steppers = {dt: aj.CollocatedStepper(g.nx, g.ny, g.dx, g.dy, w["mask"], w["sdf"], w["nu"],
+                                         1.0, w["dt"], dtype=dt, v_cycles=3)
+                for dt in (torch.float64, torch.float32)}
+    for s in steppers.values():
+        s.load(w["u"], w["v"])
+    errs = {torch.float64: [], torch.float32: []}
+    for k in range(500):
+        u64, v64, p64 = R.step_pure(u64, v64, w["mask"], w["sdf"], g.dx, g.dy, w["nu"], 1.0,
+                                    w["dt"], v_cycles=3)
+        for dt, s in steppers.items():
+            s.step()
+        if (k + 1) % 100 == 0:
+            for dt, s in steppers.items():
+                errs[dt].append(R.rel_l2(host(s.state[0]), u64))
+    print("C1 500-step u rel err FP64:", ["%.1e" % e for e in errs[torch.float64]],
+          "FP32:", ["%.1e" % e for e in errs[torch.float32]], flush=True)
+    assert max(errs[torch.float64]) < 1e-10
+    assert max(errs[torch.float32]) < 1e-3
+    for s in steppers.values():
+        assert torch.isfinite(s.state[0]).all()
