@@ -36,7 +36,7 @@ EXPORTS = (
     "aj_les_smagorinsky_total", "aj_les_dynamic_smagorinsky", "aj_divergence_rhs", "aj_divergence_periodic", "aj_project",
     "aj_project_collocated",
     "aj_flow_diagnostics", "aj_geometry", "aj_batch_geometry", "aj_tune_enable", "aj_tune_reset",
-    "aj_tune_report",
+    "aj_tune_report", "aj_tune_load",
     "aj_batch_create", "aj_batch_destroy", "aj_batch_cells",
     "aj_batch_num_instances", "aj_batch_offsets", "aj_batch_set_viscosity",
     "aj_batch_set_option", "aj_batch_step", "aj_batch_predict", "aj_batch_solve",
@@ -147,6 +147,7 @@ _SIGS = {
     "aj_tune_enable": (_i, [_i]),
     "aj_tune_reset": (_i, []),
     "aj_tune_report": (_i, [ctypes.c_char_p, _i]),
+    "aj_tune_load": (_i, [ctypes.c_char_p, ctypes.POINTER(_i)]),
     "aj_batch_create": (_i, [ctypes.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _i]),
     "aj_batch_destroy": (_i, [_vp]),
     "aj_batch_cells": (ctypes.c_long, [_vp]),
@@ -263,3 +264,11 @@ def tune_report() -> str:
     buf = ctypes.create_string_buffer(1 << 16)
     load().aj_tune_report(buf, len(buf))
     return buf.value.decode()
+
+
+def tune_load(text: str) -> int:
+    """Preset autotuner decisions from ``tune_report`` text; returns how many
+    were read."""
+    n = _i(0)
+    check(load().aj_tune_load(text.encode(), ctypes.byref(n)), "aj_tune_load")
+    return n.value

@@ -236,15 +236,22 @@ struct ConvSrc {
 
 // tile choice: 8x32-pixel patches where the layer has enough of them to
 // fill the chip, 4x16 otherwise; NT = 64 / 32 / 16 output channels
-template <int PH, int PW>
-static int conv_launch_t(const ConvArgs& c, int nt, hipStream_t st) {
+template <int PH, int PW, int SRC>
+static int conv_launch_src(const ConvArgs& c, int nt, hipStream_t st) {
   const long tiles = (long)c.B * ((c.H + PH - 1) / PH) * ((c.W + PW - 1) / PW);
   dim3 grid((unsigned)tiles, c.co / nt);
-  if (nt == 64) hipLaunchKernelGGL((k_conv3x3<PH, PW, 64>), grid, dim3(256), 0, st, c);
-  else if (nt == 32) hipLaunchKernelGGL((k_conv3x3<PH, PW, 32>), grid, dim3(256), 0, st, c);
-  else hipLaunchKernelGGL((k_conv3x3<PH, PW, 16>), grid, dim3(256), 0, st, c);
+  if (nt == 64) hipLaunchKernelGGL((k_conv3x3<PH, PW, 64, SRC>), grid, dim3(256), 0, st, c);
+  else if (nt == 32) hipLaunchKernelGGL((k_conv3x3<PH, PW, 32, SRC>), grid, dim3(256), 0, st, c);
+  else hipLaunchKernelGGL((k_conv3x3<PH, PW, 16, SRC>), grid, dim3(256), 0, st, c);
   AJ_LAUNCH_CHECK("k_conv3x3");
   return AJ_OK;
+}
+
+template <int PH, int PW>
+static int conv_launch_t(const ConvArgs& c, int nt, hipStream_t st) {
+  if (c.src == 1) return conv_launch_src<PH, PW, 1>(c, nt, st);
+  if (c.src == 2) return conv_launch_src<PH, PW, 2>(c, nt, st);
+  return conv_launch_src<PH, PW, 0>(c, nt, st);
 }
 
 // candidates: pixel tiles 8x32 and 4x16 with the widest output-channel

@@ -55,7 +55,9 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 // 32 bf16 + 8 pad (80 B): the 16 lanes of a quarter-wave then hit 16
 // distinct 4-bank groups, so the 16-byte fragment reads are conflict-free.
 // The next slice's global loads are issued before the current slice's
-// MFMAs, so their latency overlaps the arithmetic.
+// MFMAs, so their latency overlaps the arithmetic; all loads are
+// unconditional (padding is applied when staging), which keeps the
+// compiler's counter waits to the slice being staged.
 //
 // Input modes (inference, where the pooled / upsampled tensors are not
 // needed afterwards): src = 1 reads the input as the 2x2 average of `in`
@@ -84,7 +86,8 @@ struct ConvArgs {
 
 // 2x2 average of four bf16x8 rows, channel by channel (k_pool's order)
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-__device__ __forceinline__ uint4 pool4(uint4 a, uint4 b, uint4 c, uint4 d) {
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 pool4(u32x4 a, u32x4 b, u32x4 c, u32x4 d) {
   const u16x8 x00 = __builtin_bit_cast(u16x8, a), x01 = __builtin_bit_cast(u16x8, b);
   const u16x8 x10 = __builtin_bit_cast(u16x8, c), x11 = __builtin_bit_cast(u16x8, d);
   u16x8 o;
@@ -93,20 +96,23 @@ __device__ __forceinline__ uint4 pool4(uint4 a, uint4 b, uint4 c, uint4 d) {
     const float s = bf2f(x00[k]) + bf2f(x01[k]) + bf2f(x10[k]) + bf2f(x11[k]);
     o[k] = f2bf(0.25f * s);
   }
-  return __builtin_bit_cast(uint4, o);
+  return __builtin_bit_cast(u32x4, o);
 }
 
 constexpr int kLdsRow = 40;  // bf16 per LDS row (32 channels + pad)
 
-template <int PH, int PW, int NT>
+template <int PH, int PW, int NT, int SRC>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   static_assert(PH % 4 == 0 && PW % 16 == 0 && NT % 16 == 0, "conv tile");
+  static_assert(SRC >= 0 && SRC <= 2, "conv input mode");
   constexpr int RW = PW + 2, RP = (PH + 2) * RW;
   constexpr int WR = PH / 4;         // pixel rows per wave
   constexpr int SX = PW / 16;        // 16-pixel sub-tiles per row
   constexpr int MS = WR * SX, NS = NT / 16;
-  constexpr int NXL = (RP * 4 + 255) / 256;      // 16-byte input loads per thread and slice
+  constexpr int NXL = (RP * 4 + 255) / 256;      // 16-byte input elements per thread and slice
   constexpr int NWL = (9 * NT * 4 + 255) / 256;  // 16-byte weight loads per thread and slice
+  constexpr int NQ = SRC == 1 ? 4 : 1;           // loads per input element (pooling reads 2x2)
+  static_assert(NXL <= 32, "conv input patch");
   __shared__ __attribute__((aligned(16))) bf16_t sx[RP * kLdsRow];
   __shared__ __attribute__((aligned(16))) bf16_t sw[9 * NT * kLdsRow];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -125,58 +131,71 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   for (int m = 0; m < MS; ++m)
 #pragma unroll
     for (int j = 0; j < NS; ++j) acc[m][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const bf16_t* img = a.in + (long)b * a.H * a.W * a.ldi;
-  const bf16_t* img4 = a.in + (long)b * 4 * a.H * a.W * a.ldi;  // src = 1: 2H x 2W input
-  const bf16_t* img2 = a.in2 ? a.in2 + (long)b * (a.H >> 1) * (a.W >> 1) * a.ld2 : nullptr;
-  uint4 rx[NXL], rw[NWL];
-  // global -> registers: the input patch (one-pixel halo, zero outside the
-  // image) and the slice's weights, row (tap, co) holding 32 channels
-  auto fetch = [&](int c0) {
+  // Element offsets of this thread's loads, the same for every slice (the
+  // slice adds its first channel c0).  Elements outside the patch or the
+  // image load from offset 0 of the image (a valid address) and are zeroed
+  // when stashed, so every load is unconditional and the counter wait
+  // before a stash does not cover more than that slice's loads.
+  const bf16_t* img = SRC == 1 ? a.in + (long)b * 4 * a.H * a.W * a.ldi   // 2H x 2W input
+                               : a.in + (long)b * a.H * a.W * a.ldi;
+  const bf16_t* img2 = SRC == 2 ? a.in2 + (long)b * (a.H >> 1) * (a.W >> 1) * a.ld2 : nullptr;
+  int xo[NXL][NQ], xo2[NXL];
+  unsigned okx = 0;
+#pragma unroll
+  for (int k = 0; k < NXL; ++k) {
+    const int e = tid + k * 256;
+    const int px = e >> 2, q = e & 3;
+    const int rr = px / RW, cc = px - rr * RW;
+    const int hh = h0 + rr - 1, ww = w0 + cc - 1;
+    const bool ok = e < RP * 4 && hh >= 0 && hh < a.H && ww >= 0 && ww < a.W;
+    okx |= ok ? (1u << k) : 0u;
+    if (SRC == 1) {
+      const int W2 = 2 * a.W;
+      const int f = ok ? ((2 * hh) * W2 + 2 * ww) * a.ldi + q * 8 : 0;
+      xo[k][0] = f;
+      xo[k][NQ > 1 ? 1 : 0] = ok ? f + a.ldi : 0;
+      xo[k][NQ > 2 ? 2 : 0] = ok ? f + W2 * a.ldi : 0;
+      xo[k][NQ > 3 ? 3 : 0] = ok ? f + (W2 + 1) * a.ldi : 0;
+    } else {
+      xo[k][0] = ok ? (hh * a.W + ww) * a.ldi + q * 8 : 0;
+    }
+    xo2[k] = (SRC == 2 && ok) ? ((hh >> 1) * (a.W >> 1) + (ww >> 1)) * a.ld2 + q * 8 : 0;
+  }
+  int wo[NWL];
+#pragma unroll
+  for (int k = 0; k < NWL; ++k) {
+    const int e = tid + k * 256;
+    const int q = e & 3, r = e >> 2;
+    const int col = r / 9, tap = r - col * 9;
+    wo[k] = e < 9 * NT * 4 ? ((co0 + col) * 9 + tap) * a.ci + q * 8 : 0;
+  }
+  u32x4 rx[NXL][NQ], rw[NWL];
+  // global -> registers: slice at channel c0 (input patch with a one-pixel
+  // halo, and the slice's weights, row (tap, co) holding 32 channels)
+  auto fetch = [&](u32x4 (&X)[NXL][NQ], u32x4 (&Wt)[NWL], int c0) __attribute__((always_inline)) {
+    const bool up = SRC == 2 && c0 < a.cu;  // uniform: upsampled part of the concatenation
+#pragma unroll
+    for (int k = 0; k < NXL; ++k)
+#pragma unroll
+      for (int j = 0; j < NQ; ++j)
+        X[k][j] = *reinterpret_cast<const u32x4*>((up ? img2 + xo2[k] : img + xo[k][j]) + c0);
+#pragma unroll
+    for (int k = 0; k < NWL; ++k) Wt[k] = *reinterpret_cast<const u32x4*>(a.w + wo[k] + c0);
+    // keep the slice's loads together, in issue order
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  // registers -> LDS (pooling and zero padding applied here)
+  auto stash = [&](const u32x4 (&X)[NXL][NQ], const u32x4 (&Wt)[NWL]) __attribute__((always_inline)) {
 #pragma unroll
     for (int k = 0; k < NXL; ++k) {
       const int e = tid + k * 256;
-      uint4 v = make_uint4(0, 0, 0, 0);
       if (e < RP * 4) {
-        const int px = e >> 2, q = e & 3;
-        const int rr = px / RW, cc = px - rr * RW;
-        const int hh = h0 + rr - 1, ww = w0 + cc - 1;
-        if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W) {
-          if (a.src == 1) {
-            const int W2 = 2 * a.W;
-            const bf16_t* f = img4 + ((long)(2 * hh) * W2 + 2 * ww) * a.ldi + c0 + q * 8;
-            const uint4 x00 = *reinterpret_cast<const uint4*>(f);
-            const uint4 x01 = *reinterpret_cast<const uint4*>(f + a.ldi);
-            const uint4 x10 = *reinterpret_cast<const uint4*>(f + (long)W2 * a.ldi);
-            const uint4 x11 = *reinterpret_cast<const uint4*>(f + (long)(W2 + 1) * a.ldi);
-            v = pool4(x00, x01, x10, x11);
-          } else if (a.src == 2 && c0 < a.cu) {
-            v = *reinterpret_cast<const uint4*>(
-                img2 + ((long)(hh >> 1) * (a.W >> 1) + (ww >> 1)) * a.ld2 + c0 + q * 8);
-          } else {
-            v = *reinterpret_cast<const uint4*>(img + ((long)hh * a.W + ww) * a.ldi + c0 + q * 8);
-          }
-        }
+        u32x4 v = SRC == 1 ? pool4(X[k][0], X[k][NQ > 1 ? 1 : 0], X[k][NQ > 2 ? 2 : 0],
+                                   X[k][NQ > 3 ? 3 : 0])
+                           : X[k][0];
+        if (!((okx >> k) & 1u)) v = u32x4{0u, 0u, 0u, 0u};
+        *reinterpret_cast<u32x4*>(sx + (e >> 2) * kLdsRow + (e & 3) * 8) = v;
       }
-      rx[k] = v;
-    }
-#pragma unroll
-    for (int k = 0; k < NWL; ++k) {
-      const int e = tid + k * 256;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (e < 9 * NT * 4) {
-        const int q = e & 3, r = e >> 2;
-        const int col = r / 9, tap = r - col * 9;
-        v = *reinterpret_cast<const uint4*>(a.w + ((long)(co0 + col) * 9 + tap) * a.ci + c0 + q * 8);
-      }
-      rw[k] = v;
-    }
-  };
-  // registers -> LDS
-  auto stash = [&]() {
-#pragma unroll
-    for (int k = 0; k < NXL; ++k) {
-      const int e = tid + k * 256;
-      if (e < RP * 4) *reinterpret_cast<uint4*>(sx + (e >> 2) * kLdsRow + (e & 3) * 8) = rx[k];
     }
 #pragma unroll
     for (int k = 0; k < NWL; ++k) {
@@ -184,16 +203,12 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
       if (e < 9 * NT * 4) {
         const int q = e & 3, r = e >> 2;
         const int col = r / 9, tap = r - col * 9;
-        *reinterpret_cast<uint4*>(sw + (tap * NT + col) * kLdsRow + q * 8) = rw[k];
+        *reinterpret_cast<u32x4*>(sw + (tap * NT + col) * kLdsRow + q * 8) = Wt[k];
       }
     }
   };
-  fetch(0);
-  for (int s = 0; s < ns; ++s) {
-    if (s > 0) __syncthreads();  // every wave is done with the previous slice
-    stash();
-    __syncthreads();
-    if (s + 1 < ns) fetch((s + 1) * 32);
+  // one slice's MFMAs from LDS
+  auto compute = [&]() __attribute__((always_inline)) {
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
       const int dy = tap / 3, dx = tap - (tap / 3) * 3;
@@ -212,6 +227,16 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
           acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bf[j], acc[m][j], 0, 0, 0);
       }
     }
+  };
+  // the next slice (the last one again after the end) is loaded while the
+  // current one is multiplied
+  fetch(rx, rw, 0);
+  for (int s = 0; s < ns; ++s) {
+    __syncthreads();  // every wave is done with the previous slice
+    stash(rx, rw);
+    __syncthreads();
+    fetch(rx, rw, min(s + 1, ns - 1) * 32);
+    compute();
   }
 #pragma unroll
   for (int j = 0; j < NS; ++j) {

@@ -1,12 +1,15 @@
 // Launch-shape autotuner.  A kernel with several compiled tile shapes or
 // grid sizes asks tune_pick(key, n, launch) on its first launch for a given
 // problem (key = kernel + shape); every candidate is launched once to warm
-// up and then kTuneReps times between HIP events on the caller's stream,
-// and the fastest is kept for the rest of the process (the last timing
-// launch of the winner leaves the outputs in place: candidates are only
+// up, then the candidates take turns for kTuneRounds rounds of kTuneReps
+// launches between HIP events on the caller's stream, and the candidate
+// with the fastest round is kept for the rest of the process (the last
+// launch, of the winner, leaves the outputs in place: candidates are only
 // kernels whose outputs depend on nothing they write).  Decisions can be
-// read back (aj_tune_report) and tuning switched off (aj_tune_enable(0) or
-// AJ_TUNE=0: candidate 0 everywhere).  Never tunes inside stream capture.
+// read back (aj_tune_report), preset from such a report (aj_tune_load, so
+// that a profiled run launches the shapes a timed run picked) and tuning
+// switched off (aj_tune_enable(0) or AJ_TUNE=0: candidate 0 everywhere).
+// Never tunes inside stream capture.
 #include "aj_common.h"
 
 #include <algorithm>
@@ -21,6 +24,7 @@ namespace aj {
 
 namespace {
 constexpr int kTuneReps = 5;
+constexpr int kTuneRounds = 3;
 std::mutex g_tune_mu;
 std::map<std::string, std::pair<int, std::vector<float>>> g_tune;  // key -> (pick, ms per cand)
 int g_tune_on = -1;  // -1: read AJ_TUNE on first use
@@ -39,7 +43,7 @@ int tune_pick(const std::string& key, int ncand, const std::function<int(int, hi
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
+    if (it != g_tune.end() && it->second.first < ncand) {
       *pick = it->second.first;
       return launch(*pick, st);
     }
@@ -54,18 +58,27 @@ int tune_pick(const std::string& key, int ncand, const std::function<int(int, hi
   AJ_HIP(hipEventCreate(&a));
   AJ_HIP(hipEventCreate(&b));
   std::vector<float> ms(ncand, 1e30f);
+  std::vector<char> ok(ncand, 0);
   for (int c = 0; c < ncand; ++c) {
     if (int rc = launch(c, st)) {  // warm-up; a failing candidate is skipped
       (void)rc;
       (void)hipGetLastError();
       continue;
     }
-    AJ_HIP(hipEventRecord(a, st));
-    for (int r = 0; r < kTuneReps; ++r)
-      if (int rc = launch(c, st)) return rc;
-    AJ_HIP(hipEventRecord(b, st));
-    AJ_HIP(hipEventSynchronize(b));
-    AJ_HIP(hipEventElapsedTime(&ms[c], a, b));
+    ok[c] = 1;
+  }
+  for (int round = 0; round < kTuneRounds; ++round) {
+    for (int c = 0; c < ncand; ++c) {
+      if (!ok[c]) continue;
+      AJ_HIP(hipEventRecord(a, st));
+      for (int r = 0; r < kTuneReps; ++r)
+        if (int rc = launch(c, st)) return rc;
+      AJ_HIP(hipEventRecord(b, st));
+      AJ_HIP(hipEventSynchronize(b));
+      float t = 0.f;
+      AJ_HIP(hipEventElapsedTime(&t, a, b));
+      ms[c] = std::min(ms[c], t);
+    }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -91,6 +104,27 @@ extern "C" int aj_tune_enable(int enabled) {
 extern "C" int aj_tune_reset(void) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune.clear();
+  return AJ_OK;
+}
+
+extern "C" int aj_tune_load(const char* text, int* count) {
+  if (!text) return fail(AJ_ERR_INVALID, "aj_tune_load: no text");
+  if (count) *count = 0;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  int n = 0;
+  for (const char* line = text; *line;) {
+    const char* end = std::strchr(line, '\n');
+    const std::string l(line, end ? (size_t)(end - line) : std::strlen(line));
+    line = end ? end + 1 : line + l.size();
+    const size_t arrow = l.find(" -> ");
+    if (arrow == std::string::npos || arrow == 0) continue;
+    char* stop = nullptr;
+    const long pick = std::strtol(l.c_str() + arrow + 4, &stop, 10);
+    if (stop == l.c_str() + arrow + 4 || pick < 0 || pick > 64)
+      return fail(AJ_ERR_INVALID, "aj_tune_load: bad decision line");
+    g_tune[l.substr(0, arrow)] = {(int)pick, {}};
+    if (count) *count = ++n;
+  }
   return AJ_OK;
 }
 

@@ -312,6 +312,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 5 times (reference traffic for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--tune-file", default=None,
+                    help="autotuner decisions: preset from this file when it exists (a "
+                         "profiled run then launches the shapes a timed run picked), else "
+                         "written to it at the end of the run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -321,6 +325,20 @@ def main():
 
     import numpy as np
     import torch
+
+    tune_preset = bool(args.tune_file) and os.path.exists(args.tune_file)
+    if tune_preset:
+        from aerojax_amd import _lib
+        with open(args.tune_file) as fh:
+            log(f"autotuner: {_lib.tune_load(fh.read())} decisions preset from {args.tune_file}")
+    if args.tune_file and not tune_preset:
+        import atexit
+        from aerojax_amd import _lib
+
+        def _save_tune():
+            with open(args.tune_file, "w") as fh:
+                fh.write(_lib.tune_report())
+        atexit.register(_save_tune)
 
     if args.workload == "batched":
         torch.cuda.set_device(0)

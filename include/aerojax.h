@@ -219,10 +219,17 @@ int aj_mg_set_option(aj_mg_plan* plan, int option, int value);
  * time every candidate on their first launch for a problem shape and keep
  * the fastest for the process.  No counterpart in the reference.  Tuning is
  * on by default (environment AJ_TUNE=0 or aj_tune_enable(0): first
- * candidate everywhere); aj_tune_report writes "key -> pick [ms ...]" lines. */
+ * candidate everywhere); aj_tune_report writes "key -> pick [ms ...]" lines
+ * (ms = fastest round of 5 launches per candidate).  aj_tune_load presets
+ * decisions from such text (the "[ms ...]" part is optional; a pick beyond a
+ * kernel's candidate count is ignored and that kernel tunes as usual),
+ * stores the number of decisions read in *count (may be null) and fails with
+ * AJ_ERR_INVALID at a line with an arrow and no pick (the lines before it
+ * stay loaded). */
 int aj_tune_enable(int enabled);
 int aj_tune_reset(void);
 int aj_tune_report(char* buf, int len);
+int aj_tune_load(const char* text, int* count);
 
 /* ---- Flow diagnostics --------------------------------------------------
  * Replaces the viewer's and step_pure's debug residual/divergence metrics

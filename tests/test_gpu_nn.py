@@ -61,6 +61,77 @@ def test_forward_matches_oracle(net):
     assert rel(out, ref32) < 5e-2
 
 
+@pytest.mark.parametrize("B,H,W", [(2, 32, 16), (1, 128, 32)])
+def test_conv_tile_candidates_are_bit_identical(net, B, H, W):
+    """Every launch candidate of every convolution (pixel tile, output-channel
+    block) sums the same products in the same order: each
+    candidate, forced through preset tuner decisions, gives the forward
+    output of the first candidate bit for bit."""
+    from aerojax_amd import _lib
+    L = _lib.load()
+    x, _ = batch(B, H, W, 5)
+    xt = torch.from_numpy(x).cuda()
+    try:
+        L.aj_tune_enable(0)
+        L.aj_tune_reset()
+        base = net.forward(xt).clone()
+        L.aj_tune_enable(1)
+        L.aj_tune_reset()
+        net.forward(xt)
+        torch.cuda.synchronize()
+        lines = [l for l in _lib.tune_report().splitlines() if l.startswith("conv/")]
+        assert lines
+        ncand = {l.split(" -> ")[0]: len(l.split("[")[1].split()) - 1 for l in lines}
+        assert max(ncand.values()) == 4
+        for k in range(max(ncand.values())):
+            L.aj_tune_reset()
+            _lib.tune_load("\n".join(f"{key} -> {min(k, n - 1)}" for key, n in ncand.items()))
+            out = net.forward(xt)
+            torch.cuda.synchronize()
+            assert torch.equal(out, base), f"candidate {k}"
+    finally:
+        L.aj_tune_enable(1)
+        L.aj_tune_reset()
+
+
+def test_warm_start_conv_candidates_are_bit_identical(gpu):
+    """The same for the inference convolutions that read pooled and
+    upsampled+concatenated inputs (the warm-start path), each candidate in
+    a fresh network (its first warm start runs eagerly)."""
+    from aerojax_amd import _lib
+    from aerojax_amd.nn import WarmStartNet
+    L = _lib.load()
+    nx, ny = 512, 128
+    rng = np.random.default_rng(6)
+    rhs = torch.from_numpy(rng.normal(size=(nx, ny)).astype(np.float32)).cuda()
+    sdf = torch.from_numpy(rng.normal(size=(nx, ny)).astype(np.float32)).cuda()
+    def run():
+        n = WarmStartNet(base_channels=32, levels=4, seed=3)
+        try:
+            out = n.warm_start(rhs, sdf).clone()
+            torch.cuda.synchronize()
+            return out
+        finally:
+            n.close()
+    try:
+        L.aj_tune_enable(0)
+        L.aj_tune_reset()
+        base = run()
+        L.aj_tune_enable(1)
+        L.aj_tune_reset()
+        run()
+        lines = [l for l in _lib.tune_report().splitlines() if l.startswith("conv/")]
+        assert any("/pool" in l for l in lines) and any("/up" in l for l in lines)
+        ncand = {l.split(" -> ")[0]: len(l.split("[")[1].split()) - 1 for l in lines}
+        for k in range(max(ncand.values())):
+            L.aj_tune_reset()
+            _lib.tune_load("\n".join(f"{key} -> {min(k, n - 1)}" for key, n in ncand.items()))
+            assert torch.equal(run(), base), f"candidate {k}"
+    finally:
+        L.aj_tune_enable(1)
+        L.aj_tune_reset()
+
+
 def test_gradients_match_oracle(net):
     B, H, W = 2, 32, 16
     x, t = batch(B, H, W, 1)

@@ -483,9 +483,12 @@ def test_autotuned_launches_are_bit_identical(gpu):
     rng = np.random.default_rng(9)
     b = dev(rng.normal(size=(512, 128)) * 100, torch.float32)
     res = []
-    for on in (0, 1):
-        _lib.load().aj_tune_enable(on)
+    rep = ""
+    for on in (0, 1, 2):  # 2: tuning on, decisions preset from the previous pass
+        _lib.load().aj_tune_enable(1 if on else 0)
         _lib.load().aj_tune_reset()
+        if on == 2:
+            assert _lib.tune_load(rep) == len(rep.splitlines())
         us, vs = aj.rk_step_unified(dev(c["u"], torch.float32), dev(c["v"], torch.float32), 1e-3,
                                     5e-4, dx, dy, dev(c["mask"], torch.float32),
                                     sdf=dev(c["sdf"], torch.float32), brinkman_eta=0.005)
@@ -495,13 +498,17 @@ def test_autotuned_launches_are_bit_identical(gpu):
         torch.cuda.synchronize()
         res.append((us.clone(), vs.clone(), p.clone()))
         pl.close()
-        if on:
+        if on == 1:
             rep = _lib.tune_report()
             print(rep, flush=True)
             assert "rk/f32/rk3/512x128" in rep and "tb_pre/1x512x128" in rep
+        if on == 2:  # the preset picks are the ones launched, nothing re-tuned
+            picks = lambda t: sorted(l.split(" [")[0] for l in t.splitlines())  # noqa: E731
+            assert picks(_lib.tune_report()) == picks(rep)
     _lib.load().aj_tune_enable(1)
-    for a, bb in zip(res[0], res[1]):
-        assert torch.equal(a, bb)
+    for r in res[1:]:
+        for a, bb in zip(res[0], r):
+            assert torch.equal(a, bb)
 
 
 def test_config_small_500_step_trajectory(gpu):

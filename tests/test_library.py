@@ -54,6 +54,23 @@ def test_version_and_error_string_without_gpu(lib):
     assert b"null" in lib.aj_last_error()
 
 
+def test_tune_decisions_round_trip_without_gpu():
+    """aj_tune_load reads the lines aj_tune_report writes (host-only state),
+    ignores lines without an arrow and rejects an arrow without a pick."""
+    from aerojax_amd import _lib
+    L = _lib.load()
+    L.aj_tune_reset()
+    text = "tb_pre/1x2048x512 -> 2 [0.0100 0.0090 0.0080 0.0110 ms]\nrk/f32/rk3/2048x512 -> 1\nnoise\n"
+    assert _lib.tune_load(text) == 2
+    rep = _lib.tune_report().splitlines()
+    assert [l.split(" [")[0] for l in rep] == ["rk/f32/rk3/2048x512 -> 1", "tb_pre/1x2048x512 -> 2"]
+    assert _lib.tune_load("\n".join(rep)) == 2
+    with pytest.raises(_lib.AeroJaxError, match="bad decision line"):
+        _lib.tune_load("tb_post/1x8x8 -> x")
+    L.aj_tune_reset()
+    assert _lib.tune_report() == ""
+
+
 def test_struct_layouts_match_header():
     from aerojax_amd import _lib
     assert ctypes.sizeof(_lib.RkParams) == 7 * 8 + 2 * 4 + 0  # 64 bytes, no tail padding
