@@ -294,7 +294,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nn", action="store_true")
     ap.add_argument("--nn-pairs", type=int, default=20000)
@@ -476,6 +476,7 @@ def main():
         "config": {"workload": case["name"], "geometry": case["desc"], "Re": round(case["Re"], 1),
                    "grid": [nx, ny], "tolerance": case["tol"],
                    "arm": "nn_warm_start" if sps_nn is not None else "plain_mg",
+                   "cell_updates_per_s": value * nx * ny,
                    "plain_mg_steps_per_s": sps_plain, "plain_vcycles_per_step": cyc_plain,
                    "nn_over_plain": (sps_nn / sps_plain) if sps_nn else None, "nn": nn_info},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gbps"], "peak": 8000.0,
