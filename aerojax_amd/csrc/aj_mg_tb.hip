@@ -54,23 +54,6 @@ struct TbLevel {
   float inv_int;   // 1 / (2 axl + 2 ayl): inverse diagonal of interior rows
 };
 
-// Exact row solve of the level operator at (i, j) given a neighbour fetch.
-// Neighbour values outside the domain are never read.
-template <typename F>
-__device__ __forceinline__ float tb_relax(const TbLevel& L, int i, int j, float bv, F&& nb) {
-  const bool hasW = i > 0, hasS = j > 0, hasN = j < L.m - 1;
-  const bool last = (i == L.n - 1);
-  const float E = (!last) ? nb(1, 0) : 0.0f;
-  const float W = hasW ? nb(-1, 0) : 0.0f;
-  const float S = hasS ? nb(0, -1) : 0.0f;
-  const float N = hasN ? nb(0, 1) : 0.0f;
-  const float off = L.axl * (E + W) + L.ayl * (N + S);
-  if (hasW && hasS && hasN && !last) return (off - bv) * L.inv_int;
-  const float dE = (L.level > 0 && last) ? L.dE_last : L.axl;
-  const float diag = ((dE + (hasW ? L.axl : 0.0f)) + (hasS ? L.ayl : 0.0f)) + (hasN ? L.ayl : 0.0f);
-  return (off - bv) / diag;
-}
-
 // r = b - A p at (i, j) (level operator; level 0 outlet row -> 0)
 template <typename F>
 __device__ __forceinline__ float tb_resid(const TbLevel& L, int i, int j, float pc, float bv,
@@ -106,30 +89,64 @@ __device__ __forceinline__ float tb_prolong(const float* __restrict__ e, int nc,
 }
 
 // Four red-black half-sweeps on an LDS tile whose (0, 0) is global
-// (gx0, gy0); half-sweep k updates the cells of colour (k-1)&1 on the
-// region that extends MG0 - k cells beyond the tile.  Only cells of the
-// active colour are visited (W/2 per row).
+// (gx0, gy0), each over the cells of its colour within MG0 - 1 of the tile.
+// (A half-sweep spoils one more ring of the halo than the one before, so
+// after the four the cells within MG0 - 4 of the tile hold the exact
+// RB-GS iterate, the same as with regions shrinking by one per half-sweep.)
+// The region is the same for every half-sweep, so each thread's cells,
+// their LDS index, which neighbours take part and the diagonal are fixed
+// once for both colours.  The update is the exact solve of the cell's row
+// of the level operator: neighbours outside the domain are absent (the
+// level-0 outlet row is not updated, a coarse level's outlet ghost enters
+// the diagonal as dE_last).
 template <int TX, int TY, int H, int MG0, int NTH>
 __device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, const float* s_b,
                                                int gx0, int gy0) {
   constexpr int RW = TY + 2 * H;
+  constexpr int MG = MG0 - 1;                      // margin of every half-sweep
+  constexpr int W2 = (TY + 2 * MG) / 2, HH = TX + 2 * MG;
+  constexpr int NIT = (W2 * HH + NTH - 1) / NTH;   // cells per thread and colour
+  enum : int { kAct = 1, kE = 2, kW = 4, kN = 8, kS = 16, kInt = 32 };
   const int tid = threadIdx.x;
   const int active_rows = (L.level == 0) ? L.n - 1 : L.n;
+  int lidx[NIT][2], flg[NIT][2];
+  float dg[NIT][2];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = tid + it * NTH;
+    const int q = idx < W2 * HH ? idx : 0;
+    const int lx = H - MG + q / W2;
+    const int gx = gx0 + lx;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int ly0 = H - MG;
+      const int par = (c - gx - gy0 - ly0) & 1;
+      const int ly = ly0 + par + 2 * (q % W2);
+      const int gy = gy0 + ly;
+      const bool act = idx < W2 * HH && gx >= 0 && gx < active_rows && gy >= 0 && gy < L.m;
+      const bool hasW = gx > 0, hasS = gy > 0, hasN = gy < L.m - 1, last = gx == L.n - 1;
+      lidx[it][c] = lx * RW + ly;
+      flg[it][c] = (act ? kAct : 0) | (!last ? kE : 0) | (hasW ? kW : 0) | (hasN ? kN : 0) |
+                   (hasS ? kS : 0) | ((hasW && hasS && hasN && !last) ? kInt : 0);
+      const float dE = (L.level > 0 && last) ? L.dE_last : L.axl;
+      dg[it][c] = ((dE + (hasW ? L.axl : 0.0f)) + (hasS ? L.ayl : 0.0f)) + (hasN ? L.ayl : 0.0f);
+    }
+  }
 #pragma unroll
   for (int k = 1; k <= 4; ++k) {
-    const int mg = MG0 - k;
-    const int color = (k - 1) & 1;
-    const int W2 = (TY + 2 * mg) / 2, Hh = TX + 2 * mg;
-    for (int idx = tid; idx < W2 * Hh; idx += NTH) {
-      const int lx = H - mg + idx / W2;
-      const int gx = gx0 + lx;
-      const int ly0 = H - mg;
-      const int par = (color - gx - gy0 - ly0) & 1;
-      const int ly = ly0 + par + 2 * (idx % W2);
-      const int gy = gy0 + ly;
-      if (gx < 0 || gx >= active_rows || gy < 0 || gy >= L.m) continue;
-      const int l = lx * RW + ly;
-      s_p[l] = tb_relax(L, gx, gy, s_b[l], [&](int di, int dj) { return s_p[l + di * RW + dj]; });
+    const int c = (k - 1) & 1;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int f = flg[it][c];
+      if (!(f & kAct)) continue;
+      const int l = lidx[it][c];
+      const float E = (f & kE) ? s_p[l + RW] : 0.0f;
+      const float W = (f & kW) ? s_p[l - RW] : 0.0f;
+      const float S = (f & kS) ? s_p[l - 1] : 0.0f;
+      const float N = (f & kN) ? s_p[l + 1] : 0.0f;
+      const float off = L.axl * (E + W) + L.ayl * (N + S);
+      const float t = off - s_b[l];
+      s_p[l] = (f & kInt) ? t * L.inv_int : t / dg[it][c];
     }
     __syncthreads();
   }
