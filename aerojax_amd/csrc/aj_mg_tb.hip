@@ -88,13 +88,31 @@ __device__ __forceinline__ float tb_prolong(const float* __restrict__ e, int nc,
   return 0.5625f * own + 0.1875f * (xn + yn) + 0.0625f * dn;
 }
 
+// LDS layout of a smoother tile (RH x RW cells, RW even): the two colours
+// of the checkerboard live in separate planes, cell (lx, ly) of colour
+// c = (bp + lx + ly) & 1 at plane c, row lx, slot ly >> 1, where bp is the
+// parity of the tile origin.  A half-sweep then reads its four neighbours
+// (all in the other plane: slot ly >> 1 of rows lx -+ 1, slots
+// (ly -+ 1) >> 1 of row lx) and b and writes its cell at unit stride
+// across the lanes.
+template <int RH, int RW>
+struct TbTile {
+  static_assert(RW % 2 == 0, "tile width");
+  static constexpr int RW2 = RW / 2;
+  static constexpr int PS = ((RH * RW2 + 31) / 32) * 32 + 16;  // plane stride
+  static constexpr int N = 2 * PS;
+  __device__ static __forceinline__ int at(int bp, int lx, int ly) {
+    return ((bp + lx + ly) & 1) * PS + lx * RW2 + (ly >> 1);
+  }
+};
+
 // Four red-black half-sweeps on an LDS tile whose (0, 0) is global
 // (gx0, gy0), each over the cells of its colour within MG0 - 1 of the tile.
 // (A half-sweep spoils one more ring of the halo than the one before, so
 // after the four the cells within MG0 - 4 of the tile hold the exact
 // RB-GS iterate, the same as with regions shrinking by one per half-sweep.)
 // The region is the same for every half-sweep, so each thread's cells,
-// their LDS index, which neighbours take part and the diagonal are fixed
+// their LDS slots, which neighbours take part and the diagonal are fixed
 // once for both colours.  The update is the exact solve of the cell's row
 // of the level operator: neighbours outside the domain are absent (the
 // level-0 outlet row is not updated, a coarse level's outlet ghost enters
@@ -103,51 +121,67 @@ template <int TX, int TY, int H, int MG0, int NTH>
 __device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, const float* s_b,
                                                int gx0, int gy0) {
   constexpr int RW = TY + 2 * H;
+  using T = TbTile<TX + 2 * H, RW>;
   constexpr int MG = MG0 - 1;                      // margin of every half-sweep
   constexpr int W2 = (TY + 2 * MG) / 2, HH = TX + 2 * MG;
   constexpr int NIT = (W2 * HH + NTH - 1) / NTH;   // cells per thread and colour
-  enum : int { kAct = 1, kE = 2, kW = 4, kN = 8, kS = 16, kInt = 32 };
   const int tid = threadIdx.x;
+  const int bp = (gx0 + gy0) & 1;
   const int active_rows = (L.level == 0) ? L.n - 1 : L.n;
-  int lidx[NIT][2], flg[NIT][2];
+  // per cell: own slot, the other plane's slot of (lx, ly) and of
+  // (lx, ly + 1), all-ones masks of the neighbours that take part (E, W, S,
+  // N), whether it is updated and whether it is an interior cell
+  int own[NIT][2], oth[NIT][2], onx[NIT][2];
+  unsigned msk[NIT][2][4];
+  bool act[NIT][2], inr[NIT][2];
   float dg[NIT][2];
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
     const int idx = tid + it * NTH;
-    const int q = idx < W2 * HH ? idx : 0;
+    const int q = idx < W2 * HH ? idx : 0;  // idle lanes read a valid cell
     const int lx = H - MG + q / W2;
     const int gx = gx0 + lx;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const int ly0 = H - MG;
-      const int par = (c - gx - gy0 - ly0) & 1;
+      const int par = (c - bp - lx - ly0) & 1;
       const int ly = ly0 + par + 2 * (q % W2);
       const int gy = gy0 + ly;
-      const bool act = idx < W2 * HH && gx >= 0 && gx < active_rows && gy >= 0 && gy < L.m;
       const bool hasW = gx > 0, hasS = gy > 0, hasN = gy < L.m - 1, last = gx == L.n - 1;
-      lidx[it][c] = lx * RW + ly;
-      flg[it][c] = (act ? kAct : 0) | (!last ? kE : 0) | (hasW ? kW : 0) | (hasN ? kN : 0) |
-                   (hasS ? kS : 0) | ((hasW && hasS && hasN && !last) ? kInt : 0);
+      act[it][c] = idx < W2 * HH && gx >= 0 && gx < active_rows && gy >= 0 && gy < L.m;
+      inr[it][c] = hasW && hasS && hasN && !last;
+      own[it][c] = c * T::PS + lx * T::RW2 + (ly >> 1);
+      oth[it][c] = (1 - c) * T::PS + lx * T::RW2 + (ly >> 1);
+      onx[it][c] = oth[it][c] + (ly & 1);
+      msk[it][c][0] = last ? 0u : ~0u;
+      msk[it][c][1] = hasW ? ~0u : 0u;
+      msk[it][c][2] = hasS ? ~0u : 0u;
+      msk[it][c][3] = hasN ? ~0u : 0u;
       const float dE = (L.level > 0 && last) ? L.dE_last : L.axl;
       dg[it][c] = ((dE + (hasW ? L.axl : 0.0f)) + (hasS ? L.ayl : 0.0f)) + (hasN ? L.ayl : 0.0f);
     }
   }
+  // neighbour values are read unconditionally (every slot lies in the
+  // tile) and masked to 0 where the neighbour is outside the domain
+  auto nb = [](float v, unsigned m) { return __uint_as_float(__float_as_uint(v) & m); };
+  // a half-sweep writes only its colour's plane and reads only the other
+  // plane (and b), so every cell's reads are issued before any write
 #pragma unroll
   for (int k = 1; k <= 4; ++k) {
     const int c = (k - 1) & 1;
+    float t[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      const int f = flg[it][c];
-      if (!(f & kAct)) continue;
-      const int l = lidx[it][c];
-      const float E = (f & kE) ? s_p[l + RW] : 0.0f;
-      const float W = (f & kW) ? s_p[l - RW] : 0.0f;
-      const float S = (f & kS) ? s_p[l - 1] : 0.0f;
-      const float N = (f & kN) ? s_p[l + 1] : 0.0f;
-      const float off = L.axl * (E + W) + L.ayl * (N + S);
-      const float t = off - s_b[l];
-      s_p[l] = (f & kInt) ? t * L.inv_int : t / dg[it][c];
+      const int o = oth[it][c], on = onx[it][c];
+      const float E = nb(s_p[o + T::RW2], msk[it][c][0]);
+      const float W = nb(s_p[o - T::RW2], msk[it][c][1]);
+      const float S = nb(s_p[on - 1], msk[it][c][2]);
+      const float N = nb(s_p[on], msk[it][c][3]);
+      t[it] = (L.axl * (E + W) + L.ayl * (N + S)) - s_b[own[it][c]];
     }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it)
+      if (act[it][c]) s_p[own[it][c]] = inr[it][c] ? t[it] * L.inv_int : t[it] / dg[it][c];
     __syncthreads();
   }
 }
@@ -161,8 +195,9 @@ __global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restri
                                                 float* __restrict__ Y, float* __restrict__ Bc,
                                                 const aj_mg_stats* gate) {
   constexpr int H = 5, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
-  __shared__ float s_p[RN];
-  __shared__ float s_b[RN];
+  using T = TbTile<RH, RW>;
+  __shared__ float s_p[T::N];
+  __shared__ float s_b[T::N];
   __shared__ float s_r[TX * TY];
   const int n = L.n, m = L.m;
   {
@@ -175,6 +210,7 @@ __global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restri
   }
   const int gated = gate->converged;
   const int gx0 = blockIdx.y * TX - H, gy0 = blockIdx.x * TY - H;
+  const int bp = (gx0 + gy0) & 1;
   const int tid = threadIdx.x;
   {
     constexpr int NIT = (RN + NTH - 1) / NTH;
@@ -192,8 +228,10 @@ __global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restri
     for (int it = 0; it < NIT; ++it) {
       const int idx = tid + it * NTH;
       if (idx < RN) {
-        s_p[idx] = rp[it];
-        s_b[idx] = rb[it];
+        const int lx = idx / RW, ly = idx - lx * RW;
+        const int a = T::at(bp, lx, ly);
+        s_p[a] = rp[it];
+        s_b[a] = rb[it];
       }
     }
   }
@@ -205,10 +243,11 @@ __global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restri
     const int lx = H + tx, ly = H + ty;
     const int gx = gx0 + lx, gy = gy0 + ly;
     if (gx >= n || gy >= m) continue;
-    const int l = lx * RW + ly;
-    const float pc = s_p[l];
+    const int a = T::at(bp, lx, ly);
+    const float pc = s_p[a];
     Y[(long)gx * m + gy] = pc;
-    s_r[idx] = tb_resid(L, gx, gy, pc, s_b[l], [&](int di, int dj) { return s_p[l + di * RW + dj]; });
+    s_r[idx] = tb_resid(L, gx, gy, pc, s_b[a],
+                        [&](int di, int dj) { return s_p[T::at(bp, lx + di, ly + dj)]; });
   }
   __syncthreads();
   const int mc = m >> 1;
@@ -232,8 +271,9 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
                                                  float* __restrict__ E,
                                                  const aj_mg_stats* gate) {
   constexpr int H = 4, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
-  __shared__ float s_p[RN];
-  __shared__ float s_b[RN];
+  using T = TbTile<RH, RW>;
+  __shared__ float s_p[T::N];
+  __shared__ float s_b[T::N];
   const int n = L.n, m = L.m;
   {
     const long z = blockIdx.z;
@@ -245,6 +285,7 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
   }
   const int gated = gate->converged;
   const int gx0 = blockIdx.y * TX - H, gy0 = blockIdx.x * TY - H;
+  const int bp = (gx0 + gy0) & 1;
   const int tid = threadIdx.x;
   {
     constexpr int NIT = (RN + NTH - 1) / NTH;
@@ -265,8 +306,10 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
     for (int it = 0; it < NIT; ++it) {
       const int idx = tid + it * NTH;
       if (idx < RN) {
-        s_p[idx] = rp[it];
-        s_b[idx] = rb[it];
+        const int lx = idx / RW, ly = idx - lx * RW;
+        const int a = T::at(bp, lx, ly);
+        s_p[a] = rp[it];
+        s_b[a] = rb[it];
       }
     }
   }
@@ -276,7 +319,7 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
     const int lx = H + idx / TY, ly = H + idx % TY;
     const int gx = gx0 + lx, gy = gy0 + ly;
     if (gx >= n || gy >= m) continue;
-    E[(long)gx * m + gy] = s_p[lx * RW + ly];
+    E[(long)gx * m + gy] = s_p[T::at(bp, lx, ly)];
   }
 }
 
