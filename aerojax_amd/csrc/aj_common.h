@@ -61,7 +61,7 @@ int tune_pick(const std::string& key, int ncand,
 
 // ---- device helpers -----------------------------------------------------
 __device__ __forceinline__ int clampi(int x, int lo, int hi) {
-  return x < lo ? lo : (x > hi ? hi : x);
+  return max(lo, min(x, hi));
 }
 
 // Block-wide sum of a double (blockDim.x multiple of 64, <= 1024) with a

@@ -106,6 +106,17 @@ struct TbTile {
   }
 };
 
+// Thread layout of a pass over a tile region (RH x RW cells): each thread
+// keeps one column and steps over the rows RG at a time, so the row part of
+// an address is uniform per step and the column part is fixed (threads
+// past RG * RW idle).
+template <int RH, int RW, int NTH>
+struct TbPass {
+  static constexpr int RG = NTH / RW;              // rows per step
+  static constexpr int NST = (RH + RG - 1) / RG;   // steps
+  static_assert(RG >= 1, "tile wider than the workgroup");
+};
+
 // Four red-black half-sweeps on an LDS tile whose (0, 0) is global
 // (gx0, gy0), each over the cells of its colour within MG0 - 1 of the tile.
 // (A half-sweep spoils one more ring of the halo than the one before, so
@@ -194,7 +205,7 @@ __global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restri
                                                 const float* __restrict__ b,
                                                 float* __restrict__ Y, float* __restrict__ Bc,
                                                 const aj_mg_stats* gate) {
-  constexpr int H = 5, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
+  constexpr int H = 5, RH = TX + 2 * H, RW = TY + 2 * H;
   using T = TbTile<RH, RW>;
   __shared__ float s_p[T::N];
   __shared__ float s_b[T::N];
@@ -213,25 +224,25 @@ __global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restri
   const int bp = (gx0 + gy0) & 1;
   const int tid = threadIdx.x;
   {
-    constexpr int NIT = (RN + NTH - 1) / NTH;
-    float rp[NIT], rb[NIT];
+    using P = TbPass<RH, RW, NTH>;
+    const int col = tid % RW, rg = tid / RW;
+    const int gyc = clampi(gy0 + col, 0, m - 1);
+    float rp[P::NST], rb[P::NST];
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = min(tid + it * NTH, RN - 1);
-      const int lx = idx / RW, ly = idx - lx * RW;
-      const long g = (long)clampi(gx0 + lx, 0, n - 1) * m + clampi(gy0 + ly, 0, m - 1);
-      rp[it] = p_in ? p_in[g] : 0.0f;
-      rb[it] = b[g];
+    for (int st = 0; st < P::NST; ++st) {
+      const int lx = rg + st * P::RG;
+      const int g = clampi(gx0 + lx, 0, n - 1) * m + gyc;
+      rp[st] = p_in ? p_in[g] : 0.0f;
+      rb[st] = b[g];
     }
     if (gated) return;
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = tid + it * NTH;
-      if (idx < RN) {
-        const int lx = idx / RW, ly = idx - lx * RW;
-        const int a = T::at(bp, lx, ly);
-        s_p[a] = rp[it];
-        s_b[a] = rb[it];
+    for (int st = 0; st < P::NST; ++st) {
+      const int lx = rg + st * P::RG;
+      if (rg < P::RG && lx < RH) {
+        const int a = T::at(bp, lx, col);
+        s_p[a] = rp[st];
+        s_b[a] = rb[st];
       }
     }
   }
@@ -270,7 +281,7 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
                                                  float gamc, const float* __restrict__ b,
                                                  float* __restrict__ E,
                                                  const aj_mg_stats* gate) {
-  constexpr int H = 4, RH = TX + 2 * H, RW = TY + 2 * H, RN = RH * RW;
+  constexpr int H = 4, RH = TX + 2 * H, RW = TY + 2 * H;
   using T = TbTile<RH, RW>;
   __shared__ float s_p[T::N];
   __shared__ float s_b[T::N];
@@ -288,28 +299,28 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
   const int bp = (gx0 + gy0) & 1;
   const int tid = threadIdx.x;
   {
-    constexpr int NIT = (RN + NTH - 1) / NTH;
-    float rp[NIT], rb[NIT];
+    using P = TbPass<RH, RW, NTH>;
+    const int col = tid % RW, rg = tid / RW;
+    const int gy = clampi(gy0 + col, 0, m - 1);
+    float rp[P::NST], rb[P::NST];
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = min(tid + it * NTH, RN - 1);
-      const int lx = idx / RW, ly = idx - lx * RW;
-      const int gx = clampi(gx0 + lx, 0, n - 1), gy = clampi(gy0 + ly, 0, m - 1);
-      const long g = (long)gx * m + gy;
+    for (int st = 0; st < P::NST; ++st) {
+      const int lx = rg + st * P::RG;
+      const int gx = clampi(gx0 + lx, 0, n - 1);
+      const int g = gx * m + gy;
       float pv = Yin[g];
       if (!(L.level == 0 && gx == n - 1)) pv = pv + tb_prolong(Ec, nc, mc, gamc, gx, gy);
-      rp[it] = pv;
-      rb[it] = b[g];
+      rp[st] = pv;
+      rb[st] = b[g];
     }
     if (gated) return;
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = tid + it * NTH;
-      if (idx < RN) {
-        const int lx = idx / RW, ly = idx - lx * RW;
-        const int a = T::at(bp, lx, ly);
-        s_p[a] = rp[it];
-        s_b[a] = rb[it];
+    for (int st = 0; st < P::NST; ++st) {
+      const int lx = rg + st * P::RG;
+      if (rg < P::RG && lx < RH) {
+        const int a = T::at(bp, lx, col);
+        s_p[a] = rp[st];
+        s_b[a] = rb[st];
       }
     }
   }
