@@ -44,7 +44,7 @@ EXPORTS = (
     "aj_mg_create", "aj_mg_create_batched", "aj_mg_batch_count", "aj_mg_destroy",
     "aj_mg_workspace_bytes", "aj_mg_num_levels", "aj_mg_set_option", "aj_poisson_mg",
     "aj_mg_get_stats", "aj_mg_get_stats_batched",
-    "aj_profile_begin", "aj_profile_end", "aj_profile_read",
+    "aj_profile_begin", "aj_profile_set_repeat", "aj_profile_end", "aj_profile_read",
 )
 
 PROF_TAGS = ("rk", "rhs", "project", "mg_pre0", "mg_post0", "mg_pre", "mg_post", "mg_coarse",
@@ -173,6 +173,7 @@ _SIGS = {
     "aj_poisson_mg": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
     "aj_mg_get_stats": (_i, [_vp, ctypes.POINTER(MgStats), _vp]),
     "aj_profile_begin": (_i, [_i]),
+    "aj_profile_set_repeat": (_i, [_i]),
     "aj_profile_end": (_i, []),
     "aj_profile_read": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
 }
@@ -209,7 +210,10 @@ def check(rc: int, what: str):
         raise AeroJaxError(rc, f"{what}: {msg}")
 
 
-def profile_begin(max_records: int = 20000):
+def profile_begin(max_records: int = 20000, repeat: int = 1):
+    """Start live event timing; ``repeat`` launches each tunable kernel that
+    many times back to back inside its event pair (counted as launches)."""
+    check(load().aj_profile_set_repeat(int(repeat)), "aj_profile_set_repeat")
     check(load().aj_profile_begin(int(max_records)), "aj_profile_begin")
 
 

@@ -25,11 +25,14 @@ int hip_fail(hipError_t e, const char* what) {
 struct ProfSlot {
   int tag;
   hipEvent_t a, b;
-  double bytes;
+  double bytes;  // per launch
+  int launches;  // launches between the two events
 };
 static std::vector<ProfSlot> g_slots;
 static int g_used = 0;
 static bool g_on = false;
+static int g_repeat = 1;  // launches per tuned launch while profiling
+static int g_open = -1;   // slot of the instrumented launch in progress
 
 bool prof_on() { return g_on; }
 
@@ -38,6 +41,8 @@ int prof_begin_launch(int tag, hipStream_t st) {
   ProfSlot& s = g_slots[g_used];
   if (hipEventRecord(s.a, st) != hipSuccess) return -1;
   s.tag = tag;
+  s.launches = 1;
+  g_open = g_used;
   return g_used++;
 }
 
@@ -45,6 +50,13 @@ void prof_end_launch(int slot, double bytes, hipStream_t st) {
   ProfSlot& s = g_slots[slot];
   s.bytes = bytes;
   if (hipEventRecord(s.b, st) != hipSuccess) s.tag = -1;
+  g_open = -1;
+}
+
+int prof_repeat() { return (g_on && g_open >= 0) ? g_repeat : 1; }
+
+void prof_count(int launches) {
+  if (g_open >= 0) g_slots[g_open].launches = launches;
 }
 
 static void prof_release() {
@@ -76,6 +88,13 @@ extern "C" int aj_profile_begin(int max_records) {
 
 extern "C" int aj_profile_end(void) {
   aj::g_on = false;
+  aj::g_open = -1;
+  return AJ_OK;
+}
+
+extern "C" int aj_profile_set_repeat(int repeat) {
+  if (repeat < 1 || repeat > 64) return aj::fail(AJ_ERR_INVALID, "aj_profile_set_repeat: 1..64");
+  aj::g_repeat = repeat;
   return AJ_OK;
 }
 
@@ -91,9 +110,9 @@ extern "C" int aj_profile_read(int tag, int* count, double* total_ms, double* to
     AJ_HIP(hipEventSynchronize(s.b));
     float ms = 0.f;
     AJ_HIP(hipEventElapsedTime(&ms, s.a, s.b));
-    *count += 1;
+    *count += s.launches;
     *total_ms += ms;
-    *total_bytes += s.bytes;
+    *total_bytes += s.bytes * s.launches;
   }
   return AJ_OK;
 }

@@ -39,6 +39,10 @@ inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 bool prof_on();
 int prof_begin_launch(int tag, hipStream_t st);  // returns a slot or -1
 void prof_end_launch(int slot, double bytes, hipStream_t st);
+// launches a tuned kernel makes inside the open instrumented launch (the
+// repeat factor while profiling, else 1), and how many it made
+int prof_repeat();
+void prof_count(int launches);
 
 struct ProfScope {
   int slot;

@@ -42,9 +42,6 @@ static int launch_rk(const RkArgs<T>& a, hipStream_t st) {
   }
   dim3 grid(cdiv(a.ny, TY), cdiv(a.nx, TX));
   constexpr int NTH = 512;
-  // algorithmic bytes: u, v, chi source, mask (+ nu_sgs) read, u*, v* written
-  const double bytes = (double)a.nx * a.ny * sizeof(T) * (6 + (a.nu_sgs ? 1 : 0));
-  ProfScope prof(AJ_PROF_RK, bytes, st);
   hipLaunchKernelGGL((k_rk_fused<T, TX, TY, FAST, HYPER, NTH>), grid, dim3(NTH), smem, st, a);
   AJ_LAUNCH_CHECK("k_rk_fused");
   return AJ_OK;
@@ -100,6 +97,8 @@ static int rk_step_t(const void* u, const void* v, const void* mask, const void*
   char key[96];
   std::snprintf(key, sizeof(key), "rk/%s/%s%s/%dx%d", sizeof(T) == 8 ? "f64" : "f32",
                 fast ? "rk2" : "rk3", hyper ? "+hyper" : "", nx, ny);
+  // algorithmic bytes: u, v, chi source, mask (+ nu_sgs) read, u*, v* written
+  ProfScope prof(AJ_PROF_RK, (double)nx * ny * sizeof(T) * (6 + (a.nu_sgs ? 1 : 0)), st);
   int pick = 0;
   return tune_pick(key, ncand, launch, st, &pick);
 }

@@ -40,19 +40,29 @@ bool tune_enabled() {
 
 int tune_pick(const std::string& key, int ncand, const std::function<int(int, hipStream_t)>& launch,
               hipStream_t st, int* pick) {
+  // while profiling with a repeat factor, a tunable kernel (its outputs do
+  // not depend on what it writes) is launched that many times back to back
+  // inside the instrumented launch's events
+  auto run = [&](int c) -> int {
+    const int reps = ncand >= 2 ? prof_repeat() : 1;
+    for (int r = 0; r < reps; ++r)
+      if (int rc = launch(c, st)) return rc;
+    if (reps > 1) prof_count(reps);
+    return AJ_OK;
+  };
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end() && it->second.first < ncand) {
       *pick = it->second.first;
-      return launch(*pick, st);
+      return run(*pick);
     }
   }
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(st, &cap);
   if (!tune_enabled() || ncand < 2 || cap != hipStreamCaptureStatusNone || prof_on()) {
     *pick = 0;
-    return launch(0, st);
+    return run(0);
   }
   hipEvent_t a, b;
   AJ_HIP(hipEventCreate(&a));

@@ -366,7 +366,13 @@ int aj_batch_geometry(aj_batch* batch, const aj_obstacle* obstacles, void* sdf, 
  * textbook plan launches its kernels eagerly instead of replaying its
  * graph).  aj_profile_read sums, over the recorded launches with `tag`,
  * the event time and the algorithmic bytes (compulsory reads + writes) the
- * launch moves; it waits for those events. */
+ * launch moves; it waits for those events.  aj_profile_set_repeat(r)
+ * (1..64, default 1) makes every instrumented launch of a kernel with
+ * several launch shapes (RK predictor, textbook smoothers, batched grids,
+ * network convolutions: kernels whose outputs do not depend on what they
+ * write) run r times back to back inside its event pair and count r
+ * launches, so short kernels are timed without the per-event dispatch
+ * overhead. */
 enum {
   AJ_PROF_RK = 0,          /* fused RK predictor                        */
   AJ_PROF_RHS = 1,         /* divergence -> rhs                         */
@@ -384,6 +390,7 @@ enum {
   AJ_PROF_NUM_TAGS = 12
 };
 int aj_profile_begin(int max_records);
+int aj_profile_set_repeat(int repeat);
 int aj_profile_end(void);
 int aj_profile_read(int tag, int* count, double* total_ms, double* total_bytes);
 

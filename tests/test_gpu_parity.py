@@ -511,6 +511,44 @@ def test_autotuned_launches_are_bit_identical(gpu):
             assert torch.equal(a, bb)
 
 
+def test_profile_repeat_counts_and_keeps_results(gpu):
+    """With a profiling repeat factor the tunable launches run r times inside
+    their event pair and count r launches; the outputs (RK predictor, a
+    textbook solve) are bit-identical to an unprofiled run."""
+    import aerojax_amd as aj
+    from aerojax_amd import _lib
+    c = case(512, 128, 4.0, 1.0, kind="cylinder")
+    dx, dy = c["dx"], c["dy"]
+    rng = np.random.default_rng(11)
+    b = dev(rng.normal(size=(1024, 256)) * 100, torch.float32)  # two levels: pre0 runs
+    pl = aj.MultigridPlan(1024, 256, dx / 2, dy / 2, torch.float32, "textbook", 0)
+
+    def run():
+        us, vs = aj.rk_step_unified(dev(c["u"], torch.float32), dev(c["v"], torch.float32), 1e-3,
+                                    5e-4, dx, dy, dev(c["mask"], torch.float32),
+                                    sdf=dev(c["sdf"], torch.float32), brinkman_eta=0.005)
+        p = torch.empty_like(b)
+        pl.solve(b, p, None, 40, 1e-6)
+        torch.cuda.synchronize()
+        return us.clone(), vs.clone(), p.clone()
+
+    ref = run()
+    counts = {}
+    for r in (1, 4):
+        _lib.profile_begin(1000, r)
+        out = run()
+        _lib.profile_end()
+        counts[r] = _lib.profile_read()
+        for a, bb in zip(out, ref):
+            assert torch.equal(a, bb)
+    _lib.profile_begin(1, 1)
+    _lib.profile_end()
+    pl.close()
+    assert counts[4]["rk"]["count"] == 4 * counts[1]["rk"]["count"]
+    assert counts[4]["mg_pre0"]["count"] == 4 * counts[1]["mg_pre0"]["count"]
+    assert counts[4]["mg_resid"]["count"] == counts[1]["mg_resid"]["count"]  # not a tuned launch
+
+
 def test_config_small_500_step_trajectory(gpu):
     """The 512x96 airfoil configuration for its full 500 steps (reference
     multigrid, 3 V-cycles): the FP64 GPU trajectory stays on the FP64 oracle's
