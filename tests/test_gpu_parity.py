@@ -236,7 +236,8 @@ def test_reference_multigrid_large_coarse_level(gpu):
 
 
 @pytest.mark.parametrize("shape,levels", [((64, 16), 0), ((64, 16), 3), ((512, 96), 0),
-                                          ((512, 96), 4), ((2048, 512), 0), ((256, 64), 6)])
+                                          ((512, 96), 4), ((2048, 512), 0), ((256, 64), 6),
+                                          ((200, 74), 2), ((1000, 300), 0)])
 def test_textbook_multigrid_one_outer_iteration_parity(gpu, shape, levels):
     import aerojax_amd as aj
     nx, ny = shape
@@ -263,7 +264,7 @@ def test_textbook_multigrid_one_outer_iteration_parity(gpu, shape, levels):
         assert np.isclose(st["norms"][1], st_ref["norms"][1], rtol=2e-2)
 
 
-@pytest.mark.parametrize("shape", [(512, 96), (2048, 512)])
+@pytest.mark.parametrize("shape", [(512, 96), (2048, 512), (1000, 300)])
 def test_textbook_multigrid_to_tolerance(gpu, shape):
     import aerojax_amd as aj
     nx, ny = shape
