@@ -248,17 +248,22 @@ __global__ __launch_bounds__(NTH) void k_tb_pre(TbLevel L, const float* __restri
   }
   __syncthreads();
   tb_tile_sweeps<TX, TY, H, H, NTH>(L, s_p, s_b, gx0, gy0);
-  // residual on the tile, write Y
+  // residual on the tile, write Y (the four neighbours of a cell are in the
+  // other plane: slot ly >> 1 of rows lx -+ 1, slots (ly -+ 1) >> 1 of row lx)
   for (int idx = tid; idx < TX * TY; idx += NTH) {
     const int tx = idx / TY, ty = idx % TY;
     const int lx = H + tx, ly = H + ty;
     const int gx = gx0 + lx, gy = gy0 + ly;
     if (gx >= n || gy >= m) continue;
-    const int a = T::at(bp, lx, ly);
+    const int slot = lx * T::RW2 + (ly >> 1);
+    const bool odd = (bp + lx + ly) & 1;
+    const int a = odd ? slot + T::PS : slot, o = odd ? slot : slot + T::PS;
+    const int on = o + (ly & 1);
     const float pc = s_p[a];
     Y[(long)gx * m + gy] = pc;
-    s_r[idx] = tb_resid(L, gx, gy, pc, s_b[a],
-                        [&](int di, int dj) { return s_p[T::at(bp, lx + di, ly + dj)]; });
+    s_r[idx] = tb_resid(L, gx, gy, pc, s_b[a], [&](int di, int dj) {
+      return di > 0 ? s_p[o + T::RW2] : (di < 0 ? s_p[o - T::RW2] : (dj > 0 ? s_p[on] : s_p[on - 1]));
+    });
   }
   __syncthreads();
   const int mc = m >> 1;
