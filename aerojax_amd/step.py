@@ -16,11 +16,13 @@ RK step (the RK function's own contract, rk3_simple_new.py:304).
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import torch
 
 from . import _arrays as A
+from . import _lib
 from .advection_schemes import rk_step_unified
 from .operators import (constant_smagorinsky, dynamic_smagorinsky, pressure_rhs, project,
                         project_collocated, smagorinsky_total)
@@ -223,6 +225,15 @@ class CollocatedStepper:
         variant = "reference" if mg_variant == "reference" else "textbook"
         self.plan = get_plan(nx, ny, dx, dy, dtype, variant, 4 if variant == "reference" else 0,
                              dev)
+        # the per-step launches go straight to the library with arguments
+        # fixed here: the buffers are the stepper's own (shapes checked once),
+        # so the Python cost per step is one call per kernel group
+        self._lib = _lib.load()
+        self._dc = A.dtype_code(self.mask)
+        self._prm = _lib.RkParams(self.dt, self.nu, self.dx, self.dy, self.U_inf, self.eta, 0.0, 1,
+                                  int(bool(fast_mode)))
+        self._prm_ref = ctypes.byref(self._prm)
+        self._cycles = self.v_cycles if variant == "reference" else int(max_cycles)
         self.overlap = bool(overlap_guess) and warm_start is not None and variant == "textbook"
         self.guess_stats = None
         if self.overlap:
@@ -243,33 +254,46 @@ class CollocatedStepper:
         return self.u[self.cur], self.v[self.cur]
 
     def step(self, p0=None):
+        """One step (LES -> RK predictor -> rhs -> pressure -> projection) on
+        the current stream; the same launches as ``step_collocated``'s
+        functions, called without their per-call argument handling."""
+        with torch.cuda.device(self.device):
+            return self._step(p0)
+
+    def _step(self, p0):
+        L, dc, s = self._lib, self._dc, A.stream_handle(self.device)
+        nx, ny, dx, dy, dt = self.nx, self.ny, self.dx, self.dy, self.dt
         u, v = self.u[self.cur], self.v[self.cur]
         un, vn = self.u[1 - self.cur], self.v[1 - self.cur]
         nsgs = None
         if self.les:
-            delta = (self.dx * self.dy) ** 0.5
-            constant_smagorinsky(u, v, self.dx, self.dy, delta, self.cs, out=self.nsgs)
-            nsgs = self.nsgs
-        rk_step_unified(u, v, self.dt, self.nu, self.dx, self.dy, self.mask, sdf=self.sdf,
-                        U_inf=self.U_inf, nu_sgs=nsgs, fast_mode=self.fast_mode,
-                        brinkman_eta=self.eta, out=(self.us, self.vs))
-        pressure_rhs(self.us, self.vs, self.mask, self.dt, self.dx, self.dy, out=self.rhs)
+            nsgs = self.nsgs.data_ptr()
+            _lib.check(L.aj_les_smagorinsky(dc, u.data_ptr(), v.data_ptr(), nsgs, nx, ny, dx, dy,
+                                            (dx * dy) ** 0.5, self.cs, s), "aj_les_smagorinsky")
+        us, vs, mask = self.us.data_ptr(), self.vs.data_ptr(), self.mask.data_ptr()
+        _lib.check(L.aj_rk_step(dc, u.data_ptr(), v.data_ptr(), mask, A.ptr(self.sdf), nsgs,
+                                us, vs, nx, ny, self._prm_ref, s), "aj_rk_step")
+        _lib.check(L.aj_divergence_rhs(dc, us, vs, mask, self.rhs.data_ptr(), nx, ny, dx, dy, dt,
+                                       s), "aj_divergence_rhs")
         if p0 is None and self.overlap:
             self._solve_overlapped()
         else:
             if p0 is None and self.warm_start is not None:
                 p0 = self.warm_start(self.rhs, self.sdf)
             self._solve(p0)
-        project(self.us, self.vs, self.p, self.mask, self.dt, self.dx, self.dy, self.U_inf,
-                out=(un, vn))
+        _lib.check(L.aj_project(dc, us, vs, self.p.data_ptr(), mask, un.data_ptr(), vn.data_ptr(),
+                                nx, ny, dt, dx, dy, self.U_inf, s), "aj_project")
         self.cur = 1 - self.cur
         return un, vn
 
     def _solve(self, p0):
-        if self.mg_variant == "reference":
-            self.plan.solve(self.rhs, self.p, p0, self.v_cycles, self.tolerance)
-        else:
-            self.plan.solve(self.rhs, self.p, p0, self.max_cycles, self.tolerance)
+        if p0 is not None and (p0.dtype != self.rhs.dtype or tuple(p0.shape) != (self.nx, self.ny)
+                               or p0.device != self.device or not p0.is_contiguous()):
+            raise ValueError("p0 must be a contiguous tensor of the stepper's shape, dtype and device")
+        rc = self._lib.aj_poisson_mg(self.plan.handle, self.rhs.data_ptr(), A.ptr(p0),
+                                     self.p.data_ptr(), self._cycles, float(self.tolerance),
+                                     A.stream_handle(self.device))
+        _lib.check(rc, "aj_poisson_mg")
 
     def _solve_overlapped(self):
         main = torch.cuda.current_stream(self.device)

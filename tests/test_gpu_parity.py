@@ -333,6 +333,38 @@ def test_step_trajectory_error_growth(gpu):
     assert errs[-1] < 1e-4
 
 
+@pytest.mark.parametrize("variant,les", [("reference", False), ("textbook", True)])
+def test_stepper_equals_public_functions(gpu, variant, les):
+    """CollocatedStepper's direct library calls give the bits of the public
+    functions composed by hand (LES -> RK -> rhs -> multigrid -> projection)."""
+    import math
+    import aerojax_amd as aj
+    from aerojax_amd.operators import constant_smagorinsky, pressure_rhs, project
+    nx, ny = 256, 64
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder", noise=1e-3)
+    dx, dy, nu, dt = c["dx"], c["dy"], 5e-4, 1e-3
+    f = torch.float32
+    stp = aj.CollocatedStepper(nx, ny, dx, dy, c["mask"], c["sdf"], nu, 1.0, dt, mg_variant=variant,
+                               v_cycles=3, tolerance=1e-6, max_cycles=40, use_les=les,
+                               les_coupling="apply", smagorinsky_constant=0.17)
+    stp.load(c["u"], c["v"])
+    plan = aj.MultigridPlan(nx, ny, dx, dy, f, variant, 4 if variant == "reference" else 0)
+    mask, sdf = dev(c["mask"], f), dev(c["sdf"], f)
+    u, v = dev(c["u"], f), dev(c["v"], f)
+    for _ in range(3):
+        stp.step()
+        nsgs = constant_smagorinsky(u, v, dx, dy, math.sqrt(dx * dy), 0.17) if les else None
+        us, vs = aj.rk_step_unified(u, v, dt, nu, dx, dy, mask, sdf=sdf, U_inf=1.0, nu_sgs=nsgs,
+                                    brinkman_eta=0.005)
+        rhs = pressure_rhs(us, vs, mask, dt, dx, dy)
+        p = torch.empty_like(rhs)
+        plan.solve(rhs, p, None, 3 if variant == "reference" else 40, 1e-6)
+        u, v = project(us, vs, p, mask, dt, dx, dy, 1.0)
+        ug, vg = stp.state
+        assert torch.equal(ug, u) and torch.equal(vg, v) and torch.equal(stp.p, p)
+    plan.close()
+
+
 def test_step_textbook_reaches_tolerance(gpu):
     import aerojax_amd as aj
     nx, ny = 512, 128
