@@ -234,6 +234,11 @@ class CollocatedStepper:
                                   int(bool(fast_mode)))
         self._prm_ref = ctypes.byref(self._prm)
         self._cycles = self.v_cycles if variant == "reference" else int(max_cycles)
+        # a WarmStartNet writes its guess into a stepper buffer (other
+        # callables are called as warm_start(rhs, sdf) -> p0)
+        self._net = warm_start if (hasattr(warm_start, "pscale") and hasattr(warm_start, "handle")
+                                   and getattr(warm_start, "device", None) == dev) else None
+        self.p0 = e() if warm_start is not None else None
         self.overlap = bool(overlap_guess) and warm_start is not None and variant == "textbook"
         self.guess_stats = None
         if self.overlap:
@@ -242,7 +247,7 @@ class CollocatedStepper:
             self.guess_plan = MultigridPlan(nx, ny, dx, dy, dtype, "textbook", 0, dev)
             self.guess_plan.set_option(adaptive=False)
             self.side = torch.cuda.Stream(device=dev)  # default priority
-            self.p0, self.p_guess = e(), e()
+            self.p_guess = e()
             self.ev_rhs, self.ev_guess = torch.cuda.Event(), torch.cuda.Event()
 
     def load(self, u, v):
@@ -278,7 +283,13 @@ class CollocatedStepper:
         if p0 is None and self.overlap:
             self._solve_overlapped()
         else:
-            if p0 is None and self.warm_start is not None:
+            if p0 is None and self._net is not None:
+                net = self._net
+                _lib.check(L.aj_nn_warm_start(net.handle, self.rhs.data_ptr(), A.ptr(self.sdf),
+                                              self.p0.data_ptr(), nx, ny, net.r, net.sdf_ref,
+                                              net.pscale, s), "aj_nn_warm_start")
+                p0 = self.p0
+            elif p0 is None and self.warm_start is not None:
                 p0 = self.warm_start(self.rhs, self.sdf)
             self._solve(p0)
         _lib.check(L.aj_project(dc, us, vs, self.p.data_ptr(), mask, un.data_ptr(), vn.data_ptr(),
