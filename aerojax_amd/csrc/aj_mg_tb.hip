@@ -463,10 +463,20 @@ __global__ __launch_bounds__(256) void k_tb_resid_guess(const double* __restrict
 
 // keep the guess only if its residual is not larger than the zero guess's
 // (test = 0: keep any finite guess)
+// the fields the adaptive driver reads (converged, nonfinite, outer), also
+// written to its pinned host copy so that reading them needs no copy
+__device__ __forceinline__ void tb_publish(const aj_mg_stats* st, aj_mg_stats* hs) {
+  if (hs) {
+    hs->converged = st->converged;
+    hs->nonfinite = st->nonfinite;
+    hs->outer = st->outer;
+  }
+}
+
 __global__ void k_tb_guess_final(const double* __restrict__ part_g,
                                  const double* __restrict__ part_z, int np, double dxdy,
                                  int test, int rel, double* __restrict__ dtol, aj_mg_stats* st,
-                                 int* state) {
+                                 int* state, aj_mg_stats* hs) {
   __shared__ double sh[16];
   {
     const int z = blockIdx.x;  // one workgroup per instance
@@ -475,6 +485,7 @@ __global__ void k_tb_guess_final(const double* __restrict__ part_g,
     dtol += z;
     st += z;
     state += 2 * z;
+    if (hs) hs += z;
   }
   double ag = 0.0, az = 0.0;
 #pragma unroll 8
@@ -498,6 +509,7 @@ __global__ void k_tb_guess_final(const double* __restrict__ part_g,
     if (!isfinite(nrm)) st->nonfinite = 1;
     if (rel) dtol[0] *= nz;  // relative to ||b||
     st->converged = (nrm <= dtol[0]) ? 1 : 0;
+    tb_publish(st, hs);
   }
 }
 
@@ -522,7 +534,7 @@ __global__ __launch_bounds__(256) void k_tb_fallback(double* __restrict__ P,
 // to it); flip = 1: after an outer iteration
 __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, double dxdy,
                                 double* __restrict__ dtol, int flip, int cycles_added, int rel,
-                                aj_mg_stats* st, int* state) {
+                                aj_mg_stats* st, int* state, aj_mg_stats* hs) {
   __shared__ double sh[16];
   {
     const int z = blockIdx.x;  // one workgroup per instance
@@ -530,6 +542,7 @@ __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, doub
     dtol += z;
     st += z;
     state += 2 * z;
+    if (hs) hs += z;
   }
   const int skipped = st->converged;  // this outer iteration was skipped
   double acc = 0.0;
@@ -551,6 +564,7 @@ __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, doub
     if (!isfinite(nrm)) st->nonfinite = 1;
     if (rel) dtol[0] *= nrm;
     st->converged = (nrm <= dtol[0]) ? 1 : 0;
+    tb_publish(st, hs);
   }
 }
 
@@ -902,7 +916,8 @@ static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
                      (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_resid");
   hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
-                     p->dx * p->dy, p->dtol, 1, tb_cycles_per_outer(p), 0, p->stats, p->state);
+                     p->dx * p->dy, p->dtol, 1, tb_cycles_per_outer(p), 0, p->stats, p->state,
+                     p->host_stats);
   AJ_LAUNCH_CHECK("k_tb_norm_final");
   return AJ_OK;
 }
@@ -958,7 +973,7 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
     AJ_LAUNCH_CHECK("k_tb_resid_guess");
     hipLaunchKernelGGL(k_tb_guess_final, dim3(B), dim3(256), 0, st, (const double*)p->partial,
                        (const double*)p->partial2, nb, p->dx * p->dy, p->guess_test,
-                       p->relative, p->dtol, p->stats, p->state);
+                       p->relative, p->dtol, p->stats, p->state, p->host_stats);
     AJ_LAUNCH_CHECK("k_tb_guess_final");
     hipLaunchKernelGGL(k_tb_fallback<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st, p->p64[0],
                        (const T*)p->bcopy, p->tB[0], (const int*)p->state, nx, ny);
@@ -969,7 +984,8 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
                        p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
     AJ_LAUNCH_CHECK("k_tb_resid");
     hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
-                       p->dx * p->dy, p->dtol, 0, 0, p->relative, p->stats, p->state);
+                       p->dx * p->dy, p->dtol, 0, 0, p->relative, p->stats, p->state,
+                       p->host_stats);
     AJ_LAUNCH_CHECK("k_tb_norm_final");
   }
   return AJ_OK;
@@ -1027,13 +1043,10 @@ static int tb_solve_many_t(aj_mg_plan* const* plans, int count, const void* cons
   }
   for (;;) {
     bool any = false;
-    for (int q = 0; q < count; ++q)
-      if (open[q]) {
-        AJ_HIP(hipMemcpyAsync(plans[q]->host_stats, plans[q]->stats,
-                              sizeof(aj_mg_stats) * plans[q]->batch, hipMemcpyDeviceToHost, st));
-        any = true;
-      }
+    for (int q = 0; q < count; ++q) any = any || open[q];
     if (!any) break;
+    // the norm kernels wrote converged / nonfinite / outer of every
+    // instance into the pinned host copy
     AJ_HIP(hipStreamSynchronize(st));
     for (int q = 0; q < count; ++q) {
       if (!open[q]) continue;
