@@ -43,6 +43,7 @@ EXPORTS = (
     "aj_batch_project", "aj_batch_get_stats",
     "aj_mg_create", "aj_mg_create_batched", "aj_mg_batch_count", "aj_mg_destroy",
     "aj_mg_workspace_bytes", "aj_mg_num_levels", "aj_mg_set_option", "aj_poisson_mg",
+    "aj_poisson_mg_enqueue", "aj_poisson_mg_complete",
     "aj_mg_get_stats", "aj_mg_get_stats_batched",
     "aj_profile_begin", "aj_profile_set_repeat", "aj_profile_end", "aj_profile_read",
 )
@@ -171,6 +172,8 @@ _SIGS = {
     "aj_mg_num_levels": (_i, [_vp]),
     "aj_mg_set_option": (_i, [_vp, _i, _i]),
     "aj_poisson_mg": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
+    "aj_poisson_mg_enqueue": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
+    "aj_poisson_mg_complete": (_i, [_vp, ctypes.POINTER(_i)]),
     "aj_mg_get_stats": (_i, [_vp, ctypes.POINTER(MgStats), _vp]),
     "aj_profile_begin": (_i, [_i]),
     "aj_profile_set_repeat": (_i, [_i]),

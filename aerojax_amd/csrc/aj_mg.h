@@ -52,6 +52,11 @@ struct aj_mg_plan {
   void* graph_exec[2] = {nullptr, nullptr};  // hipGraphExec_t
   double* dtol = nullptr;           // textbook: tolerance of the current solve
   void* bcopy = nullptr;            // textbook: plan-owned copy of the rhs
+  // textbook: a solve enqueued without its convergence read (completed by
+  // tb_solve_complete): iterations enqueued, their bound, output, stream
+  int pend_active = 0, pend_done = 0, pend_max = 0;
+  void* pend_out = nullptr;
+  hipStream_t pend_st = nullptr;
 };
 
 namespace aj {
@@ -64,4 +69,12 @@ int tb_solve(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out, int ma
 int tb_solve_many(aj_mg_plan* const* plans, int count, const void* const* rhs,
                   const void* const* p0, void* const* p_out, int max_cycles, double tol,
                   hipStream_t st);
+// the same solve with its convergence read deferred: enqueue as many outer
+// iterations as the previous solve needed and the output; tb_solve_complete
+// then reads the device state (one synchronisation) and, if the solve has
+// not converged, enqueues further iterations and the output again, setting
+// *more
+int tb_solve_enqueue(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out, int max_cycles,
+                     double tol, hipStream_t st);
+int tb_solve_complete(aj_mg_plan* p, int* more);
 }  // namespace aj

@@ -367,6 +367,27 @@ extern "C" int aj_poisson_mg(aj_mg_plan* p, const void* rhs, const void* p0, voi
                            st);
 }
 
+extern "C" int aj_poisson_mg_enqueue(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out,
+                                     int max_cycles, double tol, void* stream) {
+  if (!p || !rhs || !p_out) return fail(AJ_ERR_INVALID, "aj_poisson_mg_enqueue: null pointer");
+  if (rhs == p_out || (p0 && p0 == p_out))
+    return fail(AJ_ERR_INVALID, "aj_poisson_mg_enqueue: p_out must not alias rhs or p0");
+  if (p->variant != AJ_MG_TEXTBOOK)
+    return aj_poisson_mg(p, rhs, p0, p_out, max_cycles, tol, stream);
+  if (!p->adaptive)  // a non-adaptive solve is fully enqueued already
+    return tb_solve(p, rhs, p0, p_out, max_cycles, tol, (hipStream_t)stream);
+  return tb_solve_enqueue(p, rhs, p0, p_out, max_cycles, tol, (hipStream_t)stream);
+}
+
+extern "C" int aj_poisson_mg_complete(aj_mg_plan* p, int* more) {
+  if (!p || !more) return fail(AJ_ERR_INVALID, "aj_poisson_mg_complete: null pointer");
+  if (p->variant != AJ_MG_TEXTBOOK) {
+    *more = 0;
+    return AJ_OK;
+  }
+  return tb_solve_complete(p, more);
+}
+
 extern "C" int aj_mg_get_stats(aj_mg_plan* p, aj_mg_stats* out, void* stream) {
   if (!p || !out) return fail(AJ_ERR_INVALID, "aj_mg_get_stats: null pointer");
   hipStream_t st = (hipStream_t)stream;

@@ -1069,6 +1069,9 @@ int tb_solve_many(aj_mg_plan* const* plans, int count, const void* const* rhs,
                   const void* const* p0, void* const* p_out, int max_cycles, double tol,
                   hipStream_t st) {
   if (count < 1) return AJ_OK;
+  for (int q = 0; q < count; ++q)
+    if (plans[q]->pend_active)
+      return fail(AJ_ERR_INVALID, "textbook multigrid: a deferred solve is open");
   for (int q = 1; q < count; ++q)
     if (plans[q]->dtype != plans[0]->dtype)
       return fail(AJ_ERR_INVALID, "textbook multigrid: plans of one solve need one dtype");
@@ -1077,8 +1080,59 @@ int tb_solve_many(aj_mg_plan* const* plans, int count, const void* const* rhs,
   return tb_solve_many_t<double>(plans, count, rhs, p0, p_out, max_cycles, tol, st);
 }
 
+template <typename T>
+static int tb_enqueue_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int max_cycles,
+                        double tol, hipStream_t st) {
+  const int cpo = tb_cycles_per_outer(p);
+  const int max_outer = (max_cycles + cpo - 1) / cpo;
+  if (int rc = tb_start<T>(p, rhs, p0, tol, st)) return rc;
+  const int n = std::min(std::max(p->last_outer, 1), max_outer);
+  for (int k = 0; k < n; ++k)
+    if (int rc = tb_outer_iteration<T>(p, k & 1, st)) return rc;
+  if (int rc = tb_finish<T>(p, p_out, st)) return rc;
+  p->pend_active = 1;
+  p->pend_done = n;
+  p->pend_max = max_outer;
+  p->pend_out = p_out;
+  p->pend_st = st;
+  return AJ_OK;
+}
+
+template <typename T>
+static int tb_complete_t(aj_mg_plan* p, int* more) {
+  hipStream_t st = p->pend_st;
+  for (;;) {
+    AJ_HIP(hipStreamSynchronize(st));  // the norm kernels published the state
+    if (tb_settled(p) || p->pend_done >= p->pend_max) break;
+    if (int rc = tb_outer_iteration<T>(p, p->pend_done & 1, st)) return rc;
+    ++p->pend_done;
+    if (int rc = tb_finish<T>(p, (T*)p->pend_out, st)) return rc;
+    *more = 1;
+  }
+  p->last_outer = tb_max_outer(p);
+  p->pend_active = 0;
+  return AJ_OK;
+}
+
+int tb_solve_enqueue(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out, int max_cycles,
+                     double tol, hipStream_t st) {
+  if (p->pend_active) return fail(AJ_ERR_INVALID, "textbook multigrid: a deferred solve is open");
+  if (p->dtype == AJ_F32)
+    return tb_enqueue_t<float>(p, (const float*)rhs, (const float*)p0, (float*)p_out, max_cycles,
+                               tol, st);
+  return tb_enqueue_t<double>(p, (const double*)rhs, (const double*)p0, (double*)p_out,
+                              max_cycles, tol, st);
+}
+
+int tb_solve_complete(aj_mg_plan* p, int* more) {
+  *more = 0;
+  if (!p->pend_active) return AJ_OK;
+  return p->dtype == AJ_F32 ? tb_complete_t<float>(p, more) : tb_complete_t<double>(p, more);
+}
+
 int tb_solve(aj_mg_plan* p, const void* rhs, const void* p0, void* p_out, int max_cycles,
              double tol, hipStream_t st) {
+  if (p->pend_active) return fail(AJ_ERR_INVALID, "textbook multigrid: a deferred solve is open");
   const void* r[1] = {rhs};
   const void* g[1] = {p0};
   void* o[1] = {p_out};

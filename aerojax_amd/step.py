@@ -200,7 +200,7 @@ class CollocatedStepper:
                  device="cuda", brinkman_eta=0.005, v_cycles=3, mg_variant="reference",
                  tolerance=1e-6, max_cycles=40, fast_mode=False, use_les=False,
                  les_coupling="discard", smagorinsky_constant=0.1, warm_start=None,
-                 overlap_guess=False):
+                 overlap_guess=False, deferred_check=False):
         A.require_gpu()
         self.nx, self.ny, self.dx, self.dy = nx, ny, float(dx), float(dy)
         self.nu, self.U_inf, self.dt = float(nu), float(U_inf), float(dt)
@@ -234,6 +234,12 @@ class CollocatedStepper:
                                   int(bool(fast_mode)))
         self._prm_ref = ctypes.byref(self._prm)
         self._cycles = self.v_cycles if variant == "reference" else int(max_cycles)
+        # deferred convergence read (textbook): a step enqueues its solve, output
+        # and projection without waiting; the next step (or finish / state)
+        # reads the convergence state and redoes the projection if the solve
+        # needed more iterations than the previous one
+        self.deferred = bool(deferred_check) and variant == "textbook"
+        self._pending = False
         # a WarmStartNet writes its guess into a stepper buffer (other
         # callables are called as warm_start(rhs, sdf) -> p0)
         self._net = warm_start if (hasattr(warm_start, "pscale") and hasattr(warm_start, "handle")
@@ -251,12 +257,31 @@ class CollocatedStepper:
             self.ev_rhs, self.ev_guess = torch.cuda.Event(), torch.cuda.Event()
 
     def load(self, u, v):
+        self.finish()
         self.u[self.cur].copy_(A.as_device(u, self.u[0].dtype, self.device, "u"))
         self.v[self.cur].copy_(A.as_device(v, self.u[0].dtype, self.device, "v"))
 
     @property
     def state(self):
+        self.finish()
         return self.u[self.cur], self.v[self.cur]
+
+    def finish(self):
+        """Complete a deferred solve: read its convergence state and, if it
+        needed more iterations, enqueue them and redo the projection."""
+        if not self._pending:
+            return
+        self._pending = False
+        more = ctypes.c_int(0)
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.aj_poisson_mg_complete(self.plan.handle, ctypes.byref(more)),
+                       "aj_poisson_mg_complete")
+            if more.value:
+                s = A.stream_handle(self.device)
+                _lib.check(self._lib.aj_project(
+                    self._dc, self.us.data_ptr(), self.vs.data_ptr(), self.p.data_ptr(),
+                    self.mask.data_ptr(), self.u[self.cur].data_ptr(), self.v[self.cur].data_ptr(),
+                    self.nx, self.ny, self.dt, self.dx, self.dy, self.U_inf, s), "aj_project")
 
     def step(self, p0=None):
         """One step (LES -> RK predictor -> rhs -> pressure -> projection) on
@@ -266,6 +291,7 @@ class CollocatedStepper:
             return self._step(p0)
 
     def _step(self, p0):
+        self.finish()
         L, dc, s = self._lib, self._dc, A.stream_handle(self.device)
         nx, ny, dx, dy, dt = self.nx, self.ny, self.dx, self.dy, self.dt
         u, v = self.u[self.cur], self.v[self.cur]
@@ -291,20 +317,21 @@ class CollocatedStepper:
                 p0 = self.p0
             elif p0 is None and self.warm_start is not None:
                 p0 = self.warm_start(self.rhs, self.sdf)
-            self._solve(p0)
+            self._solve(p0, deferred=self.deferred)
         _lib.check(L.aj_project(dc, us, vs, self.p.data_ptr(), mask, un.data_ptr(), vn.data_ptr(),
                                 nx, ny, dt, dx, dy, self.U_inf, s), "aj_project")
         self.cur = 1 - self.cur
         return un, vn
 
-    def _solve(self, p0):
+    def _solve(self, p0, deferred=False):
         if p0 is not None and (p0.dtype != self.rhs.dtype or tuple(p0.shape) != (self.nx, self.ny)
                                or p0.device != self.device or not p0.is_contiguous()):
             raise ValueError("p0 must be a contiguous tensor of the stepper's shape, dtype and device")
-        rc = self._lib.aj_poisson_mg(self.plan.handle, self.rhs.data_ptr(), A.ptr(p0),
-                                     self.p.data_ptr(), self._cycles, float(self.tolerance),
-                                     A.stream_handle(self.device))
+        fn = self._lib.aj_poisson_mg_enqueue if deferred else self._lib.aj_poisson_mg
+        rc = fn(self.plan.handle, self.rhs.data_ptr(), A.ptr(p0), self.p.data_ptr(), self._cycles,
+                float(self.tolerance), A.stream_handle(self.device))
         _lib.check(rc, "aj_poisson_mg")
+        self._pending = deferred
 
     def _solve_overlapped(self):
         main = torch.cuda.current_stream(self.device)

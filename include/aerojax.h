@@ -276,6 +276,19 @@ typedef struct aj_mg_stats {
  * number of FP32 V-cycles (2 per outer iteration). */
 int aj_poisson_mg(aj_mg_plan* plan, const void* rhs, const void* p0,
                   void* p_out, int max_cycles, double tolerance, void* stream);
+/* The same solve with its convergence read deferred (no counterpart in the
+ * reference; for callers that enqueue more work before they need the
+ * result, e.g. a time-stepping loop).  For an adaptive textbook plan
+ * aj_poisson_mg_enqueue enqueues the start, as many outer iterations as the
+ * plan's previous solve needed and the output, without waiting;
+ * aj_poisson_mg_complete then synchronises the solve's stream once and, if
+ * the tolerance was not reached, enqueues further iterations and the output
+ * again (*more = 1: work that consumed p_out must be redone) until it is or
+ * max_cycles is spent.  Until completed, the plan refuses other solves.
+ * Other plans solve in aj_poisson_mg_enqueue and complete at once. */
+int aj_poisson_mg_enqueue(aj_mg_plan* plan, const void* rhs, const void* p0,
+                          void* p_out, int max_cycles, double tolerance, void* stream);
+int aj_poisson_mg_complete(aj_mg_plan* plan, int* more);
 /* Copy the statistics of the last aj_poisson_mg call on this plan
  * (synchronises `stream`; instance 0 of a batched plan). */
 int aj_mg_get_stats(aj_mg_plan* plan, aj_mg_stats* out, void* stream);

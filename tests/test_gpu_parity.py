@@ -365,6 +365,54 @@ def test_stepper_equals_public_functions(gpu, variant, les):
     plan.close()
 
 
+def test_deferred_solve_tops_up_and_matches(gpu):
+    """aj_poisson_mg_enqueue + aj_poisson_mg_complete: a solve enqueued with
+    fewer iterations than it needs (the plan's previous solve needed one) is
+    topped up on completion (more = 1) and gives the bits of the blocking
+    solve; a stepper with the deferred convergence read equals the blocking
+    stepper step for step."""
+    import ctypes
+    import aerojax_amd as aj
+    from aerojax_amd import _lib
+    L = _lib.load()
+    nx, ny = 512, 128
+    dx, dy = 4.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(12)
+    b = dev(rng.normal(size=(nx, ny)) * 500, torch.float32)
+    pl = aj.MultigridPlan(nx, ny, dx, dy, torch.float32, "textbook", 0)
+    ref = aj.MultigridPlan(nx, ny, dx, dy, torch.float32, "textbook", 0)
+    p = torch.empty_like(b)
+    pl.solve(torch.zeros_like(b), p, None, 40, 1e-6)  # converged at once: one iteration next
+    p1, p2 = torch.empty_like(b), torch.empty_like(b)
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check(L.aj_poisson_mg_enqueue(pl.handle, b.data_ptr(), None, p1.data_ptr(), 40, 1e-6, s),
+               "enqueue")
+    with pytest.raises(_lib.AeroJaxError, match="deferred solve is open"):
+        pl.solve(b, p, None, 40, 1e-6)
+    more = ctypes.c_int(0)
+    _lib.check(L.aj_poisson_mg_complete(pl.handle, ctypes.byref(more)), "complete")
+    ref.solve(b, p2, None, 40, 1e-6)
+    torch.cuda.synchronize()
+    assert more.value == 1 and torch.equal(p1, p2)
+    s1, s2 = pl.stats(), ref.stats()
+    assert s1["converged"] and s1["cycles"] == s2["cycles"] and s1["norm"] == s2["norm"]
+    pl.close()
+    ref.close()
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder", noise=1e-3)
+    mk = lambda d: aj.CollocatedStepper(nx, ny, c["dx"], c["dy"], c["mask"], c["sdf"], 5e-4, 1.0,  # noqa
+                                        2.5e-4, mg_variant="textbook", tolerance=1e-6,
+                                        max_cycles=40, deferred_check=d)
+    a, d = mk(False), mk(True)
+    a.load(c["u"], c["v"])
+    d.load(c["u"], c["v"])
+    for _ in range(4):
+        a.step()
+        d.step()
+        ua, va = a.state
+        ud, vd = d.state
+        assert torch.equal(ua, ud) and torch.equal(va, vd) and torch.equal(a.p, d.p)
+
+
 def test_step_textbook_reaches_tolerance(gpu):
     import aerojax_amd as aj
     nx, ny = 512, 128
@@ -493,7 +541,7 @@ def test_textbook_relative_tolerance(gpu):
     dx, dy = 4.0 / nx, 1.0 / ny
     rng = np.random.default_rng(8)
     b = dev(rng.normal(size=(nx, ny)) * 300, torch.float32)
-    pl = aj.MultigridPlan(nx, ny, dx, dy, torch.float32, "textbook", 3)
+    pl = aj.MultigridPlan(nx, ny, dx, dy, torch.float32, "textbook", 0)
     pl.set_option(relative=True)
     p = torch.empty_like(b)
     for p0 in (None, torch.zeros_like(b)):
