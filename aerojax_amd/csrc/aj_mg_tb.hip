@@ -344,13 +344,26 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
 // ---------------------------------------------------------------------------
 // FP64 defect correction
 // ---------------------------------------------------------------------------
+// Row i and column j of the linear cell index c < 2^31 with ny columns,
+// from the FP64 product c * (1/ny) and one correction step (a 64-bit
+// integer division is a long instruction sequence on this target).
+__device__ __forceinline__ int tb_row(long c, int ny, double rny, int* j) {
+  const int ci = (int)c;
+  int i = (int)((double)ci * rny);
+  int r = ci - i * ny;
+  if (r < 0) { --i; r += ny; } else if (r >= ny) { ++i; r -= ny; }
+  *j = r;
+  return i;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, const T* __restrict__ rhs,
                                                  double* __restrict__ P, T* __restrict__ bcopy,
                                                  int nx, int ny) {
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= (long)nx * ny) return;
-  const int i = (int)(c / ny);
+  int j;
+  const int i = tb_row(c, ny, 1.0 / ny, &j);
   const long q = (long)blockIdx.z * nx * ny + c;
   P[q] = (p0 && i < nx - 1) ? (double)p0[q] : 0.0;
   bcopy[q] = rhs[q];
@@ -375,7 +388,8 @@ __global__ __launch_bounds__(256) void k_tb_resid(const double* __restrict__ src
                                                   const float* __restrict__ e,
                                                   const T* __restrict__ b, float* __restrict__ r32,
                                                   double* __restrict__ partial, int nx, int ny,
-                                                  double ax, double ay, const aj_mg_stats* gate) {
+                                                  double rny, double ax, double ay,
+                                                  const aj_mg_stats* gate) {
   __shared__ double sh[16];
   const long n = (long)nx * ny;
   {
@@ -393,7 +407,8 @@ __global__ __launch_bounds__(256) void k_tb_resid(const double* __restrict__ src
   const long c = blk * 256 + threadIdx.x;
   double r = 0.0, pc = 0.0;
   const bool in = c < n;
-  const int i = in ? (int)(c / ny) : 0, j = in ? (int)(c - (long)i * ny) : 0;
+  int j = 0;
+  const int i = in ? tb_row(c, ny, rny, &j) : 0;
   const bool eq = in && i < nx - 1;
   if (eq) {
     auto P = [&](long q) { return e ? src[q] + (double)e[q] : src[q]; };
@@ -436,7 +451,8 @@ __global__ __launch_bounds__(256) void k_tb_resid_guess(const double* __restrict
   double ag = 0.0, az = 0.0;
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c < n) {
-    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+    int j;
+    const int i = tb_row(c, ny, 1.0 / ny, &j);
     if (i == nx - 1) {
       r32[c] = 0.0f;
     } else {
@@ -524,7 +540,8 @@ __global__ __launch_bounds__(256) void k_tb_fallback(double* __restrict__ P,
   if (state[2 * z + 1] == 0) return;
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= (long)nx * ny) return;
-  const int i = (int)(c / ny);
+  int j;
+  const int i = tb_row(c, ny, 1.0 / ny, &j);
   const long q = z * nx * ny + c;
   P[q] = 0.0;
   r32[q] = (i < nx - 1) ? (float)b[q] : 0.0f;
@@ -912,7 +929,7 @@ static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
   ProfScope prof(AJ_PROF_MG_RESID, (double)nx * ny * B * (8 + 4 + sizeof(T) + 8 + 4), st);
   hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb, 1, B), dim3(256), 0, st,
                      (const double*)p->p64[parity], p->p64[parity ^ 1], (const float*)p->tE[0],
-                     (const T*)p->bcopy, p->tB[0], p->partial, nx, ny, ax, ay,
+                     (const T*)p->bcopy, p->tB[0], p->partial, nx, ny, 1.0 / ny, ax, ay,
                      (const aj_mg_stats*)p->stats);
   AJ_LAUNCH_CHECK("k_tb_resid");
   hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
@@ -981,7 +998,7 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
   } else {
     hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb, 1, B), dim3(256), 0, st, (const double*)p->p64[0],
                        p->p64[1], (const float*)nullptr, (const T*)p->bcopy, p->tB[0],
-                       p->partial, nx, ny, ax, ay, (const aj_mg_stats*)p->stats);
+                       p->partial, nx, ny, 1.0 / ny, ax, ay, (const aj_mg_stats*)p->stats);
     AJ_LAUNCH_CHECK("k_tb_resid");
     hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
                        p->dx * p->dy, p->dtol, 0, 0, p->relative, p->stats, p->state,
