@@ -57,6 +57,7 @@ struct aj_mg_plan {
   int pend_active = 0, pend_done = 0, pend_max = 0;
   void* pend_out = nullptr;
   hipStream_t pend_st = nullptr;
+  hipEvent_t pend_ev = nullptr;     // recorded after the enqueued output
 };
 
 namespace aj {

@@ -324,6 +324,10 @@ extern "C" int aj_mg_destroy(aj_mg_plan* p) {
     if (p->graph[k]) (void)hipGraphDestroy((hipGraph_t)p->graph[k]);
     p->graph_exec[k] = p->graph[k] = nullptr;
   }
+  if (p->pend_ev) {
+    (void)hipEventDestroy(p->pend_ev);
+    p->pend_ev = nullptr;
+  }
   if (p->host_stats) {
     (void)hipHostFree(p->host_stats);
     p->host_stats = nullptr;

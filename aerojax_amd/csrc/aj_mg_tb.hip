@@ -1107,6 +1107,10 @@ static int tb_enqueue_t(aj_mg_plan* p, const T* rhs, const T* p0, T* p_out, int 
   for (int k = 0; k < n; ++k)
     if (int rc = tb_outer_iteration<T>(p, k & 1, st)) return rc;
   if (int rc = tb_finish<T>(p, p_out, st)) return rc;
+  // the completion waits for this marker only, not for work the caller
+  // queues on the stream after the solve
+  if (!p->pend_ev) AJ_HIP(hipEventCreateWithFlags(&p->pend_ev, hipEventDisableTiming));
+  AJ_HIP(hipEventRecord(p->pend_ev, st));
   p->pend_active = 1;
   p->pend_done = n;
   p->pend_max = max_outer;
@@ -1119,11 +1123,12 @@ template <typename T>
 static int tb_complete_t(aj_mg_plan* p, int* more) {
   hipStream_t st = p->pend_st;
   for (;;) {
-    AJ_HIP(hipStreamSynchronize(st));  // the norm kernels published the state
+    AJ_HIP(hipEventSynchronize(p->pend_ev));  // the norm kernels published the state
     if (tb_settled(p) || p->pend_done >= p->pend_max) break;
     if (int rc = tb_outer_iteration<T>(p, p->pend_done & 1, st)) return rc;
     ++p->pend_done;
     if (int rc = tb_finish<T>(p, (T*)p->pend_out, st)) return rc;
+    AJ_HIP(hipEventRecord(p->pend_ev, st));
     *more = 1;
   }
   p->last_outer = tb_max_outer(p);

@@ -217,7 +217,7 @@ class CollocatedStepper:
         self.mask = A.as_device(mask, dtype, dev, "mask")
         self.sdf = A.as_device(sdf, dtype, dev, "sdf")
         e = lambda: torch.empty((nx, ny), dtype=dtype, device=dev)  # noqa: E731
-        self.us, self.vs, self.rhs, self.p = e(), e(), e(), e()
+        self.rhs, self.p = e(), e()
         self.nsgs = e() if self.les else None
         self.u = [e(), e()]
         self.v = [e(), e()]
@@ -240,6 +240,13 @@ class CollocatedStepper:
         # needed more iterations than the previous one
         self.deferred = bool(deferred_check) and variant == "textbook"
         self._pending = False
+        # predictor fields per slot: with the deferred read, a step's predictor
+        # (LES, RK, rhs, guess) is queued before the previous solve's state is
+        # read, into the other slot, so that a late top-up can still redo the
+        # previous projection from its own predictor fields
+        self._usv = [(e(), e())]
+        self._usv.append((e(), e()) if self.deferred else self._usv[0])
+        self.slot = 0
         # a WarmStartNet writes its guess into a stepper buffer (other
         # callables are called as warm_start(rhs, sdf) -> p0)
         self._net = warm_start if (hasattr(warm_start, "pscale") and hasattr(warm_start, "handle")
@@ -266,11 +273,20 @@ class CollocatedStepper:
         self.finish()
         return self.u[self.cur], self.v[self.cur]
 
-    def finish(self):
+    @property
+    def us(self):
+        """Predictor velocity u* of the last step."""
+        return self._usv[1 - self.slot][0]
+
+    @property
+    def vs(self):
+        return self._usv[1 - self.slot][1]
+
+    def finish(self) -> bool:
         """Complete a deferred solve: read its convergence state and, if it
-        needed more iterations, enqueue them and redo the projection."""
+        needed more iterations, enqueue them and redo the projection (True)."""
         if not self._pending:
-            return
+            return False
         self._pending = False
         more = ctypes.c_int(0)
         with torch.cuda.device(self.device):
@@ -278,10 +294,12 @@ class CollocatedStepper:
                        "aj_poisson_mg_complete")
             if more.value:
                 s = A.stream_handle(self.device)
+                us, vs = self._usv[1 - self.slot]
                 _lib.check(self._lib.aj_project(
-                    self._dc, self.us.data_ptr(), self.vs.data_ptr(), self.p.data_ptr(),
+                    self._dc, us.data_ptr(), vs.data_ptr(), self.p.data_ptr(),
                     self.mask.data_ptr(), self.u[self.cur].data_ptr(), self.v[self.cur].data_ptr(),
                     self.nx, self.ny, self.dt, self.dx, self.dy, self.U_inf, s), "aj_project")
+        return bool(more.value)
 
     def step(self, p0=None):
         """One step (LES -> RK predictor -> rhs -> pressure -> projection) on
@@ -291,36 +309,59 @@ class CollocatedStepper:
             return self._step(p0)
 
     def _step(self, p0):
-        self.finish()
+        if self.overlap or not self.deferred:
+            self.finish()
+            return self._correct(*self._predict(p0))
+        # queue this step's predictor before reading the previous solve's
+        # state, so that the GPU stays busy while the host waits; a top-up of
+        # the previous solve re-projects its velocity, and the predictor is
+        # then queued again from it
+        pred = self._predict(p0)
+        if self.finish():
+            pred = self._predict(p0)
+        return self._correct(*pred)
+
+    def _predict(self, p0):
+        """LES, RK predictor, rhs and the warm-start guess into this step's slot."""
         L, dc, s = self._lib, self._dc, A.stream_handle(self.device)
-        nx, ny, dx, dy, dt = self.nx, self.ny, self.dx, self.dy, self.dt
+        nx, ny, dx, dy = self.nx, self.ny, self.dx, self.dy
         u, v = self.u[self.cur], self.v[self.cur]
-        un, vn = self.u[1 - self.cur], self.v[1 - self.cur]
         nsgs = None
         if self.les:
             nsgs = self.nsgs.data_ptr()
             _lib.check(L.aj_les_smagorinsky(dc, u.data_ptr(), v.data_ptr(), nsgs, nx, ny, dx, dy,
                                             (dx * dy) ** 0.5, self.cs, s), "aj_les_smagorinsky")
-        us, vs, mask = self.us.data_ptr(), self.vs.data_ptr(), self.mask.data_ptr()
+        us, vs = self._usv[self.slot]
+        usp, vsp, mask = us.data_ptr(), vs.data_ptr(), self.mask.data_ptr()
         _lib.check(L.aj_rk_step(dc, u.data_ptr(), v.data_ptr(), mask, A.ptr(self.sdf), nsgs,
-                                us, vs, nx, ny, self._prm_ref, s), "aj_rk_step")
-        _lib.check(L.aj_divergence_rhs(dc, us, vs, mask, self.rhs.data_ptr(), nx, ny, dx, dy, dt,
-                                       s), "aj_divergence_rhs")
-        if p0 is None and self.overlap:
-            self._solve_overlapped()
-        else:
-            if p0 is None and self._net is not None:
+                                usp, vsp, nx, ny, self._prm_ref, s), "aj_rk_step")
+        _lib.check(L.aj_divergence_rhs(dc, usp, vsp, mask, self.rhs.data_ptr(), nx, ny, dx, dy,
+                                       self.dt, s), "aj_divergence_rhs")
+        if p0 is None and not self.overlap:
+            if self._net is not None:
                 net = self._net
                 _lib.check(L.aj_nn_warm_start(net.handle, self.rhs.data_ptr(), A.ptr(self.sdf),
                                               self.p0.data_ptr(), nx, ny, net.r, net.sdf_ref,
                                               net.pscale, s), "aj_nn_warm_start")
                 p0 = self.p0
-            elif p0 is None and self.warm_start is not None:
+            elif self.warm_start is not None:
                 p0 = self.warm_start(self.rhs, self.sdf)
+        return p0, (p0 is None and self.overlap)
+
+    def _correct(self, p0, overlapped):
+        """Pressure solve and projection of this step's slot into the next velocity."""
+        L, dc, s = self._lib, self._dc, A.stream_handle(self.device)
+        us, vs = self._usv[self.slot]
+        un, vn = self.u[1 - self.cur], self.v[1 - self.cur]
+        if overlapped:
+            self._solve_overlapped()
+        else:
             self._solve(p0, deferred=self.deferred)
-        _lib.check(L.aj_project(dc, us, vs, self.p.data_ptr(), mask, un.data_ptr(), vn.data_ptr(),
-                                nx, ny, dt, dx, dy, self.U_inf, s), "aj_project")
+        _lib.check(L.aj_project(dc, us.data_ptr(), vs.data_ptr(), self.p.data_ptr(),
+                                self.mask.data_ptr(), un.data_ptr(), vn.data_ptr(), self.nx,
+                                self.ny, self.dt, self.dx, self.dy, self.U_inf, s), "aj_project")
         self.cur = 1 - self.cur
+        self.slot = 1 - self.slot
         return un, vn
 
     def _solve(self, p0, deferred=False):
