@@ -808,14 +808,14 @@ static int tb_coarse(aj_mg_plan* p, hipStream_t st) {
   // b read, e written (the per-mode FP64 intermediates stay in L2)
   ProfScope prof(AJ_PROF_MG_COARSE, (double)c.n * c.m * 8 * p->batch, st);
   const unsigned nb = (unsigned)p->batch;
-  hipLaunchKernelGGL(k_tb_cfwd, dim3(cdiv(c.rows, kCT), cdiv(c.m, kCT), nb), dim3(256), 0, st, c,
+  hipLaunchKernelGGL(k_tb_cfwd, dim3(cdiv(c.rows, kCT), cdiv(c.m, kCT), nb), dim3(kCThreads), 0, st, c,
                      (const float*)p->tB[L], gate);
   AJ_LAUNCH_CHECK("k_tb_cfwd");
   const int nt = c.rows <= 128 ? 64 : (c.rows <= 256 ? 128 : (c.rows <= 512 ? 256 : 512));
   hipLaunchKernelGGL(k_tb_cpcr, dim3(c.m, 1, nb), dim3(nt), (size_t)2 * c.rows * sizeof(double), st,
                      c, gate);
   AJ_LAUNCH_CHECK("k_tb_cpcr");
-  hipLaunchKernelGGL(k_tb_cinv, dim3(cdiv(c.n, kCT), cdiv(c.m, kCT), nb), dim3(256), 0, st, c,
+  hipLaunchKernelGGL(k_tb_cinv, dim3(cdiv(c.n, kCT), cdiv(c.m, kCT), nb), dim3(kCThreads), 0, st, c,
                      p->tE[L], gate);
   AJ_LAUNCH_CHECK("k_tb_cinv");
   return AJ_OK;

@@ -8,7 +8,9 @@
 // Three kernels, each with one round trip to memory (every global load is
 // issued before the first wait; the convergence gate is read alongside):
 //   k_tb_cfwd : W[k][i] = sum_j C[k][j] b[i][j], FP64, tiles of 16 rows x
-//               16 modes with the whole contraction (<= 128) staged in LDS
+//               16 modes with the whole contraction (<= 128) staged in LDS;
+//               four waves per SIMD: each of 4 groups of 256 threads takes a
+//               quarter of the contraction, summed in a fixed order
 //   k_tb_cpcr : T_k X_k = W_k by parallel cyclic reduction, one workgroup
 //               per mode: with stride s = 1, 2, 4, ... row i eliminates its
 //               couplings to rows i -+ s,
@@ -16,7 +18,8 @@
 //               where alpha, gamma and the final 1/diagonal depend only on
 //               T_k (precomputed in FP64 at plan creation)
 //   k_tb_cinv : e[i][j] = sum_k C[k][j] X[k][i] in FP32 (an orthogonal
-//               transform: FP32 rounding stays at FP32 level)
+//               transform: FP32 rounding stays at FP32 level), split as
+//               k_tb_cfwd
 // Batched plans: instance blockIdx.z (W, X, b and e offset per instance).
 // Included by aj_mg_tb.hip.
 #pragma once
@@ -25,7 +28,10 @@ constexpr int kCT = 16;          // rows x modes (forward) / rows x columns (inv
 constexpr int kCJC = 128;        // contraction chunk staged in LDS
 constexpr int kCMaxSteps = 10;   // PCR steps: rows <= 2^10
 constexpr int kCMaxRows = 1024;  // two rows per thread, 512 threads
-constexpr int kCLoads = kCT * kCJC / 256;  // per-thread staged elements per operand
+constexpr int kCSplit = 4;       // contraction groups of 256 threads per tile
+constexpr int kCThreads = 256 * kCSplit;
+constexpr int kCLoads = kCT * kCJC / kCThreads;  // per-thread staged elements per operand
+constexpr int kCPart = kCJC / kCSplit;           // contraction length per group and chunk
 
 struct TbCoarse {
   int n, m, rows, steps;  // level shape, unknown rows, PCR steps
@@ -38,12 +44,13 @@ struct TbCoarse {
 };
 
 // W[k][i] = sum_j C[k][j] b[i][j]
-__global__ __launch_bounds__(256) void k_tb_cfwd(TbCoarse c, const float* __restrict__ b,
-                                                 const aj_mg_stats* gate) {
+__global__ __launch_bounds__(kCThreads) void k_tb_cfwd(TbCoarse c, const float* __restrict__ b,
+                                                       const aj_mg_stats* gate) {
   __shared__ float sb[kCT][kCJC + 1];
   __shared__ double sc[kCT][kCJC + 1];
+  __shared__ double sred[kCSplit - 1][256];
   const int i0 = blockIdx.x * kCT, k0 = blockIdx.y * kCT;
-  const int t = threadIdx.x, r = t % kCT, kk = t / kCT;
+  const int t = threadIdx.x, g = t >> 8, tt = t & 255, r = tt % kCT, kk = tt / kCT;
   {
     const long z = blockIdx.z;
     gate += z;
@@ -57,7 +64,7 @@ __global__ __launch_bounds__(256) void k_tb_cfwd(TbCoarse c, const float* __rest
     double vc[kCLoads];
 #pragma unroll
     for (int u = 0; u < kCLoads; ++u) {
-      const int e = t + u * 256, q = e / kCJC, jj = e - q * kCJC;
+      const int e = t + u * kCThreads, q = e / kCJC, jj = e - q * kCJC;
       const int i = i0 + q, k = k0 + q, j = j0 + jj;
       vb[u] = (i < c.rows && j < c.m) ? b[(long)i * c.m + j] : 0.0f;
       vc[u] = (k < c.m && j < c.m) ? c.C[(long)k * c.m + j] : 0.0;
@@ -66,14 +73,20 @@ __global__ __launch_bounds__(256) void k_tb_cfwd(TbCoarse c, const float* __rest
     if (j0 > 0) __syncthreads();
 #pragma unroll
     for (int u = 0; u < kCLoads; ++u) {
-      const int e = t + u * 256, q = e / kCJC, jj = e - q * kCJC;
+      const int e = t + u * kCThreads, q = e / kCJC, jj = e - q * kCJC;
       sb[q][jj] = vb[u];
       sc[q][jj] = vc[u];
     }
     __syncthreads();
+    const int jb = g * kCPart;
 #pragma unroll 8
-    for (int jj = 0; jj < kCJC; ++jj) acc += sc[kk][jj] * (double)sb[r][jj];
+    for (int jj = jb; jj < jb + kCPart; ++jj) acc += sc[kk][jj] * (double)sb[r][jj];
   }
+  if (g > 0) sred[g - 1][tt] = acc;
+  __syncthreads();
+  if (g > 0) return;
+#pragma unroll
+  for (int q = 0; q < kCSplit - 1; ++q) acc += sred[q][tt];
   const int i = i0 + r, k = k0 + kk;
   if (i < c.rows && k < c.m) c.W[(long)k * c.rows + i] = acc;
 }
@@ -134,12 +147,13 @@ __global__ __launch_bounds__(512) void k_tb_cpcr(TbCoarse c, const aj_mg_stats* 
 
 // e[i][j] = sum_k C[k][j] X[k][i] on the unknown rows, 0 on the level-0
 // Dirichlet row
-__global__ __launch_bounds__(256) void k_tb_cinv(TbCoarse c, float* __restrict__ e,
-                                                 const aj_mg_stats* gate) {
+__global__ __launch_bounds__(kCThreads) void k_tb_cinv(TbCoarse c, float* __restrict__ e,
+                                                       const aj_mg_stats* gate) {
   __shared__ float sx[kCJC][kCT + 1];
   __shared__ float sq[kCJC][kCT + 1];
+  __shared__ float sred[kCSplit - 1][256];
   const int i0 = blockIdx.x * kCT, j0 = blockIdx.y * kCT;
-  const int t = threadIdx.x, jj = t % kCT, r = t / kCT;
+  const int t = threadIdx.x, g = t >> 8, tt = t & 255, jj = tt % kCT, r = tt / kCT;
   {
     const long z = blockIdx.z;
     gate += z;
@@ -152,7 +166,7 @@ __global__ __launch_bounds__(256) void k_tb_cinv(TbCoarse c, float* __restrict__
     float vx[kCLoads], vq[kCLoads];
 #pragma unroll
     for (int u = 0; u < kCLoads; ++u) {
-      const int e2 = t + u * 256, q = e2 / kCT, rr = e2 - q * kCT;
+      const int e2 = t + u * kCThreads, q = e2 / kCT, rr = e2 - q * kCT;
       const int k = k0 + q, i = i0 + rr, j = j0 + rr;
       vx[u] = (k < c.m && i < c.rows) ? (float)c.X[(long)k * c.rows + i] : 0.0f;
       vq[u] = (k < c.m && j < c.m) ? c.Cf[(long)k * c.m + j] : 0.0f;
@@ -161,14 +175,20 @@ __global__ __launch_bounds__(256) void k_tb_cinv(TbCoarse c, float* __restrict__
     if (k0 > 0) __syncthreads();
 #pragma unroll
     for (int u = 0; u < kCLoads; ++u) {
-      const int e2 = t + u * 256, q = e2 / kCT, rr = e2 - q * kCT;
+      const int e2 = t + u * kCThreads, q = e2 / kCT, rr = e2 - q * kCT;
       sx[q][rr] = vx[u];
       sq[q][rr] = vq[u];
     }
     __syncthreads();
+    const int qb = g * kCPart;
 #pragma unroll 16
-    for (int q = 0; q < kCJC; ++q) acc += sq[q][jj] * sx[q][r];
+    for (int q = qb; q < qb + kCPart; ++q) acc += sq[q][jj] * sx[q][r];
   }
+  if (g > 0) sred[g - 1][tt] = acc;
+  __syncthreads();
+  if (g > 0) return;
+#pragma unroll
+  for (int q = 0; q < kCSplit - 1; ++q) acc += sred[q][tt];
   const int i = i0 + r, j = j0 + jj;
   if (i < c.n && j < c.m) e[(long)i * c.m + j] = i < c.rows ? acc : 0.0f;
 }
