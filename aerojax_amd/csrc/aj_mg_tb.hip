@@ -429,6 +429,91 @@ __global__ __launch_bounds__(256) void k_tb_resid(const double* __restrict__ src
   if (threadIdx.x == 0) partial[blk] = t;
 }
 
+// k_tb_resid with two cells per thread along y (ny even, so both cells share
+// a row): the same arithmetic per cell, vector loads and stores; one partial
+// sum per 512 cells.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tb_resid2(const double* __restrict__ src,
+                                                   double* __restrict__ dst,
+                                                   const float* __restrict__ e,
+                                                   const T* __restrict__ b, float* __restrict__ r32,
+                                                   double* __restrict__ partial, int nx, int ny,
+                                                   double rny, double ax, double ay,
+                                                   const aj_mg_stats* gate) {
+  __shared__ double sh[16];
+  const long n = (long)nx * ny;
+  {
+    const long z = blockIdx.z;
+    gate += z;
+    src += z * n;
+    if (dst) dst += z * n;
+    if (e) e += z * n;
+    b += z * n;
+    r32 += z * n;
+    partial += z * gridDim.x;
+  }
+  const int gated = gate->converged;
+  const long blk = xcd_block(blockIdx.x, gridDim.x);
+  const long c = (blk * 256 + threadIdx.x) * 2;
+  double r0 = 0.0, r1 = 0.0, p0 = 0.0, p1 = 0.0;
+  const bool in = c < n;
+  int j = 0;
+  const int i = in ? tb_row(c, ny, rny, &j) : 0;
+  if (in && i < nx - 1) {
+    auto P2 = [&](long q) {
+      double2 v = *reinterpret_cast<const double2*>(src + q);
+      if (e) {
+        const float2 f = *reinterpret_cast<const float2*>(e + q);
+        v.x += (double)f.x;
+        v.y += (double)f.y;
+      }
+      return v;
+    };
+    auto P = [&](long q) { return e ? src[q] + (double)e[q] : src[q]; };
+    const double2 C = P2(c);
+    p0 = C.x;
+    p1 = C.y;
+    const double2 W = i > 0 ? P2(c - ny) : C;
+    const double2 E = (i + 1 < nx - 1) ? P2(c + ny) : make_double2(0.0, 0.0);
+    const double S = j > 0 ? P(c - 1) : p0;
+    const double N = j + 2 < ny ? P(c + 2) : p1;
+    const double A0 = ax * ((E.x - p0) + (W.x - p0)) + ay * ((p1 - p0) + (S - p0));
+    const double A1 = ax * ((E.y - p1) + (W.y - p1)) + ay * ((N - p1) + (p0 - p1));
+    r0 = (double)b[c] - A0;
+    r1 = (double)b[c + 1] - A1;
+  }
+  if (gated) return;  // uniform across the grid
+  if (in) {
+    *reinterpret_cast<float2*>(r32 + c) = make_float2((float)r0, (float)r1);
+    if (e) *reinterpret_cast<double2*>(dst + c) = make_double2(p0, p1);
+  }
+  const double t = block_sum(r0 * r0 + r1 * r1, sh);
+  if (threadIdx.x == 0) partial[blk] = t;
+}
+
+// residual launch: two cells per thread when ny is even; returns the number
+// of partial sums per instance
+template <typename T>
+static int tb_resid_launch(const aj_mg_plan* p, const double* src, double* dst, const float* e,
+                           hipStream_t st) {
+  const int nx = p->nx, ny = p->ny;
+  const unsigned B = (unsigned)p->batch;
+  const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
+  const long n = (long)nx * ny;
+  if (ny % 2 == 0) {
+    const int nb = (int)cdiv(n, 512);
+    hipLaunchKernelGGL(k_tb_resid2<T>, dim3(nb, 1, B), dim3(256), 0, st, src, dst, e,
+                       (const T*)p->bcopy, p->tB[0], p->partial, nx, ny, 1.0 / ny, ax, ay,
+                       (const aj_mg_stats*)p->stats);
+    return nb;
+  }
+  const int nb = p->npartial;
+  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb, 1, B), dim3(256), 0, st, src, dst, e,
+                     (const T*)p->bcopy, p->tB[0], p->partial, nx, ny, 1.0 / ny, ax, ay,
+                     (const aj_mg_stats*)p->stats);
+  return nb;
+}
+
 // Initial guess test: partial sums of r(p0)^2 (r32 <- r(p0)) and of the
 // zero-guess residual b^2, rows < nx-1.
 template <typename T>
@@ -927,12 +1012,10 @@ static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
   if (int rc = tb_cycles(p, st)) return rc;
   // p64, e, b read; p64', r32 written
   ProfScope prof(AJ_PROF_MG_RESID, (double)nx * ny * B * (8 + 4 + sizeof(T) + 8 + 4), st);
-  hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb, 1, B), dim3(256), 0, st,
-                     (const double*)p->p64[parity], p->p64[parity ^ 1], (const float*)p->tE[0],
-                     (const T*)p->bcopy, p->tB[0], p->partial, nx, ny, 1.0 / ny, ax, ay,
-                     (const aj_mg_stats*)p->stats);
+  const int np = tb_resid_launch<T>(p, (const double*)p->p64[parity], p->p64[parity ^ 1],
+                                    (const float*)p->tE[0], st);
   AJ_LAUNCH_CHECK("k_tb_resid");
-  hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
+  hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, np,
                      p->dx * p->dy, p->dtol, 1, tb_cycles_per_outer(p), 0, p->stats, p->state,
                      p->host_stats);
   AJ_LAUNCH_CHECK("k_tb_norm_final");
@@ -996,11 +1079,9 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
                        (const T*)p->bcopy, p->tB[0], (const int*)p->state, nx, ny);
     AJ_LAUNCH_CHECK("k_tb_fallback");
   } else {
-    hipLaunchKernelGGL(k_tb_resid<T>, dim3(nb, 1, B), dim3(256), 0, st, (const double*)p->p64[0],
-                       p->p64[1], (const float*)nullptr, (const T*)p->bcopy, p->tB[0],
-                       p->partial, nx, ny, 1.0 / ny, ax, ay, (const aj_mg_stats*)p->stats);
+    const int np = tb_resid_launch<T>(p, (const double*)p->p64[0], p->p64[1], nullptr, st);
     AJ_LAUNCH_CHECK("k_tb_resid");
-    hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, nb,
+    hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, np,
                        p->dx * p->dy, p->dtol, 0, 0, p->relative, p->stats, p->state,
                        p->host_stats);
     AJ_LAUNCH_CHECK("k_tb_norm_final");
