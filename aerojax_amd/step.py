@@ -194,7 +194,16 @@ class CollocatedStepper:
     (the test is the plan's: the guess is kept only if its residual is not
     above the zero guess's).  A kept guess is then solved from to the same
     tolerance and that solution is used; a rejected one costs nothing on the
-    critical path.  ``guess_stats`` holds the last test."""
+    critical path.  ``guess_stats`` holds the last test.
+
+    ``deferred_check=True`` (textbook solver, no ``overlap_guess``): a step
+    queues its solve with the previous solve's iteration count, and the next
+    step queues its predictor (LES, RK, rhs, guess) into a second buffer slot
+    before it reads that solve's convergence state, so the host never drains
+    the stream.  A solve that needed more iterations is completed then, its
+    projection redone and the predictor queued again; results are those of
+    the blocking step.  ``state``, ``load`` and ``finish()`` complete an open
+    solve."""
 
     def __init__(self, nx, ny, dx, dy, mask, sdf, nu, U_inf, dt, *, dtype=torch.float32,
                  device="cuda", brinkman_eta=0.005, v_cycles=3, mg_variant="reference",
