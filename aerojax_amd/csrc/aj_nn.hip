@@ -572,6 +572,17 @@ __device__ __forceinline__ float nested_avg(const float* f, int m, int I, int J)
   }
 }
 
+// a / d for 0 <= a < 2^53 from the FP64 product with rd = 1/d and one
+// correction step (a 64-bit integer division is a long instruction sequence
+// on this target)
+__device__ __forceinline__ long ws_quot(long a, long d, double rd) {
+  long q = (long)((double)a * rd);
+  const long r = a - q * d;
+  if (r < 0) --q;
+  else if (r >= d) ++q;
+  return q;
+}
+
 // per coarse cell of instance blockIdx.y: restricted rhs R, tanh(sdf_r /
 // sdf_ref) T, and the block's sum of R^2 (fixed-order partials)
 template <int RR>
@@ -587,7 +598,7 @@ __global__ __launch_bounds__(256) void k_ws_prep(const float* __restrict__ rhs,
   const long b = blockIdx.y;
   double v = 0.0;
   if (e < ni) {
-    const int I = (int)(e / W), J = (int)(e - (long)I * W);
+    const int I = (int)ws_quot(e, W, 1.0 / W), J = (int)(e - (long)I * W);
     const float r = nested_avg<RR>(rhs + b * nx * ny, ny, I, J);
     const float sd = nested_avg<RR>(sdf + b * nx * ny, ny, I, J);
     Rout[b * ni + e] = r;
@@ -626,15 +637,31 @@ __global__ __launch_bounds__(256) void k_ws_input(const float* __restrict__ R,
   for (int c = 2; c < kCinPad; ++c) o[c] = 0;
 }
 
-// head (1x1 conv C -> 1 + bias, as k_head) times s pscale (as k_unscale)
+// head (1x1 conv C -> 1 + bias, as k_head) times s pscale (as k_unscale);
+// vec: rows of x 16-byte aligned and C a multiple of 8, read 8 channels per
+// load (the same channel order)
 __global__ __launch_bounds__(256) void k_ws_head(const bf16_t* x, int ldx, int C, const float* w,
                                                  const float* bias, const float* __restrict__ s,
-                                                 long ni, float pscale, float* out, long P) {
+                                                 long ni, float pscale, float* out, long P,
+                                                 int vec) {
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   if (p >= P) return;
   float v = bias[0];
-  for (int c = 0; c < C; ++c) v += bf2f(x[p * ldx + c]) * w[c];
-  out[p] = v * s[p / ni] * pscale;
+  if (vec) {
+    const u32x4* xv = reinterpret_cast<const u32x4*>(x + p * ldx);
+    for (int c8 = 0; c8 < C / 8; ++c8) {
+      const u32x4 q = xv[c8];
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const unsigned u = q[h];
+        v += bf2f((bf16_t)(u & 0xffffu)) * w[8 * c8 + 2 * h];
+        v += bf2f((bf16_t)(u >> 16)) * w[8 * c8 + 2 * h + 1];
+      }
+    }
+  } else {
+    for (int c = 0; c < C; ++c) v += bf2f(x[p * ldx + c]) * w[c];
+  }
+  out[p] = v * s[ws_quot(p, ni, 1.0 / (double)ni)] * pscale;
 }
 
 // value at (i, j) of the grid K prolongations finer than c (hc x wc), the
@@ -676,7 +703,7 @@ __global__ __launch_bounds__(256) void k_ws_prolong(const float* __restrict__ c,
   const long q = (long)blockIdx.x * 256 + threadIdx.x;
   if (q >= (long)nx * ny) return;
   const long b = blockIdx.y;
-  const int i = (int)(q / ny), j = (int)(q - (long)i * ny);
+  const int i = (int)ws_quot(q, ny, 1.0 / ny), j = (int)(q - (long)i * ny);
   p0[b * nx * ny + q] = up_val<RR>(c + b * H * W, H, W, RR, gam.g, i, j);
 }
 
@@ -970,7 +997,8 @@ static int warm_start_launches(aj_nn* n, const float* rhs, const float* sdf, flo
     if (int rc = forward(n, B, H, W, st, false, true)) return rc;
     hipLaunchKernelGGL(k_ws_head, dim3(blocks(nc)), dim3(256), 0, st, (const bf16_t*)n->act.DB[0],
                        n->c0, n->c0, (const float*)(n->params + n->head_w),
-                       (const float*)(n->params + n->head_b), (const float*)sv, ni, pscale, A, nc);
+                       (const float*)(n->params + n->head_b), (const float*)sv, ni, pscale, A, nc,
+                       (((uintptr_t)n->act.DB[0] % 16) == 0 && n->c0 % 8 == 0) ? 1 : 0);
     AJ_LAUNCH_CHECK("k_ws_head");
     Gam4 gam{};
     for (int l = 0; l <= r; ++l) gam.g[l] = (float)gamma_level(l);
