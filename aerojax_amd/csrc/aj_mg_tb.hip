@@ -1007,8 +1007,6 @@ static int tb_cycles_per_outer(const aj_mg_plan* p) { return p->ml == 0 ? 1 : kC
 template <typename T>
 static int tb_outer_launches(aj_mg_plan* p, int parity, hipStream_t st) {
   const int nx = p->nx, ny = p->ny, B = p->batch;
-  const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
-  const int nb = p->npartial;
   if (int rc = tb_cycles(p, st)) return rc;
   // p64, e, b read; p64', r32 written
   ProfScope prof(AJ_PROF_MG_RESID, (double)nx * ny * B * (8 + 4 + sizeof(T) + 8 + 4), st);
