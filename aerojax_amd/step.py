@@ -232,8 +232,16 @@ class CollocatedStepper:
         self.v = [e(), e()]
         self.cur = 0
         variant = "reference" if mg_variant == "reference" else "textbook"
-        self.plan = get_plan(nx, ny, dx, dy, dtype, variant, 4 if variant == "reference" else 0,
-                             dev)
+        deferred = bool(deferred_check) and variant == "textbook"
+        if deferred:
+            # a deferred solve stays open on the plan from one step to the next,
+            # so such a stepper owns its plan (the shared plans serve blocking
+            # callers, and a plan refuses a solve while one is open)
+            from .pressure_solvers import MultigridPlan
+            self.plan = MultigridPlan(nx, ny, dx, dy, dtype, variant, 0, dev)
+        else:
+            self.plan = get_plan(nx, ny, dx, dy, dtype, variant,
+                                 4 if variant == "reference" else 0, dev)
         # the per-step launches go straight to the library with arguments
         # fixed here: the buffers are the stepper's own (shapes checked once),
         # so the Python cost per step is one call per kernel group
@@ -247,7 +255,7 @@ class CollocatedStepper:
         # and projection without waiting; the next step (or finish / state)
         # reads the convergence state and redoes the projection if the solve
         # needed more iterations than the previous one
-        self.deferred = bool(deferred_check) and variant == "textbook"
+        self.deferred = deferred
         self._pending = False
         # predictor fields per slot: with the deferred read, a step's predictor
         # (LES, RK, rhs, guess) is queued before the previous solve's state is

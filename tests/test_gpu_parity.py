@@ -413,6 +413,38 @@ def test_deferred_solve_tops_up_and_matches(gpu):
         assert torch.equal(ua, ud) and torch.equal(va, vd) and torch.equal(a.p, d.p)
 
 
+@pytest.mark.parametrize("les,network", [(False, False), (True, True)])
+def test_deferred_stepper_speculative_predictor_matches(gpu, les, network):
+    """Steps with the deferred convergence read and no read in between (each
+    step queues its predictor before completing the previous solve; the first
+    solve of a fresh plan is enqueued with one iteration, so the next step
+    tops it up, re-projects and predicts again) give the blocking stepper's
+    bits; with LES applied and a network guess this is the composition of
+    the warm-start arm."""
+    import aerojax_amd as aj
+    from aerojax_amd.nn import WarmStartNet
+    nx, ny = 512, 128
+    c = case(nx, ny, 4.0, 1.0, kind="naca", noise=1e-3)
+    net = WarmStartNet(base_channels=32, levels=4, seed=7) if network else None
+    mk = lambda d: aj.CollocatedStepper(nx, ny, c["dx"], c["dy"], c["mask"], c["sdf"], 5e-4, 1.0,  # noqa
+                                        2.5e-4, mg_variant="textbook", tolerance=1e-6,
+                                        max_cycles=40, use_les=les, les_coupling="apply",
+                                        smagorinsky_constant=0.17, warm_start=net,
+                                        deferred_check=d)
+    a, d = mk(False), mk(True)
+    a.load(c["u"], c["v"])
+    d.load(c["u"], c["v"])
+    for n in (3, 2):
+        for _ in range(n):
+            a.step()
+            d.step()
+        ua, va = a.state
+        ud, vd = d.state
+        assert torch.equal(ua, ud) and torch.equal(va, vd) and torch.equal(a.p, d.p)
+    if net is not None:
+        net.close()
+
+
 def test_step_textbook_reaches_tolerance(gpu):
     import aerojax_amd as aj
     nx, ny = 512, 128
