@@ -319,6 +319,18 @@ extern "C" int aj_mg_batch_count(const aj_mg_plan* p) { return p ? p->batch : 0;
 
 extern "C" int aj_mg_destroy(aj_mg_plan* p) {
   if (!p) return AJ_OK;
+  // an open deferred solve and graph replays may still run on the plan's
+  // buffers: wait for the plan's device before tearing anything down
+  {
+    int cur = 0;
+    const bool have = hipGetDevice(&cur) == hipSuccess;
+    if (hipSetDevice(p->device) == hipSuccess) {
+      if (p->pend_active && p->pend_ev) (void)hipEventSynchronize(p->pend_ev);
+      (void)hipDeviceSynchronize();
+    }
+    p->pend_active = 0;
+    if (have) (void)hipSetDevice(cur);
+  }
   for (int k = 0; k < 2; ++k) {
     if (p->graph_exec[k]) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec[k]);
     if (p->graph[k]) (void)hipGraphDestroy((hipGraph_t)p->graph[k]);
@@ -394,6 +406,9 @@ extern "C" int aj_poisson_mg_complete(aj_mg_plan* p, int* more) {
 
 extern "C" int aj_mg_get_stats(aj_mg_plan* p, aj_mg_stats* out, void* stream) {
   if (!p || !out) return fail(AJ_ERR_INVALID, "aj_mg_get_stats: null pointer");
+  if (p->pend_active)
+    return fail(AJ_ERR_INVALID,
+                "aj_mg_get_stats: a deferred solve is open (call aj_poisson_mg_complete first)");
   hipStream_t st = (hipStream_t)stream;
   AJ_HIP(hipMemcpyAsync(out, p->stats, sizeof(aj_mg_stats), hipMemcpyDeviceToHost, st));
   AJ_HIP(hipStreamSynchronize(st));
@@ -402,6 +417,10 @@ extern "C" int aj_mg_get_stats(aj_mg_plan* p, aj_mg_stats* out, void* stream) {
 
 extern "C" int aj_mg_get_stats_batched(aj_mg_plan* p, aj_mg_stats* out, void* stream) {
   if (!p || !out) return fail(AJ_ERR_INVALID, "aj_mg_get_stats_batched: null pointer");
+  if (p->pend_active)
+    return fail(AJ_ERR_INVALID,
+                "aj_mg_get_stats_batched: a deferred solve is open (call aj_poisson_mg_complete "
+                "first)");
   hipStream_t st = (hipStream_t)stream;
   AJ_HIP(hipMemcpyAsync(out, p->stats, sizeof(aj_mg_stats) * p->batch, hipMemcpyDeviceToHost, st));
   AJ_HIP(hipStreamSynchronize(st));

@@ -112,9 +112,17 @@ class MultigridPlan:
         return [x.as_dict(textbook=self.variant != "reference") for x in s]
 
     def close(self):
+        """Release the plan (waits for an open deferred solve and the plan's
+        device first); idempotent."""
         if getattr(self, "handle", None):
             _lib.load().aj_mg_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def get_plan(nx, ny, dx, dy, dtype, variant, levels, device) -> MultigridPlan:

@@ -257,6 +257,9 @@ class CollocatedStepper:
         # needed more iterations than the previous one
         self.deferred = deferred
         self._pending = False
+        self._pend_stream = None  # torch stream the open solve was enqueued on
+        self.topups = 0  # deferred solves that needed more iterations than enqueued
+        self._own_plans = [self.plan] if deferred else []
         # predictor fields per slot: with the deferred read, a step's predictor
         # (LES, RK, rhs, guess) is queued before the previous solve's state is
         # read, into the other slot, so that a late top-up can still redo the
@@ -276,6 +279,7 @@ class CollocatedStepper:
             # a private plan for the guess test: stream-ordered (no host sync)
             self.guess_plan = MultigridPlan(nx, ny, dx, dy, dtype, "textbook", 0, dev)
             self.guess_plan.set_option(adaptive=False)
+            self._own_plans.append(self.guess_plan)
             self.side = torch.cuda.Stream(device=dev)  # default priority
             self.p_guess = e()
             self.ev_rhs, self.ev_guess = torch.cuda.Event(), torch.cuda.Event()
@@ -301,22 +305,46 @@ class CollocatedStepper:
 
     def finish(self) -> bool:
         """Complete a deferred solve: read its convergence state and, if it
-        needed more iterations, enqueue them and redo the projection (True)."""
+        needed more iterations, enqueue them and redo the projection (True).
+        The top-up and the re-projection run on the stream the solve was
+        enqueued on; if the current stream is another one, it waits for them."""
         if not self._pending:
             return False
-        self._pending = False
         more = ctypes.c_int(0)
         with torch.cuda.device(self.device):
             _lib.check(self._lib.aj_poisson_mg_complete(self.plan.handle, ctypes.byref(more)),
                        "aj_poisson_mg_complete")
+            self._pending = False
             if more.value:
-                s = A.stream_handle(self.device)
+                self.topups += 1
+                ps = self._pend_stream
                 us, vs = self._usv[1 - self.slot]
                 _lib.check(self._lib.aj_project(
                     self._dc, us.data_ptr(), vs.data_ptr(), self.p.data_ptr(),
                     self.mask.data_ptr(), self.u[self.cur].data_ptr(), self.v[self.cur].data_ptr(),
-                    self.nx, self.ny, self.dt, self.dx, self.dy, self.U_inf, s), "aj_project")
+                    self.nx, self.ny, self.dt, self.dx, self.dy, self.U_inf, ps.cuda_stream),
+                    "aj_project")
+                cur = torch.cuda.current_stream(self.device)
+                if cur.cuda_stream != ps.cuda_stream:
+                    cur.wait_stream(ps)
         return bool(more.value)
+
+    def close(self):
+        """Complete an open solve and release the plans this stepper owns
+        (shared plans from ``get_plan`` stay); idempotent."""
+        try:
+            if getattr(self, "_pending", False):
+                self.finish()
+        finally:
+            for pl in getattr(self, "_own_plans", []):
+                pl.close()
+            self._own_plans = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def step(self, p0=None):
         """One step (LES -> RK predictor -> rhs -> pressure -> projection) on
@@ -386,10 +414,12 @@ class CollocatedStepper:
                                or p0.device != self.device or not p0.is_contiguous()):
             raise ValueError("p0 must be a contiguous tensor of the stepper's shape, dtype and device")
         fn = self._lib.aj_poisson_mg_enqueue if deferred else self._lib.aj_poisson_mg
+        stream = torch.cuda.current_stream(self.device)
         rc = fn(self.plan.handle, self.rhs.data_ptr(), A.ptr(p0), self.p.data_ptr(), self._cycles,
-                float(self.tolerance), A.stream_handle(self.device))
+                float(self.tolerance), stream.cuda_stream)
         _lib.check(rc, "aj_poisson_mg")
         self._pending = deferred
+        self._pend_stream = stream if deferred else None
 
     def _solve_overlapped(self):
         main = torch.cuda.current_stream(self.device)

@@ -290,7 +290,9 @@ int aj_poisson_mg_enqueue(aj_mg_plan* plan, const void* rhs, const void* p0,
                           void* p_out, int max_cycles, double tolerance, void* stream);
 int aj_poisson_mg_complete(aj_mg_plan* plan, int* more);
 /* Copy the statistics of the last aj_poisson_mg call on this plan
- * (synchronises `stream`; instance 0 of a batched plan). */
+ * (synchronises `stream`; instance 0 of a batched plan).  Fails with
+ * AJ_ERR_INVALID while a deferred solve is open (complete it first);
+ * aj_mg_destroy waits for an open solve and the plan's device first. */
 int aj_mg_get_stats(aj_mg_plan* plan, aj_mg_stats* out, void* stream);
 /* Batched TEXTBOOK plan: `count` independent instances of one nx x ny grid
  * (no counterpart in the reference, which solves one grid per call).

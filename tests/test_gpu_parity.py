@@ -416,11 +416,12 @@ def test_deferred_solve_tops_up_and_matches(gpu):
 @pytest.mark.parametrize("les,network", [(False, False), (True, True)])
 def test_deferred_stepper_speculative_predictor_matches(gpu, les, network):
     """Steps with the deferred convergence read and no read in between (each
-    step queues its predictor before completing the previous solve; the first
-    solve of a fresh plan is enqueued with one iteration, so the next step
-    tops it up, re-projects and predicts again) give the blocking stepper's
-    bits; with LES applied and a network guess this is the composition of
-    the warm-start arm."""
+    step queues its predictor before completing the previous solve) give the
+    blocking stepper's bits.  The deferred stepper's plan first solves a zero
+    rhs, which settles at once, so its next solve is enqueued with a single
+    outer iteration: the following step must top it up, re-project and
+    queue the predictor again (counted in ``topups``).  With LES applied and
+    a network guess this is the composition of the warm-start arm."""
     import aerojax_amd as aj
     from aerojax_amd.nn import WarmStartNet
     nx, ny = 512, 128
@@ -432,6 +433,8 @@ def test_deferred_stepper_speculative_predictor_matches(gpu, les, network):
                                         smagorinsky_constant=0.17, warm_start=net,
                                         deferred_check=d)
     a, d = mk(False), mk(True)
+    z = torch.zeros((nx, ny), dtype=torch.float32, device=gpu)
+    d.plan.solve(z, torch.empty_like(z), None, 40, 1e-6)  # next enqueue: one iteration
     a.load(c["u"], c["v"])
     d.load(c["u"], c["v"])
     for n in (3, 2):
@@ -441,8 +444,82 @@ def test_deferred_stepper_speculative_predictor_matches(gpu, les, network):
         ua, va = a.state
         ud, vd = d.state
         assert torch.equal(ua, ud) and torch.equal(va, vd) and torch.equal(a.p, d.p)
+    print(f"deferred stepper top-ups: {d.topups}", flush=True)
+    assert d.topups >= 1
+    d.close()
     if net is not None:
         net.close()
+
+
+def test_deferred_stepper_with_accepted_guess_matches(gpu):
+    """A warm start that passes the guess test (a loose solve of the step's
+    own rhs, residual far below the zero guess's): the guess is kept at every
+    step, and the deferred stepper still gives the blocking stepper's bits."""
+    import aerojax_amd as aj
+    nx, ny = 512, 128
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder", noise=1e-3)
+    loose = aj.MultigridPlan(nx, ny, c["dx"], c["dy"], torch.float32, "textbook", 0)
+    loose.set_option(relative=True)
+
+    def guess(rhs, sdf):
+        p0 = torch.empty_like(rhs)
+        loose.solve(rhs, p0, None, 2, 1e-2)
+        return p0
+
+    mk = lambda d: aj.CollocatedStepper(nx, ny, c["dx"], c["dy"], c["mask"], c["sdf"], 5e-4, 1.0,  # noqa
+                                        2.5e-4, mg_variant="textbook", tolerance=1e-6,
+                                        max_cycles=40, warm_start=guess, deferred_check=d)
+    a, d = mk(False), mk(True)
+    a.load(c["u"], c["v"])
+    d.load(c["u"], c["v"])
+    for _ in range(4):
+        a.step()
+        d.step()
+        d.finish()
+        sa, sd = a.plan.stats(), d.plan.stats()
+        print(f"guess kept {sa['used_guess']} {sd['used_guess']}, guess/zero residual "
+              f"{sd['guess_norm'] / sd['zero_norm']:.3e}", flush=True)
+        assert sa["used_guess"] and sd["used_guess"]
+        assert sd["guess_norm"] < 0.1 * sd["zero_norm"]
+        ua, va = a.state
+        ud, vd = d.state
+        assert torch.equal(ua, ud) and torch.equal(va, vd) and torch.equal(a.p, d.p)
+    d.close()
+    loose.close()
+
+
+def test_closed_steppers_release_their_plans(gpu):
+    """Deferred steppers own their multigrid plan; closing them gives the
+    workspace back (device memory stays flat over repeated create/close)."""
+    import gc
+    import aerojax_amd as aj
+    nx, ny = 512, 128
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder", noise=1e-3)
+
+    def cycle():
+        s = aj.CollocatedStepper(nx, ny, c["dx"], c["dy"], c["mask"], c["sdf"], 5e-4, 1.0,
+                                 2.5e-4, mg_variant="textbook", tolerance=1e-6, max_cycles=40,
+                                 deferred_check=True)
+        s.load(c["u"], c["v"])
+        s.step()
+        s.step()
+        ws = s.plan.workspace_bytes
+        s.close()
+        del s
+        gc.collect()
+        return ws
+
+    cycle()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free0 = torch.cuda.mem_get_info()[0]
+    for _ in range(6):
+        ws = cycle()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free1 = torch.cuda.mem_get_info()[0]
+    print(f"plan workspace {ws} B; free device memory {free0} -> {free1}", flush=True)
+    assert free0 - free1 < ws
 
 
 def test_step_textbook_reaches_tolerance(gpu):
