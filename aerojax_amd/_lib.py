@@ -35,7 +35,7 @@ EXPORTS = (
     "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
     "aj_les_smagorinsky_total", "aj_les_dynamic_smagorinsky", "aj_divergence_rhs", "aj_divergence_periodic", "aj_project",
     "aj_project_collocated",
-    "aj_flow_diagnostics", "aj_geometry", "aj_batch_geometry", "aj_tune_enable", "aj_tune_reset",
+    "aj_flow_diagnostics", "aj_flow_diagnostics_noalloc", "aj_geometry", "aj_batch_geometry", "aj_tune_enable", "aj_tune_reset",
     "aj_tune_report", "aj_tune_load",
     "aj_batch_create", "aj_batch_destroy", "aj_batch_cells",
     "aj_batch_num_instances", "aj_batch_offsets", "aj_batch_set_viscosity",
@@ -143,6 +143,8 @@ _SIGS = {
                                    _vp]),
     "aj_flow_diagnostics": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d,
                                  ctypes.POINTER(FlowDiag), _vp]),
+    "aj_flow_diagnostics_noalloc": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d,
+                                         ctypes.POINTER(FlowDiag), _vp]),
     "aj_geometry": (_i, [_i, ctypes.POINTER(Obstacle), _i, _i, _d, _d, _vp, _vp, _vp]),
     "aj_batch_geometry": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "aj_tune_enable": (_i, [_i]),

@@ -12,6 +12,9 @@
 //   * check_cfl (advection_schemes/utils.py:14-20): max_speed =
 //     max sqrt(u^2 + v^2), cfl = max(max_speed dt / dx, max_speed dt / dy).
 // FP64 partial sums (and maxima) per block, fixed-order final reduction.
+#include <mutex>
+#include <string>
+
 #include "aj_common.h"
 
 namespace aj {
@@ -93,6 +96,69 @@ __global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ p
   }
 }
 
+// Launches the two reduction kernels into `part` (5 doubles per 256-cell
+// block) and `dres`, copies the result to the host struct and waits.
+static int diag_run(int dtype, const void* u, const void* v, const void* p, const void* mask,
+                    int nx, int ny, double dx, double dy, double dt, double* part,
+                    aj_flow_diag* dres, aj_flow_diag* out, hipStream_t st) {
+  const long n = (long)nx * ny;
+  const int nb = (int)((n + 255) / 256);
+  if (dtype == AJ_F32)
+    hipLaunchKernelGGL(k_diag<float>, dim3(nb), dim3(256), 0, st, (const float*)u,
+                       (const float*)v, (const float*)p, (const float*)mask, nx, ny, dx, dy, dt,
+                       part);
+  else
+    hipLaunchKernelGGL(k_diag<double>, dim3(nb), dim3(256), 0, st, (const double*)u,
+                       (const double*)v, (const double*)p, (const double*)mask, nx, ny, dx, dy,
+                       dt, part);
+  AJ_LAUNCH_CHECK("k_diag");
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, (const double*)part, nb, dx, dy, dt,
+                     dres);
+  AJ_LAUNCH_CHECK("k_diag_final");
+  AJ_HIP(hipMemcpyAsync(out, dres, sizeof(aj_flow_diag), hipMemcpyDeviceToHost, st));
+  AJ_HIP(hipStreamSynchronize(st));
+  return AJ_OK;
+}
+
+static int diag_check(const char* who, int dtype, const void* u, const void* v, const void* p,
+                      const void* mask, int nx, int ny, aj_flow_diag* out) {
+  if (!u || !v || !p || !mask || !out) return fail(AJ_ERR_INVALID, std::string(who) + ": null pointer");
+  if (nx < 3 || ny < 3) return fail(AJ_ERR_INVALID, std::string(who) + ": grid must be at least 3x3");
+  if (dtype != AJ_F32 && dtype != AJ_F64) return fail(AJ_ERR_INVALID, std::string(who) + ": bad dtype");
+  return AJ_OK;
+}
+
+// ---- allocation-free variant ---------------------------------------------
+// The partial sums live in a file-scope device array instead of a
+// stream-ordered allocation.  Every block of k_diag writes its 5 partials and
+// k_diag_final writes the whole result, so nothing has to start at zero.
+constexpr long kDiagNoAllocMaxCells = 1L << 23;  // 8,388,608 cells
+constexpr int kDiagNoAllocMaxBlocks = (int)(kDiagNoAllocMaxCells / 256);
+constexpr int kDiagResultDoubles = (int)((sizeof(aj_flow_diag) + sizeof(double) - 1) / sizeof(double));
+static __device__ __align__(256) double scratch[5 * (long)kDiagNoAllocMaxBlocks + kDiagResultDoubles];
+
+// The array's host-side name is not a device address: each device's copy is
+// looked up once with that device current and kept here.
+constexpr int kDiagMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kDiagMaxDevices] = {};
+
+static int diag_scratch(double** out) {
+  int dev = 0;
+  AJ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kDiagMaxDevices)
+    return fail(AJ_ERR_UNSUPPORTED, "aj_flow_diagnostics_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    AJ_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(AJ_ERR_HIP, "aj_flow_diagnostics_noalloc: no address for the scratch array");
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return AJ_OK;
+}
+
 }  // namespace aj
 
 using namespace aj;
@@ -100,8 +166,7 @@ using namespace aj;
 extern "C" int aj_flow_diagnostics(int dtype, const void* u, const void* v, const void* p,
                                    const void* mask, int nx, int ny, double dx, double dy,
                                    double dt, aj_flow_diag* out, void* stream) {
-  if (!u || !v || !p || !mask || !out) return fail(AJ_ERR_INVALID, "aj_flow_diagnostics: null pointer");
-  if (nx < 3 || ny < 3) return fail(AJ_ERR_INVALID, "aj_flow_diagnostics: grid must be at least 3x3");
+  if (int rc = diag_check("aj_flow_diagnostics", dtype, u, v, p, mask, nx, ny, out)) return rc;
   hipStream_t st = (hipStream_t)stream;
   const long n = (long)nx * ny;
   const int nb = (int)((n + 255) / 256);
@@ -109,24 +174,24 @@ extern "C" int aj_flow_diagnostics(int dtype, const void* u, const void* v, cons
   AJ_HIP(hipMallocAsync(&buf, (size_t)nb * 5 * sizeof(double) + sizeof(aj_flow_diag), st));
   double* part = (double*)buf;
   aj_flow_diag* dres = (aj_flow_diag*)(part + 5 * (long)nb);
-  if (dtype == AJ_F32)
-    hipLaunchKernelGGL(k_diag<float>, dim3(nb), dim3(256), 0, st, (const float*)u,
-                       (const float*)v, (const float*)p, (const float*)mask, nx, ny, dx, dy, dt,
-                       part);
-  else if (dtype == AJ_F64)
-    hipLaunchKernelGGL(k_diag<double>, dim3(nb), dim3(256), 0, st, (const double*)u,
-                       (const double*)v, (const double*)p, (const double*)mask, nx, ny, dx, dy,
-                       dt, part);
-  else {
-    (void)hipFreeAsync(buf, st);
-    return fail(AJ_ERR_INVALID, "aj_flow_diagnostics: bad dtype");
-  }
-  AJ_LAUNCH_CHECK("k_diag");
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, (const double*)part, nb, dx, dy, dt,
-                     dres);
-  AJ_LAUNCH_CHECK("k_diag_final");
-  AJ_HIP(hipMemcpyAsync(out, dres, sizeof(aj_flow_diag), hipMemcpyDeviceToHost, st));
-  AJ_HIP(hipStreamSynchronize(st));
-  AJ_HIP(hipFreeAsync(buf, st));
+  const int rc = diag_run(dtype, u, v, p, mask, nx, ny, dx, dy, dt, part, dres, out, st);
+  const hipError_t e = hipFreeAsync(buf, st);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "hipFreeAsync");
   return AJ_OK;
+}
+
+extern "C" int aj_flow_diagnostics_noalloc(int dtype, const void* u, const void* v, const void* p,
+                                           const void* mask, int nx, int ny, double dx, double dy,
+                                           double dt, aj_flow_diag* out, void* stream) {
+  if (int rc = diag_check("aj_flow_diagnostics_noalloc", dtype, u, v, p, mask, nx, ny, out))
+    return rc;
+  if ((long)nx * ny > kDiagNoAllocMaxCells)
+    return fail(AJ_ERR_INVALID, "aj_flow_diagnostics_noalloc: nx*ny = " +
+                                    std::to_string((long)nx * ny) + " exceeds the maximum of " +
+                                    std::to_string(kDiagNoAllocMaxCells) + " cells");
+  double* part = nullptr;
+  if (int rc = diag_scratch(&part)) return rc;
+  aj_flow_diag* dres = (aj_flow_diag*)(part + 5 * (long)kDiagNoAllocMaxBlocks);
+  return diag_run(dtype, u, v, p, mask, nx, ny, dx, dy, dt, part, dres, out, (hipStream_t)stream);
 }

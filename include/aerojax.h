@@ -251,6 +251,19 @@ int aj_flow_diagnostics(int dtype, const void* u, const void* v, const void* p,
                         const void* mask, int nx, int ny, double dx, double dy, double dt,
                         aj_flow_diag* out, void* stream);
 
+/* Allocation-free variant of aj_flow_diagnostics (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels, same result bits; the FP64 partial sums live in a fixed
+ * per-device array of the library (1.3 MB, held until the process ends)
+ * instead of a stream-ordered allocation.  At most 8,388,608 cells (nx*ny);
+ * larger grids fail with AJ_ERR_INVALID.  Calls that overlap on one GPU
+ * share that array and race: serialise them (one stream, or host order).
+ * Synchronises `stream` and fills *out. */
+int aj_flow_diagnostics_noalloc(int dtype, const void* u, const void* v, const void* p,
+                                const void* mask, int nx, int ny, double dx, double dy,
+                                double dt, aj_flow_diag* out, void* stream);
+
 #define AJ_MG_MAX_NORMS 64
 typedef struct aj_mg_stats {
   int cycles;      /* V-cycles executed                                  */
