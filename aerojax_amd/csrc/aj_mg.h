@@ -48,6 +48,12 @@ struct aj_mg_plan {
   int use_graph = 1;                // textbook: replay a captured outer iteration
   int guess_test = 1;               // textbook: keep p0 only if ||r(p0)|| <= ||b||
   int relative = 0;                 // textbook: tolerance relative to ||b||
+  // textbook: temporal initial guess from the previous solves' FP64 results
+  int temporal = 0;                 // option on
+  int hist_valid = 0;               // the last solve's iterate is still in p64[]
+  int hist_count = 0;               // entries in the ring (0..2)
+  int hist_last = 0;                // ring slot of the newest entry
+  double* hist[2] = {nullptr, nullptr};  // one allocation, 2 x batch x nx x ny
   void* graph[2] = {nullptr, nullptr};       // hipGraph_t of an even / odd outer iteration
   void* graph_exec[2] = {nullptr, nullptr};  // hipGraphExec_t
   double* dtol = nullptr;           // textbook: tolerance of the current solve

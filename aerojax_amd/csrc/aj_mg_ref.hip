@@ -340,6 +340,10 @@ extern "C" int aj_mg_destroy(aj_mg_plan* p) {
     (void)hipEventDestroy(p->pend_ev);
     p->pend_ev = nullptr;
   }
+  if (p->hist[0]) {
+    (void)hipFree(p->hist[0]);
+    p->hist[0] = p->hist[1] = nullptr;
+  }
   if (p->host_stats) {
     (void)hipHostFree(p->host_stats);
     p->host_stats = nullptr;
@@ -362,6 +366,25 @@ extern "C" int aj_mg_set_option(aj_mg_plan* p, int option, int value) {
     case AJ_MG_OPT_GRAPH: p->use_graph = value ? 1 : 0; return AJ_OK;
     case AJ_MG_OPT_GUESS_TEST: p->guess_test = value ? 1 : 0; return AJ_OK;
     case AJ_MG_OPT_RELATIVE: p->relative = value ? 1 : 0; return AJ_OK;
+    case AJ_MG_OPT_TEMPORAL: {
+      if (p->variant != AJ_MG_TEXTBOOK)
+        return fail(AJ_ERR_UNSUPPORTED, "aj_mg_set_option: the temporal guess needs a textbook plan");
+      if (p->pend_active)
+        return fail(AJ_ERR_INVALID, "aj_mg_set_option: a deferred solve is open");
+      if (value && !p->hist[0]) {
+        const size_t n = (size_t)p->batch * p->nx * p->ny;
+        void* mem = nullptr;
+        AJ_HIP(hipMalloc(&mem, 2 * n * sizeof(double)));
+        p->hist[0] = (double*)mem;
+        p->hist[1] = p->hist[0] + n;
+      }
+      // (re)enabling starts a new history: the next solve has no guess
+      p->temporal = value ? 1 : 0;
+      p->hist_valid = 0;
+      p->hist_count = 0;
+      p->hist_last = 0;
+      return AJ_OK;
+    }
     default: return fail(AJ_ERR_INVALID, "aj_mg_set_option: unknown option");
   }
 }

@@ -369,6 +369,29 @@ __global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, const
   bcopy[q] = rhs[q];
 }
 
+// Temporal initial guess (AJ_MG_OPT_TEMPORAL): k_tb_out also writes the
+// solve's FP64 result into the history ring (every output of a solve that
+// is topped up rewrites the same slot); the next solve starts from it.
+// The guess from the history: H_last (order 1) or the linear extrapolation
+// 2 H_last - H_prev (order 2), zero on the Dirichlet row; the FP64 copy of
+// the rhs as in k_tb_init.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tb_init_temporal(const double* __restrict__ Hl,
+                                                          const double* __restrict__ Hp,
+                                                          int order, const T* __restrict__ rhs,
+                                                          double* __restrict__ P,
+                                                          T* __restrict__ bcopy, int nx, int ny) {
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= (long)nx * ny) return;
+  int j;
+  const int i = tb_row(c, ny, 1.0 / ny, &j);
+  const long q = (long)blockIdx.z * nx * ny + c;
+  const double hl = Hl[q];
+  const double g = order >= 2 ? 2.0 * hl - Hp[q] : hl;
+  P[q] = i < nx - 1 ? g : 0.0;
+  bcopy[q] = rhs[q];
+}
+
 // p' = P[cur] + e (e may be null), r = b - A p' in FP64 on rows < nx-1.
 // The hardware deals consecutive workgroup ids out to the 8 XCDs in turn;
 // renumbering so that each XCD gets one contiguous range of blocks keeps
@@ -674,13 +697,15 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_tb_out(const double* __restrict__ P0,
                                                 const double* __restrict__ P1,
                                                 const int* __restrict__ state, T* __restrict__ out,
-                                                long n) {
+                                                double* __restrict__ H, long n) {
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= n) return;
   const long z = blockIdx.z, q = z * n + c;
   const int s = state[2 * z];
   const double a0 = P0[q], a1 = P1[q];  // both loads in flight with the state
-  out[q] = (T)(s ? a1 : a0);
+  const double r = s ? a1 : a0;
+  out[q] = (T)r;
+  if (H) H[q] = r;
 }
 
 __global__ void k_tb_begin(aj_mg_stats* st, int* state, double* dtol, double tol) {
@@ -1057,13 +1082,26 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
   const long n = (long)nx * ny;
   const double ax = 1.0 / (p->dx * p->dx), ay = 1.0 / (p->dy * p->dy);
   const int nb = p->npartial;
+  // temporal guess: the previous solve's output wrote the next history slot
+  if (p->temporal && p->hist_valid) {
+    p->hist_last = (p->hist_last + 1) & 1;
+    p->hist_count = std::min(p->hist_count + 1, 2);
+  }
+  p->hist_valid = p->temporal;
+  const bool temporal_guess = p->temporal && !p0 && p->hist_count > 0;
   hipLaunchKernelGGL(k_tb_begin, dim3(B), dim3(64), 0, st, p->stats, p->state, p->dtol, tol);
   AJ_LAUNCH_CHECK("k_tb_begin");
-  hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st, p0, rhs,
-                     p->p64[0], (T*)p->bcopy, nx, ny);
+  if (temporal_guess)
+    hipLaunchKernelGGL(k_tb_init_temporal<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st,
+                       (const double*)p->hist[p->hist_last],
+                       (const double*)p->hist[(p->hist_last + 1) & 1], p->hist_count, rhs,
+                       p->p64[0], (T*)p->bcopy, nx, ny);
+  else
+    hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st, p0, rhs,
+                       p->p64[0], (T*)p->bcopy, nx, ny);
   AJ_LAUNCH_CHECK("k_tb_init");
   // outer iteration 0: residual of the initial guess
-  if (p0) {
+  if (p0 || temporal_guess) {
     // a guess is kept only if its residual is not above the zero guess's
     hipLaunchKernelGGL(k_tb_resid_guess<T>, dim3(nb, 1, B), dim3(256), 0, st,
                        (const double*)p->p64[0], (const T*)p->bcopy, p->tB[0], p->partial,
@@ -1090,9 +1128,10 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
 template <typename T>
 static int tb_finish(aj_mg_plan* p, T* p_out, hipStream_t st) {
   const long n = (long)p->nx * p->ny;
+  double* H = p->temporal ? p->hist[(p->hist_last + 1) & 1] : nullptr;
   hipLaunchKernelGGL(k_tb_out<T>, dim3(cdiv(n, 256), 1, p->batch), dim3(256), 0, st,
                      (const double*)p->p64[0], (const double*)p->p64[1], (const int*)p->state,
-                     p_out, n);
+                     p_out, H, n);
   AJ_LAUNCH_CHECK("k_tb_out");
   return AJ_OK;
 }

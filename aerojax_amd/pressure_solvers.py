@@ -60,15 +60,22 @@ class MultigridPlan:
         return int(_lib.load().aj_mg_workspace_bytes(self.handle))
 
     def set_option(self, adaptive: bool | None = None, graph: bool | None = None,
-                   guess_test: bool | None = None, relative: bool | None = None):
+                   guess_test: bool | None = None, relative: bool | None = None,
+                   temporal: bool | None = None):
         """Textbook plans: ``adaptive`` reads convergence between outer
         iterations (host sync, default) instead of enqueueing all
         ``max_cycles``; ``graph`` replays a captured outer iteration
         (default) instead of launching its kernels one by one;
         ``guess_test`` keeps an initial guess only if its residual is not
         above the zero guess's (default) instead of keeping any finite one;
-        ``relative`` makes the tolerance relative to ||b||."""
+        ``relative`` makes the tolerance relative to ||b||; ``temporal``
+        starts solves without p0 from the previous FP64 results (previous
+        solution, then the linear extrapolation of the last two), setting it
+        clears that history."""
         lib = _lib.load()
+        if temporal is not None:
+            _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_TEMPORAL, int(temporal)),
+                       "aj_mg_set_option")
         if relative is not None:
             _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_RELATIVE, int(relative)),
                        "aj_mg_set_option")

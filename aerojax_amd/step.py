@@ -196,6 +196,10 @@ class CollocatedStepper:
     tolerance and that solution is used; a rejected one costs nothing on the
     critical path.  ``guess_stats`` holds the last test.
 
+    ``temporal_guess=True`` (textbook solver): a solve without another
+    guess starts from the previous step's FP64 pressure, from the second
+    step on from the linear extrapolation of the last two (same guess test).
+
     ``deferred_check=True`` (textbook solver, no ``overlap_guess``): a step
     queues its solve with the previous solve's iteration count, and the next
     step queues its predictor (LES, RK, rhs, guess) into a second buffer slot
@@ -209,7 +213,7 @@ class CollocatedStepper:
                  device="cuda", brinkman_eta=0.005, v_cycles=3, mg_variant="reference",
                  tolerance=1e-6, max_cycles=40, fast_mode=False, use_les=False,
                  les_coupling="discard", smagorinsky_constant=0.1, warm_start=None,
-                 overlap_guess=False, deferred_check=False):
+                 overlap_guess=False, deferred_check=False, temporal_guess=False):
         A.require_gpu()
         self.nx, self.ny, self.dx, self.dy = nx, ny, float(dx), float(dy)
         self.nu, self.U_inf, self.dt = float(nu), float(U_inf), float(dt)
@@ -233,12 +237,16 @@ class CollocatedStepper:
         self.cur = 0
         variant = "reference" if mg_variant == "reference" else "textbook"
         deferred = bool(deferred_check) and variant == "textbook"
-        if deferred:
+        self.temporal = bool(temporal_guess) and variant == "textbook"
+        if deferred or self.temporal:
             # a deferred solve stays open on the plan from one step to the next,
-            # so such a stepper owns its plan (the shared plans serve blocking
-            # callers, and a plan refuses a solve while one is open)
+            # and a temporal guess keeps this flow's previous pressures on the
+            # plan, so such a stepper owns its plan (the shared plans serve
+            # blocking callers, and a plan refuses a solve while one is open)
             from .pressure_solvers import MultigridPlan
             self.plan = MultigridPlan(nx, ny, dx, dy, dtype, variant, 0, dev)
+            if self.temporal:
+                self.plan.set_option(temporal=True)
         else:
             self.plan = get_plan(nx, ny, dx, dy, dtype, variant,
                                  4 if variant == "reference" else 0, dev)
@@ -259,7 +267,7 @@ class CollocatedStepper:
         self._pending = False
         self._pend_stream = None  # torch stream the open solve was enqueued on
         self.topups = 0  # deferred solves that needed more iterations than enqueued
-        self._own_plans = [self.plan] if deferred else []
+        self._own_plans = [self.plan] if (deferred or self.temporal) else []
         # predictor fields per slot: with the deferred read, a step's predictor
         # (LES, RK, rhs, guess) is queued before the previous solve's state is
         # read, into the other slot, so that a late top-up can still redo the
@@ -286,6 +294,8 @@ class CollocatedStepper:
 
     def load(self, u, v):
         self.finish()
+        if self.temporal:
+            self.plan.set_option(temporal=True)  # a new flow: no previous pressures
         self.u[self.cur].copy_(A.as_device(u, self.u[0].dtype, self.device, "u"))
         self.v[self.cur].copy_(A.as_device(v, self.u[0].dtype, self.device, "v"))
 

@@ -75,13 +75,38 @@ def normalise_targets(net: WarmStartNet, T):
     return q
 
 
-def train(net: WarmStartNet, X, T, epochs=2, batch=16, lr=5e-4, lr_min=1e-5, seed=0, log=None):
+def validation_loss(net: WarmStartNet, X, T, batch=16) -> float:
+    """Mean squared error of the network on (X, T) over whole batches of the
+    training batch size (the network's activations stay allocated for that
+    size); one host sync."""
+    n = (X.shape[0] // batch) * batch
+    if n == 0:
+        raise ValueError(f"held-out set smaller than one batch of {batch}")
+    acc = torch.zeros((), dtype=torch.float64, device=net.device)
+    for b in range(0, n, batch):
+        out = net.forward(X[b:b + batch])
+        acc += ((out - T[b:b + batch]).double() ** 2).sum()
+    return float(acc.item()) / float(n * T[0].numel())
+
+
+def train(net: WarmStartNet, X, T, epochs=2, batch=16, lr=5e-4, lr_min=1e-5, seed=0, log=None,
+          X_val=None, T_val=None):
     """Adam on shuffled mini-batches with a cosine learning-rate schedule.
-    Returns the per-epoch mean losses (one host sync per epoch)."""
+
+    With a held-out set (X_val, T_val), its loss is evaluated after every
+    epoch and the parameters of the best epoch are kept at the end (the
+    reference keeps its best checkpoint, neural_operators/training_utils.py:
+    195-248).  Returns {"train": per-epoch mean training loss, "val": per-epoch
+    held-out loss (or []), "best_epoch": epoch whose parameters are kept}."""
     n = X.shape[0]
     steps = max(1, (n // batch) * epochs)
     g = torch.Generator(device="cpu").manual_seed(seed)
-    hist = []
+    hist, val = [], []
+    best, best_ep, best_params = math.inf, epochs - 1, None
+    if X_val is not None:
+        val0 = validation_loss(net, X_val, T_val)
+        if log:
+            log(f"  held-out loss before training: {val0:.4e}")
     it = 0
     t0 = time.perf_counter()
     for ep in range(epochs):
@@ -94,6 +119,15 @@ def train(net: WarmStartNet, X, T, epochs=2, batch=16, lr=5e-4, lr_min=1e-5, see
             acc += net.train_step(X, T, idx, cur)[0]
             it += 1
         hist.append(float(acc.item()) / max(nb, 1))
+        msg = f"  epoch {ep}: loss {hist[-1]:.4e}"
+        if X_val is not None:
+            v = validation_loss(net, X_val, T_val)
+            val.append(v)
+            msg += f", held-out {v:.4e}"
+            if v < best:
+                best, best_ep, best_params = v, ep, net.get_params()
         if log:
-            log(f"  epoch {ep}: loss {hist[-1]:.4e} ({time.perf_counter() - t0:.1f}s)")
-    return hist
+            log(msg + f" ({time.perf_counter() - t0:.1f}s)")
+    if best_params is not None and best_ep != epochs - 1:
+        net.set_params(best_params)
+    return {"train": hist, "val": val, "best_epoch": best_ep}

@@ -29,14 +29,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_arm(aj, case, dev, steps, warmup, warm_start=None, overlap=False):
+def run_arm(aj, case, dev, steps, warmup, warm_start=None, overlap=False, temporal=False):
     """Warmup then `steps` timed steps; returns (steps/s, stepper, per-step stats)."""
     import torch
     g = case["grid"]
     stp = aj.CollocatedStepper(g.nx, g.ny, g.dx, g.dy, case["mask"], case["sdf"], case["nu"],
                                case["U_inf"], case["dt"], dtype=torch.float32, device=dev,
                                mg_variant="textbook", tolerance=case["tol"], max_cycles=60,
-                               warm_start=warm_start, overlap_guess=overlap, deferred_check=True)
+                               warm_start=warm_start, overlap_guess=overlap, deferred_check=True,
+                               temporal_guess=temporal)
     stp.load(case["u"], case["v"])
     stats = []
     for _ in range(warmup):
@@ -66,7 +67,8 @@ def cycles_per_step(aj, stp, n):
         s = stp.plan.stats()
         cyc.append(s["cycles"])
         g = stp.guess_stats if stp.guess_stats is not None else s
-        if "used_guess" in g and g.get("zero_norm", 0) > 0 and stp.warm_start is not None:
+        if "used_guess" in g and g.get("zero_norm", 0) > 0 and \
+                (stp.warm_start is not None or stp.temporal):
             used.append(g["used_guess"])
             ratio.append(g["guess_norm"] / g["zero_norm"])
     return cyc, used, ratio
@@ -125,26 +127,186 @@ def run_batched(args):
 
 
 def train_net(args, dev, nx=2048, ny=512):
-    """Warm-start network trained on solver-generated pairs at nx x ny."""
+    """Warm-start network trained on solver-generated pairs at nx x ny: 10
+    pairs per flow (every second step from step 4 to 22) so that the pairs
+    come from many flows; a held-out set from other flows picks the best
+    epoch."""
     import torch
     from aerojax_amd.nn import WarmStartNet
-    from aerojax_amd.warmstart import generate_pairs, normalise_targets, train
+    from aerojax_amd.warmstart import generate_pairs, normalise_targets, train, validation_loss
     net = WarmStartNet(base_channels=32, levels=4, seed=0, r=args.nn_r, device=dev)
     t0 = time.perf_counter()
-    X, T = generate_pairs(net, args.nn_pairs, range(1000, 100000), nx, ny,
-                          steps_per_seed=40, stride=2, skip=2)
+    kw = dict(steps_per_seed=24, stride=2, skip=4)
+    X, T = generate_pairs(net, args.nn_pairs, range(1000, 1000000), nx, ny, **kw)
+    Xv, Tv = generate_pairs(net, args.nn_val_pairs, range(2000000, 3000000), nx, ny, **kw)
     t_gen = time.perf_counter() - t0
-    log(f"generated {X.shape[0]} pairs at {X.shape[1]}x{X.shape[2]} in {t_gen:.1f}s")
+    log(f"generated {X.shape[0]} training + {Xv.shape[0]} held-out pairs at "
+        f"{X.shape[1]}x{X.shape[2]} in {t_gen:.1f}s")
     pscale = normalise_targets(net, T)
+    Tv.div_(pscale)
     t0 = time.perf_counter()
-    hist = train(net, X, T, epochs=args.nn_epochs, batch=16, lr=5e-4, log=log)
+    hist = train(net, X, T, epochs=args.nn_epochs, batch=16, lr=5e-4, log=log, X_val=Xv,
+                 T_val=Tv)
+    val_best = validation_loss(net, Xv, Tv)
     t_train = time.perf_counter() - t0
-    del X, T
+    info = {"train_pairs": int(X.shape[0]), "heldout_pairs": int(Xv.shape[0]),
+            "pairs_per_flow": 10, "train_epochs": args.nn_epochs, "train_loss": hist["train"],
+            "heldout_loss": hist["val"], "best_epoch": hist["best_epoch"],
+            "heldout_loss_kept": val_best, "gen_s": round(t_gen, 1),
+            "train_s": round(t_train, 1), "target_scale": pscale, "params": net.num_params}
+    del X, T, Xv, Tv
     torch.cuda.empty_cache()
-    info = {"train_pairs": args.nn_pairs, "train_epochs": args.nn_epochs, "train_loss": hist,
-            "gen_s": round(t_gen, 1), "train_s": round(t_train, 1), "target_scale": pscale,
-            "params": net.num_params}
     return net, info
+
+
+def ablation(aj, case, dev, net, steps=40, at=(10, 20, 30, 39)):
+    """Untimed: V-cycles to the tolerance and guess/zero residual ratio from
+    initial guesses of the same pressure problems (the rhs of a plain
+    trajectory at the listed steps), each solved with the guess test off:
+      zero        the reference's start
+      net         the trained network's guess (if a network is given)
+      exact_rK    the converged pressure restricted K times (2x2 average) and
+                  prolonged back (bilinear): the best a network running on
+                  the grid restricted K times could return
+      exact       the converged FP32 pressure itself: the floor of any FP32
+                  guess (its rounding leaves a residual of ~1e-5 of ||b||)
+      previous    the previous step's pressure
+      extrapolated 2 p_prev - p_prevprev
+    Returns {guess: {"vcycles": [...], "residual_ratio": [...]}}."""
+    import torch
+    import torch.nn.functional as F
+    g = case["grid"]
+    nx, ny = g.nx, g.ny
+    stp = aj.CollocatedStepper(nx, ny, g.dx, g.dy, case["mask"], case["sdf"], case["nu"],
+                               case["U_inf"], case["dt"], dtype=torch.float32, device=dev,
+                               mg_variant="textbook", tolerance=case["tol"], max_cycles=60)
+    stp.load(case["u"], case["v"])
+    probe = aj.MultigridPlan(nx, ny, g.dx, g.dy, torch.float32, "textbook", 0, dev)
+    probe.set_option(guess_test=False)
+    out = torch.empty((nx, ny), device=dev)
+
+    def rp(p, r):
+        x = p[None, None]
+        for _ in range(r):
+            x = F.avg_pool2d(x, 2)
+        for _ in range(r):
+            x = F.interpolate(x, scale_factor=2, mode="bilinear", align_corners=False)
+        return x[0, 0].contiguous()
+
+    res = {}
+
+    def run(name, rhs, p0):
+        probe.solve(rhs, out, p0, 60, case["tol"])
+        st = probe.stats()
+        assert st["converged"], (name, st)
+        d = res.setdefault(name, {"vcycles": [], "residual_ratio": []})
+        d["vcycles"].append(st["cycles"])
+        d["residual_ratio"].append(st["norms"][0] / st["zero_norm"] if p0 is not None else 1.0)
+
+    prev = prev2 = None
+    for k in range(steps):
+        stp.step()
+        rhs, p = stp.rhs.clone(), stp.p.clone()
+        if k in at:
+            run("zero", rhs, None)
+            if net is not None:
+                run("net", rhs, net.warm_start(rhs, stp.sdf))
+            for r in (1, 2, 3):
+                run(f"exact_r{r}", rhs, rp(p, r))
+            run("exact", rhs, p)
+            run("previous", rhs, prev)
+            run("extrapolated", rhs, (2.0 * prev - prev2).contiguous())
+        prev2, prev = prev, p
+    probe.close()
+    stp.close()
+    for d in res.values():
+        d["mean_vcycles"] = sum(d["vcycles"]) / len(d["vcycles"])
+    res["steps"] = list(at)
+    return res
+
+
+def gpu_first_step(aj, case, dev):
+    """Untimed: one plain step from the initial state; host copies of
+    (u, v, p) for the error against the FP64 oracle."""
+    import torch
+    g = case["grid"]
+    stp = aj.CollocatedStepper(g.nx, g.ny, g.dx, g.dy, case["mask"], case["sdf"], case["nu"],
+                               case["U_inf"], case["dt"], dtype=torch.float32, device=dev,
+                               mg_variant="textbook", tolerance=case["tol"], max_cycles=60)
+    stp.load(case["u"], case["v"])
+    stp.step()
+    u, v = stp.state
+    out = tuple(t.double().cpu().numpy() for t in (u, v, stp.p))
+    stp.close()
+    return out
+
+
+def cpu_baseline_leg(case, gpu_step, seconds, numpy_steps):
+    """CPU baseline and the error field (the only part of the bench that uses
+    oracle/, as the checker and the timed CPU restatement):
+      * the C++/OpenMP restatement of the same to-tolerance step on all the
+        threads OpenMP is given, steps from the initial condition until
+        `seconds` have passed;
+      * the NumPy restatement (one core) for `numpy_steps` steps;
+      * one FP64 oracle step from the initial state (pressure to a relative
+        residual of 1e-13) against the GPU's first step."""
+    import numpy as np
+    from oracle import cpu_step as C
+    from oracle import reference_step as R
+    from oracle import textbook_mg as TM
+    g = case["grid"]
+    nx, ny = g.nx, g.ny
+    f = np.float32
+    mask, sdf = case["mask"].astype(f), case["sdf"].astype(f)
+    out = {}
+    cs = C.CpuStepper(nx, ny, g.dx, g.dy)
+    uu, vv = case["u"].astype(f), case["v"].astype(f)
+    n, cyc = 0, []
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and n < 10000:
+        uu, vv, _, c, _ = cs.step(uu, vv, mask, sdf, case["nu"], case["dt"], tol=case["tol"])
+        if c < 0:
+            raise RuntimeError("CPU restatement did not reach the tolerance")
+        cyc.append(c)
+        n += 1
+    el = time.perf_counter() - t0
+    cs.close()
+    out["cpp"] = {"value": n / el, "steps": n, "seconds": el, "threads": C.threads(),
+                  "vcycles_per_step": cyc[:5] + ["..."] + cyc[-3:] if n > 8 else cyc}
+    log(f"cpu baseline (C++/OpenMP, {C.threads()} threads): {n / el:.2f} steps/s over {n} steps")
+    if numpy_steps > 0:
+        mg = TM.TextbookMG(nx, ny, g.dx, g.dy)
+        uu, vv = case["u"].astype(f), case["v"].astype(f)
+        t0 = time.perf_counter()
+        for _ in range(numpy_steps):
+            us, vs = R.rk_step_unified(uu, vv, case["dt"], case["nu"], g.dx, g.dy, mask, sdf=sdf,
+                                       U_inf=1.0, brinkman_eta=0.005)
+            us, vs = R.predictor_bcs(us, vs, f(1.0))
+            rhs = R.pressure_rhs(us, vs, mask, case["dt"], g.dx, g.dy)
+            p64, _ = mg.solve(rhs, tol=case["tol"])
+            uu, vv = R.project(us, vs, p64.astype(f), mask, case["dt"], g.dx, g.dy, 1.0)
+        el = time.perf_counter() - t0
+        out["numpy"] = {"value": numpy_steps / el, "steps": numpy_steps, "seconds": el,
+                        "threads": 1}
+        log(f"cpu baseline (NumPy, 1 core): {numpy_steps / el:.3f} steps/s")
+    # FP64 oracle step from the same initial state
+    m64, s64 = case["mask"].astype(np.float64), case["sdf"].astype(np.float64)
+    u0, v0 = case["u"].astype(f).astype(np.float64), case["v"].astype(f).astype(np.float64)
+    us, vs = R.rk_step_unified(u0, v0, case["dt"], case["nu"], g.dx, g.dy, m64, sdf=s64,
+                               U_inf=1.0, brinkman_eta=0.005)
+    us, vs = R.predictor_bcs(us, vs, 1.0)
+    rhs = R.pressure_rhs(us, vs, m64, case["dt"], g.dx, g.dy)
+    mg = TM.TextbookMG(nx, ny, g.dx, g.dy)
+    b = rhs.copy()
+    b[-1] = 0.0
+    p64, _ = mg.solve(rhs, tol=1e-13 * mg.norm(b), max_outer=40)
+    un, vn = R.project(us, vs, p64, m64, case["dt"], g.dx, g.dy, 1.0)
+    gu, gv, gp = gpu_step
+    out["error"] = {"u": R.rel_l2(gu, un), "v": R.rel_l2(gv, vn), "p": R.rel_l2(gp, p64),
+                    "reference": "FP64 oracle step from the same initial state, pressure to a "
+                                 "relative residual of 1e-13; GPU step FP32 to ||div|| <= 1e-6"}
+    log(f"GPU first step vs FP64 oracle: {out['error']}")
+    return out
 
 
 def run_uneven(args):
@@ -297,11 +459,17 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="time budget of the C++/OpenMP CPU baseline (steps from the initial "
+                         "condition)")
+    ap.add_argument("--cpu-numpy-steps", type=int, default=3,
+                    help="steps of the single-core NumPy restatement reported beside it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nn", action="store_true")
+    ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--nn-pairs", type=int, default=20000)
-    ap.add_argument("--nn-epochs", type=int, default=1)
+    ap.add_argument("--nn-val-pairs", type=int, default=1024)
+    ap.add_argument("--nn-epochs", type=int, default=3)
     ap.add_argument("--nn-r", type=int, default=2, help="network runs on the grid restricted r times")
     ap.add_argument("--workload", choices=["warmstart", "batched", "large", "uneven"],
                     default="warmstart",
@@ -374,14 +542,27 @@ def main():
     nx, ny = g.nx, g.ny
     log(f"workload {case['name']}: {case['desc']}, Re={case['Re']:.0f}")
 
-    # ---- plain multigrid arm
+    # ---- plain multigrid arm (zero initial guess, the reference's start)
     sps_plain, el_plain, stp_plain, _ = run_arm(aj, case, dev, args.steps, args.warmup)
     cyc_plain, _, _ = cycles_per_step(aj, stp_plain, 5)
     log(f"plain MG: {sps_plain:.1f} steps/s, V-cycles/step {cyc_plain}")
 
+    # ---- temporal arm: the plan starts each solve from the extrapolation of
+    # the previous FP64 pressures (not the metric's subject; the measured
+    # best practical initial guess, reported beside it)
+    sps_tmp, _, stp_tmp, _ = run_arm(aj, case, dev, args.steps, args.warmup, temporal=True)
+    cyc_tmp, used_tmp, ratio_tmp = cycles_per_step(aj, stp_tmp, 5)
+    log(f"temporal guess: {sps_tmp:.1f} steps/s, V-cycles/step {cyc_tmp}, kept {used_tmp}, "
+        f"guess/zero residual {['%.4f' % r for r in ratio_tmp]}")
+    stp_tmp.close()
+    temporal_info = {"steps_per_s": sps_tmp, "over_plain": sps_tmp / sps_plain,
+                     "vcycles_per_step": cyc_tmp, "guess_used": used_tmp,
+                     "guess_over_zero_residual": ratio_tmp}
+
     # ---- NN warm-start arm
     nn_info = None
     sps_nn = None
+    net = None
     if not args.no_nn:
         net, tinfo = train_net(args, dev, nx, ny)
         # overlapped variant: the zero-guess solve runs while the network and its
@@ -389,6 +570,7 @@ def main():
         sps_ov, _, stp_ov, _ = run_arm(aj, case, dev, args.steps, args.warmup, warm_start=net,
                                        overlap=True)
         log(f"NN warm start, overlapped: {sps_ov:.1f} steps/s")
+        stp_ov.close()
         del stp_ov
         # sequential (the stepper's default): inference, guess test, then the solve
         # from the kept guess or from zero
@@ -400,6 +582,14 @@ def main():
                    "overlapped_steps_per_s": sps_ov, "vcycles_per_step": cyc_nn,
                    "guess_used": used, "guess_over_zero_residual": ratio,
                    "net_resolution": [nx >> args.nn_r, ny >> args.nn_r], **tinfo}
+
+    # ---- ablation of initial guesses (untimed)
+    abl = None
+    if not args.no_ablation:
+        abl = ablation(aj, case, dev, net)
+        log("ablation (mean V-cycles to the tolerance, mean guess/zero residual): " + ", ".join(
+            f"{k} {v['mean_vcycles']:.1f} {sum(v['residual_ratio']) / len(v['residual_ratio']):.3g}"
+            for k, v in abl.items() if k != "steps"))
 
     # ---- live per-kernel timing (HIP event pairs on the launching stream):
     # the stencil / multigrid kernels from the plain arm, the network's from
@@ -455,29 +645,22 @@ def main():
                 "peak_tflops": 2500.0, "frac": c["gbps"] / 1e3 / 2500.0,
                 "avg_launch_us": c["avg_us"]}
 
-    # ---- CPU baseline: NumPy restatement of the same step, bounded sample
+    # ---- CPU baseline (C++/OpenMP restatement on the host's cores, NumPy on
+    # one core) and the first step's error against the FP64 oracle
+    gstep = gpu_first_step(aj, case, dev)
     cpu = None
-    if not args.no_cpu and args.cpu_steps > 0:
-        from oracle import reference_step as R
-        from oracle import textbook_mg as TM
-        f = np.float32
-        mg = TM.TextbookMG(nx, ny, g.dx, g.dy)
-        uu, vv = case["u"].astype(f), case["v"].astype(f)
-        mask, sdf = case["mask"].astype(f), case["sdf"].astype(f)
-        t0 = time.perf_counter()
-        for _ in range(args.cpu_steps):
-            us, vs = R.rk_step_unified(uu, vv, case["dt"], case["nu"], g.dx, g.dy, mask, sdf=sdf,
-                                       U_inf=1.0, brinkman_eta=0.005)
-            us, vs = R.predictor_bcs(us, vs, f(1.0))
-            rhs = R.pressure_rhs(us, vs, mask, case["dt"], g.dx, g.dy)
-            p64, _ = mg.solve(rhs, tol=case["tol"])
-            uu, vv = R.project(us, vs, p64.astype(f), mask, case["dt"], g.dx, g.dy, 1.0)
-        cpu_el = time.perf_counter() - t0
-        cpu = {"value": args.cpu_steps / cpu_el, "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the NumPy restatement (FP32 fields, FP64 "
-                         "defect correction, plain multigrid) of the same 2048x512 "
-                         "to-tolerance step, first steps from the initial condition"}
-        log(f"cpu baseline: {cpu['value']:.3f} steps/s")
+    err = None
+    if not args.no_cpu:
+        leg = cpu_baseline_leg(case, gstep, args.cpu_seconds, args.cpu_numpy_steps)
+        cc = leg["cpp"]
+        cpu = {"value": cc["value"], "unit": "steps/s", "cores": cc["threads"], "kind": "port",
+               "sample": f"{cc['steps']} steps ({cc['seconds']:.1f} s) from the initial condition "
+                         "of the same 2048x512 to-tolerance step (FP32 fields, FP64 defect "
+                         "correction, plain multigrid), CPU restatement of the reference algorithm "
+                         f"in C++/OpenMP on {cc['threads']} threads "
+                         f"(os.sched_getaffinity: {len(os.sched_getaffinity(0))} cores)",
+               "numpy_one_core": leg.get("numpy")}
+        err = leg["error"]
 
     value = sps_nn if sps_nn is not None else sps_plain
     elapsed = el_nn if sps_nn is not None else el_plain
@@ -499,9 +682,15 @@ def main():
                    "arm": "nn_warm_start" if sps_nn is not None else "plain_mg",
                    "cell_updates_per_s": value * nx * ny,
                    "plain_mg_steps_per_s": sps_plain, "plain_vcycles_per_step": cyc_plain,
-                   "nn_over_plain": (sps_nn / sps_plain) if sps_nn else None, "nn": nn_info},
+                   "nn_over_plain": (sps_nn / sps_plain) if sps_nn else None, "nn": nn_info,
+                   "temporal_guess": temporal_info, "guess_ablation": abl},
+        "error_vs_fp64_oracle": err,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gbps"], "peak": 8000.0,
                      "unit": "GB/s", "frac": d["gbps"] / 8000.0, "traffic": traffic,
+                     "traffic_source": "profiles/hbm_traffic_2048x512.json: rocprofv3 --pmc "
+                                       "FETCH_SIZE / WRITE_SIZE of the same kernel, scaled by a "
+                                       "copy calibration; read from that file, not counted in "
+                                       "this run" if traffic is not None else None,
                      "avg_launch_us": d["avg_us"], "bytes_per_launch": d["bytes"] / d["count"],
                      "single_launch_event_us": d["single_launch_us"],
                      "timing": f"HIP events on the plan's stream around {repeat} back-to-back "

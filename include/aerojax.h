@@ -208,9 +208,15 @@ int aj_mg_num_levels(const aj_mg_plan* plan);
  *     to measure what a guess is worth).
  *   AJ_MG_OPT_RELATIVE = 0 (default): `tol` of aj_poisson_mg is absolute
  *     (||b - A p|| <= tol); 1: relative to the right-hand side
- *     (||b - A p|| <= tol ||b||). */
+ *     (||b - A p|| <= tol ||b||).
+ *   AJ_MG_OPT_TEMPORAL = 0 (default): 1 starts every solve called without
+ *     p0 from the plan's previous FP64 results: the previous solution after
+ *     one solve, the linear extrapolation 2 p_prev - p_prevprev after two,
+ *     under the same guess test (a worse guess falls back to zero).  Setting
+ *     the option (to 1 or 0) clears the history, e.g. when the plan moves to
+ *     another flow; enabling it allocates 2 FP64 fields per instance. */
 enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1, AJ_MG_OPT_GUESS_TEST = 2,
-       AJ_MG_OPT_RELATIVE = 3 };
+       AJ_MG_OPT_RELATIVE = 3, AJ_MG_OPT_TEMPORAL = 4 };
 int aj_mg_set_option(aj_mg_plan* plan, int option, int value);
 
 /* ---- Launch-shape autotuner ----------------------------------------------
