@@ -33,7 +33,7 @@ AJ_MG_OPT_TEMPORAL = 4
 
 # Every symbol the header declares; tests check the library exports them.
 EXPORTS = (
-    "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_nu_field", "aj_les_smagorinsky",
+    "aj_version", "aj_last_error", "aj_rk_step", "aj_rk_step_rhs", "aj_rk_step_nu_field", "aj_les_smagorinsky",
     "aj_les_smagorinsky_total", "aj_les_dynamic_smagorinsky", "aj_divergence_rhs", "aj_divergence_periodic", "aj_project",
     "aj_project_collocated",
     "aj_flow_diagnostics", "aj_flow_diagnostics_noalloc", "aj_geometry", "aj_batch_geometry", "aj_tune_enable", "aj_tune_reset",
@@ -131,6 +131,8 @@ _SIGS = {
     "aj_last_error": (ctypes.c_char_p, []),
     "aj_rk_step": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
                         ctypes.POINTER(RkParams), _vp]),
+    "aj_rk_step_rhs": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,
+                            ctypes.POINTER(RkParams), _vp]),
     "aj_rk_step_nu_field": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i,
                                  ctypes.POINTER(RkParams), _vp]),
     "aj_les_smagorinsky": (_i, [_i, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp]),

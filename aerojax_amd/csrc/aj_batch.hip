@@ -106,6 +106,8 @@ __global__ __launch_bounds__(NTH) void kb_rk(BatchWork w, BatchStepArgs<T> s) {
     a.nu_field = nullptr;
     a.u_out = s.us + it.off;
     a.v_out = s.vs + it.off;
+    a.rhs = nullptr;
+    a.rhs_masked = 0;
     a.nx = w.nx[g];
     a.ny = w.ny[g];
     a.dt = T(s.dt);

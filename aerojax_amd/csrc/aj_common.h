@@ -68,6 +68,34 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) {
   return max(lo, min(x, hi));
 }
 
+// The hardware deals consecutive workgroup ids out to the 8 XCDs in turn;
+// renumbering so that each XCD gets one contiguous range of blocks keeps
+// the +-row neighbours of a block in the same L2.
+__device__ __forceinline__ long xcd_block(long b, long nb) {
+  if (nb % 8) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+// (x, y) block coordinates of a 2D tile grid, numbered so that each XCD
+// works on one contiguous band of tile rows (blockIdx.y): the halo rows a
+// tile shares with the tiles above and below are then read into one L2.
+__device__ __forceinline__ void tile_block(int* bx, int* by) {
+  const long nb = (long)gridDim.x * gridDim.y;
+  const long b = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, nb);
+  *by = (int)(b / gridDim.x);
+  *bx = (int)(b - (long)*by * gridDim.x);
+}
+
+// Row i and first column j0 of a block of a row kernel (grid
+// (cdiv(ny, blockDim.x), nx), one row segment per block), numbered so that
+// each XCD works on one contiguous band of rows.
+__device__ __forceinline__ void row_block(int* i, int* j0) {
+  const long nb = (long)gridDim.x * gridDim.y;
+  const long b = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, nb);
+  *i = (int)(b / gridDim.x);
+  *j0 = (int)(b - (long)*i * gridDim.x) * (int)blockDim.x;
+}
+
 // Block-wide sum of a double (blockDim.x multiple of 64, <= 1024) with a
 // fixed reduction tree, so repeated runs give identical bits.
 __device__ __forceinline__ double block_sum(double v, double* sh) {

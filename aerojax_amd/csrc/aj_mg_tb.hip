@@ -392,14 +392,8 @@ __global__ __launch_bounds__(256) void k_tb_init_temporal(const double* __restri
   bcopy[q] = rhs[q];
 }
 
-// p' = P[cur] + e (e may be null), r = b - A p' in FP64 on rows < nx-1.
-// The hardware deals consecutive workgroup ids out to the 8 XCDs in turn;
-// renumbering so that each XCD gets one contiguous range of blocks keeps
-// the +-row neighbours of a block in the same L2.
-__device__ __forceinline__ long xcd_block(long b, long nb) {
-  if (nb % 8) return b;
-  return (b % 8) * (nb / 8) + b / 8;
-}
+// p' = P[cur] + e (e may be null), r = b - A p' in FP64 on rows < nx-1
+// (blocks renumbered by xcd_block).
 
 // r = b - A (src + e) in FP64 on rows < nx-1 (r32 <- r, FP64 partial sums
 // of r^2); with e, dst <- src + e.  src / dst are fixed per captured outer

@@ -15,34 +15,33 @@
 
 namespace aj {
 
-template <typename T, int TX, int TY, bool FAST, bool HYPER, int NTH>
+template <typename T, int TX, int TY, bool FAST, bool HYPER, int NTH, bool DIV>
 __global__ __launch_bounds__(NTH) void k_rk_fused(RkArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
-  rk_tile<T, TX, TY, FAST, HYPER, NTH>(a, blockIdx.y * TX, blockIdx.x * TY,
-                                       reinterpret_cast<T*>(smem_raw));
+  int bx, by;
+  tile_block(&bx, &by);
+  rk_tile<T, TX, TY, FAST, HYPER, NTH, DIV>(a, by * TX, bx * TY,
+                                            reinterpret_cast<T*>(smem_raw));
 }
 
-template <typename T, int TX, int TY, bool FAST, bool HYPER>
+template <typename T, int TX, int TY, bool FAST, bool HYPER, bool DIV = false>
 static int launch_rk(const RkArgs<T>& a, hipStream_t st) {
-  constexpr int S = FAST ? 2 : 3;
-  constexpr int R1 = HYPER ? 2 : 1;
-  constexpr int H = S * R1;
-  constexpr size_t RN = (size_t)(TX + 2 * H) * (TY + 2 * H);
-  constexpr size_t NARR = HYPER ? 11 : 8;
-  const size_t smem = RN * NARR * sizeof(T);
-  static_assert(RN * NARR * sizeof(T) <= 160 * 1024, "LDS budget");
+  constexpr int NTH = 512;
+  constexpr size_t N = RkTileShape<TX, TY, FAST, HYPER, DIV>::N;
+  const size_t smem = N * sizeof(T);
+  static_assert(N * sizeof(T) <= 160 * 1024, "LDS budget");
   if (smem > 65536) {
     // dynamic LDS beyond 64 KiB has to be enabled per kernel (gfx950: 160 KiB)
     static bool raised = false;
     if (!raised) {
-      AJ_HIP(hipFuncSetAttribute((const void*)k_rk_fused<T, TX, TY, FAST, HYPER, 512>,
+      AJ_HIP(hipFuncSetAttribute((const void*)k_rk_fused<T, TX, TY, FAST, HYPER, NTH, DIV>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
       raised = true;
     }
   }
   dim3 grid(cdiv(a.ny, TY), cdiv(a.nx, TX));
-  constexpr int NTH = 512;
-  hipLaunchKernelGGL((k_rk_fused<T, TX, TY, FAST, HYPER, NTH>), grid, dim3(NTH), smem, st, a);
+  hipLaunchKernelGGL((k_rk_fused<T, TX, TY, FAST, HYPER, NTH, DIV>), grid, dim3(NTH), smem, st,
+                     a);
   AJ_LAUNCH_CHECK("k_rk_fused");
   return AJ_OK;
 }
@@ -50,11 +49,17 @@ static int launch_rk(const RkArgs<T>& a, hipStream_t st) {
 template <typename T>
 static int rk_step_t(const void* u, const void* v, const void* mask, const void* sdf,
                      const void* nu_field, const void* nu_sgs, void* u_out, void* v_out, int nx,
-                     int ny, const aj_rk_params* p, hipStream_t st) {
+                     int ny, const aj_rk_params* p, hipStream_t st, void* rhs = nullptr,
+                     int rhs_masked = 1) {
   RkArgs<T> a;
   a.u = (const T*)u; a.v = (const T*)v; a.mask = (const T*)mask;
   a.sdf = (const T*)sdf; a.nu_sgs = (const T*)nu_sgs; a.nu_field = (const T*)nu_field;
   a.u_out = (T*)u_out; a.v_out = (T*)v_out;
+  a.rhs = (T*)rhs;
+  a.rhs_masked = rhs_masked;
+  a.inv_2dx = T(1.0 / (2.0 * p->dx));
+  a.inv_2dy = T(1.0 / (2.0 * p->dy));
+  a.inv_dt = T(1.0 / p->dt);
   a.nx = nx; a.ny = ny;
   a.dt = T(p->dt); a.nu = T(p->nu);
   a.inv_dx = T(1.0 / p->dx); a.inv_dy = T(1.0 / p->dy);
@@ -65,6 +70,34 @@ static int rk_step_t(const void* u, const void* v, const void* mask, const void*
   a.slip = p->slip_walls ? 1 : 0;
   const bool hyper = p->nu_hyper_ratio > 0.0;
   const bool fast = p->fast_mode != 0;
+  if (rhs) {
+    // RK predictor with the pressure rhs in its epilogue (no hyperviscosity:
+    // the GUI step that uses it keeps the separate rhs launch)
+    auto launch = [&](int c, hipStream_t s) -> int {
+      if constexpr (sizeof(T) == 8) {
+        if (c == 1) return fast ? launch_rk<T, 8, 64, true, false, true>(a, s)
+                                : launch_rk<T, 8, 64, false, false, true>(a, s);
+        return fast ? launch_rk<T, 4, 64, true, false, true>(a, s)
+                    : launch_rk<T, 4, 64, false, false, true>(a, s);
+      } else {
+        switch (c) {
+          case 1: return fast ? launch_rk<T, 8, 64, true, false, true>(a, s)
+                              : launch_rk<T, 8, 64, false, false, true>(a, s);
+          case 2: return fast ? launch_rk<T, 32, 64, true, false, true>(a, s)
+                              : launch_rk<T, 32, 64, false, false, true>(a, s);
+          default: return fast ? launch_rk<T, 16, 64, true, false, true>(a, s)
+                               : launch_rk<T, 16, 64, false, false, true>(a, s);
+        }
+      }
+    };
+    char key[96];
+    std::snprintf(key, sizeof(key), "rkdiv/%s/%s/%dx%d", sizeof(T) == 8 ? "f64" : "f32",
+                  fast ? "rk2" : "rk3", nx, ny);
+    // u, v, chi source, mask (+ nu_sgs) read; u*, v*, rhs written
+    ProfScope prof(AJ_PROF_RK, (double)nx * ny * sizeof(T) * (7 + (a.nu_sgs ? 1 : 0)), st);
+    int pick = 0;
+    return tune_pick(key, sizeof(T) == 8 ? 2 : 3, launch, st, &pick);
+  }
   // tile candidates (rows x columns), tuned per grid shape and variant
   auto launch = [&](int c, hipStream_t s) -> int {
     if constexpr (sizeof(T) == 8) {
@@ -111,8 +144,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_les(const T* __restrict__ u, const T* __restrict__ v,
                                              T* __restrict__ nsgs, T* __restrict__ ntot, T nu,
                                              int nx, int ny, T inv_2dx, T inv_2dy, T coef) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
+  int i, j;
+  row_block(&i, &j);
+  j += threadIdx.x;
   if (j >= ny) return;
   const long c = (long)i * ny;
   const T s = les_cell(u, v, nx, ny, i, j, inv_2dx, inv_2dy, coef);
@@ -129,8 +163,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_div_rhs(const T* __restrict__ us, const T* __restrict__ vs,
                                                  const T* __restrict__ mask, T* __restrict__ rhs,
                                                  int nx, int ny, T inv_2dx, T inv_2dy, T inv_dt) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
+  int i, j;
+  row_block(&i, &j);
+  j += threadIdx.x;
   if (j >= ny) return;
   div_rhs_cell(us, vs, mask, rhs, nx, ny, i, j, inv_2dx, inv_2dy, inv_dt);
 }
@@ -145,12 +180,15 @@ __global__ __launch_bounds__(256) void k_project(const T* __restrict__ us, const
                                                  T* __restrict__ uo, T* __restrict__ vo, int nx,
                                                  int ny, T dt, T inv_2dx, T inv_2dy, T iface_thr,
                                                  T U_inf) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
+  int i, j;
+  row_block(&i, &j);
+  j += threadIdx.x;
   if (j >= ny) return;
   project_cell(us, vs, p, mask, uo, vo, nx, ny, i, j, dt, inv_2dx, inv_2dy, iface_thr, U_inf);
 }
 
+// one block per 256-column segment of a row; blocks are renumbered per XCD
+// (row_block) so that the +-row stencil reads of a band of rows share an L2
 static dim3 row_grid(int nx, int ny) { return dim3(cdiv(ny, 256), nx); }
 
 static int check_grid(int nx, int ny) {
@@ -175,8 +213,9 @@ __global__ __launch_bounds__(256) void k_project_gui(const T* __restrict__ us,
                                                      T* __restrict__ uo, T* __restrict__ vo,
                                                      int nx, int ny, T dt, T inv_2dx, T inv_2dy,
                                                      T U_in) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
+  int i, j;
+  row_block(&i, &j);
+  j += threadIdx.x;
   if (j >= ny) return;
   const long g = (long)i * ny + j;
   T un = T(0), vn = T(0);
@@ -218,6 +257,28 @@ extern "C" int aj_rk_step(int dtype, const void* u, const void* v, const void* m
                           const void* sdf, const void* nu_sgs, void* u_out, void* v_out,
                           int nx, int ny, const aj_rk_params* prm, void* stream) {
   return rk_entry(dtype, u, v, mask, sdf, nullptr, nu_sgs, u_out, v_out, nx, ny, prm, stream);
+}
+
+extern "C" int aj_rk_step_rhs(int dtype, const void* u, const void* v, const void* mask,
+                              const void* sdf, const void* nu_sgs, void* u_out, void* v_out,
+                              void* rhs, int rhs_masked, int nx, int ny, const aj_rk_params* prm,
+                              void* stream) {
+  if (!u || !v || !mask || !u_out || !v_out || !rhs || !prm)
+    return fail(AJ_ERR_INVALID, "aj_rk_step_rhs: null pointer");
+  if (int rc = check_grid(nx, ny)) return rc;
+  if (!(prm->brinkman_eta > 0)) return fail(AJ_ERR_INVALID, "aj_rk_step_rhs: brinkman_eta must be > 0");
+  if (prm->nu_hyper_ratio > 0.0)
+    return fail(AJ_ERR_UNSUPPORTED, "aj_rk_step_rhs: hyperviscosity needs aj_rk_step + aj_divergence_rhs");
+  if (rhs == u_out || rhs == v_out || rhs == u || rhs == v || rhs == mask)
+    return fail(AJ_ERR_INVALID, "aj_rk_step_rhs: rhs must not alias another field");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == AJ_F32)
+    return rk_step_t<float>(u, v, mask, sdf, nullptr, nu_sgs, u_out, v_out, nx, ny, prm, st, rhs,
+                            rhs_masked ? 1 : 0);
+  if (dtype == AJ_F64)
+    return rk_step_t<double>(u, v, mask, sdf, nullptr, nu_sgs, u_out, v_out, nx, ny, prm, st, rhs,
+                             rhs_masked ? 1 : 0);
+  return fail(AJ_ERR_INVALID, "aj_rk_step_rhs: bad dtype");
 }
 
 extern "C" int aj_rk_step_nu_field(int dtype, const void* u, const void* v, const void* mask,
@@ -292,8 +353,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_div_periodic(const T* __restrict__ u,
                                                       const T* __restrict__ v, T* __restrict__ div,
                                                       int nx, int ny, T inv_2dx, T inv_2dy) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
+  int i, j;
+  row_block(&i, &j);
+  j += threadIdx.x;
   if (j >= ny) return;
   const int ip = (i + 1 == nx) ? 0 : i + 1, im = (i == 0) ? nx - 1 : i - 1;
   const int jp = (j + 1 == ny) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;

@@ -42,10 +42,13 @@ struct RkArgs {
   const T* nu_field;  // may be null: per-cell viscosity replacing nu
   T* u_out;
   T* v_out;
+  T* rhs;  // fused pressure rhs (DIV tiles): div(u*, v*) [* mask] / dt
   int nx, ny;
   T dt, nu, inv_dx, inv_dy, inv_dx2, inv_dy2, U_inf, inv_eta, sdf_scale;
   T hyper;  // nu_hyper_ratio * dx * dy (the biharmonic coefficient is nu * hyper)
+  T inv_2dx, inv_2dy, inv_dt;  // rhs coefficients (DIV tiles)
   int slip;
+  int rhs_masked;  // DIV tiles: multiply the divergence by the mask
 };
 
 // One stencil evaluation of rhs = -upwind(f; u, v) + nu_tot * lap(f)
@@ -82,14 +85,30 @@ __device__ __forceinline__ T masked(T val, T m) {
   return (m > T(0.5)) ? val : T(0);
 }
 
+// LDS elements of a fused RK tile
+template <int TX, int TY, bool FAST, bool HYPER, bool DIV>
+struct RkTileShape {
+  static constexpr int S = FAST ? 2 : 3;
+  static constexpr int R1 = HYPER ? 2 : 1;          // per-stage stencil radius
+  static constexpr int H = S * R1 + (DIV ? 1 : 0);  // total halo
+  static constexpr int RN = (TX + 2 * H) * (TY + 2 * H);
+  static constexpr int RF = DIV ? (TX + 2) * (TY + 2) : 0;  // final values on margin 1
+  static constexpr int N = RN * (HYPER ? 11 : 8) + (DIV ? RN + 2 * RF : 0);
+};
+
 // Fused RK predictor on the tile with origin (x0, y0) (LDS at `sm`,
-// RN * (HYPER ? 11 : 8) elements).  FAST = RK2 (2 stages), else RK3.  HYPER
-// adds the biharmonic term (per-stage stencil radius 2).
-template <typename T, int TX, int TY, bool FAST, bool HYPER, int NTH>
+// RkTileShape::N elements).  FAST = RK2 (2 stages), else RK3.  HYPER adds
+// the biharmonic term (per-stage stencil radius 2).  DIV also writes the
+// pressure rhs of step_pure (divergence_nonperiodic(u*, v*) * mask / dt,
+// solver/operators.py:81-96, step.py:194-199): the final stage runs on the
+// tile plus one ring (one more halo cell), and the divergence reads those
+// values with the boundary values the BCs give them.
+template <typename T, int TX, int TY, bool FAST, bool HYPER, int NTH, bool DIV = false>
 __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* sm) {
-  constexpr int S = FAST ? 2 : 3;
-  constexpr int R1 = HYPER ? 2 : 1;  // per-stage stencil radius
-  constexpr int H = S * R1;          // total halo
+  using Sh = RkTileShape<TX, TY, FAST, HYPER, DIV>;
+  constexpr int S = Sh::S;
+  constexpr int R1 = Sh::R1;         // per-stage stencil radius
+  constexpr int H = Sh::H;           // total halo
   constexpr int RH = TX + 2 * H;     // region rows (x)
   constexpr int RW = TY + 2 * H;     // region cols (y, contiguous)
   constexpr int RN = RH * RW;
@@ -105,6 +124,9 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
   T* s_lu = s_bv + RN;      // HYPER only: laplacians of the stage input
   T* s_lv = s_lu + RN;
   T* s_nh = s_lv + RN;      // HYPER only: biharmonic coefficient nu * hyper
+  T* s_m = sm + RN * (HYPER ? 11 : 8);  // DIV only: mask
+  T* s_fu = s_m + RN;                   // DIV only: final u*, v* (masked) on margin 1
+  T* s_fv = s_fu + Sh::RF;
 
   const int nx = a.nx, ny = a.ny;
   const int gx0 = x0 - H, gy0 = y0 - H;  // global index of LDS (0, 0)
@@ -128,7 +150,7 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
   // domain hold edge copies and are never read by the stencils).
   {
     constexpr int NIT = (RN + NTH - 1) / NTH;
-    T ru[NIT], rv[NIT], rc[NIT], rn[NIT], rf[NIT];
+    T ru[NIT], rv[NIT], rc[NIT], rn[NIT], rf[NIT], rm[DIV ? NIT : 1];
     const T* csrc = has_sdf ? a.sdf : a.mask;
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -140,6 +162,7 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
       rc[it] = csrc[g];
       rn[it] = has_nsgs ? a.nu_sgs[g] : T(0);
       rf[it] = has_nuf ? a.nu_field[g] : a.nu;
+      if constexpr (DIV) rm[it] = a.mask[g];
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -154,6 +177,7 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
       // (rk3_simple_new.py:304-309)
       s_nt[idx] = has_nsgs ? rf[it] + rn[it] : rf[it];
       if constexpr (HYPER) s_nh[idx] = rf[it] * a.hyper;
+      if constexpr (DIV) s_m[idx] = rm[it];
     }
   }
   __syncthreads();
@@ -162,7 +186,7 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
   const T* fv = s_v;
 #pragma unroll
   for (int s = 1; s <= S; ++s) {
-    const int m = (S - s) * R1;  // output margin of this stage
+    const int m = (S - s) * R1 + (DIV ? 1 : 0);  // output margin of this stage
     if constexpr (HYPER) {
       // laplacian of the stage input on margin m + 1
       const int mm = m + 1;
@@ -236,6 +260,22 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
       fv = ov;
       continue;
     }
+    if constexpr (DIV) {
+      // final stage, masked, on the tile plus one ring
+      constexpr int FW = TY + 2, FH = TX + 2;
+      for (int idx = tid; idx < FW * FH; idx += NTH) {
+        const int lx = H - 1 + idx / FW, ly = H - 1 + idx % FW;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+        const int l = lx * RW + ly;
+        T nu_, nv_;
+        stage(l, gx, gy, nu_, nv_);
+        const T m0 = s_m[l];
+        s_fu[idx] = masked(nu_, m0);
+        s_fv[idx] = masked(nv_, m0);
+      }
+      __syncthreads();
+    }
     // ---- final stage on the tile: hard mask, then BCs
     // (rk3_simple_new.py:332-336, 264-279).  Boundary cells copy the masked
     // value of an interior source; each source writes its own targets so
@@ -252,14 +292,35 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
         const bool wall = (gy == 0 || gy == ny - 1);
         a.u_out[g] = (wall && !a.slip) ? T(0) : a.U_inf;
         a.v_out[g] = T(0);
+        if constexpr (DIV) a.rhs[g] = T(0);
         continue;
       }
       const bool interior = gx < nx - 1 && gy > 0 && gy < ny - 1;
-      if (!interior) continue;
-      T nu_, nv_;
-      stage(lx * RW + ly, gx, gy, nu_, nv_);
+      if (!interior) {
+        if constexpr (DIV) a.rhs[g] = T(0);  // zero on every boundary cell
+        continue;
+      }
       const T m0 = rmask[it];
-      const T fu_ = masked(nu_, m0), fv_ = masked(nv_, m0);
+      T fu_, fv_;
+      if constexpr (DIV) {
+        constexpr int FW = TY + 2;
+        const int f = (idx / TY + 1) * FW + (idx % TY + 1);
+        fu_ = s_fu[f];
+        fv_ = s_fv[f];
+        // neighbours with the values the BCs give them: inlet u = U_inf,
+        // outlet row = copy of row nx-2, wall v = 0
+        const T uE = (gx == nx - 2) ? fu_ : s_fu[f + FW];
+        const T uW = (gx == 1) ? a.U_inf : s_fu[f - FW];
+        const T vN = (gy == ny - 2) ? T(0) : s_fv[f + 1];
+        const T vS = (gy == 1) ? T(0) : s_fv[f - 1];
+        const T d = (uE - uW) * a.inv_2dx + (vN - vS) * a.inv_2dy;
+        a.rhs[g] = a.rhs_masked ? d * m0 * a.inv_dt : d * a.inv_dt;
+      } else {
+        T nu_, nv_;
+        stage(lx * RW + ly, gx, gy, nu_, nv_);
+        fu_ = masked(nu_, m0);
+        fv_ = masked(nv_, m0);
+      }
       a.u_out[g] = fu_;
       a.v_out[g] = fv_;
       const T uw = a.slip ? fu_ : T(0);

@@ -388,10 +388,10 @@ class CollocatedStepper:
                                             (dx * dy) ** 0.5, self.cs, s), "aj_les_smagorinsky")
         us, vs = self._usv[self.slot]
         usp, vsp, mask = us.data_ptr(), vs.data_ptr(), self.mask.data_ptr()
-        _lib.check(L.aj_rk_step(dc, u.data_ptr(), v.data_ptr(), mask, A.ptr(self.sdf), nsgs,
-                                usp, vsp, nx, ny, self._prm_ref, s), "aj_rk_step")
-        _lib.check(L.aj_divergence_rhs(dc, usp, vsp, mask, self.rhs.data_ptr(), nx, ny, dx, dy,
-                                       self.dt, s), "aj_divergence_rhs")
+        # RK predictor with rhs = div(u*, v*) * mask / dt in its epilogue
+        _lib.check(L.aj_rk_step_rhs(dc, u.data_ptr(), v.data_ptr(), mask, A.ptr(self.sdf), nsgs,
+                                    usp, vsp, self.rhs.data_ptr(), 1, nx, ny, self._prm_ref, s),
+                   "aj_rk_step_rhs")
         if p0 is None and not self.overlap:
             if self._net is not None:
                 net = self._net

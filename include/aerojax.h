@@ -70,6 +70,18 @@ int aj_rk_step(int dtype, const void* u, const void* v, const void* mask,
                const void* sdf, const void* nu_sgs, void* u_out, void* v_out,
                int nx, int ny, const aj_rk_params* prm, void* stream);
 
+/* The same RK step with the pressure right-hand side of step_pure in its
+ * epilogue: rhs = divergence_nonperiodic(u*, v*) [* mask] / prm->dt
+ * (solver/operators.py:81-96, solver/pure_functions/step.py:182-199; the
+ * predictor BCs step.py re-imposes are the RK step's own last step), zero
+ * on every boundary cell; rhs_masked = 0 gives div / dt (the GUI step,
+ * step_handlers.py:92-98).  One launch instead of aj_rk_step +
+ * aj_divergence_rhs, with their results.  Hyperviscosity (nu_hyper_ratio >
+ * 0) is not supported here (AJ_ERR_UNSUPPORTED). */
+int aj_rk_step_rhs(int dtype, const void* u, const void* v, const void* mask,
+                   const void* sdf, const void* nu_sgs, void* u_out, void* v_out, void* rhs,
+                   int rhs_masked, int nx, int ny, const aj_rk_params* prm, void* stream);
+
 /* Same step with the reference's `nu` argument given as a field: the GUI
  * step passes nu_total = nu + nu_sgs as `nu` together with nu_sgs
  * (solver/simulation/step_handlers.py:76-90), so the viscosity of the

@@ -300,13 +300,20 @@ def test_dynamic_smagorinsky_parity(gpu, dtype):
     # L_ij = u~u~ - (uu)~ cancels (|u| ~ 1 against a sub-filter variance
     # ~1e-3), so FP32 rounding (6e-8) reaches ~1e-4 in C_d whatever the
     # summation order: the FP32 restatement of the reference formula is itself
-    # ~7e-4 from FP64.  FP32 bar: 2e-3 against FP64; FP64 bar: 1e-12.
+    # ~7e-4 from FP64.  FP32 bars: 2e-3 against the FP64 oracle, against the
+    # FP64 oracle on the FP32-rounded inputs and against the FP32 oracle;
+    # FP64 bar: 1e-12.
     tol = 2e-3 if dtype == torch.float32 else 1e-12
-    for name, got, r64, rdt in (("nu_sgs", nu_sgs, rnu, rnu_d), ("C_d", C, rC, rC_d)):
-        e64, edt = R.rel_l2(host(got), r64), R.rel_l2(host(got), rdt)
-        print(f"dynamic smagorinsky {name} {dtype}: vs fp64 {e64:.2e}, vs same dtype {edt:.2e}",
-              flush=True)
+    r32 = lambda a: a.astype(f).astype(np.float64)  # noqa: E731
+    rnu_r, rC_r = R.dynamic_smagorinsky(r32(c["u"]), r32(c["v"]), dx, dy, delta)
+    for name, got, r64, rdt, rr in (("nu_sgs", nu_sgs, rnu, rnu_d, rnu_r), ("C_d", C, rC, rC_d,
+                                                                           rC_r)):
+        e64, edt, er = (R.rel_l2(host(got), x) for x in (r64, rdt, rr))
+        print(f"dynamic smagorinsky {name} {dtype}: vs fp64 {e64:.2e}, vs same dtype {edt:.2e}, "
+              f"vs fp64 on rounded inputs {er:.2e}", flush=True)
         assert e64 <= tol, (name, e64)
+        assert edt <= tol, (name, edt)
+        assert er <= tol, (name, er)
     assert R.rel_l2(host(nu_tot), 1e-3 + rnu) <= tol
     assert host(C).max() <= 0.2 and host(C).min() >= 0.0
     # GUI step with the dynamic model

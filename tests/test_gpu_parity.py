@@ -115,6 +115,45 @@ def test_rk_viscosity_field_parity(gpu, dtype, fast):
         check(host(out[k]), ref[k], None, dtype, f"rk nu-field fast={fast} {dtype} {name}")
 
 
+@pytest.mark.parametrize("shape", [(512, 96), (130, 70), (2048, 512), (6, 5)])
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("variant", ["sdf", "fast", "nsgs", "nosdf", "unmasked", "noslip"])
+def test_rk_step_rhs_equals_separate_launches(gpu, shape, dtype, variant):
+    """aj_rk_step_rhs (the RK predictor with the pressure rhs in its
+    epilogue) gives the bits of aj_rk_step followed by aj_divergence_rhs."""
+    import ctypes
+    from aerojax_amd import _lib
+    L = _lib.load()
+    nx, ny = shape
+    c = case(nx, ny, 4.0, 1.0, kind="cylinder" if nx >= 64 else "naca")
+    dc = _lib.AJ_F32 if dtype == torch.float32 else _lib.AJ_F64
+    u, v, mask, sdf = (dev(c[k], dtype) for k in ("u", "v", "mask", "sdf"))
+    if variant == "nosdf":
+        sdf = None
+    nsgs = None
+    if variant == "nsgs":
+        nsgs = dev(50.0 * R.constant_smagorinsky(c["u"], c["v"], c["dx"], c["dy"],
+                                                 math.sqrt(c["dx"] * c["dy"]), 0.17), dtype)
+    dt = 1e-3
+    prm = _lib.RkParams(dt, 5e-4, c["dx"], c["dy"], 1.0, 0.005, 0.0,
+                        0 if variant == "noslip" else 1, 1 if variant == "fast" else 0)
+    masked = 0 if variant == "unmasked" else 1
+    s = torch.cuda.current_stream().cuda_stream
+    p = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+    e = lambda: torch.empty_like(u)  # noqa: E731
+    us1, vs1, r1, us2, vs2, r2 = e(), e(), e(), e(), e(), e()
+    _lib.check(L.aj_rk_step(dc, p(u), p(v), p(mask), p(sdf), p(nsgs), p(us1), p(vs1), nx, ny,
+                            ctypes.byref(prm), s), "aj_rk_step")
+    _lib.check(L.aj_divergence_rhs(dc, p(us1), p(vs1), p(mask) if masked else None, p(r1), nx, ny,
+                                   c["dx"], c["dy"], dt, s), "aj_divergence_rhs")
+    _lib.check(L.aj_rk_step_rhs(dc, p(u), p(v), p(mask), p(sdf), p(nsgs), p(us2), p(vs2), p(r2),
+                                masked, nx, ny, ctypes.byref(prm), s), "aj_rk_step_rhs")
+    torch.cuda.synchronize()
+    assert torch.equal(us1, us2) and torch.equal(vs1, vs2)
+    assert torch.equal(r1, r2), float((r1 - r2).abs().max())
+    assert float(r1.abs().max()) > 0
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("Re,les,fast", [(800.0, False, False), (5e4, True, False),
                                          (5e3, True, True)])
