@@ -515,23 +515,35 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
     AJ_LAUNCH_CHECK("kb_jacobi");
     return AJ_OK;
   };
-  if (int rc = batch_launch(b, "jacobi", wt.total, jacobi, st)) return rc;
+  // algorithmic bytes of this level's launches (all instances)
+  const double cl = (double)b->level_len[lev] * sizeof(T);
+  {
+    // p (warm start) and b read, p written
+    ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, cl * (p_src ? 3 : 2), st);
+    if (int rc = batch_launch(b, "jacobi", wt.total, jacobi, st)) return rc;
+  }
   T* Bc = (T*)b->B[lev + 1];
   BatchWork wr = make_work(b, lev, 2);
   BatchMgArgs<T> r = m;
   r.p_out = Yl;
-  if (int rc = batch_launch(b, "resrestrict", wr.total, [&](dim3 g, hipStream_t s) -> int {
-        hipLaunchKernelGGL(kb_resrestrict<T>, g, dim3(256), 0, s, wr, r, Bc);
-        AJ_LAUNCH_CHECK("kb_resrestrict");
-        return AJ_OK;
-      }, st))
-    return rc;
+  {
+    // residual + restriction: p, b read, the quarter-size coarse rhs written
+    ProfScope prof(AJ_PROF_MG_RESID, cl * 2.25, st);
+    if (int rc = batch_launch(b, "resrestrict", wr.total, [&](dim3 g, hipStream_t s) -> int {
+          hipLaunchKernelGGL(kb_resrestrict<T>, g, dim3(256), 0, s, wr, r, Bc);
+          AJ_LAUNCH_CHECK("kb_resrestrict");
+          return AJ_OK;
+        }, st))
+      return rc;
+  }
   T* Ec = (T*)b->E[lev + 1];
   if (int rc = batch_vcycle<T>(b, lev + 1, nullptr, Ec, Bc, 0, st)) return rc;
   m.p_in = Yl;
   m.e_c = Ec;
   m.p_out = p_dst;
   m.clamp = 1;
+  // Y, b and the quarter-size coarse correction read, p written
+  ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, cl * 3.25, st);
   return batch_launch(b, "jacobi_post", wt.total, jacobi, st);
 }
 

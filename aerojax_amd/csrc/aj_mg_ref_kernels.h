@@ -40,12 +40,32 @@ __device__ __forceinline__ T prolong_ref(const T* __restrict__ c, int ncx, int n
                     c[(long)(I + 1) * ncy + J + 1]);
 }
 
-// NS Jacobi sweeps (smooth, multigrid_solver.py:158-171) on one tile.
+// the same prolongation from a coarse patch in LDS whose (0, 0) is coarse
+// cell (cI0, cJ0), row length CW
+template <typename T>
+__device__ __forceinline__ T prolong_ref_lds(const T* __restrict__ c, int cI0, int cJ0, int CW,
+                                             int ncx, int ncy, int i, int j) {
+  const int nxf = 2 * ncx, nyf = 2 * ncy;
+  if (i == nxf - 1 || j == nyf - 1) return T(0);
+  const int I = i >> 1, J = j >> 1;
+  const bool io = i & 1, jo = j & 1;
+  const int l = (I - cI0) * CW + (J - cJ0);
+  const T c00 = c[l];
+  if (!io && !jo) return c00;
+  if (io && !jo) return T(0.5) * (c00 + c[l + CW]);
+  if (!io && jo) return T(0.5) * (c00 + c[l + 1]);
+  return T(0.25) * (((c00 + c[l + CW]) + c[l + 1]) + c[l + CW + 1]);
+}
+
+// NS Jacobi sweeps (smooth, multigrid_solver.py:158-171) on one tile; `c`
+// holds the coarse patch the prolongation of the tile region reads
 template <typename T, int TX, int TY, int NS>
 struct JacobiLds {
   static constexpr int RN = (TX + 2 * NS) * (TY + 2 * NS);
+  static constexpr int CH = (TX + 2 * NS) / 2 + 2, CW = (TY + 2 * NS) / 2 + 2;
   T p[2][RN];
   T b[RN];
+  T c[CH * CW];
 };
 
 // NS Jacobi sweeps on the tile with origin (x0, y0), 256 threads
@@ -59,7 +79,22 @@ __device__ __forceinline__ void ref_jacobi_tile(const RefJacobiArgs<T>& a, int x
   const int gx0 = x0 - NS, gy0 = y0 - NS;
   const int tid = threadIdx.x;
   {
-    // all loads of a thread in flight together (clamped, always in bounds)
+    // all loads of a thread in flight together (clamped, always in bounds);
+    // with a coarse correction, its patch is staged in LDS first and the
+    // prolongation of every region cell reads it there
+    using J = JacobiLds<T, TX, TY, NS>;
+    const int cI0 = clampi(gx0, 0, nx - 1) >> 1, cJ0 = clampi(gy0, 0, ny - 1) >> 1;
+    constexpr int NC = (J::CH * J::CW + 255) / 256;
+    T rc[NC];
+    if (a.e_c) {
+#pragma unroll
+      for (int it = 0; it < NC; ++it) {
+        const int idx = min(tid + it * 256, J::CH * J::CW - 1);
+        const int cx = idx / J::CW, cy = idx - cx * J::CW;
+        const int I = min(cI0 + cx, a.ncx - 1), Jc = min(cJ0 + cy, a.ncy - 1);
+        rc[it] = a.e_c[(long)I * a.ncy + Jc];
+      }
+    }
     constexpr int NIT = (RN + 255) / 256;
     T rp[NIT], rb[NIT];
 #pragma unroll
@@ -68,17 +103,29 @@ __device__ __forceinline__ void ref_jacobi_tile(const RefJacobiArgs<T>& a, int x
       const int lx = idx / RW, ly = idx - lx * RW;
       const int gx = clampi(gx0 + lx, 0, nx - 1), gy = clampi(gy0 + ly, 0, ny - 1);
       const long g = (long)gx * ny + gy;
-      T pv = a.p_in ? a.p_in[g] : T(0);
-      if (a.e_c) pv = pv + prolong_ref(a.e_c, a.ncx, a.ncy, gx, gy);
-      rp[it] = pv;
+      rp[it] = a.p_in ? a.p_in[g] : T(0);
       rb[it] = (a.b_zero_last && gx == nx - 1) ? T(0) : a.b[g];
     }
     if (gated(a.gate)) return;
+    if (a.e_c) {
+#pragma unroll
+      for (int it = 0; it < NC; ++it) {
+        const int idx = tid + it * 256;
+        if (idx < J::CH * J::CW) L.c[idx] = rc[it];
+      }
+      __syncthreads();
+    }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int idx = tid + it * 256;
       if (idx < RN) {
-        s_p[0][idx] = rp[it];
+        T pv = rp[it];
+        if (a.e_c) {
+          const int lx = idx / RW, ly = idx - lx * RW;
+          const int gx = clampi(gx0 + lx, 0, nx - 1), gy = clampi(gy0 + ly, 0, ny - 1);
+          pv = pv + prolong_ref_lds(L.c, cI0, cJ0, J::CW, a.ncx, a.ncy, gx, gy);
+        }
+        s_p[0][idx] = pv;
         s_b[idx] = rb[it];
       }
     }

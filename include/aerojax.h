@@ -424,11 +424,14 @@ enum {
   AJ_PROF_RHS = 1,         /* divergence -> rhs                         */
   AJ_PROF_PROJECT = 2,     /* projection                                */
   AJ_PROF_MG_PRE0 = 3,     /* textbook pre-smooth + restrict, level 0   */
+                           /* (batched reference: pre Jacobi sweeps)    */
   AJ_PROF_MG_POST0 = 4,    /* textbook prolong + post-smooth, level 0   */
+                           /* (batched reference: post Jacobi sweeps)   */
   AJ_PROF_MG_PRE = 5,      /* same, levels >= 1                         */
   AJ_PROF_MG_POST = 6,
   AJ_PROF_MG_COARSE = 7,   /* exact coarsest-level solve (3 kernels)    */
-  AJ_PROF_MG_RESID = 8,    /* FP64 defect-correction residual           */
+  AJ_PROF_MG_RESID = 8,    /* FP64 defect-correction residual (batched  */
+                           /* reference multigrid: residual + restrict)  */
   AJ_PROF_NN_CONV = 9,     /* network 3x3 conv (forward / input grad);  */
                            /* "bytes" holds FLOPs for this tag          */
   AJ_PROF_NN_DW = 10,      /* network weight gradient (FLOPs)           */
