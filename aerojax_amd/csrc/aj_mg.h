@@ -48,6 +48,7 @@ struct aj_mg_plan {
   int use_graph = 1;                // textbook: replay a captured outer iteration
   int guess_test = 1;               // textbook: keep p0 only if ||r(p0)|| <= ||b||
   int relative = 0;                 // textbook: tolerance relative to ||b||
+  double omega = 1.2;               // textbook: over-relaxation of the red-black smoother
   // textbook: temporal initial guess from the previous solves' FP64 results
   int temporal = 0;                 // option on
   int hist_valid = 0;               // the last solve's iterate is still in p64[]

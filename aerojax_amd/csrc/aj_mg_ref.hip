@@ -366,6 +366,23 @@ extern "C" int aj_mg_set_option(aj_mg_plan* p, int option, int value) {
     case AJ_MG_OPT_GRAPH: p->use_graph = value ? 1 : 0; return AJ_OK;
     case AJ_MG_OPT_GUESS_TEST: p->guess_test = value ? 1 : 0; return AJ_OK;
     case AJ_MG_OPT_RELATIVE: p->relative = value ? 1 : 0; return AJ_OK;
+    case AJ_MG_OPT_OMEGA: {
+      if (p->variant != AJ_MG_TEXTBOOK)
+        return fail(AJ_ERR_UNSUPPORTED, "aj_mg_set_option: omega applies to textbook plans");
+      if (value < 500 || value > 1900)
+        return fail(AJ_ERR_INVALID, "aj_mg_set_option: omega (in thousandths) must be in 500..1900");
+      if (p->pend_active)
+        return fail(AJ_ERR_INVALID, "aj_mg_set_option: a deferred solve is open");
+      // the captured outer iterations hold the old factor: capture again
+      AJ_HIP(hipDeviceSynchronize());
+      for (int k = 0; k < 2; ++k) {
+        if (p->graph_exec[k]) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec[k]);
+        if (p->graph[k]) (void)hipGraphDestroy((hipGraph_t)p->graph[k]);
+        p->graph_exec[k] = p->graph[k] = nullptr;
+      }
+      p->omega = value / 1000.0;
+      return AJ_OK;
+    }
     case AJ_MG_OPT_TEMPORAL: {
       if (p->variant != AJ_MG_TEXTBOOK)
         return fail(AJ_ERR_UNSUPPORTED, "aj_mg_set_option: the temporal guess needs a textbook plan");

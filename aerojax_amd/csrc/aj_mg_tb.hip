@@ -10,7 +10,9 @@
 // l has coefficients ax/4^l, ay/4^l and an outlet ghost gamma_l * last that
 // puts its Dirichlet point where the fine grid has it
 // (gamma_l = 1 - 1/(1/2 - 2^-(l+1))).  Smoother: red-black Gauss-Seidel
-// with the exact row solve.  Restriction: 2x2 average.  Prolongation:
+// with the exact row solve, over-relaxed by omega (default 1.2: red-black
+// SOR smoothing, about 5x the residual reduction per outer iteration of
+// omega = 1 here).  Restriction: 2x2 average.  Prolongation:
 // cell-centred bilinear.  Coarsest level: exact FP64 solve (cosine
 // transform along y, one tridiagonal solve per mode along x, three kernels).
 //
@@ -52,6 +54,7 @@ struct TbLevel {
   float gam;       // outlet ghost factor (level >= 1)
   float dE_last;   // (1 - gamma) * axl
   float inv_int;   // 1 / (2 axl + 2 ayl): inverse diagonal of interior rows
+  float omega;     // over-relaxation of the red-black sweeps (1 = Gauss-Seidel)
 };
 
 // r = b - A p at (i, j) (level operator; level 0 outlet row -> 0)
@@ -190,9 +193,20 @@ __device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, con
       const float N = nb(s_p[on], msk[it][c][3]);
       t[it] = (L.axl * (E + W) + L.ayl * (N + S)) - s_b[own[it][c]];
     }
+    if (L.omega == 1.0f) {
 #pragma unroll
-    for (int it = 0; it < NIT; ++it)
-      if (act[it][c]) s_p[own[it][c]] = inr[it][c] ? t[it] * L.inv_int : t[it] / dg[it][c];
+      for (int it = 0; it < NIT; ++it)
+        if (act[it][c]) s_p[own[it][c]] = inr[it][c] ? t[it] * L.inv_int : t[it] / dg[it][c];
+    } else {
+      // over-relaxed: p + omega (p_GS - p)
+#pragma unroll
+      for (int it = 0; it < NIT; ++it) {
+        if (!act[it][c]) continue;
+        const float gs = inr[it][c] ? t[it] * L.inv_int : t[it] / dg[it][c];
+        const float po = s_p[own[it][c]];
+        s_p[own[it][c]] = po + L.omega * (gs - po);
+      }
+    }
     __syncthreads();
   }
 }
@@ -742,6 +756,7 @@ static TbLevel make_level(const aj_mg_plan* p, int l) {
   L.gam = (float)g;
   L.dE_last = (float)(1.0 - g) * L.axl;
   L.inv_int = 1.0f / (2.0f * L.axl + 2.0f * L.ayl);
+  L.omega = (float)p->omega;
   return L;
 }
 

@@ -61,7 +61,7 @@ class MultigridPlan:
 
     def set_option(self, adaptive: bool | None = None, graph: bool | None = None,
                    guess_test: bool | None = None, relative: bool | None = None,
-                   temporal: bool | None = None):
+                   temporal: bool | None = None, omega: float | None = None):
         """Textbook plans: ``adaptive`` reads convergence between outer
         iterations (host sync, default) instead of enqueueing all
         ``max_cycles``; ``graph`` replays a captured outer iteration
@@ -71,8 +71,12 @@ class MultigridPlan:
         ``relative`` makes the tolerance relative to ||b||; ``temporal``
         starts solves without p0 from the previous FP64 results (previous
         solution, then the linear extrapolation of the last two), setting it
-        clears that history."""
+        clears that history; ``omega`` sets the over-relaxation of the
+        red-black smoother (default 1.2, 1.0 = Gauss-Seidel)."""
         lib = _lib.load()
+        if omega is not None:
+            _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_OMEGA,
+                                            int(round(1000 * float(omega)))), "aj_mg_set_option")
         if temporal is not None:
             _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_TEMPORAL, int(temporal)),
                        "aj_mg_set_option")

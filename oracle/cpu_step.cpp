@@ -11,7 +11,8 @@
 //   predictor BCs, rhs = divergence_nonperiodic(u*, v*) mask / dt
 //     (solver/pure_functions/step.py:182-199, solver/operators.py:81-96)
 //   the textbook multigrid for the outlet-Dirichlet operator: FP32 V(2,2)
-//     cycles (red-black Gauss-Seidel with the exact row solve, 2x2 average
+//     cycles (red-black Gauss-Seidel with the exact row solve over-relaxed
+//     by omega = 1.2, 2x2 average
 //     restriction, cell-centred bilinear prolongation, exact FP64 coarsest
 //     solve by a cosine transform along y and one tridiagonal solve per
 //     mode) inside an FP64 defect correction, 2 V-cycles per outer
@@ -30,6 +31,7 @@ namespace {
 
 constexpr int kCoarseCells = 65536;
 constexpr int kMaxCoarseRows = 1024;
+constexpr float kOmega = 1.2f;  // over-relaxation of the red-black smoother
 
 double outlet_gamma(int level) { return 1.0 - 1.0 / (0.5 - std::ldexp(1.0, -level - 1)); }
 
@@ -98,7 +100,8 @@ void rbgs(const Level& l, float* p, const float* b, int sweeps) {
           const float N = hN ? p[c + 1] : 0.0f;
           const float diag = dE + (hW ? axl : 0.0f) + (hS ? ayl : 0.0f) + (hN ? ayl : 0.0f);
           const float off = axl * (E + W) + ayl * (N + S);
-          p[c] = (off - b[c]) / diag;
+          const float gs = (off - b[c]) / diag;
+          p[c] = p[c] + kOmega * (gs - p[c]);
         }
       }
     }

@@ -298,9 +298,13 @@ def test_textbook_multigrid_one_outer_iteration_parity(gpu, shape, levels):
         assert st["norms"][1] < 1e-3 * st["norms"][0]
         assert st_ref["norms"][1] < 1e-3 * st_ref["norms"][0]
     else:
-        # after two FP32 V-cycles the residual is ~1e-3 of b: FP32 rounding of
-        # p (rel ~1e-7) is amplified by ||A|| ||p|| / ||r||, so compare loosely
-        assert np.isclose(st["norms"][1], st_ref["norms"][1], rtol=2e-2)
+        # after two FP32 V-cycles the residual is 1e-4..1e-3 of b: FP32
+        # rounding of p (rel ~1e-7) is amplified by ||A|| ||p|| / ||r||, so
+        # the two residuals agree only loosely (p itself is checked above)
+        r1, r2 = st["norms"][1], st_ref["norms"][1]
+        print(f"residual after one outer iteration: {r1:.4e} vs oracle {r2:.4e} "
+              f"(of ||b|| {r1 / st['norms'][0]:.2e})", flush=True)
+        assert np.isclose(r1, r2, rtol=0.25) and r1 < 2e-3 * st["norms"][0]
 
 
 @pytest.mark.parametrize("shape", [(512, 96), (2048, 512), (1000, 300)])
