@@ -338,12 +338,14 @@ int conv_bwd(aj_nn* n, int li, const float* gpost, int ldg, const bf16_t* pre, c
 }
 
 // fused = true (inference): pooling and upsampling are read by the
-// convolutions directly instead of being materialised (the backward pass
-// needs them materialised, so training uses fused = false)
+// convolutions directly instead of being materialised, and the
+// pre-activations are not stored (the backward pass needs both, so
+// training uses fused = false)
 int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true,
             bool fused = false) {
   Act& a = n->act;
   const int L = n->L;
+  auto pre = [&](bf16_t* buf) -> bf16_t* { return fused ? nullptr : buf; };
   for (int l = 0; l < L; ++l) {
     const int Hl = H >> l, Wl = W >> l, c = chan(n, l);
     const long P = (long)B * Hl * Wl;
@@ -368,12 +370,12 @@ int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true,
       in = a.POOL[l];
       ldi = cp;
     }
-    if (int rc = conv_fwd(n, enc_a(l), in, ldi, a.A[l], c, a.PA[l], B, Hl, Wl, st, src)) return rc;
+    if (int rc = conv_fwd(n, enc_a(l), in, ldi, a.A[l], c, pre(a.PA[l]), B, Hl, Wl, st, src)) return rc;
     bf16_t* ob;
     int ldo;
     if (l < L - 1) { ob = a.CAT[l] + chan(n, l + 1); ldo = chan(n, l + 1) + c; }
     else { ob = a.Ebot; ldo = c; }
-    if (int rc = conv_fwd(n, enc_b(l), a.A[l], c, ob, ldo, a.PB[l], B, Hl, Wl, st)) return rc;
+    if (int rc = conv_fwd(n, enc_b(l), a.A[l], c, ob, ldo, pre(a.PB[l]), B, Hl, Wl, st)) return rc;
   }
   for (int l = L - 2; l >= 0; --l) {
     const int Hl = H >> l, Wl = W >> l, c = chan(n, l), cu = chan(n, l + 1);
@@ -390,10 +392,10 @@ int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true,
                          cu + c, cu, B, Hl, Wl);
       AJ_LAUNCH_CHECK("k_up");
     }
-    if (int rc = conv_fwd(n, dec_a(n, l), a.CAT[l], cu + c, a.DA[l], c, a.DPA[l], B, Hl, Wl, st,
+    if (int rc = conv_fwd(n, dec_a(n, l), a.CAT[l], cu + c, a.DA[l], c, pre(a.DPA[l]), B, Hl, Wl, st,
                           src))
       return rc;
-    if (int rc = conv_fwd(n, dec_b(n, l), a.DA[l], c, a.DB[l], c, a.DPB[l], B, Hl, Wl, st))
+    if (int rc = conv_fwd(n, dec_b(n, l), a.DA[l], c, a.DB[l], c, pre(a.DPB[l]), B, Hl, Wl, st))
       return rc;
   }
   if (!head) return AJ_OK;
