@@ -63,6 +63,30 @@ struct BatchItem {
   long off, coff;
 };
 
+// Persistent-grid order of the work items: workgroups go to the 8 XCDs
+// round-robin, so with a grid that is a multiple of 8 the workgroups of XCD x
+// take the contiguous eighth x of the items between them.  Neighbouring tiles
+// and rows (which share halo rows and cache lines) are then read through the
+// same L2 instead of one per XCD.  Loop: for (XcdItems q(total); q.t < q.end;
+// q.t += q.step) { item = q.base + q.t; ... }
+struct XcdItems {
+  int base, t, step, end;
+  __device__ __forceinline__ explicit XcdItems(int total) {
+    if ((gridDim.x & 7) == 0) {
+      const int per = (total + 7) >> 3;
+      base = (int)(blockIdx.x & 7) * per;
+      t = (int)(blockIdx.x >> 3);
+      step = (int)(gridDim.x >> 3);
+      end = min(per, total - base);
+    } else {
+      base = 0;
+      t = (int)blockIdx.x;
+      step = (int)gridDim.x;
+      end = total;
+    }
+  }
+};
+
 __device__ __forceinline__ BatchItem batch_item(const BatchWork& w, int item) {
   int g = 0;
 #pragma unroll
@@ -94,7 +118,8 @@ template <typename T, int TX, int TY, int NTH>
 __global__ __launch_bounds__(NTH) void kb_rk(BatchWork w, BatchStepArgs<T> s) {
   extern __shared__ unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g;
     RkArgs<T> a;
@@ -134,7 +159,8 @@ constexpr int kRowsPerItem = 4;
 // nu_sgs = constant Smagorinsky with delta = sqrt(dx dy) (step.py:111-122)
 template <typename T>
 __global__ __launch_bounds__(256) void kb_les(BatchWork w, BatchStepArgs<T> s) {
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
     const int cpr = (ny + 255) / 256;
@@ -162,7 +188,8 @@ struct ObstacleDev {
 template <typename T>
 __global__ __launch_bounds__(256) void kb_geometry(BatchWork w, const ObstacleDev* __restrict__ obs,
                                                    T* __restrict__ sdf, T* __restrict__ mask) {
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
     const int cpr = (ny + 255) / 256;
@@ -187,7 +214,8 @@ __global__ __launch_bounds__(256) void kb_geometry(BatchWork w, const ObstacleDe
 // rhs = div * mask / dt
 template <typename T>
 __global__ __launch_bounds__(256) void kb_div(BatchWork w, BatchStepArgs<T> s) {
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
     const int cpr = (ny + 255) / 256;
@@ -205,7 +233,8 @@ __global__ __launch_bounds__(256) void kb_div(BatchWork w, BatchStepArgs<T> s) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void kb_project(BatchWork w, BatchStepArgs<T> s) {
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
     const int cpr = (ny + 255) / 256;
@@ -237,7 +266,8 @@ struct BatchMgArgs {
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(256) void kb_jacobi(BatchWork w, BatchMgArgs<T> m) {
   __shared__ JacobiLds<T, TX, TY, 2> L;
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g;
     RefJacobiArgs<T> a;
@@ -265,7 +295,8 @@ __global__ __launch_bounds__(256) void kb_jacobi(BatchWork w, BatchMgArgs<T> m) 
 // residual + restriction, rows of 256 coarse cells
 template <typename T>
 __global__ __launch_bounds__(256) void kb_resrestrict(BatchWork w, BatchMgArgs<T> m, T* bc) {
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g;
     if (gated(m.stats + it.inst)) continue;
@@ -289,7 +320,8 @@ __global__ __launch_bounds__(256) void kb_resrestrict(BatchWork w, BatchMgArgs<T
 template <typename T>
 __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(BatchWork w, BatchMgArgs<T> m) {
   extern __shared__ unsigned char smem_raw[];
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g;
     if (gated(m.stats + it.inst)) continue;
@@ -306,7 +338,8 @@ __global__ __launch_bounds__(256) void kb_resnorm(BatchWork w, const T* __restri
                                                   const T* __restrict__ b,
                                                   double* __restrict__ partial) {
   __shared__ double sh[16];
-  for (int item = blockIdx.x; item < w.total; item += gridDim.x) {
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
     const long n = (long)nx * ny;
