@@ -68,11 +68,18 @@ struct JacobiLds {
   T c[CH * CW];
 };
 
-// NS Jacobi sweeps on the tile with origin (x0, y0), 256 threads
+// NS Jacobi sweeps on the tile with origin (x0, y0), 256 threads.  Every
+// pass over a region keeps each thread on one column and steps it over the
+// rows (W columns per pass: 256 / W rows at a time), so a thread's column
+// index, its wall tests and its y-neighbour offsets are fixed for the pass
+// and only the row advances; the update of a cell is the same arithmetic in
+// any thread layout.
 template <typename T, int TX, int TY, int NS>
 __device__ __forceinline__ void ref_jacobi_tile(const RefJacobiArgs<T>& a, int x0, int y0,
                                                 JacobiLds<T, TX, TY, NS>& L) {
-  constexpr int RH = TX + 2 * NS, RW = TY + 2 * NS, RN = RH * RW;
+  constexpr int NT = 256;
+  constexpr int RH = TX + 2 * NS, RW = TY + 2 * NS;
+  static_assert(RW <= NT && TY <= NT, "tile wider than the workgroup");
   auto& s_p = L.p;
   auto& s_b = L.b;
   const int nx = a.nx, ny = a.ny;
@@ -84,49 +91,49 @@ __device__ __forceinline__ void ref_jacobi_tile(const RefJacobiArgs<T>& a, int x
     // prolongation of every region cell reads it there
     using J = JacobiLds<T, TX, TY, NS>;
     const int cI0 = clampi(gx0, 0, nx - 1) >> 1, cJ0 = clampi(gy0, 0, ny - 1) >> 1;
-    constexpr int NC = (J::CH * J::CW + 255) / 256;
+    constexpr int NC = (J::CH * J::CW + NT - 1) / NT;
     T rc[NC];
     if (a.e_c) {
 #pragma unroll
       for (int it = 0; it < NC; ++it) {
-        const int idx = min(tid + it * 256, J::CH * J::CW - 1);
+        const int idx = min(tid + it * NT, J::CH * J::CW - 1);
         const int cx = idx / J::CW, cy = idx - cx * J::CW;
         const int I = min(cI0 + cx, a.ncx - 1), Jc = min(cJ0 + cy, a.ncy - 1);
         rc[it] = a.e_c[(long)I * a.ncy + Jc];
       }
     }
-    constexpr int NIT = (RN + 255) / 256;
-    T rp[NIT], rb[NIT];
+    constexpr int RG = NT / RW;                 // region rows per pass
+    constexpr int NST = (RH + RG - 1) / RG;     // passes
+    const int col = tid % RW, rg = tid / RW;
+    const int gyc = clampi(gy0 + col, 0, ny - 1);
+    T rp[NST], rb[NST];
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = min(tid + it * 256, RN - 1);
-      const int lx = idx / RW, ly = idx - lx * RW;
-      const int gx = clampi(gx0 + lx, 0, nx - 1), gy = clampi(gy0 + ly, 0, ny - 1);
-      const long g = (long)gx * ny + gy;
-      rp[it] = a.p_in ? a.p_in[g] : T(0);
-      rb[it] = (a.b_zero_last && gx == nx - 1) ? T(0) : a.b[g];
+    for (int st = 0; st < NST; ++st) {
+      const int gx = clampi(gx0 + rg + st * RG, 0, nx - 1);
+      const long g = (long)gx * ny + gyc;
+      rp[st] = a.p_in ? a.p_in[g] : T(0);
+      rb[st] = (a.b_zero_last && gx == nx - 1) ? T(0) : a.b[g];
     }
     if (gated(a.gate)) return;
     if (a.e_c) {
 #pragma unroll
       for (int it = 0; it < NC; ++it) {
-        const int idx = tid + it * 256;
+        const int idx = tid + it * NT;
         if (idx < J::CH * J::CW) L.c[idx] = rc[it];
       }
       __syncthreads();
     }
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-      const int idx = tid + it * 256;
-      if (idx < RN) {
-        T pv = rp[it];
+    for (int st = 0; st < NST; ++st) {
+      const int lx = rg + st * RG;
+      if (rg < RG && lx < RH) {
+        T pv = rp[st];
         if (a.e_c) {
-          const int lx = idx / RW, ly = idx - lx * RW;
-          const int gx = clampi(gx0 + lx, 0, nx - 1), gy = clampi(gy0 + ly, 0, ny - 1);
-          pv = pv + prolong_ref_lds(L.c, cI0, cJ0, J::CW, a.ncx, a.ncy, gx, gy);
+          const int gx = clampi(gx0 + lx, 0, nx - 1);
+          pv = pv + prolong_ref_lds(L.c, cI0, cJ0, J::CW, a.ncx, a.ncy, gx, gyc);
         }
-        s_p[0][idx] = pv;
-        s_b[idx] = rb[it];
+        s_p[0][lx * RW + col] = pv;
+        s_b[lx * RW + col] = rb[st];
       }
     }
   }
@@ -135,27 +142,39 @@ __device__ __forceinline__ void ref_jacobi_tile(const RefJacobiArgs<T>& a, int x
 #pragma unroll
   for (int s = 1; s <= NS; ++s) {
     const int m = NS - s;
-    const int W = TY + 2 * m, Hh = TX + 2 * m;
+    const int W = TY + 2 * m, Hh = TX + 2 * m;  // compile-time per unrolled sweep
+    const int RG = NT / W;
     const T* src = s_p[cur];
     T* dst = s_p[cur ^ 1];
-    for (int idx = tid; idx < W * Hh; idx += 256) {
-      const int lx = NS - m + idx / W, ly = NS - m + idx % W;
-      const int gx = gx0 + lx, gy = gy0 + ly;
-      if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-      const int l = lx * RW + ly;
-      const T pW = src[gx > 0 ? l - RW : l], pE = src[gx < nx - 1 ? l + RW : l];
-      const T pS = src[gy > 0 ? l - 1 : l], pN = src[gy < ny - 1 ? l + 1 : l];
-      dst[l] = (a.ax * (pE + pW) + a.ay * (pN + pS) - s_b[l]) * a.inv_diag;
+    const int c2 = tid % W, r2 = tid / W;
+    const int ly = NS - m + c2, gy = gy0 + ly;
+    const bool yok = r2 < RG && gy >= 0 && gy < ny;
+    const int dS = gy > 0 ? 1 : 0, dN = gy < ny - 1 ? 1 : 0;
+    if (yok) {
+      for (int r = r2; r < Hh; r += RG) {
+        const int lx = NS - m + r, gx = gx0 + lx;
+        if (gx < 0 || gx >= nx) continue;
+        const int l = lx * RW + ly;
+        const T pW = src[gx > 0 ? l - RW : l], pE = src[gx < nx - 1 ? l + RW : l];
+        const T pS = src[l - dS], pN = src[l + dN];
+        dst[l] = (a.ax * (pE + pW) + a.ay * (pN + pS) - s_b[l]) * a.inv_diag;
+      }
     }
     __syncthreads();
     cur ^= 1;
   }
-  for (int idx = tid; idx < TX * TY; idx += 256) {
-    const int lx = NS + idx / TY, ly = NS + idx % TY;
-    const int gx = gx0 + lx, gy = gy0 + ly;
-    if (gx >= nx || gy >= ny) continue;
-    const T val = (a.clamp_last && gx == nx - 1) ? T(0) : s_p[cur][lx * RW + ly];
-    a.p_out[(long)gx * ny + gy] = val;
+  {
+    constexpr int RGT = NT / TY;
+    const int ty = tid % TY, rt = tid / TY;
+    const int ly = NS + ty, gy = gy0 + ly;
+    if (rt < RGT && gy < ny) {
+      for (int tx = rt; tx < TX; tx += RGT) {
+        const int lx = NS + tx, gx = gx0 + lx;
+        if (gx >= nx) break;
+        const T val = (a.clamp_last && gx == nx - 1) ? T(0) : s_p[cur][lx * RW + ly];
+        a.p_out[(long)gx * ny + gy] = val;
+      }
+    }
   }
 }
 
