@@ -370,10 +370,41 @@ __device__ __forceinline__ int tb_row(long c, int ny, double rny, int* j) {
   return i;
 }
 
+// Reset of a solve's device state (statistics, live-buffer parity, the
+// tolerance), made by workgroup 0 of each instance of the solve's first
+// launch; every later launch of the solve reads it.
+struct TbBegin {
+  aj_mg_stats* st;
+  int* state;
+  double* dtol;
+  double tol;
+};
+
+__device__ __forceinline__ void tb_begin(const TbBegin& bg) {
+  if (blockIdx.x != 0) return;
+  const int z = blockIdx.z;
+  aj_mg_stats* st = bg.st + z;
+  if (threadIdx.x == 0) {
+    bg.dtol[z] = bg.tol;
+    st->cycles = 0;
+    st->outer = 0;
+    st->converged = 0;
+    st->nonfinite = 0;
+    st->norm = 0.0;
+    st->guess_norm = 0.0;
+    st->zero_norm = 0.0;
+    st->used_guess = 0;
+    bg.state[2 * z] = 0;
+    bg.state[2 * z + 1] = 0;
+  }
+  for (int k = threadIdx.x; k < AJ_MG_MAX_NORMS; k += blockDim.x) st->norms[k] = 0.0;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, const T* __restrict__ rhs,
                                                  double* __restrict__ P, T* __restrict__ bcopy,
-                                                 int nx, int ny) {
+                                                 int nx, int ny, TbBegin bg) {
+  tb_begin(bg);
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= (long)nx * ny) return;
   int j;
@@ -394,7 +425,9 @@ __global__ __launch_bounds__(256) void k_tb_init_temporal(const double* __restri
                                                           const double* __restrict__ Hp,
                                                           int order, const T* __restrict__ rhs,
                                                           double* __restrict__ P,
-                                                          T* __restrict__ bcopy, int nx, int ny) {
+                                                          T* __restrict__ bcopy, int nx, int ny,
+                                                          TbBegin bg) {
+  tb_begin(bg);
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= (long)nx * ny) return;
   int j;
@@ -517,6 +550,48 @@ __global__ __launch_bounds__(256) void k_tb_resid2(const double* __restrict__ sr
   if (in) {
     *reinterpret_cast<float2*>(r32 + c) = make_float2((float)r0, (float)r1);
     if (e) *reinterpret_cast<double2*>(dst + c) = make_double2(p0, p1);
+  }
+  const double t = block_sum(r0 * r0 + r1 * r1, sh);
+  if (threadIdx.x == 0) partial[blk] = t;
+}
+
+// The start of a solve without an initial guess, ny even, in one launch:
+// the state reset, the FP64 iterate P = 0, the copy of the rhs, and the
+// initial residual r = b - A 0 = b (zero on the Dirichlet row) with its
+// partial sums -- what k_tb_init followed by k_tb_resid2 on the zero
+// iterate write, in the same cells, blocks and summation order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tb_start0(const T* __restrict__ rhs,
+                                                   double* __restrict__ P,
+                                                   T* __restrict__ bcopy, float* __restrict__ r32,
+                                                   double* __restrict__ partial, int nx, int ny,
+                                                   double rny, TbBegin bg) {
+  __shared__ double sh[16];
+  tb_begin(bg);
+  const long n = (long)nx * ny;
+  {
+    const long z = blockIdx.z;
+    rhs += z * n;
+    P += z * n;
+    bcopy += z * n;
+    r32 += z * n;
+    partial += z * gridDim.x;
+  }
+  const long blk = xcd_block(blockIdx.x, gridDim.x);
+  const long c = (blk * 256 + threadIdx.x) * 2;
+  double r0 = 0.0, r1 = 0.0;
+  if (c < n) {
+    int j = 0;
+    const int i = tb_row(c, ny, rny, &j);
+    const T b0 = rhs[c], b1 = rhs[c + 1];
+    if (i < nx - 1) {
+      r0 = (double)b0;
+      r1 = (double)b1;
+    }
+    *reinterpret_cast<double2*>(P + c) = make_double2(0.0, 0.0);
+    bcopy[c] = b0;
+    bcopy[c + 1] = b1;
+    *reinterpret_cast<float2*>(r32 + c) = make_float2((float)r0, (float)r1);
   }
   const double t = block_sum(r0 * r0 + r1 * r1, sh);
   if (threadIdx.x == 0) partial[blk] = t;
@@ -716,28 +791,6 @@ __global__ __launch_bounds__(256) void k_tb_out(const double* __restrict__ P0,
   if (H) H[q] = r;
 }
 
-__global__ void k_tb_begin(aj_mg_stats* st, int* state, double* dtol, double tol) {
-  {
-    const int z = blockIdx.x;  // one workgroup per instance
-    st += z;
-    state += 2 * z;
-    dtol += z;
-  }
-  if (threadIdx.x == 0) {
-    dtol[0] = tol;
-    st->cycles = 0;
-    st->outer = 0;
-    st->converged = 0;
-    st->nonfinite = 0;
-    st->norm = 0.0;
-    st->guess_norm = 0.0;
-    st->zero_norm = 0.0;
-    st->used_guess = 0;
-    state[0] = 0;
-    state[1] = 0;
-  }
-  for (int k = threadIdx.x; k < AJ_MG_MAX_NORMS; k += blockDim.x) st->norms[k] = 0.0;
-}
 
 // ---------------------------------------------------------------------------
 // host side
@@ -1098,16 +1151,27 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
   }
   p->hist_valid = p->temporal;
   const bool temporal_guess = p->temporal && !p0 && p->hist_count > 0;
-  hipLaunchKernelGGL(k_tb_begin, dim3(B), dim3(64), 0, st, p->stats, p->state, p->dtol, tol);
-  AJ_LAUNCH_CHECK("k_tb_begin");
+  const TbBegin bg{p->stats, p->state, p->dtol, tol};
+  if (!p0 && !temporal_guess && ny % 2 == 0) {
+    // zero guess: reset, P = 0, rhs copy and the initial residual in one launch
+    const int np = (int)cdiv(n, 512);
+    hipLaunchKernelGGL(k_tb_start0<T>, dim3(np, 1, B), dim3(256), 0, st, rhs, p->p64[0],
+                       (T*)p->bcopy, p->tB[0], p->partial, nx, ny, 1.0 / ny, bg);
+    AJ_LAUNCH_CHECK("k_tb_start0");
+    hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, np,
+                       p->dx * p->dy, p->dtol, 0, 0, p->relative, p->stats, p->state,
+                       p->host_stats);
+    AJ_LAUNCH_CHECK("k_tb_norm_final");
+    return AJ_OK;
+  }
   if (temporal_guess)
     hipLaunchKernelGGL(k_tb_init_temporal<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st,
                        (const double*)p->hist[p->hist_last],
                        (const double*)p->hist[(p->hist_last + 1) & 1], p->hist_count, rhs,
-                       p->p64[0], (T*)p->bcopy, nx, ny);
+                       p->p64[0], (T*)p->bcopy, nx, ny, bg);
   else
     hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st, p0, rhs,
-                       p->p64[0], (T*)p->bcopy, nx, ny);
+                       p->p64[0], (T*)p->bcopy, nx, ny, bg);
   AJ_LAUNCH_CHECK("k_tb_init");
   // outer iteration 0: residual of the initial guess
   if (p0 || temporal_guess) {
