@@ -31,6 +31,7 @@ AJ_MG_OPT_GUESS_TEST = 2
 AJ_MG_OPT_RELATIVE = 3
 AJ_MG_OPT_TEMPORAL = 4
 AJ_MG_OPT_OMEGA = 5
+AJ_MG_OPT_CYCLES = 6
 
 # Every symbol the header declares; tests check the library exports them.
 EXPORTS = (

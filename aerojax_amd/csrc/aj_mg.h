@@ -49,6 +49,7 @@ struct aj_mg_plan {
   int guess_test = 1;               // textbook: keep p0 only if ||r(p0)|| <= ||b||
   int relative = 0;                 // textbook: tolerance relative to ||b||
   double omega = 1.2;               // textbook: over-relaxation of the red-black smoother
+  int cycles_per_outer = 2;         // textbook: V-cycles per FP64 defect-correction iteration
   // textbook: temporal initial guess from the previous solves' FP64 results
   int temporal = 0;                 // option on
   int hist_valid = 0;               // the last solve's iterate is still in p64[]

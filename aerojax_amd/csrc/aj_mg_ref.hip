@@ -383,6 +383,25 @@ extern "C" int aj_mg_set_option(aj_mg_plan* p, int option, int value) {
       p->omega = value / 1000.0;
       return AJ_OK;
     }
+    case AJ_MG_OPT_CYCLES: {
+      if (p->variant != AJ_MG_TEXTBOOK)
+        return fail(AJ_ERR_UNSUPPORTED, "aj_mg_set_option: cycles per outer iteration apply to textbook plans");
+      if (value < 1 || value > 8)
+        return fail(AJ_ERR_INVALID, "aj_mg_set_option: V-cycles per outer iteration must be in 1..8");
+      if (p->pend_active)
+        return fail(AJ_ERR_INVALID, "aj_mg_set_option: a deferred solve is open");
+      // the captured outer iterations hold the old cycle count: capture again
+      AJ_HIP(hipDeviceSynchronize());
+      for (int k = 0; k < 2; ++k) {
+        if (p->graph_exec[k]) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec[k]);
+        if (p->graph[k]) (void)hipGraphDestroy((hipGraph_t)p->graph[k]);
+        p->graph_exec[k] = p->graph[k] = nullptr;
+      }
+      // the predicted iteration count of the next solve keeps its V-cycles
+      p->last_outer = std::max(1, (p->last_outer * p->cycles_per_outer + value - 1) / value);
+      p->cycles_per_outer = value;
+      return AJ_OK;
+    }
     case AJ_MG_OPT_TEMPORAL: {
       if (p->variant != AJ_MG_TEXTBOOK)
         return fail(AJ_ERR_UNSUPPORTED, "aj_mg_set_option: the temporal guess needs a textbook plan");

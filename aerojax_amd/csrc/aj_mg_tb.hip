@@ -41,7 +41,6 @@ namespace aj {
 
 constexpr int kTbCoarseCells = 65536;  // coarsening stops at or below this many cells
 constexpr int kMaxLevels = 24;
-constexpr int kCyclesPerOuter = 2;
 
 static double outlet_gamma(int level) {
   return 1.0 - 1.0 / (0.5 - std::ldexp(1.0, -level - 1));
@@ -1079,12 +1078,12 @@ static int tb_vcycle(aj_mg_plan* p, int lev, bool warm, hipStream_t st) {
 // tE[0]; a single-level hierarchy is one exact solve
 static int tb_cycles(aj_mg_plan* p, hipStream_t st) {
   if (p->ml == 0) return tb_coarse(p, st);
-  for (int c = 0; c < kCyclesPerOuter; ++c)
+  for (int c = 0; c < p->cycles_per_outer; ++c)
     if (int rc = tb_vcycle(p, 0, c > 0, st)) return rc;
   return AJ_OK;
 }
 
-static int tb_cycles_per_outer(const aj_mg_plan* p) { return p->ml == 0 ? 1 : kCyclesPerOuter; }
+static int tb_cycles_per_outer(const aj_mg_plan* p) { return p->ml == 0 ? 1 : p->cycles_per_outer; }
 
 // launches of one defect-correction outer iteration (2 V-cycles, FP64
 // residual with the update p64 += e, norm and gate); only plan-owned

@@ -61,7 +61,8 @@ class MultigridPlan:
 
     def set_option(self, adaptive: bool | None = None, graph: bool | None = None,
                    guess_test: bool | None = None, relative: bool | None = None,
-                   temporal: bool | None = None, omega: float | None = None):
+                   temporal: bool | None = None, omega: float | None = None,
+                   cycles_per_outer: int | None = None):
         """Textbook plans: ``adaptive`` reads convergence between outer
         iterations (host sync, default) instead of enqueueing all
         ``max_cycles``; ``graph`` replays a captured outer iteration
@@ -72,8 +73,13 @@ class MultigridPlan:
         starts solves without p0 from the previous FP64 results (previous
         solution, then the linear extrapolation of the last two), setting it
         clears that history; ``omega`` sets the over-relaxation of the
-        red-black smoother (default 1.2, 1.0 = Gauss-Seidel)."""
+        red-black smoother (default 1.2, 1.0 = Gauss-Seidel); ``cycles_per_outer``
+        sets the FP32 V-cycles per FP64 defect-correction iteration (default
+        2, 1..8)."""
         lib = _lib.load()
+        if cycles_per_outer is not None:
+            _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_CYCLES,
+                                            int(cycles_per_outer)), "aj_mg_set_option")
         if omega is not None:
             _lib.check(lib.aj_mg_set_option(self.handle, _lib.AJ_MG_OPT_OMEGA,
                                             int(round(1000 * float(omega)))), "aj_mg_set_option")

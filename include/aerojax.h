@@ -231,9 +231,15 @@ int aj_mg_num_levels(const aj_mg_plan* plan);
  *     red-black smoother in thousandths (1000 = plain Gauss-Seidel; 1200 =
  *     red-black SOR, about 5x the residual reduction per outer iteration at
  *     2048x512), 500..1900.  Changing it synchronises the device and drops
- *     the captured outer iterations. */
+ *     the captured outer iterations.
+ *   AJ_MG_OPT_CYCLES = 2 (default): FP32 V-cycles per FP64
+ *     defect-correction iteration, 1..8 (convergence is tested after each
+ *     iteration; fewer, longer iterations spend fewer residual launches, more
+ *     V-cycles may run past the tolerance).  Changing it synchronises the
+ *     device and drops the captured outer iterations. */
 enum { AJ_MG_OPT_ADAPTIVE = 0, AJ_MG_OPT_GRAPH = 1, AJ_MG_OPT_GUESS_TEST = 2,
-       AJ_MG_OPT_RELATIVE = 3, AJ_MG_OPT_TEMPORAL = 4, AJ_MG_OPT_OMEGA = 5 };
+       AJ_MG_OPT_RELATIVE = 3, AJ_MG_OPT_TEMPORAL = 4, AJ_MG_OPT_OMEGA = 5,
+       AJ_MG_OPT_CYCLES = 6 };
 int aj_mg_set_option(aj_mg_plan* plan, int option, int value);
 
 /* ---- Launch-shape autotuner ----------------------------------------------

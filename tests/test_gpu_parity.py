@@ -686,6 +686,37 @@ def test_batched_graph_replays_match_eager(gpu):
     assert torch.equal(a.u, e.u) and torch.equal(a.v, e.v) and torch.equal(a.p, e.p)
 
 
+@pytest.mark.parametrize("cpo", [1, 3])
+def test_textbook_cycles_per_outer_iteration(gpu, cpo):
+    """AJ_MG_OPT_CYCLES: V-cycles per FP64 defect-correction iteration.  The
+    solve reaches the tolerance, counts cpo V-cycles per outer iteration, and
+    its first outer iteration equals cpo V-cycles of the oracle's FP32
+    multigrid on the same rhs (the check of one outer iteration with 2)."""
+    import aerojax_amd as aj
+    nx, ny = 512, 128
+    dx, dy = 4.0 / nx, 1.0 / ny
+    rng = np.random.default_rng(12)
+    b = rng.normal(size=(nx, ny)) * 300
+    pl = aj.MultigridPlan(nx, ny, dx, dy, torch.float64, "textbook", 2)
+    pl.set_option(cycles_per_outer=cpo)
+    p = torch.empty((nx, ny), dtype=torch.float64, device=gpu)
+    pl.solve(dev(b, torch.float64), p, None, 40, 1e-6)
+    st = pl.stats()
+    print(f"cycles per outer {cpo}: cycles {st['cycles']} outer {st['outer']} "
+          f"norms {['%.2e' % x for x in st['norms'][:st['outer'] + 1]]}", flush=True)
+    assert st["converged"] and st["norm"] <= 1e-6
+    assert st["cycles"] == cpo * st["outer"]
+    pl.solve(dev(b, torch.float64), p, None, cpo, 0.0)
+    st1 = pl.stats()
+    mg = T.TextbookMG(nx, ny, dx, dy, levels=2)
+    ref, st_ref = mg.solve(b, tol=0.0, max_outer=1, cycles_per_outer=cpo)
+    assert st1["cycles"] == cpo == st_ref["cycles"] and st1["outer"] == 1
+    check(host(p), ref, None, torch.float32, f"mg-textbook {cpo} V-cycles")
+    with pytest.raises(RuntimeError):
+        pl.set_option(cycles_per_outer=9)
+    pl.close()
+
+
 def test_textbook_relative_tolerance(gpu):
     """AJ_MG_OPT_RELATIVE: ||b - A p|| <= tol ||b||, with and without a guess."""
     import aerojax_amd as aj
