@@ -415,28 +415,9 @@ __global__ __launch_bounds__(256) void k_tb_init(const T* __restrict__ p0, const
 
 // Temporal initial guess (AJ_MG_OPT_TEMPORAL): k_tb_out also writes the
 // solve's FP64 result into the history ring (every output of a solve that
-// is topped up rewrites the same slot); the next solve starts from it.
-// The guess from the history: H_last (order 1) or the linear extrapolation
-// 2 H_last - H_prev (order 2), zero on the Dirichlet row; the FP64 copy of
-// the rhs as in k_tb_init.
-template <typename T>
-__global__ __launch_bounds__(256) void k_tb_init_temporal(const double* __restrict__ Hl,
-                                                          const double* __restrict__ Hp,
-                                                          int order, const T* __restrict__ rhs,
-                                                          double* __restrict__ P,
-                                                          T* __restrict__ bcopy, int nx, int ny,
-                                                          TbBegin bg) {
-  tb_begin(bg);
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
-  if (c >= (long)nx * ny) return;
-  int j;
-  const int i = tb_row(c, ny, 1.0 / ny, &j);
-  const long q = (long)blockIdx.z * nx * ny + c;
-  const double hl = Hl[q];
-  const double g = order >= 2 ? 2.0 * hl - Hp[q] : hl;
-  P[q] = i < nx - 1 ? g : 0.0;
-  bcopy[q] = rhs[q];
-}
+// is topped up rewrites the same slot); the next solve starts from the last
+// one (order 1) or the linear extrapolation 2 H_last - H_prev (order 2),
+// formed by k_tb_start_guess.
 
 // p' = P[cur] + e (e may be null), r = b - A p' in FP64 on rows < nx-1
 // (blocks renumbered by xcd_block).
@@ -619,40 +600,63 @@ static int tb_resid_launch(const aj_mg_plan* p, const double* src, double* dst, 
   return nb;
 }
 
-// Initial guess test: partial sums of r(p0)^2 (r32 <- r(p0)) and of the
-// zero-guess residual b^2, rows < nx-1.
-template <typename T>
-__global__ __launch_bounds__(256) void k_tb_resid_guess(const double* __restrict__ P,
-                                                        const T* __restrict__ b,
-                                                        float* __restrict__ r32,
-                                                        double* __restrict__ part_g,
-                                                        double* __restrict__ part_z, int nx, int ny,
-                                                        double ax, double ay) {
+// The start of a solve with an initial guess in one launch: the state
+// reset, the FP64 iterate P = guess (p0, or from the history when HIST;
+// zero on the Dirichlet row), the copy of the rhs, r32 = r(guess) on the
+// equation rows (rows < nx-1) and the partial sums of the guess test:
+// r(guess)^2 and of the zero guess's residual b^2.  Neighbours' guess
+// values are formed where they are read.
+template <typename T, bool HIST>
+__global__ __launch_bounds__(256) void k_tb_start_guess(
+    const T* __restrict__ p0, const double* __restrict__ Hl, const double* __restrict__ Hp,
+    int order, const T* __restrict__ rhs, double* __restrict__ P, T* __restrict__ bcopy,
+    float* __restrict__ r32, double* __restrict__ part_g, double* __restrict__ part_z, int nx,
+    int ny, double ax, double ay, TbBegin bg) {
   __shared__ double sh[16];
+  tb_begin(bg);
   const long n = (long)nx * ny;
   {
     const long z = blockIdx.z;
+    if (HIST) {
+      Hl += z * n;
+      Hp += z * n;
+    } else {
+      p0 += z * n;
+    }
+    rhs += z * n;
     P += z * n;
-    b += z * n;
+    bcopy += z * n;
     r32 += z * n;
     part_g += z * gridDim.x;
     part_z += z * gridDim.x;
   }
+  // the guess at cell q of an equation row
+  auto G = [&](long q) -> double {
+    if (HIST) {
+      const double hl = Hl[q];
+      return order >= 2 ? 2.0 * hl - Hp[q] : hl;
+    }
+    return (double)p0[q];
+  };
   double ag = 0.0, az = 0.0;
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c < n) {
     int j;
     const int i = tb_row(c, ny, 1.0 / ny, &j);
+    const T bt = rhs[c];
+    bcopy[c] = bt;
     if (i == nx - 1) {
+      P[c] = 0.0;
       r32[c] = 0.0f;
     } else {
-      const double pc = P[c];
-      const double W = i > 0 ? P[c - ny] : pc;
-      const double E = (i + 1 < nx - 1) ? P[c + ny] : 0.0;
-      const double S = j > 0 ? P[c - 1] : pc;
-      const double N = j < ny - 1 ? P[c + 1] : pc;
+      const double pc = G(c);
+      P[c] = pc;
+      const double W = i > 0 ? G(c - ny) : pc;
+      const double E = (i + 1 < nx - 1) ? G(c + ny) : 0.0;
+      const double S = j > 0 ? G(c - 1) : pc;
+      const double N = j < ny - 1 ? G(c + 1) : pc;
       const double Ap = ax * ((E - pc) + (W - pc)) + ay * ((N - pc) + (S - pc));
-      const double bv = (double)b[c];
+      const double bv = (double)bt;
       const double r = bv - Ap;
       r32[c] = (float)r;
       ag = r * r;
@@ -1163,22 +1167,20 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
     AJ_LAUNCH_CHECK("k_tb_norm_final");
     return AJ_OK;
   }
-  if (temporal_guess)
-    hipLaunchKernelGGL(k_tb_init_temporal<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st,
-                       (const double*)p->hist[p->hist_last],
-                       (const double*)p->hist[(p->hist_last + 1) & 1], p->hist_count, rhs,
-                       p->p64[0], (T*)p->bcopy, nx, ny, bg);
-  else
-    hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st, p0, rhs,
-                       p->p64[0], (T*)p->bcopy, nx, ny, bg);
-  AJ_LAUNCH_CHECK("k_tb_init");
-  // outer iteration 0: residual of the initial guess
   if (p0 || temporal_guess) {
-    // a guess is kept only if its residual is not above the zero guess's
-    hipLaunchKernelGGL(k_tb_resid_guess<T>, dim3(nb, 1, B), dim3(256), 0, st,
-                       (const double*)p->p64[0], (const T*)p->bcopy, p->tB[0], p->partial,
-                       p->partial2, nx, ny, ax, ay);
-    AJ_LAUNCH_CHECK("k_tb_resid_guess");
+    // reset, P = guess, rhs copy and the guess test's sums in one launch; a
+    // guess is kept only if its residual is not above the zero guess's
+    if (temporal_guess)
+      hipLaunchKernelGGL((k_tb_start_guess<T, true>), dim3(nb, 1, B), dim3(256), 0, st,
+                         (const T*)nullptr, (const double*)p->hist[p->hist_last],
+                         (const double*)p->hist[(p->hist_last + 1) & 1], p->hist_count, rhs,
+                         p->p64[0], (T*)p->bcopy, p->tB[0], p->partial, p->partial2, nx, ny, ax,
+                         ay, bg);
+    else
+      hipLaunchKernelGGL((k_tb_start_guess<T, false>), dim3(nb, 1, B), dim3(256), 0, st, p0,
+                         (const double*)nullptr, (const double*)nullptr, 0, rhs, p->p64[0],
+                         (T*)p->bcopy, p->tB[0], p->partial, p->partial2, nx, ny, ax, ay, bg);
+    AJ_LAUNCH_CHECK("k_tb_start_guess");
     hipLaunchKernelGGL(k_tb_guess_final, dim3(B), dim3(256), 0, st, (const double*)p->partial,
                        (const double*)p->partial2, nb, p->dx * p->dy, p->guess_test,
                        p->relative, p->dtol, p->stats, p->state, p->host_stats);
@@ -1187,6 +1189,11 @@ static int tb_start(aj_mg_plan* p, const T* rhs, const T* p0, double tol, hipStr
                        (const T*)p->bcopy, p->tB[0], (const int*)p->state, nx, ny);
     AJ_LAUNCH_CHECK("k_tb_fallback");
   } else {
+    // zero guess with ny odd: reset, P = 0 and the rhs copy, then the
+    // one-cell-per-thread residual
+    hipLaunchKernelGGL(k_tb_init<T>, dim3(cdiv(n, 256), 1, B), dim3(256), 0, st, (const T*)nullptr,
+                       rhs, p->p64[0], (T*)p->bcopy, nx, ny, bg);
+    AJ_LAUNCH_CHECK("k_tb_init");
     const int np = tb_resid_launch<T>(p, (const double*)p->p64[0], p->p64[1], nullptr, st);
     AJ_LAUNCH_CHECK("k_tb_resid");
     hipLaunchKernelGGL(k_tb_norm_final, dim3(B), dim3(256), 0, st, (const double*)p->partial, np,
