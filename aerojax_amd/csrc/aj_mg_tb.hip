@@ -639,7 +639,10 @@ __global__ __launch_bounds__(256) void k_tb_start_guess(
     return (double)p0[q];
   };
   double ag = 0.0, az = 0.0;
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  // blocks numbered so that each XCD works through a contiguous band of
+  // rows (the rows above and below a block are then in the same L2)
+  const long blk = xcd_block(blockIdx.x, gridDim.x);
+  const long c = blk * 256 + threadIdx.x;
   if (c < n) {
     int j;
     const int i = tb_row(c, ny, 1.0 / ny, &j);
@@ -666,8 +669,8 @@ __global__ __launch_bounds__(256) void k_tb_start_guess(
   const double tg = block_sum(ag, sh);
   const double tz = block_sum(az, sh);
   if (threadIdx.x == 0) {
-    part_g[blockIdx.x] = tg;
-    part_z[blockIdx.x] = tz;
+    part_g[blk] = tg;
+    part_z[blk] = tz;
   }
 }
 
