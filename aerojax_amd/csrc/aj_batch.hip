@@ -332,7 +332,9 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(BatchWork w, BatchMg
   }
 }
 
-// sum(r^2) partials: kResChunks items per instance
+// sum(r^2) partials: kResChunks items per instance, item k taking the
+// contiguous rows [k nx / kResChunks, (k + 1) nx / kResChunks); a thread
+// steps its (row, column) by 256 cells without a division in the loop
 template <typename T>
 __global__ __launch_bounds__(256) void kb_resnorm(BatchWork w, const T* __restrict__ p,
                                                   const T* __restrict__ b,
@@ -342,13 +344,22 @@ __global__ __launch_bounds__(256) void kb_resnorm(BatchWork w, const T* __restri
     const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
-    const long n = (long)nx * ny;
+    const int r0 = (int)((long)it.sub * nx / kResChunks);
+    const int r1 = (int)((long)(it.sub + 1) * nx / kResChunks);
     const T ax = T(1) / (T(w.dx[g]) * T(w.dx[g])), ay = T(1) / (T(w.dy[g]) * T(w.dy[g]));
+    const int di = 256 / ny, dj = 256 - di * ny;
+    int i = r0 + (int)threadIdx.x / ny, j = (int)threadIdx.x % ny;
     double acc = 0.0;
-    for (long c = (long)it.sub * 256 + threadIdx.x; c < n; c += (long)kResChunks * 256) {
-      const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+#pragma unroll 4
+    for (; i < r1;) {
       const double r = (double)ref_residual(p + it.off, b + it.off, 1, nx, ny, ax, ay, i, j);
       acc += r * r;
+      i += di;
+      j += dj;
+      if (j >= ny) {
+        j -= ny;
+        ++i;
+      }
     }
     const double t = block_sum(acc, sh);
     if (threadIdx.x == 0) partial[(long)it.inst * kResChunks + it.sub] = t;

@@ -56,12 +56,20 @@ __global__ __launch_bounds__(256) void k_ref_resnorm(const T* __restrict__ p, co
                                                      int nx, int ny, T ax, T ay,
                                                      double* __restrict__ partial) {
   __shared__ double sh[16];
-  const long n = (long)nx * ny;
+  // grid-stride over the cells; (row, column) stepped without a division
+  const int c0 = (int)(blockIdx.x * 256 + threadIdx.x), stride = (int)(gridDim.x * 256);
+  const int di = stride / ny, dj = stride - di * ny;
+  int i = c0 / ny, j = c0 - (c0 / ny) * ny;
   double acc = 0.0;
-  for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long)gridDim.x * 256) {
-    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+  for (; i < nx;) {
     const double r = (double)ref_residual(p, b, 1, nx, ny, ax, ay, i, j);
     acc += r * r;
+    i += di;
+    j += dj;
+    if (j >= ny) {
+      j -= ny;
+      ++i;
+    }
   }
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
