@@ -41,20 +41,21 @@ __device__ __forceinline__ T prolong_ref(const T* __restrict__ c, int ncx, int n
 }
 
 // the same prolongation from a coarse patch in LDS whose (0, 0) is coarse
-// cell (cI0, cJ0), row length CW
+// cell (cI0, cJ0), row length CW.  All four corners are read and the case
+// is selected, so the lanes of a wave (alternating parities) do not diverge;
+// each case adds the same values in the same order as prolong_ref.  The
+// patch holds the cells (I + 1, J + 1) of every cell of the region it serves.
 template <typename T>
 __device__ __forceinline__ T prolong_ref_lds(const T* __restrict__ c, int cI0, int cJ0, int CW,
                                              int ncx, int ncy, int i, int j) {
   const int nxf = 2 * ncx, nyf = 2 * ncy;
-  if (i == nxf - 1 || j == nyf - 1) return T(0);
   const int I = i >> 1, J = j >> 1;
   const bool io = i & 1, jo = j & 1;
   const int l = (I - cI0) * CW + (J - cJ0);
-  const T c00 = c[l];
-  if (!io && !jo) return c00;
-  if (io && !jo) return T(0.5) * (c00 + c[l + CW]);
-  if (!io && jo) return T(0.5) * (c00 + c[l + 1]);
-  return T(0.25) * (((c00 + c[l + CW]) + c[l + 1]) + c[l + CW + 1]);
+  const T c00 = c[l], c10 = c[l + CW], c01 = c[l + 1], c11 = c[l + CW + 1];
+  const T s1 = c00 + (io ? c10 : c01);
+  const T r = (io && jo) ? T(0.25) * ((s1 + c01) + c11) : (io || jo) ? T(0.5) * s1 : c00;
+  return (i == nxf - 1 || j == nyf - 1) ? T(0) : r;
 }
 
 // NS Jacobi sweeps (smooth, multigrid_solver.py:158-171) on one tile; `c`
