@@ -41,7 +41,7 @@ constexpr int kBatchRkNTH = 512;
 template <typename T> struct BatchRkTile;  // RK tile: 16x64 (FP32) / 8x64 (FP64) keeps LDS < 64 KiB
 template <> struct BatchRkTile<float> { static constexpr int TX = 16, TY = 64; };
 template <> struct BatchRkTile<double> { static constexpr int TX = 8, TY = 64; };
-constexpr int kBatchJX = 16, kBatchJY = 64;  // Jacobi tile
+constexpr int kBatchJX = 16, kBatchJY = 64;  // Jacobi tile (rows of >= 64 cells)
 constexpr int kResChunks = 16;  // residual-norm partial sums per instance
 constexpr int kWgCoarse = 2;    // coarsest solve: 1024-thread workgroups per CU
 
@@ -292,7 +292,10 @@ __global__ __launch_bounds__(256) void kb_jacobi(BatchWork w, BatchMgArgs<T> m) 
   }
 }
 
-// residual + restriction, rows of 256 coarse cells
+// residual + restriction: kCoarseCellsPerItem consecutive coarse cells
+// (row-major) per item, 4 per thread, so that every lane has a cell whatever
+// the coarse row length; a thread steps its (row, column) by 256 cells
+constexpr int kCoarseCellsPerItem = 1024;
 template <typename T>
 __global__ __launch_bounds__(256) void kb_resrestrict(BatchWork w, BatchMgArgs<T> m, T* bc) {
   for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
@@ -301,17 +304,22 @@ __global__ __launch_bounds__(256) void kb_resrestrict(BatchWork w, BatchMgArgs<T
     const int g = it.g;
     if (gated(m.stats + it.inst)) continue;
     const int ncx = w.cnx[g], ncy = w.cny[g];
-    const int cpr = (ncy + 255) / 256;
-    const int rb = it.sub / cpr, J = (it.sub - rb * cpr) * 256 + threadIdx.x;
-    if (J >= ncy) continue;
     const T ax = T(1) / (T(w.dx[g]) * T(w.dx[g])), ay = T(1) / (T(w.dy[g]) * T(w.dy[g]));
     const int degenerate = (w.nx[g] / 2 < 2 || w.ny[g] / 2 < 2) ? 1 : 0;
+    const int c0 = it.sub * kCoarseCellsPerItem + (int)threadIdx.x;
+    const int di = 256 / ncy, dj = 256 - di * ncy;
+    int I = c0 / ncy, J = c0 - (c0 / ncy) * ncy;
 #pragma unroll
-    for (int q = 0; q < kRowsPerItem; ++q) {
-      const int I = rb * kRowsPerItem + q;
+    for (int k = 0; k < kCoarseCellsPerItem / 256; ++k) {
       if (I < ncx)
         ref_resrestrict_cell(m.p_out + it.off, m.b + it.off, m.bzl, w.nx[g], w.ny[g], ax, ay,
                              bc + it.coff, ncy, degenerate, I, J);
+      I += di;
+      J += dj;
+      if (J >= ncy) {
+        J -= ncy;
+        ++I;
+      }
     }
   }
 }
@@ -442,9 +450,9 @@ struct aj_batch {
 
 namespace {
 
-// kind: 0 = tiles tx x ty, 1 = row items (this level), 2 = row items of
-// the next coarser level, 3 = one item per instance, 4 = kResChunks per
-// instance
+// kind: 0 = tiles tx x ty, 1 = row items (this level), 2 = blocks of
+// kCoarseCellsPerItem cells of the next coarser level, 3 = one item per
+// instance, 4 = kResChunks per instance
 BatchWork make_work(const aj_batch* b, int level, int kind, int tx = 16, int ty = 64) {
   BatchWork w;
   std::memset(&w, 0, sizeof(w));
@@ -457,7 +465,7 @@ BatchWork make_work(const aj_batch* b, int level, int kind, int tx = 16, int ty 
       case 0: per = (int)(cdiv(nx, tx) * cdiv(ny, ty)); break;
       case 1: per = (int)cdiv(nx, kRowsPerItem) * (int)cdiv(ny, 256); break;
       case 2:
-        per = (int)cdiv(b->lnx[g][level + 1], kRowsPerItem) * (int)cdiv(b->lny[g][level + 1], 256);
+        per = (int)cdiv((long)b->lnx[g][level + 1] * b->lny[g][level + 1], kCoarseCellsPerItem);
         break;
       case 3: per = 1; break;
       default: per = kResChunks; break;
@@ -504,6 +512,32 @@ int batch_launch(const aj_batch* b, const char* name, int total, F&& launch, hip
   return tune_pick(key, 4, L, st, &pick);
 }
 
+template <int TX_, int TY_>
+struct JacobiTile {
+  static constexpr int TX = TX_, TY = TY_;
+};
+
+// Jacobi launch of one level: 16x64 tiles while every instance's row length
+// at the level is at least 64, else 32x32, else 64x16 (a tile wider than the
+// grid leaves lanes without cells)
+template <typename T>
+int batch_jacobi(aj_batch* b, int lev, const char* name, const BatchMgArgs<T>& m, hipStream_t st) {
+  int nymin = b->lny[0][lev];
+  for (int g = 1; g < b->ng; ++g) nymin = std::min(nymin, b->lny[g][lev]);
+  auto go = [&](auto tile) -> int {
+    constexpr int TX = decltype(tile)::TX, TY = decltype(tile)::TY;
+    const BatchWork wt = make_work(b, lev, 0, TX, TY);
+    return batch_launch(b, name, wt.total, [&](dim3 g, hipStream_t s) -> int {
+      hipLaunchKernelGGL((kb_jacobi<T, TX, TY>), g, dim3(256), 0, s, wt, m);
+      AJ_LAUNCH_CHECK("kb_jacobi");
+      return AJ_OK;
+    }, st);
+  };
+  if (nymin >= 64) return go(JacobiTile<kBatchJX, kBatchJY>{});
+  if (nymin >= 32) return go(JacobiTile<32, 32>{});
+  return go(JacobiTile<64, 16>{});
+}
+
 template <typename T>
 int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, int bzl,
                  hipStream_t st) {
@@ -527,7 +561,6 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
       return AJ_OK;
     }
     // larger coarsest grids: 5 fused pairs of Jacobi sweeps, ping-pong
-    BatchWork wt = make_work(b, lev, 0, kBatchJX, kBatchJY);
     T* other = (T*)b->Y[lev];
     const T* src = nullptr;
     for (int k = 0; k < 5; ++k) {
@@ -537,34 +570,23 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
       m.b = bb;
       m.p_out = dst;
       m.clamp = 0;
-      if (int rc = batch_launch(b, "jacobi", wt.total, [&](dim3 g, hipStream_t s) -> int {
-            hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), g, dim3(256), 0, s, wt, m);
-            AJ_LAUNCH_CHECK("kb_jacobi");
-            return AJ_OK;
-          }, st))
-        return rc;
+      if (int rc = batch_jacobi<T>(b, lev, "jacobi", m, st)) return rc;
       src = dst;
     }
     return AJ_OK;
   }
   T* Yl = (T*)b->Y[lev];
-  BatchWork wt = make_work(b, lev, 0, kBatchJX, kBatchJY);
   m.p_in = p_src;
   m.e_c = nullptr;
   m.b = bb;
   m.p_out = Yl;
   m.clamp = 1;
-  auto jacobi = [&](dim3 g, hipStream_t s) -> int {
-    hipLaunchKernelGGL((kb_jacobi<T, kBatchJX, kBatchJY>), g, dim3(256), 0, s, wt, m);
-    AJ_LAUNCH_CHECK("kb_jacobi");
-    return AJ_OK;
-  };
   // algorithmic bytes of this level's launches (all instances)
   const double cl = (double)b->level_len[lev] * sizeof(T);
   {
     // p (warm start) and b read, p written
     ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, cl * (p_src ? 3 : 2), st);
-    if (int rc = batch_launch(b, "jacobi", wt.total, jacobi, st)) return rc;
+    if (int rc = batch_jacobi<T>(b, lev, "jacobi", m, st)) return rc;
   }
   T* Bc = (T*)b->B[lev + 1];
   BatchWork wr = make_work(b, lev, 2);
@@ -588,7 +610,7 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
   m.clamp = 1;
   // Y, b and the quarter-size coarse correction read, p written
   ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, cl * 3.25, st);
-  return batch_launch(b, "jacobi_post", wt.total, jacobi, st);
+  return batch_jacobi<T>(b, lev, "jacobi_post", m, st);
 }
 
 template <typename T>
