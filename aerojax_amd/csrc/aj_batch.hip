@@ -152,9 +152,28 @@ __global__ __launch_bounds__(NTH) void kb_rk(BatchWork w, BatchStepArgs<T> s) {
   }
 }
 
-// row items: kRowsPerItem rows x 256 columns, one cell per thread and row
-// (the rows' loads are independent, so they are in flight together)
-constexpr int kRowsPerItem = 4;
+// cell items: kCellsPerItem consecutive cells (row-major) of one instance,
+// 4 per thread, so that every lane has a cell whatever the row length (a row
+// of 128 cells would leave half of a 256-thread workgroup idle) and the four
+// cells' loads are in flight together; a thread steps its (row, column) by
+// 256 cells.  f(i, j) runs for each cell of item `sub` of an nx x ny grid.
+constexpr int kCellsPerItem = 1024;
+template <typename F>
+__device__ __forceinline__ void for_item_cells(int sub, int nx, int ny, F&& f) {
+  const int c0 = sub * kCellsPerItem + (int)threadIdx.x;
+  const int di = 256 / ny, dj = 256 - di * ny;
+  int i = c0 / ny, j = c0 - (c0 / ny) * ny;
+#pragma unroll
+  for (int k = 0; k < kCellsPerItem / 256; ++k) {
+    if (i < nx) f(i, j);
+    i += di;
+    j += dj;
+    if (j >= ny) {
+      j -= ny;
+      ++i;
+    }
+  }
+}
 
 // nu_sgs = constant Smagorinsky with delta = sqrt(dx dy) (step.py:111-122)
 template <typename T>
@@ -163,19 +182,13 @@ __global__ __launch_bounds__(256) void kb_les(BatchWork w, BatchStepArgs<T> s) {
     const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
-    const int cpr = (ny + 255) / 256;
-    const int rb = it.sub / cpr, j = (it.sub - rb * cpr) * 256 + threadIdx.x;
-    if (j >= ny) continue;
     const double cd = s.cs * sqrt(w.dx[g] * w.dy[g]);
     const T coef = T(cd * cd);
-#pragma unroll
-    for (int q = 0; q < kRowsPerItem; ++q) {
-      const int i = rb * kRowsPerItem + q;
-      if (i < nx)
-        s.nsgs[it.off + (long)i * ny + j] =
-            les_cell(s.u + it.off, s.v + it.off, nx, ny, i, j, T(1.0 / (2.0 * w.dx[g])),
-                     T(1.0 / (2.0 * w.dy[g])), coef);
-    }
+    for_item_cells(it.sub, nx, ny, [&](int i, int j) {
+      s.nsgs[it.off + (long)i * ny + j] =
+          les_cell(s.u + it.off, s.v + it.off, nx, ny, i, j, T(1.0 / (2.0 * w.dx[g])),
+                   T(1.0 / (2.0 * w.dy[g])), coef);
+    });
   }
 }
 
@@ -192,22 +205,14 @@ __global__ __launch_bounds__(256) void kb_geometry(BatchWork w, const ObstacleDe
     const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
-    const int cpr = (ny + 255) / 256;
-    const int rb = it.sub / cpr, j = (it.sub - rb * cpr) * 256 + threadIdx.x;
-    if (j >= ny) continue;
     const ObstacleDev od = obs[it.inst];
     const double lx = nx * w.dx[g], ly = ny * w.dy[g];
-    const double Y = linspace_at(j, ny, ly);
-#pragma unroll
-    for (int q = 0; q < kRowsPerItem; ++q) {
-      const int i = rb * kRowsPerItem + q;
-      if (i < nx) {
-        const double s = obstacle_sdf(od.o, od.ca, od.sa, linspace_at(i, nx, lx), Y);
-        const long c = it.off + (long)i * ny + j;
-        if (sdf) sdf[c] = (T)s;
-        if (mask) mask[c] = (T)obstacle_mask(s, od.o.eps);
-      }
-    }
+    for_item_cells(it.sub, nx, ny, [&](int i, int j) {
+      const double s = obstacle_sdf(od.o, od.ca, od.sa, linspace_at(i, nx, lx), linspace_at(j, ny, ly));
+      const long c = it.off + (long)i * ny + j;
+      if (sdf) sdf[c] = (T)s;
+      if (mask) mask[c] = (T)obstacle_mask(s, od.o.eps);
+    });
   }
 }
 
@@ -218,16 +223,10 @@ __global__ __launch_bounds__(256) void kb_div(BatchWork w, BatchStepArgs<T> s) {
     const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
-    const int cpr = (ny + 255) / 256;
-    const int rb = it.sub / cpr, j = (it.sub - rb * cpr) * 256 + threadIdx.x;
-    if (j >= ny) continue;
-#pragma unroll
-    for (int q = 0; q < kRowsPerItem; ++q) {
-      const int i = rb * kRowsPerItem + q;
-      if (i < nx)
-        div_rhs_cell(s.us + it.off, s.vs + it.off, s.mask + it.off, s.rhs + it.off, nx, ny, i, j,
-                     T(1.0 / (2.0 * w.dx[g])), T(1.0 / (2.0 * w.dy[g])), T(1.0 / s.dt));
-    }
+    for_item_cells(it.sub, nx, ny, [&](int i, int j) {
+      div_rhs_cell(s.us + it.off, s.vs + it.off, s.mask + it.off, s.rhs + it.off, nx, ny, i, j,
+                   T(1.0 / (2.0 * w.dx[g])), T(1.0 / (2.0 * w.dy[g])), T(1.0 / s.dt));
+    });
   }
 }
 
@@ -237,17 +236,11 @@ __global__ __launch_bounds__(256) void kb_project(BatchWork w, BatchStepArgs<T> 
     const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
-    const int cpr = (ny + 255) / 256;
-    const int rb = it.sub / cpr, j = (it.sub - rb * cpr) * 256 + threadIdx.x;
-    if (j >= ny) continue;
-#pragma unroll
-    for (int q = 0; q < kRowsPerItem; ++q) {
-      const int i = rb * kRowsPerItem + q;
-      if (i < nx)
-        project_cell(s.us + it.off, s.vs + it.off, s.p + it.off, s.mask + it.off, s.uo + it.off,
-                     s.vo + it.off, nx, ny, i, j, T(s.dt), T(1.0 / (2.0 * w.dx[g])),
-                     T(1.0 / (2.0 * w.dy[g])), T(0.1 / w.dx[g]), T(s.U_inf));
-    }
+    for_item_cells(it.sub, nx, ny, [&](int i, int j) {
+      project_cell(s.us + it.off, s.vs + it.off, s.p + it.off, s.mask + it.off, s.uo + it.off,
+                   s.vo + it.off, nx, ny, i, j, T(s.dt), T(1.0 / (2.0 * w.dx[g])),
+                   T(1.0 / (2.0 * w.dy[g])), T(0.1 / w.dx[g]), T(s.U_inf));
+    });
   }
 }
 
@@ -292,10 +285,7 @@ __global__ __launch_bounds__(256) void kb_jacobi(BatchWork w, BatchMgArgs<T> m) 
   }
 }
 
-// residual + restriction: kCoarseCellsPerItem consecutive coarse cells
-// (row-major) per item, 4 per thread, so that every lane has a cell whatever
-// the coarse row length; a thread steps its (row, column) by 256 cells
-constexpr int kCoarseCellsPerItem = 1024;
+// residual + restriction: cell items of the coarse grid
 template <typename T>
 __global__ __launch_bounds__(256) void kb_resrestrict(BatchWork w, BatchMgArgs<T> m, T* bc) {
   for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
@@ -306,21 +296,10 @@ __global__ __launch_bounds__(256) void kb_resrestrict(BatchWork w, BatchMgArgs<T
     const int ncx = w.cnx[g], ncy = w.cny[g];
     const T ax = T(1) / (T(w.dx[g]) * T(w.dx[g])), ay = T(1) / (T(w.dy[g]) * T(w.dy[g]));
     const int degenerate = (w.nx[g] / 2 < 2 || w.ny[g] / 2 < 2) ? 1 : 0;
-    const int c0 = it.sub * kCoarseCellsPerItem + (int)threadIdx.x;
-    const int di = 256 / ncy, dj = 256 - di * ncy;
-    int I = c0 / ncy, J = c0 - (c0 / ncy) * ncy;
-#pragma unroll
-    for (int k = 0; k < kCoarseCellsPerItem / 256; ++k) {
-      if (I < ncx)
-        ref_resrestrict_cell(m.p_out + it.off, m.b + it.off, m.bzl, w.nx[g], w.ny[g], ax, ay,
-                             bc + it.coff, ncy, degenerate, I, J);
-      I += di;
-      J += dj;
-      if (J >= ncy) {
-        J -= ncy;
-        ++I;
-      }
-    }
+    for_item_cells(it.sub, ncx, ncy, [&](int I, int J) {
+      ref_resrestrict_cell(m.p_out + it.off, m.b + it.off, m.bzl, w.nx[g], w.ny[g], ax, ay,
+                           bc + it.coff, ncy, degenerate, I, J);
+    });
   }
 }
 
@@ -450,9 +429,9 @@ struct aj_batch {
 
 namespace {
 
-// kind: 0 = tiles tx x ty, 1 = row items (this level), 2 = blocks of
-// kCoarseCellsPerItem cells of the next coarser level, 3 = one item per
-// instance, 4 = kResChunks per instance
+// kind: 0 = tiles tx x ty, 1 = cell items (this level), 2 = cell items of
+// the next coarser level, 3 = one item per instance, 4 = kResChunks per
+// instance
 BatchWork make_work(const aj_batch* b, int level, int kind, int tx = 16, int ty = 64) {
   BatchWork w;
   std::memset(&w, 0, sizeof(w));
@@ -463,9 +442,9 @@ BatchWork make_work(const aj_batch* b, int level, int kind, int tx = 16, int ty 
     w.tiles_y[ng] = (int)cdiv(ny, ty);
     switch (kind) {
       case 0: per = (int)(cdiv(nx, tx) * cdiv(ny, ty)); break;
-      case 1: per = (int)cdiv(nx, kRowsPerItem) * (int)cdiv(ny, 256); break;
+      case 1: per = (int)cdiv((long)nx * ny, kCellsPerItem); break;
       case 2:
-        per = (int)cdiv((long)b->lnx[g][level + 1] * b->lny[g][level + 1], kCoarseCellsPerItem);
+        per = (int)cdiv((long)b->lnx[g][level + 1] * b->lny[g][level + 1], kCellsPerItem);
         break;
       case 3: per = 1; break;
       default: per = kResChunks; break;
