@@ -254,6 +254,11 @@ struct BatchMgArgs {
   T* p_out;
   int bzl, clamp;
   aj_mg_stats* stats;  // [instances]
+  // non-null: each tile item also writes the sums of the squared residual
+  // of its input iterate (one per wave) to
+  // res_partial[instance * res_stride + 4 tile + wave]
+  double* res_partial;
+  int res_stride;
 };
 
 template <typename T, int TX, int TY>
@@ -279,6 +284,7 @@ __global__ __launch_bounds__(256) void kb_jacobi(BatchWork w, BatchMgArgs<T> m) 
     a.inv_diag = T(1) / (T(2) * (a.ax + a.ay));
     a.clamp_last = m.clamp;
     a.gate = m.stats + it.inst;
+    a.res_out = m.res_partial ? m.res_partial + (long)it.inst * m.res_stride + 4 * it.sub : nullptr;
     const int ty = it.sub % w.tiles_y[g], tx = it.sub / w.tiles_y[g];
     ref_jacobi_tile<T, TX, TY, 2>(a, tx * TX, ty * TY, L);
     __syncthreads();
@@ -320,12 +326,13 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(BatchWork w, BatchMg
 }
 
 // sum(r^2) partials: kResChunks items per instance, item k taking the
-// contiguous rows [k nx / kResChunks, (k + 1) nx / kResChunks); a thread
-// steps its (row, column) by 256 cells without a division in the loop
+// contiguous rows [k nx / kResChunks, (k + 1) nx / kResChunks), written to
+// partial[instance * stride + k]; a thread steps its (row, column) by 256
+// cells without a division in the loop
 template <typename T>
 __global__ __launch_bounds__(256) void kb_resnorm(BatchWork w, const T* __restrict__ p,
                                                   const T* __restrict__ b,
-                                                  double* __restrict__ partial) {
+                                                  double* __restrict__ partial, int stride) {
   __shared__ double sh[16];
   for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
     const int item = q.base + q.t;
@@ -349,23 +356,31 @@ __global__ __launch_bounds__(256) void kb_resnorm(BatchWork w, const T* __restri
       }
     }
     const double t = block_sum(acc, sh);
-    if (threadIdx.x == 0) partial[(long)it.inst * kResChunks + it.sub] = t;
+    if (threadIdx.x == 0) partial[(long)it.inst * stride + it.sub] = t;
   }
 }
 
-// per instance: sum the partials, record the norm, set the gate
-// (multigrid_solver.py:222-234)
+// per instance (one wave each): sum its w.per_inst * per_item partials
+// (layout w, partial[instance * stride + k]) in a fixed order, record the
+// norm, set the gate (multigrid_solver.py:222-234).  An instance whose gate
+// was already set ran no items: its iterate is unchanged and keeps its norm.
 __global__ __launch_bounds__(64) void kb_resnorm_final(const double* __restrict__ partial,
-                                                       int ninst, const double* __restrict__ dxdy,
+                                                       BatchWork w, int per_item, int stride,
+                                                       const double* __restrict__ dxdy,
                                                        const double* __restrict__ tol2,
                                                        aj_mg_stats* __restrict__ stats) {
-  const int inst = blockIdx.x * 64 + threadIdx.x;
-  if (inst >= ninst) return;
-  double tot = 0.0;
-  for (int k = 0; k < kResChunks; ++k) tot += partial[(long)inst * kResChunks + k];
+  const int inst = blockIdx.x;
+  int g = 0;
+  for (int k = 1; k < w.ng; ++k)
+    if (inst >= w.first[k]) g = k;
+  const int n = w.per_inst[g] * per_item;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += 64) acc += partial[(long)inst * stride + k];
+  const double tot = wave_sum(acc);
+  if (threadIdx.x != 0) return;
   aj_mg_stats* st = stats + inst;
-  const double nsq = tot * dxdy[inst];
   const int executed = st->converged == 0;
+  const double nsq = executed ? tot * dxdy[inst] : st->norm;
   const int k = st->outer;
   if (k < AJ_MG_MAX_NORMS) st->norms[k] = nsq;
   st->outer = k + 1;
@@ -409,7 +424,8 @@ struct aj_batch {
   void *us = nullptr, *vs = nullptr, *rhs = nullptr, *nsgs = nullptr;
   std::vector<aj_mg_plan*> tb;  // textbook solver: one batched plan per group (created on first use)
   int last_solver = AJ_MG_REFERENCE;
-  double* partial = nullptr;
+  double* partial = nullptr;  // [instances][res_stride] residual-norm partial sums
+  int res_stride = 0;
   double* dxdy = nullptr;
   double* tol2 = nullptr;
   double* nu = nullptr;
@@ -498,14 +514,16 @@ struct JacobiTile {
 
 // Jacobi launch of one level: 16x64 tiles while every instance's row length
 // at the level is at least 64, else 32x32, else 64x16 (a tile wider than the
-// grid leaves lanes without cells)
+// grid) and the tile layout in *wt_out when given
 template <typename T>
-int batch_jacobi(aj_batch* b, int lev, const char* name, const BatchMgArgs<T>& m, hipStream_t st) {
+int batch_jacobi(aj_batch* b, int lev, const char* name, const BatchMgArgs<T>& m, hipStream_t st,
+                 BatchWork* wt_out = nullptr) {
   int nymin = b->lny[0][lev];
   for (int g = 1; g < b->ng; ++g) nymin = std::min(nymin, b->lny[g][lev]);
   auto go = [&](auto tile) -> int {
     constexpr int TX = decltype(tile)::TX, TY = decltype(tile)::TY;
     const BatchWork wt = make_work(b, lev, 0, TX, TY);
+    if (wt_out) *wt_out = wt;
     return batch_launch(b, name, wt.total, [&](dim3 g, hipStream_t s) -> int {
       hipLaunchKernelGGL((kb_jacobi<T, TX, TY>), g, dim3(256), 0, s, wt, m);
       AJ_LAUNCH_CHECK("kb_jacobi");
@@ -517,12 +535,30 @@ int batch_jacobi(aj_batch* b, int lev, const char* name, const BatchMgArgs<T>& m
   return go(JacobiTile<64, 16>{});
 }
 
+// per instance: sum the residual partials of layout w, record the norm and
+// set the gate
+int resnorm_final(aj_batch* b, const BatchWork& w, int per_item, hipStream_t st) {
+  hipLaunchKernelGGL(kb_resnorm_final, dim3(b->ninst), dim3(64), 0, st,
+                     (const double*)b->partial, w, per_item, b->res_stride,
+                     (const double*)b->dxdy, (const double*)b->tol2, b->stats);
+  AJ_LAUNCH_CHECK("kb_resnorm_final");
+  return AJ_OK;
+}
+
+// One V-cycle from level lev down.  With eval_norm (level 0, p_src the
+// iterate after the previous cycle), the pre-smoothing launch also sums the
+// squared residual of p_src per tile, and the norm check after the previous
+// cycle (its record and gate) runs right after it, before anything of this
+// cycle writes p: an instance that converged with the previous cycle then
+// skips the rest of this one, and its pre-smoothed scratch is not used.
 template <typename T>
 int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, int bzl,
-                 hipStream_t st) {
+                 hipStream_t st, bool eval_norm = false) {
   BatchMgArgs<T> m;
   m.stats = b->stats;
   m.bzl = bzl;
+  m.res_partial = nullptr;
+  m.res_stride = 0;
   if (lev >= b->ml) {
     // coarsest: 10 sweeps from zero; one workgroup per instance when it fits
     BatchWork w = make_work(b, lev, 3);
@@ -562,11 +598,19 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
   m.clamp = 1;
   // algorithmic bytes of this level's launches (all instances)
   const double cl = (double)b->level_len[lev] * sizeof(T);
+  BatchWork wt;
   {
     // p (warm start) and b read, p written
     ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, cl * (p_src ? 3 : 2), st);
-    if (int rc = batch_jacobi<T>(b, lev, "jacobi", m, st)) return rc;
+    if (eval_norm && p_src) {
+      m.res_partial = b->partial;
+      m.res_stride = b->res_stride;
+    }
+    if (int rc = batch_jacobi<T>(b, lev, "jacobi", m, st, &wt)) return rc;
+    m.res_partial = nullptr;
   }
+  if (eval_norm && p_src)
+    if (int rc = resnorm_final(b, wt, 4, st)) return rc;
   T* Bc = (T*)b->B[lev + 1];
   BatchWork wr = make_work(b, lev, 2);
   BatchMgArgs<T> r = m;
@@ -657,23 +701,22 @@ int refmg_launches(aj_batch* b, const aj_batch_step_params* prm, const T* rhs, T
                    hipStream_t st) {
   hipLaunchKernelGGL(kb_mg_begin, dim3(b->ninst), dim3(64), 0, st, b->stats, b->ninst);
   AJ_LAUNCH_CHECK("kb_mg_begin");
-  BatchWork wn = make_work(b, 0, 4);
+  // the norm after cycle c < v_cycles - 1 is evaluated inside cycle c + 1
+  // (batch_vcycle's eval_norm); the one after the last cycle by its own pass
   for (int c = 0; c < prm->v_cycles; ++c) {
     const T* src = (c == 0) ? nullptr : p_out;
-    if (int rc = batch_vcycle<T>(b, 0, src, p_out, rhs, 1, st)) return rc;
-    if (int rc = batch_launch(b, "resnorm", wn.total, [&](dim3 g, hipStream_t t) -> int {
-          hipLaunchKernelGGL(kb_resnorm<T>, g, dim3(256), 0, t, wn, (const T*)p_out, rhs,
-                             b->partial);
-          AJ_LAUNCH_CHECK("kb_resnorm");
-          return AJ_OK;
-        }, st))
-      return rc;
-    hipLaunchKernelGGL(kb_resnorm_final, dim3(cdiv(b->ninst, 64)), dim3(64), 0, st,
-                       (const double*)b->partial, b->ninst, (const double*)b->dxdy,
-                       (const double*)b->tol2, b->stats);
-    AJ_LAUNCH_CHECK("kb_resnorm_final");
+    if (int rc = batch_vcycle<T>(b, 0, src, p_out, rhs, 1, st, c > 0)) return rc;
   }
-  return AJ_OK;
+  if (prm->v_cycles <= 0) return AJ_OK;
+  BatchWork wn = make_work(b, 0, 4);
+  if (int rc = batch_launch(b, "resnorm", wn.total, [&](dim3 g, hipStream_t t) -> int {
+        hipLaunchKernelGGL(kb_resnorm<T>, g, dim3(256), 0, t, wn, (const T*)p_out, rhs,
+                           b->partial, b->res_stride);
+        AJ_LAUNCH_CHECK("kb_resnorm");
+        return AJ_OK;
+      }, st))
+    return rc;
+  return resnorm_final(b, wn, 1, st);
 }
 
 template <typename T>
@@ -836,7 +879,16 @@ extern "C" int aj_batch_create(aj_batch** out, int dtype, int ngroups, const int
     oY[l] = take(b->level_len[l] * es);
     oE[l] = take(b->level_len[l] * es);
   }
-  const size_t o_part = take((size_t)b->ninst * kResChunks * 8);
+  // partial sums per instance: kResChunks (norm pass) or four per level-0
+  // Jacobi tile (any of the three tile shapes)
+  b->res_stride = kResChunks;
+  for (int g = 0; g < ngroups; ++g) {
+    const long nx = b->lnx[g][0], ny = b->lny[g][0];
+    const long t = std::max({cdiv(nx, kBatchJX) * cdiv(ny, kBatchJY), cdiv(nx, 32) * cdiv(ny, 32),
+                             cdiv(nx, 64) * cdiv(ny, 16)});
+    b->res_stride = (int)std::max<long>(b->res_stride, 4 * t);
+  }
+  const size_t o_part = take((size_t)b->ninst * b->res_stride * 8);
   const size_t o_dxdy = take((size_t)b->ninst * 8), o_tol2 = take((size_t)b->ninst * 8);
   const size_t o_nu = take((size_t)b->ninst * 8);
   const size_t o_obs = take((size_t)b->ninst * sizeof(ObstacleDev));

@@ -96,6 +96,13 @@ __device__ __forceinline__ void row_block(int* i, int* j0) {
   *j0 = (int)(b - (long)*i * gridDim.x) * (int)blockDim.x;
 }
 
+// Sum of a double over the 64 lanes of a wave (fixed butterfly, every lane
+// gets the same bits).
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // Block-wide sum of a double (blockDim.x multiple of 64, <= 1024) with a
 // fixed reduction tree, so repeated runs give identical bits.
 __device__ __forceinline__ double block_sum(double v, double* sh) {

@@ -19,6 +19,10 @@ struct RefJacobiArgs {
   T ax, ay, inv_diag;
   int clamp_last;
   const aj_mg_stats* gate;
+  // non-null (no coarse correction only): sums of the squared residual
+  // b - A p of the input iterate over the tile cells of each of the 4 waves,
+  // in FP64, written to res_out[0..3]
+  double* res_out = nullptr;
 };
 
 __device__ __forceinline__ bool gated(const aj_mg_stats* g) {
@@ -56,6 +60,26 @@ __device__ __forceinline__ T prolong_ref_lds(const T* __restrict__ c, int cI0, i
   const T s1 = c00 + (io ? c10 : c01);
   const T r = (io && jo) ? T(0.25) * ((s1 + c01) + c11) : (io || jo) ? T(0.5) * s1 : c00;
   return (i == nxf - 1 || j == nyf - 1) ? T(0) : r;
+}
+
+// b - apply_laplacian(p) from the centre value, its edge-padded
+// neighbours and b (multigrid_solver.py:173-183, 198)
+template <typename T>
+__device__ __forceinline__ T ref_residual_of(T pc, T pW, T pE, T pS, T pN, T bv, T ax, T ay) {
+  const T lap = ax * (pE + pW - T(2) * pc) + ay * (pN + pS - T(2) * pc);
+  return bv - lap;
+}
+
+// b - apply_laplacian(p) at (i, j)
+template <typename T>
+__device__ __forceinline__ T ref_residual(const T* __restrict__ p, const T* __restrict__ b,
+                                          int bzl, int nx, int ny, T ax, T ay, int i, int j) {
+  const long g = (long)i * ny + j;
+  const T pc = p[g];
+  const T pW = i > 0 ? p[g - ny] : pc, pE = i < nx - 1 ? p[g + ny] : pc;
+  const T pS = j > 0 ? p[g - 1] : pc, pN = j < ny - 1 ? p[g + 1] : pc;
+  const T bv = (bzl && i == nx - 1) ? T(0) : b[g];
+  return ref_residual_of(pc, pW, pE, pS, pN, bv, ax, ay);
 }
 
 // NS Jacobi sweeps (smooth, multigrid_solver.py:158-171) on one tile; `c`
@@ -139,6 +163,29 @@ __device__ __forceinline__ void ref_jacobi_tile(const RefJacobiArgs<T>& a, int x
     }
   }
   __syncthreads();
+  if (a.res_out) {
+    // residual of the input iterate on the tile cells, from the loaded
+    // region (b already zero on the last row when b_zero_last)
+    constexpr int RGT = NT / TY;
+    const int ty = tid % TY, rt = tid / TY;
+    const int ly = NS + ty, gy = gy0 + ly;
+    double acc = 0.0;
+    if (rt < RGT && gy < ny) {
+      const int dS = gy > 0 ? 1 : 0, dN = gy < ny - 1 ? 1 : 0;
+      for (int tx = rt; tx < TX; tx += RGT) {
+        const int lx = NS + tx, gx = gx0 + lx;
+        if (gx >= nx) break;
+        const int l = lx * RW + ly;
+        const T* src = s_p[0];
+        const T pc = src[l];
+        const T pW = gx > 0 ? src[l - RW] : pc, pE = gx < nx - 1 ? src[l + RW] : pc;
+        const T r = ref_residual_of(pc, pW, pE, src[l - dS], src[l + dN], s_b[l], a.ax, a.ay);
+        acc += (double)r * (double)r;
+      }
+    }
+    const double t = wave_sum(acc);
+    if ((tid & 63) == 0) a.res_out[tid >> 6] = t;
+  }
   int cur = 0;
 #pragma unroll
   for (int s = 1; s <= NS; ++s) {
@@ -177,19 +224,6 @@ __device__ __forceinline__ void ref_jacobi_tile(const RefJacobiArgs<T>& a, int x
       }
     }
   }
-}
-
-// b - apply_laplacian(p) at (i, j) (multigrid_solver.py:173-183, 198)
-template <typename T>
-__device__ __forceinline__ T ref_residual(const T* __restrict__ p, const T* __restrict__ b,
-                                          int bzl, int nx, int ny, T ax, T ay, int i, int j) {
-  const long g = (long)i * ny + j;
-  const T pc = p[g];
-  const T pW = i > 0 ? p[g - ny] : pc, pE = i < nx - 1 ? p[g + ny] : pc;
-  const T pS = j > 0 ? p[g - 1] : pc, pN = j < ny - 1 ? p[g + 1] : pc;
-  const T lap = ax * (pE + pW - T(2) * pc) + ay * (pN + pS - T(2) * pc);
-  const T bv = (bzl && i == nx - 1) ? T(0) : b[g];
-  return bv - lap;
 }
 
 // residual + restrict (multigrid_solver.py:108-132, 198-199) for coarse
