@@ -325,6 +325,30 @@ __global__ __launch_bounds__(kCoarseThreads) void kb_coarse(BatchWork w, BatchMg
   }
 }
 
+// levels sub_lev .. coarsest of every group, for the in-LDS V-cycle
+struct SubDims {
+  int nl;
+  int nx[kMaxGroups][kMaxRefLevels], ny[kMaxGroups][kMaxRefLevels];
+};
+
+// the V-cycle from level sub_lev down for one instance per workgroup, all
+// in LDS (ref_subcycle_wg), in place of the per-level launches
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void kb_subcycle(BatchWork w, SubDims d,
+                                                              BatchMgArgs<T> m) {
+  extern __shared__ unsigned char smem_raw[];
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const int item = q.base + q.t;
+    const BatchItem it = batch_item(w, item);
+    const int g = it.g;
+    if (gated(m.stats + it.inst)) continue;
+    const T ax = T(1) / (T(w.dx[g]) * T(w.dx[g])), ay = T(1) / (T(w.dy[g]) * T(w.dy[g]));
+    ref_subcycle_wg(m.b + it.off, m.p_out + it.off, d.nx[g], d.ny[g], d.nl, ax, ay,
+                    T(1) / (T(2) * (ax + ay)), reinterpret_cast<T*>(smem_raw));
+    __syncthreads();  // LDS reused by the next instance
+  }
+}
+
 // sum(r^2) partials: kResChunks items per instance, item k taking the
 // contiguous rows [k nx / kResChunks, (k + 1) nx / kResChunks), written to
 // partial[instance * stride + k]; a thread steps its (row, column) by 256
@@ -424,6 +448,9 @@ struct aj_batch {
   void *us = nullptr, *vs = nullptr, *rhs = nullptr, *nsgs = nullptr;
   std::vector<aj_mg_plan*> tb;  // textbook solver: one batched plan per group (created on first use)
   int last_solver = AJ_MG_REFERENCE;
+  // levels sub_lev .. ml run as one in-LDS V-cycle per instance (0: none)
+  int sub_lev = 0;
+  size_t sub_lds = 0;
   double* partial = nullptr;  // [instances][res_stride] residual-norm partial sums
   int res_stride = 0;
   double* dxdy = nullptr;
@@ -559,6 +586,28 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
   m.bzl = bzl;
   m.res_partial = nullptr;
   m.res_stride = 0;
+  if (lev > 0 && lev == b->sub_lev) {
+    // this level and all coarser ones in one launch, one instance per
+    // workgroup in LDS (bzl = 0 and a zero guess below level 0)
+    BatchWork w = make_work(b, lev, 3);
+    SubDims d;
+    std::memset(&d, 0, sizeof(d));
+    d.nl = b->ml - lev + 1;
+    for (int g = 0; g < b->ng; ++g)
+      for (int q = 0; q < d.nl; ++q) {
+        d.nx[g][q] = b->lnx[g][lev + q];
+        d.ny[g][q] = b->lny[g][lev + q];
+      }
+    m.p_in = nullptr;
+    m.e_c = nullptr;
+    m.b = bb;
+    m.p_out = p_dst;
+    m.clamp = 1;
+    hipLaunchKernelGGL(kb_subcycle<T>, dim3(persistent_grid(b, w.total, kWgCoarse)),
+                       dim3(kCoarseThreads), b->sub_lds, st, w, d, m);
+    AJ_LAUNCH_CHECK("kb_subcycle");
+    return AJ_OK;
+  }
   if (lev >= b->ml) {
     // coarsest: 10 sweeps from zero; one workgroup per instance when it fits
     BatchWork w = make_work(b, lev, 3);
@@ -895,6 +944,26 @@ extern "C" int aj_batch_create(aj_batch** out, int dtype, int ngroups, const int
   const size_t o_st = take((size_t)b->ninst * sizeof(aj_mg_stats));
   hipError_t e = hipGetDevice(&b->device);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  // the in-LDS V-cycle starts at the first level below 0 whose levels fit
+  // one workgroup's LDS (w0, w1 of the top level, b and y of every level)
+  for (int l = 1; l < L && b->ml - l + 1 <= kMaxRefLevels; ++l) {
+    size_t need = 0;
+    for (int g = 0; g < ngroups; ++g) {
+      const size_t n0 = (size_t)b->lnx[g][l] * b->lny[g][l];
+      size_t tot = 2 * n0;
+      for (int q = l; q <= L; ++q) tot += 2 * (size_t)b->lnx[g][q] * b->lny[g][q];
+      need = std::max(need, tot * es);
+    }
+    if (need <= 160 * 1024) {
+      b->sub_lev = l;
+      b->sub_lds = need;
+      break;
+    }
+  }
+  if (e == hipSuccess && b->sub_lds > 65536)
+    e = hipFuncSetAttribute(dtype == AJ_F64 ? (const void*)kb_subcycle<double>
+                                            : (const void*)kb_subcycle<float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->sub_lds);
   if (e == hipSuccess) e = hipMalloc(&b->ws, bytes);
   if (e != hipSuccess) {
     delete b;

@@ -62,6 +62,24 @@ __device__ __forceinline__ T prolong_ref_lds(const T* __restrict__ c, int cI0, i
   return (i == nxf - 1 || j == nyf - 1) ? T(0) : r;
 }
 
+// prolong_ref from a whole coarse grid (in LDS) with the case selected
+// rather than branched on; coarse indices are clamped into the grid as the
+// tile kernels' coarse patch is, and each case adds the same values in the
+// same order as prolong_ref
+template <typename T>
+__device__ __forceinline__ T prolong_ref_sel(const T* __restrict__ c, int ncx, int ncy, int i,
+                                             int j) {
+  const int nxf = 2 * ncx, nyf = 2 * ncy;
+  const int I = min(i >> 1, ncx - 1), J = min(j >> 1, ncy - 1);
+  const int I1 = min(I + 1, ncx - 1), J1 = min(J + 1, ncy - 1);
+  const bool io = i & 1, jo = j & 1;
+  const T c00 = c[I * ncy + J], c10 = c[I1 * ncy + J], c01 = c[I * ncy + J1];
+  const T c11 = c[I1 * ncy + J1];
+  const T s1 = c00 + (io ? c10 : c01);
+  const T r = (io && jo) ? T(0.25) * ((s1 + c01) + c11) : (io || jo) ? T(0.5) * s1 : c00;
+  return (i == nxf - 1 || j == nyf - 1) ? T(0) : r;
+}
+
 // b - apply_laplacian(p) from the centre value, its edge-padded
 // neighbours and b (multigrid_solver.py:173-183, 198)
 template <typename T>
@@ -248,6 +266,7 @@ __device__ __forceinline__ void ref_resrestrict_cell(const T* __restrict__ p,
 // coarsest level: NSW Jacobi sweeps from zero in one workgroup
 constexpr int kCoarseThreads = 1024;
 constexpr int kCoarsePerThread = 8;
+constexpr int kMaxRefLevels = 8;  // levels of one workgroup's in-LDS V-cycle
 
 // nsw sweeps from zero on the whole coarsest grid (n <= kCoarseThreads *
 // kCoarsePerThread cells) by one workgroup; s0 holds 2 n elements of LDS
@@ -286,6 +305,121 @@ __device__ __forceinline__ void ref_coarse_wg(const T* __restrict__ b, T* __rest
   for (int k = 0; k < kCoarsePerThread; ++k) {
     const int c = threadIdx.x + k * kCoarseThreads;
     if (c < n) out[c] = src[c];
+  }
+}
+
+// One Jacobi sweep (multigrid_solver.py:158-171) of a whole nx x ny grid in
+// LDS by the kCoarseThreads threads of a workgroup: dst = J(src) with rhs bq
+// (ZERO: src is zero everywhere and is not read; CLAMP: the last row of dst
+// is stored as zero), with the per-cell arithmetic of ref_jacobi_tile and
+// ref_coarse_wg.  Each thread steps its (row, column) by kCoarseThreads
+// cells; the barrier at the end makes dst visible to the workgroup.
+template <typename T, bool ZERO, bool CLAMP>
+__device__ __forceinline__ void ref_sweep_lds(const T* __restrict__ src, T* __restrict__ dst,
+                                              const T* __restrict__ bq, int nx, int ny, T ax,
+                                              T ay, T inv_diag) {
+  const int n = nx * ny;
+  const int di = kCoarseThreads / ny, dj = kCoarseThreads - di * ny;
+  int i = (int)threadIdx.x / ny, j = (int)threadIdx.x - ((int)threadIdx.x / ny) * ny;
+#pragma unroll 4
+  for (int c = threadIdx.x; c < n; c += kCoarseThreads) {
+    T pc = T(0), pW = T(0), pE = T(0), pS = T(0), pN = T(0);
+    if (!ZERO) {
+      pc = src[c];
+      pW = i > 0 ? src[c - ny] : pc;
+      pE = i < nx - 1 ? src[c + ny] : pc;
+      pS = j > 0 ? src[c - 1] : pc;
+      pN = j < ny - 1 ? src[c + 1] : pc;
+    }
+    const T val = (ax * (pE + pW) + ay * (pN + pS) - bq[c]) * inv_diag;
+    dst[c] = (CLAMP && i == nx - 1) ? T(0) : val;
+    i += di;
+    j += dj;
+    if (j >= ny) {
+      j -= ny;
+      ++i;
+    }
+  }
+  __syncthreads();
+}
+
+// The reference V-cycle (multigrid_solver.py:186-203) from a level >= 1 down
+// to the coarsest, for one grid by one workgroup of kCoarseThreads threads,
+// every level in LDS: on each level but the coarsest 2 sweeps from zero
+// (last row stored as zero after them), residual + restriction, the next
+// level, then the prolonged correction added and 2 more sweeps (last row
+// zero); on the coarsest 10 sweeps from zero.  The per-cell arithmetic is
+// that of the per-level launches (ref_jacobi_tile, ref_resrestrict_cell,
+// prolong_ref_lds, ref_coarse_wg), so the result is the same.  nl levels with
+// dims nxl[q] x nyl[q] (q = 0 the top), rhs b (global, top level),
+// correction out (global, top level).  LDS: w0, w1 (n0 each), then b_q and
+// y_q per level.
+template <typename T>
+__device__ __forceinline__ void ref_subcycle_wg(const T* __restrict__ b, T* __restrict__ out,
+                                                const int* nxl, const int* nyl, int nl, T ax,
+                                                T ay, T inv_diag, T* sm) {
+  const int n0 = nxl[0] * nyl[0];
+  T* w0 = sm;
+  T* w1 = w0 + n0;
+  T* bq[kMaxRefLevels];
+  T* yq[kMaxRefLevels];
+  {
+    T* at = w1 + n0;
+    for (int q = 0; q < nl; ++q) {
+      const int n = nxl[q] * nyl[q];
+      bq[q] = at;
+      yq[q] = at + n;
+      at += 2 * n;
+    }
+  }
+  for (int c = threadIdx.x; c < n0; c += kCoarseThreads) bq[0][c] = b[c];
+  __syncthreads();
+  // down: pre-smoothing from zero, residual + restriction
+  for (int q = 0; q + 1 < nl; ++q) {
+    const int nx = nxl[q], ny = nyl[q];
+    ref_sweep_lds<T, true, false>(w1, w0, bq[q], nx, ny, ax, ay, inv_diag);
+    ref_sweep_lds<T, false, true>(w0, yq[q], bq[q], nx, ny, ax, ay, inv_diag);
+    const int ncx = nxl[q + 1], ncy = nyl[q + 1];
+    const int degenerate = (nx / 2 < 2 || ny / 2 < 2) ? 1 : 0;
+    for (int c = threadIdx.x; c < ncx * ncy; c += kCoarseThreads) {
+      const int I = c / ncy, J = c - I * ncy;
+      ref_resrestrict_cell(yq[q], bq[q], 0, nx, ny, ax, ay, bq[q + 1], ncy, degenerate, I, J);
+    }
+    __syncthreads();
+  }
+  // coarsest: 10 sweeps from zero, the last one into y
+  {
+    const int q = nl - 1, nx = nxl[q], ny = nyl[q];
+    ref_sweep_lds<T, true, false>(w1, w0, bq[q], nx, ny, ax, ay, inv_diag);
+    for (int s = 0; s < 4; ++s) {
+      ref_sweep_lds<T, false, false>(w0, w1, bq[q], nx, ny, ax, ay, inv_diag);
+      ref_sweep_lds<T, false, false>(w1, w0, bq[q], nx, ny, ax, ay, inv_diag);
+    }
+    ref_sweep_lds<T, false, false>(w0, yq[q], bq[q], nx, ny, ax, ay, inv_diag);
+  }
+  // up: y + prolong(correction), 2 sweeps, last row zero
+  for (int q = nl - 2; q >= 0; --q) {
+    const int nx = nxl[q], ny = nyl[q], n = nx * ny;
+    const int ncx = nxl[q + 1], ncy = nyl[q + 1];
+    {
+      // prolong_ref from LDS with the case selected rather than branched on
+      // (the lanes of a wave alternate parities)
+      const int di = kCoarseThreads / ny, dj = kCoarseThreads - di * ny;
+      int i = (int)threadIdx.x / ny, j = (int)threadIdx.x - ((int)threadIdx.x / ny) * ny;
+#pragma unroll 4
+      for (int c = threadIdx.x; c < n; c += kCoarseThreads) {
+        w0[c] = yq[q][c] + prolong_ref_sel(yq[q + 1], ncx, ncy, i, j);
+        i += di;
+        j += dj;
+        if (j >= ny) {
+          j -= ny;
+          ++i;
+        }
+      }
+    }
+    __syncthreads();
+    ref_sweep_lds<T, false, false>(w0, w1, bq[q], nx, ny, ax, ay, inv_diag);
+    ref_sweep_lds<T, false, true>(w1, q == 0 ? out : yq[q], bq[q], nx, ny, ax, ay, inv_diag);
   }
 }
 
