@@ -56,6 +56,40 @@ struct TbLevel {
   float omega;     // over-relaxation of the red-black sweeps (1 = Gauss-Seidel)
 };
 
+// The per-cell arithmetic of the smoothers, written out operation by
+// operation (explicit fused multiply-adds): every smoother kernel (LDS
+// tiles and register columns) evaluates a cell with these functions, so
+// any of them gives the same bits.
+//
+// Weight of a red-black update: omega / diagonal of the cell's row of the
+// level operator (omega * inv_int for interior cells).  Absent neighbours
+// (outside the domain) leave the diagonal; a coarse level's outlet ghost
+// enters it as dE_last.  The diagonal sums dE, W, S, N in that order.
+__device__ __forceinline__ float tb_weight(const TbLevel& L, bool hasW, bool last, bool hasS,
+                                           bool hasN) {
+  const bool inr = hasW && hasS && hasN && !last;
+  const float dE = (L.level > 0 && last) ? L.dE_last : L.axl;
+  const float dg = ((dE + (hasW ? L.axl : 0.0f)) + (hasS ? L.ayl : 0.0f)) + (hasN ? L.ayl : 0.0f);
+  return L.omega * (inr ? L.inv_int : 1.0f / dg);
+}
+
+// Over-relaxed exact row solve p + omega (p_GS - p) as p (1 - omega) + t wr,
+// t = ax (E + W) + ay (N + S) - b; xs = E + W and ys = N + S with absent
+// neighbours 0, omw = 1 - omega.
+__device__ __forceinline__ float tb_update(const TbLevel& L, float xs, float ys, float bv,
+                                           float po, float wr, float omw) {
+  const float t = fmaf(L.axl, xs, fmaf(L.ayl, ys, -bv));
+  return fmaf(t, wr, po * omw);
+}
+
+// r = b - A p from the cell and its four neighbours (an absent neighbour is
+// the cell itself, the coarse outlet ghost gamma * p)
+__device__ __forceinline__ float tb_res(const TbLevel& L, float pc, float bv, float W, float E,
+                                        float S, float N) {
+  const float Ap = fmaf(L.axl, (E - pc) + (W - pc), L.ayl * ((N - pc) + (S - pc)));
+  return bv - Ap;
+}
+
 // r = b - A p at (i, j) (level operator; level 0 outlet row -> 0)
 template <typename F>
 __device__ __forceinline__ float tb_resid(const TbLevel& L, int i, int j, float pc, float bv,
@@ -67,8 +101,13 @@ __device__ __forceinline__ float tb_resid(const TbLevel& L, int i, int j, float 
   else E = L.gam * pc;  // level >= 1 only (level 0 returned above)
   const float S = j > 0 ? nb(0, -1) : pc;
   const float N = j < L.m - 1 ? nb(0, 1) : pc;
-  const float Ap = L.axl * ((E - pc) + (W - pc)) + L.ayl * ((N - pc) + (S - pc));
-  return bv - Ap;
+  return tb_res(L, pc, bv, W, E, S, N);
+}
+
+// bilinear weights of the cell-centred prolongation: the own coarse cell,
+// its x and y neighbours towards the fine cell, the diagonal one
+__device__ __forceinline__ float tb_prolong_val(float own, float xn, float yn, float dn) {
+  return fmaf(0.0625f, dn, fmaf(0.1875f, xn + yn, 0.5625f * own));
 }
 
 // cell-centred bilinear prolongation of coarse e (nc x mc) at fine (i, j);
@@ -87,7 +126,7 @@ __device__ __forceinline__ float tb_prolong(const float* __restrict__ e, int nc,
   const float xn = e[In * mc + J] * sx;
   const float yn = e[I * mc + Jn];
   const float dn = e[In * mc + Jn] * sx;
-  return 0.5625f * own + 0.1875f * (xn + yn) + 0.0625f * dn;
+  return tb_prolong_val(own, xn, yn, dn);
 }
 
 // LDS layout of a smoother tile (RH x RW cells, RW even): the two colours
@@ -143,11 +182,11 @@ __device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, con
   const int active_rows = (L.level == 0) ? L.n - 1 : L.n;
   // per cell: own slot, the other plane's slot of (lx, ly) and of
   // (lx, ly + 1), all-ones masks of the neighbours that take part (E, W, S,
-  // N), whether it is updated and whether it is an interior cell
+  // N), whether it is updated and its update weight
   int own[NIT][2], oth[NIT][2], onx[NIT][2];
   unsigned msk[NIT][2][4];
-  bool act[NIT][2], inr[NIT][2];
-  float dg[NIT][2];
+  bool act[NIT][2];
+  float wr[NIT][2];
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
     const int idx = tid + it * NTH;
@@ -162,7 +201,6 @@ __device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, con
       const int gy = gy0 + ly;
       const bool hasW = gx > 0, hasS = gy > 0, hasN = gy < L.m - 1, last = gx == L.n - 1;
       act[it][c] = idx < W2 * HH && gx >= 0 && gx < active_rows && gy >= 0 && gy < L.m;
-      inr[it][c] = hasW && hasS && hasN && !last;
       own[it][c] = c * T::PS + lx * T::RW2 + (ly >> 1);
       oth[it][c] = (1 - c) * T::PS + lx * T::RW2 + (ly >> 1);
       onx[it][c] = oth[it][c] + (ly & 1);
@@ -170,10 +208,10 @@ __device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, con
       msk[it][c][1] = hasW ? ~0u : 0u;
       msk[it][c][2] = hasS ? ~0u : 0u;
       msk[it][c][3] = hasN ? ~0u : 0u;
-      const float dE = (L.level > 0 && last) ? L.dE_last : L.axl;
-      dg[it][c] = ((dE + (hasW ? L.axl : 0.0f)) + (hasS ? L.ayl : 0.0f)) + (hasN ? L.ayl : 0.0f);
+      wr[it][c] = tb_weight(L, hasW, last, hasS, hasN);
     }
   }
+  const float omw = 1.0f - L.omega;
   // neighbour values are read unconditionally (every slot lies in the
   // tile) and masked to 0 where the neighbour is outside the domain
   auto nb = [](float v, unsigned m) { return __uint_as_float(__float_as_uint(v) & m); };
@@ -182,7 +220,7 @@ __device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, con
 #pragma unroll
   for (int k = 1; k <= 4; ++k) {
     const int c = (k - 1) & 1;
-    float t[NIT];
+    float xs[NIT], ys[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int o = oth[it][c], on = onx[it][c];
@@ -190,21 +228,14 @@ __device__ __forceinline__ void tb_tile_sweeps(const TbLevel& L, float* s_p, con
       const float W = nb(s_p[o - T::RW2], msk[it][c][1]);
       const float S = nb(s_p[on - 1], msk[it][c][2]);
       const float N = nb(s_p[on], msk[it][c][3]);
-      t[it] = (L.axl * (E + W) + L.ayl * (N + S)) - s_b[own[it][c]];
+      xs[it] = E + W;
+      ys[it] = N + S;
     }
-    if (L.omega == 1.0f) {
 #pragma unroll
-      for (int it = 0; it < NIT; ++it)
-        if (act[it][c]) s_p[own[it][c]] = inr[it][c] ? t[it] * L.inv_int : t[it] / dg[it][c];
-    } else {
-      // over-relaxed: p + omega (p_GS - p)
-#pragma unroll
-      for (int it = 0; it < NIT; ++it) {
-        if (!act[it][c]) continue;
-        const float gs = inr[it][c] ? t[it] * L.inv_int : t[it] / dg[it][c];
-        const float po = s_p[own[it][c]];
-        s_p[own[it][c]] = po + L.omega * (gs - po);
-      }
+    for (int it = 0; it < NIT; ++it) {
+      if (!act[it][c]) continue;
+      const int a = own[it][c];
+      s_p[a] = tb_update(L, xs[it], ys[it], s_b[a], s_p[a], wr[it][c], omw);
     }
     __syncthreads();
   }
@@ -353,6 +384,7 @@ __global__ __launch_bounds__(NTH) void k_tb_post(TbLevel L, const float* __restr
 }
 
 #include "aj_mg_tb_coarse.h"
+#include "aj_mg_tb_rc.h"
 
 // ---------------------------------------------------------------------------
 // FP64 defect correction
@@ -1020,8 +1052,34 @@ static int tb_launch_post(const TbLevel& Lv, int nb, const float* Y, const float
   return AJ_OK;
 }
 
-// smoother tile candidates (rows x columns, threads): tuned per level shape
-constexpr int kTbTileCands = 4;
+template <int W, int R>
+static int tb_launch_pre_rc(const TbLevel& Lv, int nb, const float* pin, const float* b, float* Y,
+                            float* Bc, const aj_mg_stats* gate, hipStream_t st) {
+  if ((Lv.n | Lv.m) & 1) return fail(AJ_ERR_INVALID, "register-column smoother: odd level shape");
+  using G = rc::PreGeo<W, R>;
+  const int XT = (int)cdiv(Lv.n, G::U), YT = (int)cdiv(Lv.m, G::UY);
+  hipLaunchKernelGGL((k_tb_pre_rc<W, R>), dim3(8 * cdiv((long)XT * YT, 8), 1, nb), dim3(W * 64), 0,
+                     st, Lv, XT, YT, pin, b, Y, Bc, gate);
+  AJ_LAUNCH_CHECK("k_tb_pre_rc");
+  return AJ_OK;
+}
+
+template <int W, int R>
+static int tb_launch_post_rc(const TbLevel& Lv, int nb, const float* Y, const float* Ec, int nc,
+                             int mc, float gamc, const float* b, float* E,
+                             const aj_mg_stats* gate, hipStream_t st) {
+  if ((Lv.n | Lv.m) & 1) return fail(AJ_ERR_INVALID, "register-column smoother: odd level shape");
+  using G = rc::PostGeo<W, R>;
+  const int XT = (int)cdiv(Lv.n, G::U), YT = (int)cdiv(Lv.m, G::UY);
+  hipLaunchKernelGGL((k_tb_post_rc<W, R>), dim3(8 * cdiv((long)XT * YT, 8), 1, nb), dim3(W * 64), 0,
+                     st, Lv, XT, YT, Y, Ec, nc, mc, gamc, b, E, gate);
+  AJ_LAUNCH_CHECK("k_tb_post_rc");
+  return AJ_OK;
+}
+
+// smoother candidates, tuned per level shape: LDS tiles (rows x columns,
+// threads) 0-3, register columns (waves x rows per wave) 4-7
+constexpr int kTbTileCands = 8;
 
 // pre-smoothing on level lev (lev < coarsest): pin (null = zero start) ->
 // Y, coarse rhs tB[lev + 1]
@@ -1036,6 +1094,10 @@ static int tb_pre(aj_mg_plan* p, int lev, const float* pin, float* Y, hipStream_
       case 1: return tb_launch_pre<16, 64, 512>(Lv, nb, pin, b, Y, Bc, gate, s);
       case 2: return tb_launch_pre<32, 32, 512>(Lv, nb, pin, b, Y, Bc, gate, s);
       case 3: return tb_launch_pre<16, 32, 256>(Lv, nb, pin, b, Y, Bc, gate, s);
+      case 4: return tb_launch_pre_rc<8, 4>(Lv, nb, pin, b, Y, Bc, gate, s);
+      case 5: return tb_launch_pre_rc<4, 8>(Lv, nb, pin, b, Y, Bc, gate, s);
+      case 6: return tb_launch_pre_rc<8, 6>(Lv, nb, pin, b, Y, Bc, gate, s);
+      case 7: return tb_launch_pre_rc<16, 4>(Lv, nb, pin, b, Y, Bc, gate, s);
       default: return tb_launch_pre<32, 64, 1024>(Lv, nb, pin, b, Y, Bc, gate, s);
     }
   };
@@ -1061,6 +1123,10 @@ static int tb_post(aj_mg_plan* p, int lev, const float* Y, float* E, hipStream_t
       case 1: return tb_launch_post<16, 64, 512>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
       case 2: return tb_launch_post<32, 32, 512>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
       case 3: return tb_launch_post<16, 32, 256>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      case 4: return tb_launch_post_rc<8, 4>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      case 5: return tb_launch_post_rc<4, 8>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      case 6: return tb_launch_post_rc<8, 6>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      case 7: return tb_launch_post_rc<16, 4>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
       default: return tb_launch_post<32, 64, 1024>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
     }
   };
