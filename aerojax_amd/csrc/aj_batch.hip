@@ -27,6 +27,7 @@
 // synchronisation per round.
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "aj_geometry_kernels.h"
@@ -430,6 +431,8 @@ __global__ __launch_bounds__(64) void kb_mg_begin(aj_mg_stats* stats, int ninst)
 
 }  // namespace aj
 
+#include "aj_batch_rc.h"
+
 using namespace aj;
 
 // ---------------------------------------------------------------- host side
@@ -562,6 +565,38 @@ int batch_jacobi(aj_batch* b, int lev, const char* name, const BatchMgArgs<T>& m
   return go(JacobiTile<64, 16>{});
 }
 
+// register-column Jacobi tiles (FP32): waves per workgroup, rows per wave
+constexpr int kJacRcW = 8, kJacRcR = 4;
+
+// the register-column Jacobi kernels apply on level lev when every group
+// has even nx, ny with ny <= 128 (a row is at most one wave of two-column
+// lanes) and the next level is the half-size grid with both sides >= 2
+bool jac_rc_ok(const aj_batch* b, int lev) {
+  if (b->dtype != AJ_F32 || lev + 1 > b->ml) return false;
+  for (int g = 0; g < b->ng; ++g) {
+    const int nx = b->lnx[g][lev], ny = b->lny[g][lev];
+    if ((nx | ny) & 1 || ny > 128 || nx < 4 || ny < 4) return false;
+    if (b->lnx[g][lev + 1] != nx / 2 || b->lny[g][lev + 1] != ny / 2) return false;
+  }
+  return true;
+}
+
+// register-column Jacobi launch of one level (PRE: with the residual and
+// its restriction into bc); the item layout in *wt_out when given
+template <bool PRE>
+int batch_jacobi_rc(aj_batch* b, int lev, const char* name, const BatchMgArgs<float>& m,
+                    float* bc, hipStream_t st, BatchWork* wt_out = nullptr) {
+  using G = JacRcGeo<kJacRcW, kJacRcR, PRE>;
+  const BatchWork wt = make_work(b, lev, 0, G::U, 128);
+  if (wt_out) *wt_out = wt;
+  return batch_launch(b, name, wt.total, [&](dim3 g, hipStream_t s) -> int {
+    hipLaunchKernelGGL((kb_jacobi_rc<kJacRcW, kJacRcR, PRE>), g, dim3(kJacRcW * 64), 0, s, wt, m,
+                       bc);
+    AJ_LAUNCH_CHECK("kb_jacobi_rc");
+    return AJ_OK;
+  }, st);
+}
+
 // per instance: sum the residual partials of layout w, record the norm and
 // set the gate
 int resnorm_final(aj_batch* b, const BatchWork& w, int per_item, hipStream_t st) {
@@ -647,6 +682,35 @@ int batch_vcycle(aj_batch* b, int lev, const T* p_src, T* p_dst, const T* bb, in
   m.clamp = 1;
   // algorithmic bytes of this level's launches (all instances)
   const double cl = (double)b->level_len[lev] * sizeof(T);
+  if constexpr (std::is_same<T, float>::value) {
+    if (jac_rc_ok(b, lev)) {
+      // register columns: smoothing with the residual and its restriction
+      // in one launch, then the coarser levels, then post-smoothing
+      T* Bc = (T*)b->B[lev + 1];
+      BatchWork wt;
+      {
+        // p (warm start) and b read, p and the quarter-size coarse rhs written
+        ProfScope prof(lev == 0 ? AJ_PROF_MG_PRE0 : AJ_PROF_MG_PRE, cl * (p_src ? 3.25 : 2.25),
+                       st);
+        if (eval_norm && p_src) {
+          m.res_partial = b->partial;
+          m.res_stride = b->res_stride;
+        }
+        if (int rc = batch_jacobi_rc<true>(b, lev, "jacobi_rc", m, Bc, st, &wt)) return rc;
+        m.res_partial = nullptr;
+      }
+      if (eval_norm && p_src)
+        if (int rc = resnorm_final(b, wt, kJacRcW, st)) return rc;
+      T* Ec = (T*)b->E[lev + 1];
+      if (int rc = batch_vcycle<T>(b, lev + 1, nullptr, Ec, Bc, 0, st)) return rc;
+      m.p_in = Yl;
+      m.e_c = Ec;
+      m.p_out = p_dst;
+      m.clamp = 1;
+      ProfScope prof(lev == 0 ? AJ_PROF_MG_POST0 : AJ_PROF_MG_POST, cl * 3.25, st);
+      return batch_jacobi_rc<false>(b, lev, "jacobi_rc_post", m, nullptr, st);
+    }
+  }
   BatchWork wt;
   {
     // p (warm start) and b read, p written
@@ -936,6 +1000,9 @@ extern "C" int aj_batch_create(aj_batch** out, int dtype, int ngroups, const int
     const long t = std::max({cdiv(nx, kBatchJX) * cdiv(ny, kBatchJY), cdiv(nx, 32) * cdiv(ny, 32),
                              cdiv(nx, 64) * cdiv(ny, 16)});
     b->res_stride = (int)std::max<long>(b->res_stride, 4 * t);
+    // register-column tiles: one partial per wave
+    const long trc = cdiv(nx, JacRcGeo<kJacRcW, kJacRcR, true>::U);
+    b->res_stride = (int)std::max<long>(b->res_stride, kJacRcW * trc);
   }
   const size_t o_part = take((size_t)b->ninst * b->res_stride * 8);
   const size_t o_dxdy = take((size_t)b->ninst * 8), o_tol2 = take((size_t)b->ninst * 8);

@@ -68,6 +68,18 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) {
   return max(lo, min(x, hi));
 }
 
+// Value of x in lane l - 1 of the wave, read by lane l (DPP wave_shr:1);
+// lane 0, which has no such lane, gets fb.
+__device__ __forceinline__ float dpp_from_left(float x, float fb) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(fb), __float_as_int(x), 0x138, 0xF, 0xF, false));
+}
+// Value of x in lane l + 1, read by lane l (DPP wave_shl:1); lane 63 gets fb.
+__device__ __forceinline__ float dpp_from_right(float x, float fb) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(fb), __float_as_int(x), 0x130, 0xF, 0xF, false));
+}
+
 // The hardware deals consecutive workgroup ids out to the 8 XCDs in turn;
 // renumbering so that each XCD gets one contiguous range of blocks keeps
 // the +-row neighbours of a block in the same L2.

@@ -80,11 +80,23 @@ __device__ __forceinline__ T prolong_ref_sel(const T* __restrict__ c, int ncx, i
   return (i == nxf - 1 || j == nyf - 1) ? T(0) : r;
 }
 
+// The per-cell arithmetic of the reference multigrid, written out operation
+// by operation (explicit fused multiply-adds), so that every kernel that
+// sweeps or forms residuals (LDS tiles, register columns, the in-LDS
+// coarse cycles) gives the same bits.
+//
+// One Jacobi update (multigrid_solver.py:158-171) from the edge-padded
+// neighbours: (ax (E + W) + ay (N + S) - b) / (2 (ax + ay)).
+template <typename T>
+__device__ __forceinline__ T ref_jacobi_val(T pW, T pE, T pS, T pN, T bv, T ax, T ay, T inv_diag) {
+  return fma(ax, pE + pW, fma(ay, pN + pS, -bv)) * inv_diag;
+}
+
 // b - apply_laplacian(p) from the centre value, its edge-padded
 // neighbours and b (multigrid_solver.py:173-183, 198)
 template <typename T>
 __device__ __forceinline__ T ref_residual_of(T pc, T pW, T pE, T pS, T pN, T bv, T ax, T ay) {
-  const T lap = ax * (pE + pW - T(2) * pc) + ay * (pN + pS - T(2) * pc);
+  const T lap = fma(ax, fma(T(-2), pc, pE + pW), ay * fma(T(-2), pc, pN + pS));
   return bv - lap;
 }
 
@@ -223,7 +235,7 @@ __device__ __forceinline__ void ref_jacobi_tile(const RefJacobiArgs<T>& a, int x
         const int l = lx * RW + ly;
         const T pW = src[gx > 0 ? l - RW : l], pE = src[gx < nx - 1 ? l + RW : l];
         const T pS = src[l - dS], pN = src[l + dN];
-        dst[l] = (a.ax * (pE + pW) + a.ay * (pN + pS) - s_b[l]) * a.inv_diag;
+        dst[l] = ref_jacobi_val(pW, pE, pS, pN, s_b[l], a.ax, a.ay, a.inv_diag);
       }
     }
     __syncthreads();
@@ -295,7 +307,7 @@ __device__ __forceinline__ void ref_coarse_wg(const T* __restrict__ b, T* __rest
         const T pc = src[c];
         const T pW = i > 0 ? src[c - ny] : pc, pE = i < nx - 1 ? src[c + ny] : pc;
         const T pS = j > 0 ? src[c - 1] : pc, pN = j < ny - 1 ? src[c + 1] : pc;
-        dst[c] = (ax * (pE + pW) + ay * (pN + pS) - breg[k]) * inv_diag;
+        dst[c] = ref_jacobi_val(pW, pE, pS, pN, breg[k], ax, ay, inv_diag);
       }
     }
     __syncthreads();
@@ -331,7 +343,7 @@ __device__ __forceinline__ void ref_sweep_lds(const T* __restrict__ src, T* __re
       pS = j > 0 ? src[c - 1] : pc;
       pN = j < ny - 1 ? src[c + 1] : pc;
     }
-    const T val = (ax * (pE + pW) + ay * (pN + pS) - bq[c]) * inv_diag;
+    const T val = ref_jacobi_val(pW, pE, pS, pN, bq[c], ax, ay, inv_diag);
     dst[c] = (CLAMP && i == nx - 1) ? T(0) : val;
     i += di;
     j += dj;

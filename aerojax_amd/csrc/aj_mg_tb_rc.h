@@ -36,14 +36,10 @@ namespace rc {
 constexpr int kLanes = 64;
 constexpr int kCols = 2 * kLanes;  // columns of a wave's strip
 
-// value of lane l - 1 in lane l (0 in lane 0): DPP wave_shr:1
-__device__ __forceinline__ float from_left(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, false));
-}
-// value of lane l + 1 in lane l (0 in lane 63): DPP wave_shl:1
-__device__ __forceinline__ float from_right(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, false));
-}
+// value of lane l - 1 in lane l (0 in lane 0)
+__device__ __forceinline__ float from_left(float x) { return aj::dpp_from_left(x, 0.0f); }
+// value of lane l + 1 in lane l (0 in lane 63)
+__device__ __forceinline__ float from_right(float x) { return aj::dpp_from_right(x, 0.0f); }
 
 // Tile of W waves x R rows with HX halo rows and HY halo columns per side.
 template <int W, int R, int HX, int HY>
