@@ -34,6 +34,7 @@
 #include "aj_mg.h"
 #include "aj_mg_ref_kernels.h"
 #include "aj_step_kernels.h"
+#include "aj_rk_rc.h"
 
 namespace aj {
 
@@ -150,6 +151,47 @@ __global__ __launch_bounds__(NTH) void kb_rk(BatchWork w, BatchStepArgs<T> s) {
     const int ty = it.sub % w.tiles_y[g], tx = it.sub / w.tiles_y[g];
     rk_tile<T, TX, TY, false, false, NTH>(a, tx * TX, ty * TY, sm);
     __syncthreads();  // LDS reused by the next item
+  }
+}
+
+// register-column RK3 predictor with the pressure rhs in its epilogue (FP32,
+// every group with even ny in [4, 128]): items are x-tiles of one instance,
+// each a strip of the whole row width
+template <int W, int R>
+__global__ __launch_bounds__(W * 64) void kb_rk_rc(BatchWork w, BatchStepArgs<float> s) {
+  using G = RkRcGeo<W, R, false, true>;
+  __shared__ float4 xb[2][W][2][64];
+  for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
+    const BatchItem it = batch_item(w, q.base + q.t);
+    const int g = it.g;
+    RkRcArgs a;
+    a.u = s.u + it.off;
+    a.v = s.v + it.off;
+    a.mask = s.mask + it.off;
+    a.sdf = s.sdf ? s.sdf + it.off : nullptr;
+    a.nu_sgs = s.nsgs ? s.nsgs + it.off : nullptr;
+    a.nu_field = nullptr;
+    a.u_out = s.us + it.off;
+    a.v_out = s.vs + it.off;
+    a.rhs = s.rhs + it.off;
+    a.rhs_masked = 1;
+    a.nx = w.nx[g];
+    a.ny = w.ny[g];
+    a.dt = (float)s.dt;
+    a.nu = (float)s.nu[it.inst];
+    a.inv_dx = (float)(1.0 / w.dx[g]);
+    a.inv_dy = (float)(1.0 / w.dy[g]);
+    a.inv_dx2 = (float)(1.0 / (w.dx[g] * w.dx[g]));
+    a.inv_dy2 = (float)(1.0 / (w.dy[g] * w.dy[g]));
+    a.U_inf = (float)s.U_inf;
+    a.inv_eta = (float)s.inv_eta;
+    a.sdf_scale = (float)(1.0 / (0.1 * w.dx[g]));
+    a.inv_2dx = (float)(1.0 / (2.0 * w.dx[g]));
+    a.inv_2dy = (float)(1.0 / (2.0 * w.dy[g]));
+    a.inv_dt = (float)(1.0 / s.dt);
+    a.slip = 1;
+    rk_rc_tile<W, R, false, true>(a, it.sub * G::U - G::HX, 0, 0, a.ny, xb);
+    __syncthreads();  // the exchange buffers are reused by the next item
   }
 }
 
@@ -567,6 +609,8 @@ int batch_jacobi(aj_batch* b, int lev, const char* name, const BatchMgArgs<T>& m
 
 // register-column Jacobi tiles (FP32): waves per workgroup, rows per wave
 constexpr int kJacRcW = 8, kJacRcR = 4;
+// register-column RK predictor tiles (FP32)
+constexpr int kRkRcW = 8, kRkRcR = 4;
 
 // the register-column Jacobi kernels apply on level lev when every group
 // has even nx, ny with ny <= 128 (a row is at most one wave of two-column
@@ -785,6 +829,24 @@ int predict_launches(aj_batch* b, const BatchStepArgs<T>& s, hipStream_t st) {
           return AJ_OK;
         }, st))
       return rc;
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    bool rc_ok = true;
+    for (int g = 0; g < b->ng; ++g) {
+      const int nx = b->lnx[g][0], ny = b->lny[g][0];
+      if ((ny & 1) || ny < 4 || ny > 128 || nx < 4) rc_ok = false;
+    }
+    if (rc_ok) {
+      // register columns: RK3 predictor and the pressure rhs in one launch
+      using G = RkRcGeo<kRkRcW, kRkRcR, false, true>;
+      BatchWork w = make_work(b, 0, 0, G::U, 128);
+      ProfScope prof(AJ_PROF_RK, (double)b->cells * sizeof(T) * (s.nsgs ? 8 : 7), st);
+      return batch_launch(b, "rk_rc", w.total, [&](dim3 g, hipStream_t t) -> int {
+        hipLaunchKernelGGL((kb_rk_rc<kRkRcW, kRkRcR>), g, dim3(kRkRcW * 64), 0, t, w, s);
+        AJ_LAUNCH_CHECK("kb_rk_rc");
+        return AJ_OK;
+      }, st);
+    }
   }
   // RK3 predictor (the predictor BCs of step.py:182-191 are its own last step)
   {

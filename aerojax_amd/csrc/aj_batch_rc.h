@@ -44,7 +44,7 @@ __device__ __forceinline__ void jac_rc_item(float (&p)[R][2], const float (&bb)[
                                             int ny, int ncy, int gx0, float ax, float ay,
                                             float inv, double* res_out) {
   using G = JacRcGeo<W, R, PRE>;
-  const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int wv = wave_id(), lane = (int)(threadIdx.x & 63);
   const int wu = wv > 0 ? wv - 1 : wv, wd = wv < W - 1 ? wv + 1 : wv;
   const int gxw = gx0 + wv * R;
   const int gy = 2 * lane;
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(W * 64) void kb_jacobi_rc(BatchWork w, BatchMgArgs<
   using G = JacRcGeo<W, R, PRE>;
   constexpr int NQ = R / 2 + 1;  // coarse rows of a wave's rows (post)
   __shared__ float2 xb[2][W][2][64];
-  const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int wv = wave_id(), lane = (int)(threadIdx.x & 63);
   for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
     const BatchItem it = batch_item(w, q.base + q.t);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];

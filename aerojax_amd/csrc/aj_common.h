@@ -68,6 +68,13 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) {
   return max(lo, min(x, hi));
 }
 
+// Index of the calling thread's wave in its workgroup, as a wave-uniform
+// (scalar) value, so that row offsets and row tests derived from it stay
+// in scalar registers.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // Value of x in lane l - 1 of the wave, read by lane l (DPP wave_shr:1);
 // lane 0, which has no such lane, gets fb.
 __device__ __forceinline__ float dpp_from_left(float x, float fb) {

@@ -32,6 +32,7 @@ __device__ __forceinline__ bool gated(const aj_mg_stats* g) {
 // prolong (multigrid_solver.py:134-156) evaluated at fine cell (i, j)
 template <typename T>
 __device__ __forceinline__ T prolong_ref(const T* __restrict__ c, int ncx, int ncy, int i, int j) {
+#pragma clang fp contract(off)
   const int nxf = 2 * ncx, nyf = 2 * ncy;
   if (i == nxf - 1 || j == nyf - 1) return T(0);
   const int I = i >> 1, J = j >> 1;
@@ -52,6 +53,7 @@ __device__ __forceinline__ T prolong_ref(const T* __restrict__ c, int ncx, int n
 template <typename T>
 __device__ __forceinline__ T prolong_ref_lds(const T* __restrict__ c, int cI0, int cJ0, int CW,
                                              int ncx, int ncy, int i, int j) {
+#pragma clang fp contract(off)
   const int nxf = 2 * ncx, nyf = 2 * ncy;
   const int I = i >> 1, J = j >> 1;
   const bool io = i & 1, jo = j & 1;
@@ -69,6 +71,7 @@ __device__ __forceinline__ T prolong_ref_lds(const T* __restrict__ c, int cI0, i
 template <typename T>
 __device__ __forceinline__ T prolong_ref_sel(const T* __restrict__ c, int ncx, int ncy, int i,
                                              int j) {
+#pragma clang fp contract(off)
   const int nxf = 2 * ncx, nyf = 2 * ncy;
   const int I = min(i >> 1, ncx - 1), J = min(j >> 1, ncy - 1);
   const int I1 = min(I + 1, ncx - 1), J1 = min(J + 1, ncy - 1);
@@ -89,6 +92,7 @@ __device__ __forceinline__ T prolong_ref_sel(const T* __restrict__ c, int ncx, i
 // neighbours: (ax (E + W) + ay (N + S) - b) / (2 (ax + ay)).
 template <typename T>
 __device__ __forceinline__ T ref_jacobi_val(T pW, T pE, T pS, T pN, T bv, T ax, T ay, T inv_diag) {
+#pragma clang fp contract(off)
   return fma(ax, pE + pW, fma(ay, pN + pS, -bv)) * inv_diag;
 }
 
@@ -96,6 +100,7 @@ __device__ __forceinline__ T ref_jacobi_val(T pW, T pE, T pS, T pN, T bv, T ax, 
 // neighbours and b (multigrid_solver.py:173-183, 198)
 template <typename T>
 __device__ __forceinline__ T ref_residual_of(T pc, T pW, T pE, T pS, T pN, T bv, T ax, T ay) {
+#pragma clang fp contract(off)
   const T lap = fma(ax, fma(T(-2), pc, pE + pW), ay * fma(T(-2), pc, pN + pS));
   return bv - lap;
 }

@@ -67,6 +67,7 @@ struct TbLevel {
 // enters it as dE_last.  The diagonal sums dE, W, S, N in that order.
 __device__ __forceinline__ float tb_weight(const TbLevel& L, bool hasW, bool last, bool hasS,
                                            bool hasN) {
+#pragma clang fp contract(off)
   const bool inr = hasW && hasS && hasN && !last;
   const float dE = (L.level > 0 && last) ? L.dE_last : L.axl;
   const float dg = ((dE + (hasW ? L.axl : 0.0f)) + (hasS ? L.ayl : 0.0f)) + (hasN ? L.ayl : 0.0f);
@@ -78,6 +79,7 @@ __device__ __forceinline__ float tb_weight(const TbLevel& L, bool hasW, bool las
 // neighbours 0, omw = 1 - omega.
 __device__ __forceinline__ float tb_update(const TbLevel& L, float xs, float ys, float bv,
                                            float po, float wr, float omw) {
+#pragma clang fp contract(off)
   const float t = fmaf(L.axl, xs, fmaf(L.ayl, ys, -bv));
   return fmaf(t, wr, po * omw);
 }
@@ -86,6 +88,7 @@ __device__ __forceinline__ float tb_update(const TbLevel& L, float xs, float ys,
 // the cell itself, the coarse outlet ghost gamma * p)
 __device__ __forceinline__ float tb_res(const TbLevel& L, float pc, float bv, float W, float E,
                                         float S, float N) {
+#pragma clang fp contract(off)
   const float Ap = fmaf(L.axl, (E - pc) + (W - pc), L.ayl * ((N - pc) + (S - pc)));
   return bv - Ap;
 }
@@ -107,6 +110,7 @@ __device__ __forceinline__ float tb_resid(const TbLevel& L, int i, int j, float 
 // bilinear weights of the cell-centred prolongation: the own coarse cell,
 // its x and y neighbours towards the fine cell, the diagonal one
 __device__ __forceinline__ float tb_prolong_val(float own, float xn, float yn, float dn) {
+#pragma clang fp contract(off)
   return fmaf(0.0625f, dn, fmaf(0.1875f, xn + yn, 0.5625f * own));
 }
 

@@ -181,7 +181,7 @@ __device__ __forceinline__ void pre_body(const aj::TbLevel& L, float (&p)[R][2],
                                          int gy0, float* __restrict__ Y, float* __restrict__ Bc,
                                          float (*xb)[W][2][kLanes]) {
   using G = PreGeo<W, R>;
-  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int w = aj::wave_id(), lane = (int)(threadIdx.x & 63);
   const int gxw = gx0 + w * R;
   const int n = L.n, m = L.m;
   RowInfo ri[R];
@@ -246,7 +246,7 @@ __device__ __forceinline__ void post_body(const aj::TbLevel& L, float (&p)[R][2]
                                           const float (&b)[R][2], const LaneInfo& li, int gx0,
                                           float* __restrict__ E, float (*xb)[W][2][kLanes]) {
   using G = PostGeo<W, R>;
-  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int w = aj::wave_id(), lane = (int)(threadIdx.x & 63);
   const int gxw = gx0 + w * R;
   const int n = L.n, m = L.m;
   RowInfo ri[R];
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(W * 64) void k_tb_pre_rc(TbLevel L, int XT, int YT,
     Bc += z * (n >> 1) * (m >> 1);
   }
   const int gated = gate->converged;
-  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int w = aj::wave_id(), lane = (int)(threadIdx.x & 63);
   const int gx0 = xt * G::U - 6, gy0 = yt * G::UY - 6;
   const rc::LaneInfo li = rc::lane_info(gy0, lane, m);
   const int gyl = li.in ? li.gy : 0;  // loads of lanes outside the grid read column 0
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(W * 64) void k_tb_post_rc(TbLevel L, int XT, int YT
     E += z * n * m;
   }
   const int gated = gate->converged;
-  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int w = aj::wave_id(), lane = (int)(threadIdx.x & 63);
   const int gx0 = xt * G::U - 4, gy0 = yt * G::UY - 6;
   const int gxw = gx0 + w * R;  // even
   const rc::LaneInfo li = rc::lane_info(gy0, lane, m);

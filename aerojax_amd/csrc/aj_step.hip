@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <cstdio>
 
+#include "aj_rk_rc.h"
+
 namespace aj {
 
 template <typename T, int TX, int TY, bool FAST, bool HYPER, int NTH, bool DIV>
@@ -43,6 +45,29 @@ static int launch_rk(const RkArgs<T>& a, hipStream_t st) {
   hipLaunchKernelGGL((k_rk_fused<T, TX, TY, FAST, HYPER, NTH, DIV>), grid, dim3(NTH), smem, st,
                      a);
   AJ_LAUNCH_CHECK("k_rk_fused");
+  return AJ_OK;
+}
+
+// register-column predictor (FP32, no hyperviscosity): W waves x R rows per
+// tile; refused (a failing tuner candidate) for odd or tiny grids
+template <int W, int R, bool FAST, bool DIV>
+static int launch_rk_rc(const RkArgs<float>& a, hipStream_t st) {
+  if ((a.ny & 1) || a.ny < 4 || a.nx < 4)
+    return fail(AJ_ERR_INVALID, "register-column RK predictor: odd or tiny grid");
+  RkRcArgs r;
+  r.u = a.u; r.v = a.v; r.mask = a.mask; r.sdf = a.sdf;
+  r.nu_sgs = a.nu_sgs; r.nu_field = a.nu_field;
+  r.u_out = a.u_out; r.v_out = a.v_out; r.rhs = a.rhs;
+  r.nx = a.nx; r.ny = a.ny;
+  r.dt = a.dt; r.nu = a.nu; r.inv_dx = a.inv_dx; r.inv_dy = a.inv_dy;
+  r.inv_dx2 = a.inv_dx2; r.inv_dy2 = a.inv_dy2; r.U_inf = a.U_inf; r.inv_eta = a.inv_eta;
+  r.sdf_scale = a.sdf_scale; r.inv_2dx = a.inv_2dx; r.inv_2dy = a.inv_2dy; r.inv_dt = a.inv_dt;
+  r.slip = a.slip; r.rhs_masked = a.rhs_masked;
+  using G = RkRcGeo<W, R, FAST, DIV>;
+  const int XT = (int)cdiv(a.nx, G::U), YT = a.ny <= 128 ? 1 : (int)cdiv(a.ny, G::UY);
+  hipLaunchKernelGGL((k_rk_rc<W, R, FAST, DIV>), dim3(8 * cdiv((long)XT * YT, 8)), dim3(W * 64),
+                     0, st, r, XT, YT);
+  AJ_LAUNCH_CHECK("k_rk_rc");
   return AJ_OK;
 }
 
@@ -85,6 +110,10 @@ static int rk_step_t(const void* u, const void* v, const void* mask, const void*
                               : launch_rk<T, 8, 64, false, false, true>(a, s);
           case 2: return fast ? launch_rk<T, 32, 64, true, false, true>(a, s)
                               : launch_rk<T, 32, 64, false, false, true>(a, s);
+          case 3: return fast ? launch_rk_rc<8, 4, true, true>(a, s)
+                              : launch_rk_rc<8, 4, false, true>(a, s);
+          case 4: return fast ? launch_rk_rc<8, 2, true, true>(a, s)
+                              : launch_rk_rc<8, 2, false, true>(a, s);
           default: return fast ? launch_rk<T, 16, 64, true, false, true>(a, s)
                                : launch_rk<T, 16, 64, false, false, true>(a, s);
         }
@@ -96,7 +125,7 @@ static int rk_step_t(const void* u, const void* v, const void* mask, const void*
     // u, v, chi source, mask (+ nu_sgs) read; u*, v*, rhs written
     ProfScope prof(AJ_PROF_RK, (double)nx * ny * sizeof(T) * (7 + (a.nu_sgs ? 1 : 0)), st);
     int pick = 0;
-    return tune_pick(key, sizeof(T) == 8 ? 2 : 3, launch, st, &pick);
+    return tune_pick(key, sizeof(T) == 8 ? 2 : 5, launch, st, &pick);
   }
   // tile candidates (rows x columns), tuned per grid shape and variant
   auto launch = [&](int c, hipStream_t s) -> int {
@@ -121,12 +150,16 @@ static int rk_step_t(const void* u, const void* v, const void* mask, const void*
                             : launch_rk<T, 32, 64, false, false>(a, s);
         case 3: return fast ? launch_rk<T, 8, 128, true, false>(a, s)
                             : launch_rk<T, 8, 128, false, false>(a, s);
+        case 4: return fast ? launch_rk_rc<8, 4, true, false>(a, s)
+                            : launch_rk_rc<8, 4, false, false>(a, s);
+        case 5: return fast ? launch_rk_rc<8, 2, true, false>(a, s)
+                            : launch_rk_rc<8, 2, false, false>(a, s);
         default: return fast ? launch_rk<T, 16, 64, true, false>(a, s)
                              : launch_rk<T, 16, 64, false, false>(a, s);
       }
     }
   };
-  const int ncand = hyper ? 1 : (sizeof(T) == 8 ? 3 : 4);
+  const int ncand = hyper ? 1 : (sizeof(T) == 8 ? 3 : 6);
   char key[96];
   std::snprintf(key, sizeof(key), "rk/%s/%s%s/%dx%d", sizeof(T) == 8 ? "f64" : "f32",
                 fast ? "rk2" : "rk3", hyper ? "+hyper" : "", nx, ny);

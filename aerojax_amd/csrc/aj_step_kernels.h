@@ -51,37 +51,93 @@ struct RkArgs {
   int rhs_masked;  // DIV tiles: multiply the divergence by the mask
 };
 
+// The per-cell arithmetic of the RK predictor, written out operation by
+// operation (explicit fused multiply-adds): the LDS tiles and the
+// register-column kernels evaluate every cell with these functions, so any
+// of them gives the same bits.
+//
+// upwind advection vel * df/dh (rk3_simple_new.py:287-298): backward
+// difference where vel > 0, forward otherwise
+template <typename T>
+__device__ __forceinline__ T rk_upwind(T vel, T fc, T fm, T fp, T inv_h) {
+#pragma clang fp contract(off)
+  return (vel > T(0)) ? vel * (fc - fm) * inv_h : vel * (fp - fc) * inv_h;
+}
+
+// 5-point laplacian from the edge-padded neighbours (rk3_simple_new.py:215-219)
+template <typename T>
+__device__ __forceinline__ T rk_lap(T c, T W, T E, T S, T N, T inv_dx2, T inv_dy2) {
+#pragma clang fp contract(off)
+  return fma(fma(T(-2), c, E + W), inv_dx2, fma(T(-2), c, N + S) * inv_dy2);
+}
+
+// rhs = -upwind(f; u, v) + nu_tot lap(f) for f = u and f = v
+template <typename T>
+__device__ __forceinline__ void rk_rhs_cell(T uc, T uW, T uE, T uS, T uN, T vc, T vW, T vE, T vS,
+                                            T vN, T nu_tot, T inv_dx, T inv_dy, T inv_dx2,
+                                            T inv_dy2, T& ru, T& rv) {
+#pragma clang fp contract(off)
+  const T adv_u = rk_upwind(uc, uc, uW, uE, inv_dx) + rk_upwind(vc, uc, uS, uN, inv_dy);
+  const T adv_v = rk_upwind(uc, vc, vW, vE, inv_dx) + rk_upwind(vc, vc, vS, vN, inv_dy);
+  ru = fma(nu_tot, rk_lap(uc, uW, uE, uS, uN, inv_dx2, inv_dy2), -adv_u);
+  rv = fma(nu_tot, rk_lap(vc, vW, vE, vS, vN, inv_dx2, inv_dy2), -adv_v);
+}
+
+// Brinkman penalisation chi: sigmoid(-sdf / (0.1 dx)) from the SDF
+// (rk3_simple_new.py:198; scale = 1 / (0.1 dx)), else 1 - mask
+template <typename T>
+__device__ __forceinline__ T rk_chi(T c, bool from_sdf, T scale) {
+#pragma clang fp contract(off)
+  return from_sdf ? T(1) / (T(1) + exp(c * scale)) : T(1) - c;
+}
+
+// 1 / (1 + dt chi / eta): the Brinkman division of every stage
+// (rk3_simple_new.py:196-213)
+template <typename T>
+__device__ __forceinline__ T rk_inv_d(T chi, T dt, T inv_eta) {
+#pragma clang fp contract(off)
+  return T(1) / fma(dt * chi, inv_eta, T(1));
+}
+
+// stage s of RK3 (rk3_simple_new.py:315-331) or RK2 (FAST, :339-347) from
+// the step's start value f0, this stage's rhs k (RK2: k1 the first stage's)
+template <typename T, bool FAST>
+__device__ __forceinline__ T rk_stage(int s, T f0, T k, T k1, T dt, T iD) {
+#pragma clang fp contract(off)
+  if (s == 1) return fma(dt, k, f0) * iD;
+  if (FAST) return fma(T(0.5) * dt, k1 + k, f0) * iD;
+  const T fs = fma(dt, k, f0);
+  if (s == 2) return fma(T(0.75), f0, T(0.25) * fs) * iD;
+  return fma(T(1.0 / 3.0), f0, T(2.0 / 3.0) * fs) * iD;
+}
+
+// pressure rhs from the central differences of u* along x and v* along y
+// (divergence_nonperiodic, solver/operators.py:81-96), before the mask and 1/dt
+template <typename T>
+__device__ __forceinline__ T rk_div(T uE, T uW, T vN, T vS, T inv_2dx, T inv_2dy) {
+#pragma clang fp contract(off)
+  return fma(uE - uW, inv_2dx, (vN - vS) * inv_2dy);
+}
+
 // One stencil evaluation of rhs = -upwind(f; u, v) + nu_tot * lap(f)
 // [- nu_h * lap(lap f)] for f = u and f = v at LDS cell l.
 template <typename T, bool HYPER>
 __device__ __forceinline__ void rk_rhs(const T* fu, const T* fv, const T* lu, const T* lv,
                                        int l, int lW, int lE, int lS, int lN, T nu_tot,
                                        T nu_h, const RkArgs<T>& a, T& ru, T& rv) {
-  const T uc = fu[l], vc = fv[l];
-  const T uW = fu[lW], uE = fu[lE], uS = fu[lS], uN = fu[lN];
-  const T vW = fv[lW], vE = fv[lE], vS = fv[lS], vN = fv[lN];
-  // upwind_advection (rk3_simple_new.py:287-298): vel > 0 -> backward diff
-  T ax_u = (uc > T(0)) ? uc * (uc - uW) * a.inv_dx : uc * (uE - uc) * a.inv_dx;
-  T ay_u = (vc > T(0)) ? vc * (uc - uS) * a.inv_dy : vc * (uN - uc) * a.inv_dy;
-  T ax_v = (uc > T(0)) ? uc * (vc - vW) * a.inv_dx : uc * (vE - vc) * a.inv_dx;
-  T ay_v = (vc > T(0)) ? vc * (vc - vS) * a.inv_dy : vc * (vN - vc) * a.inv_dy;
-  // laplacian (rk3_simple_new.py:215-219)
-  T lap_u = (uE + uW - T(2) * uc) * a.inv_dx2 + (uN + uS - T(2) * uc) * a.inv_dy2;
-  T lap_v = (vE + vW - T(2) * vc) * a.inv_dx2 + (vN + vS - T(2) * vc) * a.inv_dy2;
-  ru = -(ax_u + ay_u) + nu_tot * lap_u;
-  rv = -(ax_v + ay_v) + nu_tot * lap_v;
+  rk_rhs_cell(fu[l], fu[lW], fu[lE], fu[lS], fu[lN], fv[l], fv[lW], fv[lE], fv[lS], fv[lN],
+              nu_tot, a.inv_dx, a.inv_dy, a.inv_dx2, a.inv_dy2, ru, rv);
   if constexpr (HYPER) {
-    const T bu = (lu[lE] + lu[lW] - T(2) * lu[l]) * a.inv_dx2 +
-                 (lu[lN] + lu[lS] - T(2) * lu[l]) * a.inv_dy2;
-    const T bv = (lv[lE] + lv[lW] - T(2) * lv[l]) * a.inv_dx2 +
-                 (lv[lN] + lv[lS] - T(2) * lv[l]) * a.inv_dy2;
-    ru = ru - nu_h * bu;
-    rv = rv - nu_h * bv;
+    const T bu = rk_lap(lu[l], lu[lW], lu[lE], lu[lS], lu[lN], a.inv_dx2, a.inv_dy2);
+    const T bv = rk_lap(lv[l], lv[lW], lv[lE], lv[lS], lv[lN], a.inv_dx2, a.inv_dy2);
+    ru = fma(-nu_h, bu, ru);
+    rv = fma(-nu_h, bv, rv);
   }
 }
 
 template <typename T>
 __device__ __forceinline__ T masked(T val, T m) {
+#pragma clang fp contract(off)
   return (m > T(0.5)) ? val : T(0);
 }
 
@@ -170,9 +226,7 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
       if (idx >= RN) break;
       s_u[idx] = ru[it];
       s_v[idx] = rv[it];
-      // sigmoid(-sdf / (0.1 dx)) (rk3_simple_new.py:198), else 1 - mask
-      const T chi = has_sdf ? T(1) / (T(1) + exp(rc[it] * a.sdf_scale)) : T(1) - rc[it];
-      s_iD[idx] = T(1) / (T(1) + a.dt * chi * a.inv_eta);
+      s_iD[idx] = rk_inv_d(rk_chi(rc[it], has_sdf, a.sdf_scale), a.dt, a.inv_eta);
       // nu_total = nu (+ nu_sgs) with nu a scalar or a field
       // (rk3_simple_new.py:304-309)
       s_nt[idx] = has_nsgs ? rf[it] + rn[it] : rf[it];
@@ -198,10 +252,8 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
         const int l = lx * RW + ly;
         const int lW = gx > 0 ? l - RW : l, lE = gx < nx - 1 ? l + RW : l;
         const int lS = gy > 0 ? l - 1 : l, lN = gy < ny - 1 ? l + 1 : l;
-        s_lu[l] = (fu[lE] + fu[lW] - T(2) * fu[l]) * a.inv_dx2 +
-                  (fu[lN] + fu[lS] - T(2) * fu[l]) * a.inv_dy2;
-        s_lv[l] = (fv[lE] + fv[lW] - T(2) * fv[l]) * a.inv_dx2 +
-                  (fv[lN] + fv[lS] - T(2) * fv[l]) * a.inv_dy2;
+        s_lu[l] = rk_lap(fu[l], fu[lW], fu[lE], fu[lS], fu[lN], a.inv_dx2, a.inv_dy2);
+        s_lv[l] = rk_lap(fv[l], fv[lW], fv[lE], fv[lS], fv[lN], a.inv_dx2, a.inv_dy2);
       }
       __syncthreads();
     }
@@ -215,33 +267,19 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
       rk_rhs<T, HYPER>(fu, fv, s_lu, s_lv, l, lW, lE, lS, lN, s_nt[l], HYPER ? s_nh[l] : T(0), a,
                        ku, kv);
       const T u0 = s_u[l], v0 = s_v[l], iD = s_iD[l];
+      T k1u = T(0), k1v = T(0);
       if constexpr (FAST) {
         if (s == 1) {
-          nu_ = (u0 + a.dt * ku) * iD;
-          nv_ = (v0 + a.dt * kv) * iD;
           // k1 is needed again at the final stage (margin 0 within margin R1)
           s_bu[l] = ku;
           s_bv[l] = kv;
         } else {
-          const T k1u = s_bu[l], k1v = s_bv[l];
-          nu_ = (u0 + T(0.5) * a.dt * (k1u + ku)) * iD;
-          nv_ = (v0 + T(0.5) * a.dt * (k1v + kv)) * iD;
-        }
-      } else {
-        if (s == 1) {
-          nu_ = (u0 + a.dt * ku) * iD;
-          nv_ = (v0 + a.dt * kv) * iD;
-        } else {
-          const T us = u0 + a.dt * ku, vs = v0 + a.dt * kv;
-          if (s == 2) {
-            nu_ = (T(0.75) * u0 + T(0.25) * us) * iD;
-            nv_ = (T(0.75) * v0 + T(0.25) * vs) * iD;
-          } else {
-            nu_ = (T(1.0 / 3.0) * u0 + T(2.0 / 3.0) * us) * iD;
-            nv_ = (T(1.0 / 3.0) * v0 + T(2.0 / 3.0) * vs) * iD;
-          }
+          k1u = s_bu[l];
+          k1v = s_bv[l];
         }
       }
+      nu_ = rk_stage<T, FAST>(s, u0, ku, k1u, a.dt, iD);
+      nv_ = rk_stage<T, FAST>(s, v0, kv, k1v, a.dt, iD);
     };
     if (s < S) {
       const int W = TY + 2 * m, Hh = TX + 2 * m;
@@ -313,7 +351,7 @@ __device__ __forceinline__ void rk_tile(const RkArgs<T>& a, int x0, int y0, T* s
         const T uW = (gx == 1) ? a.U_inf : s_fu[f - FW];
         const T vN = (gy == ny - 2) ? T(0) : s_fv[f + 1];
         const T vS = (gy == 1) ? T(0) : s_fv[f - 1];
-        const T d = (uE - uW) * a.inv_2dx + (vN - vS) * a.inv_2dy;
+        const T d = rk_div(uE, uW, vN, vS, a.inv_2dx, a.inv_2dy);
         a.rhs[g] = a.rhs_masked ? d * m0 * a.inv_dt : d * a.inv_dt;
       } else {
         T nu_, nv_;
@@ -353,7 +391,7 @@ __device__ __forceinline__ void div_rhs_cell(const T* __restrict__ us, const T* 
   const long g = (long)i * ny + j;
   T r = T(0);
   if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
-    const T d = (us[g + ny] - us[g - ny]) * inv_2dx + (vs[g + 1] - vs[g - 1]) * inv_2dy;
+    const T d = rk_div(us[g + ny], us[g - ny], vs[g + 1], vs[g - 1], inv_2dx, inv_2dy);
     r = mask ? d * mask[g] * inv_dt : d * inv_dt;
   }
   rhs[g] = r;
