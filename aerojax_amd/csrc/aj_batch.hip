@@ -155,8 +155,9 @@ __global__ __launch_bounds__(NTH) void kb_rk(BatchWork w, BatchStepArgs<T> s) {
 }
 
 // register-column RK3 predictor with the pressure rhs in its epilogue (FP32,
-// every group with even ny in [4, 128]): items are x-tiles of one instance,
-// each a strip of the whole row width
+// every group with even ny >= 4): items are tiles of one instance, x-tiles
+// of the whole row width when ny <= 128, else x-tiles x column strips
+// (w.tiles_y strips per x-tile)
 template <int W, int R>
 __global__ __launch_bounds__(W * 64) void kb_rk_rc(BatchWork w, BatchStepArgs<float> s) {
   using G = RkRcGeo<W, R, false, true>;
@@ -190,7 +191,14 @@ __global__ __launch_bounds__(W * 64) void kb_rk_rc(BatchWork w, BatchStepArgs<fl
     a.inv_2dy = (float)(1.0 / (2.0 * w.dy[g]));
     a.inv_dt = (float)(1.0 / s.dt);
     a.slip = 1;
-    rk_rc_tile<W, R, false, true>(a, it.sub * G::U - G::HX, 0, 0, a.ny, xb);
+    const int ty = it.sub % w.tiles_y[g], tx = it.sub / w.tiles_y[g];
+    if (a.ny <= 128) {
+      rk_rc_tile<W, R, false, true>(a, tx * G::U - G::HX, 0, 0, a.ny, xb);
+    } else {
+      const int gy0 = ty * G::UY - G::HY;
+      rk_rc_tile<W, R, false, true>(a, tx * G::U - G::HX, gy0, gy0 + G::HY, gy0 + G::HY + G::UY,
+                                    xb);
+    }
     __syncthreads();  // the exchange buffers are reused by the next item
   }
 }
@@ -834,12 +842,25 @@ int predict_launches(aj_batch* b, const BatchStepArgs<T>& s, hipStream_t st) {
     bool rc_ok = true;
     for (int g = 0; g < b->ng; ++g) {
       const int nx = b->lnx[g][0], ny = b->lny[g][0];
-      if ((ny & 1) || ny < 4 || ny > 128 || nx < 4) rc_ok = false;
+      if ((ny & 1) || ny < 4 || nx < 4) rc_ok = false;
     }
     if (rc_ok) {
-      // register columns: RK3 predictor and the pressure rhs in one launch
+      // register columns: RK3 predictor and the pressure rhs in one launch;
+      // tiles of U rows x (the whole row, or UY-column strips past 128 columns)
       using G = RkRcGeo<kRkRcW, kRkRcR, false, true>;
       BatchWork w = make_work(b, 0, 0, G::U, 128);
+      {
+        int tot = 0;
+        for (int g = 0; g < w.ng; ++g) {
+          const int ny = w.ny[g];
+          w.tiles_y[g] = ny <= 128 ? 1 : (int)cdiv(ny, G::UY);
+          w.per_inst[g] = (int)cdiv(w.nx[g], G::U) * w.tiles_y[g];
+          w.wstart[g] = tot;
+          tot += w.per_inst[g] * b->gcount[g];
+        }
+        w.wstart[w.ng] = tot;
+        w.total = tot;
+      }
       ProfScope prof(AJ_PROF_RK, (double)b->cells * sizeof(T) * (s.nsgs ? 8 : 7), st);
       return batch_launch(b, "rk_rc", w.total, [&](dim3 g, hipStream_t t) -> int {
         hipLaunchKernelGGL((kb_rk_rc<kRkRcW, kRkRcR>), g, dim3(kRkRcW * 64), 0, t, w, s);

@@ -61,7 +61,8 @@ struct RkArgs {
 template <typename T>
 __device__ __forceinline__ T rk_upwind(T vel, T fc, T fm, T fp, T inv_h) {
 #pragma clang fp contract(off)
-  return (vel > T(0)) ? vel * (fc - fm) * inv_h : vel * (fp - fc) * inv_h;
+  const T d = (vel > T(0)) ? fc - fm : fp - fc;
+  return vel * d * inv_h;
 }
 
 // 5-point laplacian from the edge-padded neighbours (rk3_simple_new.py:215-219)
