@@ -1082,8 +1082,8 @@ static int tb_launch_post_rc(const TbLevel& Lv, int nb, const float* Y, const fl
 }
 
 // smoother candidates, tuned per level shape: LDS tiles (rows x columns,
-// threads) 0-3, register columns (waves x rows per wave) 4-7
-constexpr int kTbTileCands = 8;
+// threads) 0-3, register columns (waves x rows per wave) 4-8
+constexpr int kTbTileCands = 9;
 
 // pre-smoothing on level lev (lev < coarsest): pin (null = zero start) ->
 // Y, coarse rhs tB[lev + 1]
@@ -1102,6 +1102,7 @@ static int tb_pre(aj_mg_plan* p, int lev, const float* pin, float* Y, hipStream_
       case 5: return tb_launch_pre_rc<4, 8>(Lv, nb, pin, b, Y, Bc, gate, s);
       case 6: return tb_launch_pre_rc<8, 6>(Lv, nb, pin, b, Y, Bc, gate, s);
       case 7: return tb_launch_pre_rc<16, 4>(Lv, nb, pin, b, Y, Bc, gate, s);
+      case 8: return tb_launch_pre_rc<16, 2>(Lv, nb, pin, b, Y, Bc, gate, s);
       default: return tb_launch_pre<32, 64, 1024>(Lv, nb, pin, b, Y, Bc, gate, s);
     }
   };
@@ -1131,6 +1132,7 @@ static int tb_post(aj_mg_plan* p, int lev, const float* Y, float* E, hipStream_t
       case 5: return tb_launch_post_rc<4, 8>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
       case 6: return tb_launch_post_rc<8, 6>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
       case 7: return tb_launch_post_rc<16, 4>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
+      case 8: return tb_launch_post_rc<16, 2>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
       default: return tb_launch_post<32, 64, 1024>(Lv, nb, Y, Ec, nc, mc, gamc, b, E, gate, s);
     }
   };

@@ -228,14 +228,14 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
       }
     }
   };
-  // the next slice (the last one again after the end) is loaded while the
-  // current one is multiplied
+  // the next slice is loaded while the current one is multiplied (none
+  // after the last: a wave retires only once its loads have landed)
   fetch(rx, rw, 0);
   for (int s = 0; s < ns; ++s) {
     __syncthreads();  // every wave is done with the previous slice
     stash(rx, rw);
     __syncthreads();
-    fetch(rx, rw, min(s + 1, ns - 1) * 32);
+    if (s + 1 < ns) fetch(rx, rw, (s + 1) * 32);
     compute();
   }
 #pragma unroll

@@ -74,7 +74,7 @@ def test_rk_candidates_bit_identical(gpu, shape, variant):
 @pytest.mark.parametrize("shape", [(2048, 512), (1000, 300), (1024, 96), (512, 256)])
 def test_smoother_candidates_bit_identical(gpu, shape):
     """Textbook plans with every level's pre- and post-smoother forced to
-    one candidate (LDS tiles 0-3, register columns 4-7): two V-cycles of
+    one candidate (LDS tiles 0-3, register columns 4-8): two V-cycles of
     the same rhs give the same bits."""
     import aerojax_amd as aj
     from aerojax_amd import _lib
@@ -92,7 +92,7 @@ def test_smoother_candidates_bit_identical(gpu, shape):
         m //= 2
     assert shapes, "no smoothed level"
     res = {}
-    for cand in range(8):
+    for cand in range(9):
         L.aj_tune_reset()
         text = "".join(f"tb_pre/1x{a}x{c} -> {cand}\ntb_post/1x{a}x{c} -> {cand}\n"
                        for a, c in shapes)
