@@ -256,18 +256,32 @@ static int conv_launch_t(const ConvArgs& c, int nt, hipStream_t st) {
 
 // candidates: pixel tiles 8x32 and 4x16 with the widest output-channel
 // block NT that divides co, then 4x16 with narrower blocks (more workgroups
-// for layers with few pixel tiles); tuned per layer shape.  Launches that
-// accumulate into their output (input gradients of a concatenated input)
-// reuse the choice of the plain launch of the same shape.
+// for layers with few pixel tiles), then 8x16 with the widest and the next
+// block (two 16-pixel rows per wave: each LDS fragment feeds two MFMAs,
+// where the 4x16 tiles' A fragments feed NT / 16); tuned per layer shape.
+// Launches that accumulate into their output (input gradients of a
+// concatenated input) reuse the choice of the plain launch of the same
+// shape.
 int conv_launch(const ConvArgs& c, hipStream_t st) {
   if (c.ci % 32 || c.co % 16) return fail(AJ_ERR_INVALID, "conv: ci % 32 or co % 16 != 0");
   const int nt = (c.co % 64 == 0) ? 64 : (c.co % 32 == 0 ? 32 : 16);
-  int cands[4][2] = {{0, nt}, {1, nt}, {1, 0}, {1, 0}};
+  enum { kT8x32 = 0, kT4x16 = 1, kT8x16 = 2 };
+  int cands[6][2] = {{kT8x32, nt}, {kT4x16, nt}};
   int ncand = 2;
-  for (int w = nt / 2; w >= 16 && ncand < 4; w /= 2) cands[ncand++][1] = w;
+  for (int w = nt / 2; w >= 16; w /= 2) {
+    cands[ncand][0] = kT4x16;
+    cands[ncand++][1] = w;
+  }
+  for (int w = nt; w >= 16 && w >= nt / 2; w /= 2) {
+    cands[ncand][0] = kT8x16;
+    cands[ncand++][1] = w;
+  }
   auto launch = [&](int k, hipStream_t s) -> int {
-    return cands[k][0] == 1 ? conv_launch_t<4, 16>(c, cands[k][1], s)
-                            : conv_launch_t<8, 32>(c, cands[k][1], s);
+    switch (cands[k][0]) {
+      case kT4x16: return conv_launch_t<4, 16>(c, cands[k][1], s);
+      case kT8x16: return conv_launch_t<8, 16>(c, cands[k][1], s);
+      default: return conv_launch_t<8, 32>(c, cands[k][1], s);
+    }
   };
   char key[96];
   std::snprintf(key, sizeof(key), "conv/%dx%dx%d/%d->%d/%s%s", c.B, c.H, c.W, c.ci, c.co,

@@ -113,8 +113,10 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   constexpr int NWL = (9 * NT * 4 + 255) / 256;  // 16-byte weight loads per thread and slice
   constexpr int NQ = SRC == 1 ? 4 : 1;           // loads per input element (pooling reads 2x2)
   static_assert(NXL <= 32, "conv input patch");
-  __shared__ __attribute__((aligned(16))) bf16_t sx[RP * kLdsRow];
-  __shared__ __attribute__((aligned(16))) bf16_t sw[9 * NT * kLdsRow];
+  // one spare row each: a thread without an element stages into it, so
+  // that staging has no branches (and the counter waits no joins)
+  __shared__ __attribute__((aligned(16))) bf16_t sx[(RP + 1) * kLdsRow];
+  __shared__ __attribute__((aligned(16))) bf16_t sw[(9 * NT + 1) * kLdsRow];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int tilesW = (a.W + PW - 1) / PW, tilesH = (a.H + PH - 1) / PH;
   int t = blockIdx.x;
@@ -169,7 +171,6 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     const int col = r / 9, tap = r - col * 9;
     wo[k] = e < 9 * NT * 4 ? ((co0 + col) * 9 + tap) * a.ci + q * 8 : 0;
   }
-  u32x4 rx[NXL][NQ], rw[NWL];
   // global -> registers: slice at channel c0 (input patch with a one-pixel
   // halo, and the slice's weights, row (tap, co) holding 32 channels)
   auto fetch = [&](u32x4 (&X)[NXL][NQ], u32x4 (&Wt)[NWL], int c0) __attribute__((always_inline)) {
@@ -189,22 +190,20 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
 #pragma unroll
     for (int k = 0; k < NXL; ++k) {
       const int e = tid + k * 256;
-      if (e < RP * 4) {
-        u32x4 v = SRC == 1 ? pool4(X[k][0], X[k][NQ > 1 ? 1 : 0], X[k][NQ > 2 ? 2 : 0],
-                                   X[k][NQ > 3 ? 3 : 0])
-                           : X[k][0];
-        if (!((okx >> k) & 1u)) v = u32x4{0u, 0u, 0u, 0u};
-        *reinterpret_cast<u32x4*>(sx + (e >> 2) * kLdsRow + (e & 3) * 8) = v;
-      }
+      u32x4 v = SRC == 1 ? pool4(X[k][0], X[k][NQ > 1 ? 1 : 0], X[k][NQ > 2 ? 2 : 0],
+                                 X[k][NQ > 3 ? 3 : 0])
+                         : X[k][0];
+      if (!((okx >> k) & 1u)) v = u32x4{0u, 0u, 0u, 0u};
+      const int row = e < RP * 4 ? (e >> 2) : RP;
+      *reinterpret_cast<u32x4*>(sx + row * kLdsRow + (e & 3) * 8) = v;
     }
 #pragma unroll
     for (int k = 0; k < NWL; ++k) {
       const int e = tid + k * 256;
-      if (e < 9 * NT * 4) {
-        const int q = e & 3, r = e >> 2;
-        const int col = r / 9, tap = r - col * 9;
-        *reinterpret_cast<u32x4*>(sw + (tap * NT + col) * kLdsRow + q * 8) = Wt[k];
-      }
+      const int r = e >> 2;
+      const int col = r / 9, tap = r - col * 9;
+      const int row = e < 9 * NT * 4 ? tap * NT + col : 9 * NT;
+      *reinterpret_cast<u32x4*>(sw + row * kLdsRow + (e & 3) * 8) = Wt[k];
     }
   };
   // one slice's MFMAs from LDS
@@ -230,6 +229,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   };
   // the next slice is loaded while the current one is multiplied (none
   // after the last: a wave retires only once its loads have landed)
+  u32x4 rx[NXL][NQ], rw[NWL];
   fetch(rx, rw, 0);
   for (int s = 0; s < ns; ++s) {
     __syncthreads();  // every wave is done with the previous slice

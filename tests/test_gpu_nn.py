@@ -82,7 +82,7 @@ def test_conv_tile_candidates_are_bit_identical(net, B, H, W):
         lines = [l for l in _lib.tune_report().splitlines() if l.startswith("conv/")]
         assert lines
         ncand = {l.split(" -> ")[0]: len(l.split("[")[1].split()) - 1 for l in lines}
-        assert max(ncand.values()) == 4
+        assert max(ncand.values()) == 6
         for k in range(max(ncand.values())):
             L.aj_tune_reset()
             _lib.tune_load("\n".join(f"{key} -> {min(k, n - 1)}" for key, n in ncand.items()))
