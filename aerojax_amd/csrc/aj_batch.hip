@@ -825,6 +825,31 @@ BatchStepArgs<T> step_args(aj_batch* b, const aj_batch_step_params* prm, const v
   return s;
 }
 
+// one register-column RK launch of W waves x R rows per wave: tiles of U
+// rows x (the whole row, or UY-column strips past 128 columns)
+template <int W, int R>
+int batch_rk_rc(aj_batch* b, const BatchStepArgs<float>& s, hipStream_t st) {
+  using G = RkRcGeo<W, R, false, true>;
+  BatchWork w = make_work(b, 0, 0, G::U, 128);
+  {
+    int tot = 0;
+    for (int g = 0; g < w.ng; ++g) {
+      const int ny = w.ny[g];
+      w.tiles_y[g] = ny <= 128 ? 1 : (int)cdiv(ny, G::UY);
+      w.per_inst[g] = (int)cdiv(w.nx[g], G::U) * w.tiles_y[g];
+      w.wstart[g] = tot;
+      tot += w.per_inst[g] * b->gcount[g];
+    }
+    w.wstart[w.ng] = tot;
+    w.total = tot;
+  }
+  return batch_launch(b, "rk_rc", w.total, [&](dim3 g, hipStream_t t) -> int {
+    hipLaunchKernelGGL((kb_rk_rc<W, R>), g, dim3(W * 64), 0, t, w, s);
+    AJ_LAUNCH_CHECK("kb_rk_rc");
+    return AJ_OK;
+  }, st);
+}
+
 // [LES] + RK3 predictor (u*, v* in the batch's buffers) + rhs
 template <typename T>
 int predict_launches(aj_batch* b, const BatchStepArgs<T>& s, hipStream_t st) {
@@ -845,28 +870,9 @@ int predict_launches(aj_batch* b, const BatchStepArgs<T>& s, hipStream_t st) {
       if ((ny & 1) || ny < 4 || nx < 4) rc_ok = false;
     }
     if (rc_ok) {
-      // register columns: RK3 predictor and the pressure rhs in one launch;
-      // tiles of U rows x (the whole row, or UY-column strips past 128 columns)
-      using G = RkRcGeo<kRkRcW, kRkRcR, false, true>;
-      BatchWork w = make_work(b, 0, 0, G::U, 128);
-      {
-        int tot = 0;
-        for (int g = 0; g < w.ng; ++g) {
-          const int ny = w.ny[g];
-          w.tiles_y[g] = ny <= 128 ? 1 : (int)cdiv(ny, G::UY);
-          w.per_inst[g] = (int)cdiv(w.nx[g], G::U) * w.tiles_y[g];
-          w.wstart[g] = tot;
-          tot += w.per_inst[g] * b->gcount[g];
-        }
-        w.wstart[w.ng] = tot;
-        w.total = tot;
-      }
+      // register columns: RK3 predictor and the pressure rhs in one launch
       ProfScope prof(AJ_PROF_RK, (double)b->cells * sizeof(T) * (s.nsgs ? 8 : 7), st);
-      return batch_launch(b, "rk_rc", w.total, [&](dim3 g, hipStream_t t) -> int {
-        hipLaunchKernelGGL((kb_rk_rc<kRkRcW, kRkRcR>), g, dim3(kRkRcW * 64), 0, t, w, s);
-        AJ_LAUNCH_CHECK("kb_rk_rc");
-        return AJ_OK;
-      }, st);
+      return batch_rk_rc<kRkRcW, kRkRcR>(b, s, st);
     }
   }
   // RK3 predictor (the predictor BCs of step.py:182-191 are its own last step)

@@ -50,13 +50,17 @@ int tune_pick(const std::string& key, int ncand, const std::function<int(int, hi
     if (reps > 1) prof_count(reps);
     return AJ_OK;
   };
+  // the decision is read under the lock and launched outside it: a launch
+  // may tune a kernel of its own (nested keys)
+  int known = -1;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
-    if (it != g_tune.end() && it->second.first < ncand) {
-      *pick = it->second.first;
-      return run(*pick);
-    }
+    if (it != g_tune.end() && it->second.first < ncand) known = it->second.first;
+  }
+  if (known >= 0) {
+    *pick = known;
+    return run(known);
   }
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   (void)hipStreamIsCapturing(st, &cap);
