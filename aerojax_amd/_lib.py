@@ -266,7 +266,9 @@ _NN_SIGS = {
                               _vp]),
     "aj_nn_warm_start_batched": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, ctypes.c_float,
                                       ctypes.c_float, _vp]),
+    "aj_nn_set_option": (_i, [_vp, _i, _i]),
 }
+AJ_NN_OPT_FUSED_IO = 0
 _SIGS.update(_NN_SIGS)
 EXPORTS = EXPORTS + tuple(_NN_SIGS)
 PROF_TAGS = PROF_TAGS + ("nn_conv", "nn_dw", "les")

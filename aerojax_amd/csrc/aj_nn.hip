@@ -52,6 +52,7 @@ struct Act {  // activations and gradients of one batch shape
 struct aj_nn {
   int c0 = 32, L = 4, cin = 2;
   int device = 0;
+  int fuse_io = 1;  // AJ_NN_OPT_FUSED_IO
   std::vector<Layer> layers;  // enc (2 per level), dec (2 per level above the bottom)
   long head_w = 0, head_b = 0, nparams = 0;
   float* params = nullptr;
@@ -232,25 +233,57 @@ struct ConvSrc {
   int mode = 0;
   const bf16_t* in2 = nullptr;
   int ld2 = 0, cu = 0;
+  // mode 3: the network input row from R, T and the partial sums of R^2
+  const float* inR = nullptr;
+  const float* inT = nullptr;
+  const double* part = nullptr;
+  int nblk = 0;
+  float* s_out = nullptr;
+};
+
+// the 1x1 head applied in a convolution's epilogue (its activation is then
+// not stored): weights, bias, per-instance s, pscale, output
+struct HeadOut {
+  const float* w;
+  const float* b;
+  const float* s;
+  float scale;
+  float* out;
 };
 
 // tile choice: 8x32-pixel patches where the layer has enough of them to
 // fill the chip, 4x16 otherwise; NT = 64 / 32 / 16 output channels
-template <int PH, int PW, int SRC>
+// the head epilogue stages the tile's activations in the input patch's LDS
+template <int PH, int PW, int NT>
+constexpr bool kHeadFits = PH * PW * NT <= ((PH + 2) * (PW + 2) + 1) * kLdsRow;
+
+template <int PH, int PW, int NT, int SRC, bool HEAD>
+static int conv_launch_nt(const ConvArgs& c, dim3 grid, hipStream_t st) {
+  if constexpr (HEAD && !kHeadFits<PH, PW, NT>) {
+    (void)c, (void)grid, (void)st;
+    return fail(AJ_ERR_INVALID, "conv: head tile larger than the LDS patch");
+  } else {
+    hipLaunchKernelGGL((k_conv3x3<PH, PW, NT, SRC, HEAD>), grid, dim3(256), 0, st, c);
+    AJ_LAUNCH_CHECK("k_conv3x3");
+    return AJ_OK;
+  }
+}
+
+template <int PH, int PW, int SRC, bool HEAD = false>
 static int conv_launch_src(const ConvArgs& c, int nt, hipStream_t st) {
   const long tiles = (long)c.B * ((c.H + PH - 1) / PH) * ((c.W + PW - 1) / PW);
-  dim3 grid((unsigned)tiles, c.co / nt);
-  if (nt == 64) hipLaunchKernelGGL((k_conv3x3<PH, PW, 64, SRC>), grid, dim3(256), 0, st, c);
-  else if (nt == 32) hipLaunchKernelGGL((k_conv3x3<PH, PW, 32, SRC>), grid, dim3(256), 0, st, c);
-  else hipLaunchKernelGGL((k_conv3x3<PH, PW, 16, SRC>), grid, dim3(256), 0, st, c);
-  AJ_LAUNCH_CHECK("k_conv3x3");
-  return AJ_OK;
+  const dim3 grid((unsigned)tiles, c.co / nt);
+  if (nt == 64) return conv_launch_nt<PH, PW, 64, SRC, HEAD>(c, grid, st);
+  if (nt == 32) return conv_launch_nt<PH, PW, 32, SRC, HEAD>(c, grid, st);
+  return conv_launch_nt<PH, PW, 16, SRC, HEAD>(c, grid, st);
 }
 
 template <int PH, int PW>
 static int conv_launch_t(const ConvArgs& c, int nt, hipStream_t st) {
+  if (c.hout) return conv_launch_src<PH, PW, 0, true>(c, nt, st);  // plain input only
   if (c.src == 1) return conv_launch_src<PH, PW, 1>(c, nt, st);
   if (c.src == 2) return conv_launch_src<PH, PW, 2>(c, nt, st);
+  if (c.src == 3) return conv_launch_src<PH, PW, 3>(c, nt, st);
   return conv_launch_src<PH, PW, 0>(c, nt, st);
 }
 
@@ -261,20 +294,28 @@ static int conv_launch_t(const ConvArgs& c, int nt, hipStream_t st) {
 // where the 4x16 tiles' A fragments feed NT / 16); tuned per layer shape.
 // Launches that accumulate into their output (input gradients of a
 // concatenated input) reuse the choice of the plain launch of the same
-// shape.
+// shape.  A launch with the head epilogue needs every output channel in one
+// workgroup: its candidates are the three pixel tiles with NT = co.
 int conv_launch(const ConvArgs& c, hipStream_t st) {
   if (c.ci % 32 || c.co % 16) return fail(AJ_ERR_INVALID, "conv: ci % 32 or co % 16 != 0");
+  if (c.hout && (c.src != 0 || (c.co != 16 && c.co != 32 && c.co != 64)))
+    return fail(AJ_ERR_INVALID, "conv: the head epilogue needs a plain input and co of 16, 32 or 64");
   const int nt = (c.co % 64 == 0) ? 64 : (c.co % 32 == 0 ? 32 : 16);
   enum { kT8x32 = 0, kT4x16 = 1, kT8x16 = 2 };
   int cands[6][2] = {{kT8x32, nt}, {kT4x16, nt}};
   int ncand = 2;
-  for (int w = nt / 2; w >= 16; w /= 2) {
-    cands[ncand][0] = kT4x16;
-    cands[ncand++][1] = w;
-  }
-  for (int w = nt; w >= 16 && w >= nt / 2; w /= 2) {
+  if (c.hout) {
     cands[ncand][0] = kT8x16;
-    cands[ncand++][1] = w;
+    cands[ncand++][1] = nt;
+  } else {
+    for (int w = nt / 2; w >= 16; w /= 2) {
+      cands[ncand][0] = kT4x16;
+      cands[ncand++][1] = w;
+    }
+    for (int w = nt; w >= 16 && w >= nt / 2; w /= 2) {
+      cands[ncand][0] = kT8x16;
+      cands[ncand++][1] = w;
+    }
   }
   auto launch = [&](int k, hipStream_t s) -> int {
     switch (cands[k][0]) {
@@ -284,15 +325,18 @@ int conv_launch(const ConvArgs& c, hipStream_t st) {
     }
   };
   char key[96];
-  std::snprintf(key, sizeof(key), "conv/%dx%dx%d/%d->%d/%s%s", c.B, c.H, c.W, c.ci, c.co,
-                c.outf ? "f32" : "bf16", c.src == 1 ? "/pool" : (c.src == 2 ? "/up" : ""));
+  std::snprintf(key, sizeof(key), "conv/%dx%dx%d/%d->%d/%s%s%s", c.B, c.H, c.W, c.ci, c.co,
+                c.outf ? "f32" : "bf16",
+                c.src == 1 ? "/pool" : (c.src == 2 ? "/up" : (c.src == 3 ? "/in" : "")),
+                c.hout ? "/head" : "");
   int pick = 0;
   return tune_pick(key, c.accum ? 1 : ncand, launch, st, &pick);
 }
 
 // forward conv + GELU: out (post) and pre both stored
 int conv_fwd(aj_nn* n, int li, const bf16_t* in, int ldi, bf16_t* out, int ldo, bf16_t* pre,
-             int B, int H, int W, hipStream_t st, const ConvSrc& src = ConvSrc()) {
+             int B, int H, int W, hipStream_t st, const ConvSrc& src = ConvSrc(),
+             const HeadOut* ho = nullptr) {
   const Layer& Ly = n->layers[li];
   ConvArgs c{};
   c.in = in; c.ldi = ldi; c.ci = Ly.ci_pad;
@@ -301,6 +345,11 @@ int conv_fwd(aj_nn* n, int li, const bf16_t* in, int ldi, bf16_t* out, int ldo, 
   c.out = out; c.ldo = ldo; c.pre = pre; c.ldp = Ly.co;
   c.outf = nullptr; c.ldf = 0; c.accum = 0; c.act = 1;
   c.src = src.mode; c.in2 = src.in2; c.ld2 = src.ld2; c.cu = src.cu;
+  c.inR = src.inR; c.inT = src.inT; c.part = src.part; c.nblk = src.nblk; c.s_out = src.s_out;
+  if (ho) {
+    c.out = nullptr;
+    c.hw = ho->w; c.hb = ho->b; c.hs = ho->s; c.hscale = ho->scale; c.hout = ho->out;
+  }
   ProfScope prof(AJ_PROF_NN_CONV, 2.0 * 9 * Ly.ci * Ly.co * (double)B * H * W, st);
   return conv_launch(c, st);
 }
@@ -355,8 +404,11 @@ int conv_bwd(aj_nn* n, int li, const float* gpost, int ldg, const bf16_t* pre, c
 // convolutions directly instead of being materialised, and the
 // pre-activations are not stored (the backward pass needs both, so
 // training uses fused = false)
+// in0: the first convolution's input mode instead of the staged input row
+// (warm start: the row is built in its loads); ho: the 1x1 head in the last
+// convolution's epilogue (warm start; its activation is then not stored)
 int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true,
-            bool fused = false) {
+            bool fused = false, const ConvSrc* in0 = nullptr, const HeadOut* ho = nullptr) {
   Act& a = n->act;
   const int L = n->L;
   auto pre = [&](bf16_t* buf) -> bf16_t* { return fused ? nullptr : buf; };
@@ -366,7 +418,11 @@ int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true,
     const bf16_t* in;
     int ldi;
     ConvSrc src{};
-    if (l == 0) { in = a.X0; ldi = kCinPad; }
+    if (l == 0) {
+      in = a.X0;
+      ldi = kCinPad;
+      if (in0) src = *in0;
+    }
     else if (fused) {
       // 2x2 average of the skip of level l-1 (channels chan(l) .. of CAT[l-1])
       const int cp = chan(n, l - 1);
@@ -409,7 +465,8 @@ int forward(aj_nn* n, int B, int H, int W, hipStream_t st, bool head = true,
     if (int rc = conv_fwd(n, dec_a(n, l), a.CAT[l], cu + c, a.DA[l], c, pre(a.DPA[l]), B, Hl, Wl, st,
                           src))
       return rc;
-    if (int rc = conv_fwd(n, dec_b(n, l), a.DA[l], c, a.DB[l], c, pre(a.DPB[l]), B, Hl, Wl, st))
+    if (int rc = conv_fwd(n, dec_b(n, l), a.DA[l], c, a.DB[l], c, pre(a.DPB[l]), B, Hl, Wl, st,
+                          ConvSrc(), l == 0 ? ho : nullptr))
       return rc;
   }
   if (!head) return AJ_OK;
@@ -670,12 +727,12 @@ __global__ __launch_bounds__(256) void k_ws_head(const bf16_t* x, int ldx, int C
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
         const unsigned u = q[h];
-        v += bf2f((bf16_t)(u & 0xffffu)) * w[8 * c8 + 2 * h];
-        v += bf2f((bf16_t)(u >> 16)) * w[8 * c8 + 2 * h + 1];
+        v = head_acc(v, bf2f((bf16_t)(u & 0xffffu)), w[8 * c8 + 2 * h]);
+        v = head_acc(v, bf2f((bf16_t)(u >> 16)), w[8 * c8 + 2 * h + 1]);
       }
     }
   } else {
-    for (int c = 0; c < C; ++c) v += bf2f(x[p * ldx + c]) * w[c];
+    for (int c = 0; c < C; ++c) v = head_acc(v, bf2f(x[p * ldx + c]), w[c]);
   }
   out[p] = v * s[ws_quot(p, ni, 1.0 / (double)ni)] * pscale;
 }
@@ -836,6 +893,17 @@ extern "C" int aj_nn_destroy(aj_nn* n) {
 }
 
 extern "C" long aj_nn_num_params(const aj_nn* n) { return n ? n->nparams : 0; }
+
+extern "C" int aj_nn_set_option(aj_nn* n, int option, int value) {
+  if (!n) return fail(AJ_ERR_INVALID, "aj_nn_set_option: null network");
+  switch (option) {
+    case AJ_NN_OPT_FUSED_IO:
+      n->fuse_io = value ? 1 : 0;
+      return AJ_OK;
+    default:
+      return fail(AJ_ERR_INVALID, "aj_nn_set_option: unknown option");
+  }
+}
 
 extern "C" int aj_nn_get_params(aj_nn* n, float* host, long count, int which) {
   if (!n || !host || count != n->nparams) return fail(AJ_ERR_INVALID, "aj_nn_get_params: bad args");
@@ -1007,15 +1075,30 @@ static int warm_start_launches(aj_nn* n, const float* rhs, const float* sdf, flo
       default: hipLaunchKernelGGL(k_ws_prep<3>, g2, dim3(256), 0, st, rhs, sdf, nx, ny, sdf_ref, R, S, part); break;
     }
     AJ_LAUNCH_CHECK("k_ws_prep");
-    hipLaunchKernelGGL(k_ws_input, g2, dim3(256), 0, st, (const float*)R, (const float*)S,
-                       (const double*)part, nblk, ni, n->act.X0, sv);
-    AJ_LAUNCH_CHECK("k_ws_input");
-    if (int rc = forward(n, B, H, W, st, false, true)) return rc;
-    hipLaunchKernelGGL(k_ws_head, dim3(blocks(nc)), dim3(256), 0, st, (const bf16_t*)n->act.DB[0],
-                       n->c0, n->c0, (const float*)(n->params + n->head_w),
-                       (const float*)(n->params + n->head_b), (const float*)sv, ni, pscale, A, nc,
-                       (((uintptr_t)n->act.DB[0] % 16) == 0 && n->c0 % 8 == 0) ? 1 : 0);
-    AJ_LAUNCH_CHECK("k_ws_head");
+    const float* hw = n->params + n->head_w;
+    const float* hb = n->params + n->head_b;
+    if (n->fuse_io && (n->c0 == 32 || n->c0 == 64)) {
+      // the input row in the first convolution's loads, the head in the
+      // last one's epilogue (the arithmetic of k_ws_input and k_ws_head)
+      ConvSrc in0{};
+      in0.mode = 3;
+      in0.inR = R;
+      in0.inT = S;
+      in0.part = part;
+      in0.nblk = nblk;
+      in0.s_out = sv;
+      const HeadOut ho{hw, hb, sv, pscale, A};
+      if (int rc = forward(n, B, H, W, st, false, true, &in0, &ho)) return rc;
+    } else {
+      hipLaunchKernelGGL(k_ws_input, g2, dim3(256), 0, st, (const float*)R, (const float*)S,
+                         (const double*)part, nblk, ni, n->act.X0, sv);
+      AJ_LAUNCH_CHECK("k_ws_input");
+      if (int rc = forward(n, B, H, W, st, false, true)) return rc;
+      hipLaunchKernelGGL(k_ws_head, dim3(blocks(nc)), dim3(256), 0, st, (const bf16_t*)n->act.DB[0],
+                         n->c0, n->c0, hw, hb, (const float*)sv, ni, pscale, A, nc,
+                         (((uintptr_t)n->act.DB[0] % 16) == 0 && n->c0 % 8 == 0) ? 1 : 0);
+      AJ_LAUNCH_CHECK("k_ws_head");
+    }
     Gam4 gam{};
     for (int l = 0; l <= r; ++l) gam.g[l] = (float)gamma_level(l);
     const dim3 gp(blocks((long)nx * ny), B);
@@ -1081,7 +1164,7 @@ extern "C" int aj_nn_warm_start_batched(aj_nn* n, const float* rhs, const float*
                                      (uint64_t)(uintptr_t)p0, (uint64_t)B, (uint64_t)nx,
                                      (uint64_t)ny, (uint64_t)r, bits(sdf_ref), bits(pscale),
                                      (uint64_t)(uintptr_t)n->ws_mem,
-                                     (uint64_t)(uintptr_t)n->act.mem};
+                                     (uint64_t)(uintptr_t)n->act.mem, (uint64_t)n->fuse_io};
   if (prof_on() || !(r >= 1 && r <= 3))
     return warm_start_launches(n, rhs, sdf, p0, B, nx, ny, r, sdf_ref, pscale, st);
   for (auto& g : n->ws_graphs)

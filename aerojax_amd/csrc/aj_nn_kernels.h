@@ -63,7 +63,16 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 // needed afterwards): src = 1 reads the input as the 2x2 average of `in`
 // (2H x 2W, the arithmetic of k_pool); src = 2 reads channels < cu as the
 // nearest upsampling of in2 (H/2 x W/2, stride ld2, as k_up) and channels
-// >= cu from `in` (the concatenation's skip part).
+// >= cu from `in` (the concatenation's skip part); src = 3 builds the
+// warm-start network's input row from the restricted rhs R and T =
+// tanh(sdf_r / sdf_ref) (FP32, B x H x W): channel 0 = R / s, channel 1 =
+// T, the others 0, with s the instance's RMS of R from the restriction's
+// partial sums (the arithmetic of k_ws_input, whose row it replaces).
+//
+// HEAD epilogue (inference, NT = co): the activations of the workgroup's
+// pixels go through LDS instead of memory and each pixel's 1x1 head (bias
+// + the channel sum in channel order, as k_ws_head) times s pscale is
+// written.
 // ---------------------------------------------------------------------------
 struct ConvArgs {
   const bf16_t* in;
@@ -82,7 +91,21 @@ struct ConvArgs {
   int src;              // 0 = in, 1 = 2x2 average of in, 2 = [up(in2), in]
   const bf16_t* in2;    // src = 2: lower-resolution tensor
   int ld2, cu;          // its channel stride, channels taken from it
+  const float* inR;     // src = 3: restricted rhs and tanh(sdf_r / sdf_ref)
+  const float* inT;
+  const double* part;   // src = 3: per instance nblk partial sums of R^2
+  int nblk;
+  float* s_out;         // src = 3: per-instance s, written by the instance's first workgroup
+  const float* hw;      // HEAD: head weights [co], bias, per-instance s, pscale, output
+  const float* hb;
+  const float* hs;
+  float hscale;
+  float* hout;          // B x H x W
 };
+
+// one term of the 1x1 head's channel sum (the same operation in the fused
+// epilogue and in k_ws_head)
+__device__ __forceinline__ float head_acc(float v, float x, float w) { return fmaf(x, w, v); }
 
 // 2x2 average of four bf16x8 rows, channel by channel (k_pool's order)
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
@@ -101,10 +124,10 @@ __device__ __forceinline__ u32x4 pool4(u32x4 a, u32x4 b, u32x4 c, u32x4 d) {
 
 constexpr int kLdsRow = 40;  // bf16 per LDS row (32 channels + pad)
 
-template <int PH, int PW, int NT, int SRC>
+template <int PH, int PW, int NT, int SRC, bool HEAD = false>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   static_assert(PH % 4 == 0 && PW % 16 == 0 && NT % 16 == 0, "conv tile");
-  static_assert(SRC >= 0 && SRC <= 2, "conv input mode");
+  static_assert(SRC >= 0 && SRC <= 3, "conv input mode");
   constexpr int RW = PW + 2, RP = (PH + 2) * RW;
   constexpr int WR = PH / 4;         // pixel rows per wave
   constexpr int SX = PW / 16;        // 16-pixel sub-tiles per row
@@ -141,6 +164,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   const bf16_t* img = SRC == 1 ? a.in + (long)b * 4 * a.H * a.W * a.ldi   // 2H x 2W input
                                : a.in + (long)b * a.H * a.W * a.ldi;
   const bf16_t* img2 = SRC == 2 ? a.in2 + (long)b * (a.H >> 1) * (a.W >> 1) * a.ld2 : nullptr;
+  const float* Rin = SRC == 3 ? a.inR + (long)b * a.H * a.W : nullptr;
+  const float* Tin = SRC == 3 ? a.inT + (long)b * a.H * a.W : nullptr;
   int xo[NXL][NQ], xo2[NXL];
   unsigned okx = 0;
 #pragma unroll
@@ -158,6 +183,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
       xo[k][NQ > 1 ? 1 : 0] = ok ? f + a.ldi : 0;
       xo[k][NQ > 2 ? 2 : 0] = ok ? f + W2 * a.ldi : 0;
       xo[k][NQ > 3 ? 3 : 0] = ok ? f + (W2 + 1) * a.ldi : 0;
+    } else if (SRC == 3) {
+      xo[k][0] = ok ? hh * a.W + ww : 0;  // pixel of the instance
     } else {
       xo[k][0] = ok ? (hh * a.W + ww) * a.ldi + q * 8 : 0;
     }
@@ -176,23 +203,37 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   auto fetch = [&](u32x4 (&X)[NXL][NQ], u32x4 (&Wt)[NWL], int c0) __attribute__((always_inline)) {
     const bool up = SRC == 2 && c0 < a.cu;  // uniform: upsampled part of the concatenation
 #pragma unroll
-    for (int k = 0; k < NXL; ++k)
+    for (int k = 0; k < NXL; ++k) {
+      if constexpr (SRC == 3) {
+        X[k][0] = u32x4{__float_as_uint(Rin[xo[k][0]]), __float_as_uint(Tin[xo[k][0]]), 0u, 0u};
+      } else {
 #pragma unroll
-      for (int j = 0; j < NQ; ++j)
-        X[k][j] = *reinterpret_cast<const u32x4*>((up ? img2 + xo2[k] : img + xo[k][j]) + c0);
+        for (int j = 0; j < NQ; ++j)
+          X[k][j] = *reinterpret_cast<const u32x4*>((up ? img2 + xo2[k] : img + xo[k][j]) + c0);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < NWL; ++k) Wt[k] = *reinterpret_cast<const u32x4*>(a.w + wo[k] + c0);
     // keep the slice's loads together, in issue order
     __builtin_amdgcn_sched_barrier(0);
   };
+  float s_in = 1.0f;  // src = 3: the instance's s
   // registers -> LDS (pooling and zero padding applied here)
   auto stash = [&](const u32x4 (&X)[NXL][NQ], const u32x4 (&Wt)[NWL]) __attribute__((always_inline)) {
 #pragma unroll
     for (int k = 0; k < NXL; ++k) {
       const int e = tid + k * 256;
-      u32x4 v = SRC == 1 ? pool4(X[k][0], X[k][NQ > 1 ? 1 : 0], X[k][NQ > 2 ? 2 : 0],
-                                 X[k][NQ > 3 ? 3 : 0])
-                         : X[k][0];
+      u32x4 v;
+      if constexpr (SRC == 3) {
+        // channels 0..7 of the pixel: R / s, T, 0 ...; channels 8.. zero
+        const unsigned lo = f2bf(__uint_as_float(X[k][0][0]) / s_in);
+        const unsigned hi = f2bf(__uint_as_float(X[k][0][1]));
+        v = (e & 3) == 0 ? u32x4{lo | (hi << 16), 0u, 0u, 0u} : u32x4{0u, 0u, 0u, 0u};
+      } else {
+        v = SRC == 1 ? pool4(X[k][0], X[k][NQ > 1 ? 1 : 0], X[k][NQ > 2 ? 2 : 0],
+                             X[k][NQ > 3 ? 3 : 0])
+                     : X[k][0];
+      }
       if (!((okx >> k) & 1u)) v = u32x4{0u, 0u, 0u, 0u};
       const int row = e < RP * 4 ? (e >> 2) : RP;
       *reinterpret_cast<u32x4*>(sx + row * kLdsRow + (e & 3) * 8) = v;
@@ -231,12 +272,56 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
   // after the last: a wave retires only once its loads have landed)
   u32x4 rx[NXL][NQ], rw[NWL];
   fetch(rx, rw, 0);
+  if constexpr (SRC == 3) {
+    // s = sqrt(mean R^2) + 1e-30 of the instance, its partial sums in the
+    // order and reduction tree of k_ws_input
+    __shared__ double ssum[16];
+    __shared__ float s_sh;
+    double acc0 = 0.0;
+    for (int k = tid; k < a.nblk; k += 256) acc0 += a.part[(long)b * a.nblk + k];
+    const double tot = block_sum(acc0, ssum);
+    if (tid == 0) {
+      s_sh = (float)(sqrt(tot / (double)((long)a.H * a.W)) + 1e-30);
+      if (tw == 0 && th == 0 && blockIdx.y == 0) a.s_out[b] = s_sh;
+    }
+    __syncthreads();
+    s_in = s_sh;
+  }
   for (int s = 0; s < ns; ++s) {
     __syncthreads();  // every wave is done with the previous slice
     stash(rx, rw);
     __syncthreads();
     if (s + 1 < ns) fetch(rx, rw, (s + 1) * 32);
     compute();
+  }
+  if constexpr (HEAD) {
+    // the workgroup's activations (NT = co channels) into LDS, then one
+    // thread per pixel: head bias + the channel sum in channel order
+    static_assert(PH * PW * NT <= (RP + 1) * kLdsRow, "head tile in LDS");
+    __syncthreads();  // every wave is done reading the last slice
+    bf16_t* ht = sx;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int c = 16 * j + (lane & 15);
+      const float bv = a.bias ? a.bias[co0 + c] : 0.0f;
+#pragma unroll
+      for (int m = 0; m < MS; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int lp = (wv * WR + m / SX) * PW + (m % SX) * 16 + (lane >> 4) * 4 + i;
+          const float v = acc[m][j][i] + bv;
+          ht[lp * NT + c] = f2bf(a.act ? gelu(v) : v);
+        }
+    }
+    __syncthreads();
+    for (int lp = tid; lp < PH * PW; lp += 256) {
+      const int hh = h0 + lp / PW, ww = w0 + lp % PW;
+      if (hh >= a.H || ww >= a.W) continue;
+      float v = a.hb[0];
+      for (int c = 0; c < NT; ++c) v = head_acc(v, bf2f(ht[lp * NT + c]), a.hw[c]);
+      a.hout[((long)b * a.H + hh) * a.W + ww] = v * a.hs[b] * a.hscale;
+    }
+    return;
   }
 #pragma unroll
   for (int j = 0; j < NS; ++j) {

@@ -39,6 +39,15 @@ class WarmStartNet:
     def num_params(self) -> int:
         return int(_lib.load().aj_nn_num_params(self.handle))
 
+    def set_option(self, fused_io: bool | None = None):
+        """``fused_io`` (default True): the warm start builds the network
+        input in its first convolution and applies the 1x1 head in its last
+        one (two launches fewer, the same bits); False runs them as
+        separate launches."""
+        if fused_io is not None:
+            _lib.check(_lib.load().aj_nn_set_option(self.handle, _lib.AJ_NN_OPT_FUSED_IO,
+                                                    int(bool(fused_io))), "aj_nn_set_option")
+
     def get_params(self, grads=False) -> np.ndarray:
         out = np.empty(self.num_params, dtype=np.float32)
         rc = _lib.load().aj_nn_get_params(self.handle, out.ctypes.data, out.size, int(grads))

@@ -501,6 +501,15 @@ int aj_nn_warm_start(aj_nn* net, const float* rhs, const float* sdf, float* p0, 
  * normalised by its own rhs RMS), in one network inference of batch B. */
 int aj_nn_warm_start_batched(aj_nn* net, const float* rhs, const float* sdf, float* p0, int B,
                              int nx, int ny, int r, float sdf_ref, float pscale, void* stream);
+/* Network options (no counterpart in the reference, whose networks are not
+ * used as warm starts):
+ *   AJ_NN_OPT_FUSED_IO = 1 (default): for r = 1..3 and 32 or 64 base
+ *     channels, the warm start builds the network input row in its first
+ *     convolution's loads and applies the 1x1 head in its last
+ *     convolution's epilogue (two launches fewer); 0: separate launches.
+ *     The same bits either way. */
+enum { AJ_NN_OPT_FUSED_IO = 0 };
+int aj_nn_set_option(aj_nn* net, int option, int value);
 
 #ifdef __cplusplus
 }

@@ -234,3 +234,55 @@ def test_warm_start_pipeline(net):
     for l in range(r - 1, -1, -1):
         e = T.prolong(e, l, (e.shape[0] * 2, e.shape[1] * 2))
     assert np.allclose(p0, e, rtol=1e-4, atol=1e-4 * np.abs(e).max())
+
+
+def test_fused_warm_start_io_equals_separate_launches(gpu):
+    """The warm start with the network input built in the first
+    convolution's loads and the 1x1 head in the last convolution's epilogue
+    gives the bits of the separate launches (input row, head), single and
+    batched, on its eager, captured and replayed calls and for each of the
+    head launch's candidates."""
+    from aerojax_amd import _lib
+    from aerojax_amd.nn import WarmStartNet
+    L = _lib.load()
+    rng = np.random.default_rng(12)
+    nx, ny, B = 1024, 256, 3
+    rhs = torch.from_numpy((rng.normal(size=(B, nx, ny)) * 30).astype(np.float32)).cuda()
+    sdf = torch.from_numpy(rng.normal(size=(B, nx, ny)).astype(np.float32)).cuda()
+    net = WarmStartNet(base_channels=32, levels=4, seed=9)
+    try:
+        def runs(fused):
+            net.set_option(fused_io=fused)
+            outs = []
+            for _ in range(3):  # eager (first signature), captured, replayed
+                outs.append(net.warm_start(rhs[0], sdf[0]).clone())
+                p0 = torch.empty_like(rhs)
+                net.warm_start_batched(rhs, sdf, p0, B, nx, ny)
+                outs.append(p0.clone())
+            torch.cuda.synchronize()
+            return outs
+        sep = runs(False)
+        fus = runs(True)
+        for k, (a, b) in enumerate(zip(sep, fus)):
+            print(f"fused warm start output {k}: {int((a != b).sum())} cells differ", flush=True)
+            assert torch.equal(a, b), k
+        assert torch.equal(sep[0], sep[2]) and torch.isfinite(sep[0]).all()
+        # every candidate of the head launch (batch 1 and batch B shapes)
+        keys = [l.split(" -> ")[0] for l in _lib.tune_report().splitlines() if l.endswith("ms]")
+                and "/head" in l]
+        assert keys, "no head launch was tuned"
+        # (a fresh network per candidate: its first warm start runs eagerly)
+        for cand in range(3):
+            L.aj_tune_reset()
+            assert _lib.tune_load("".join(f"{k} -> {cand}\n" for k in keys)) == len(keys)
+            n2 = WarmStartNet(base_channels=32, levels=4, seed=9)
+            try:
+                a = n2.warm_start(rhs[0], sdf[0]).clone()
+                torch.cuda.synchronize()
+            finally:
+                n2.close()
+            print(f"head candidate {cand}: {int((a != fus[0]).sum())} cells differ", flush=True)
+            assert torch.equal(a, fus[0]), cand
+    finally:
+        L.aj_tune_reset()
+        net.close()
