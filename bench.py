@@ -424,6 +424,41 @@ def run_uneven(args):
     print(json.dumps(line), flush=True)
 
 
+def run_inference(args):
+    """The warm start alone: restriction, network inference at 2048x512 / 2^r,
+    head and prolongation on seeded 2048x512 fields with a seeded untrained
+    network (the launches and their cost do not depend on the weights);
+    `warmup` calls (the first one tunes and runs eagerly, the second
+    captures the graph), then `steps` timed calls.  Prints one JSON line."""
+    import numpy as np
+    import torch
+
+    from aerojax_amd.nn import WarmStartNet
+
+    nx, ny = 2048, 512
+    rng = np.random.default_rng(args.seed)
+    rhs = torch.from_numpy((rng.normal(size=(nx, ny)) * 30).astype(np.float32)).cuda()
+    sdf = torch.from_numpy(rng.normal(size=(nx, ny)).astype(np.float32)).cuda()
+    net = WarmStartNet(base_channels=32, levels=4, seed=args.seed, r=args.nn_r)
+    p0 = torch.empty_like(rhs)
+    for _ in range(max(args.warmup, 2)):
+        net.warm_start(rhs, sdf, p0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.warm_start(rhs, sdf, p0)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    net.close()
+    H, W = nx >> args.nn_r, ny >> args.nn_r
+    line = {"metric": f"warm-start inferences/s, 2048x512 fields, network at {H}x{W} (batch 1)",
+            "value": args.steps / el, "unit": "inferences/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic", "config": {"workload": f"warm_start_inference_{nx}x{ny}_r{args.nn_r}"}}
+    print(json.dumps(line), flush=True)
+
+
 def run_large(args):
     """4096x1024 cylinder at Re 1e5 with Smagorinsky LES (Cs 0.17), textbook
     multigrid in FP32 inside FP64 defect correction to a relative residual
@@ -496,13 +531,14 @@ def main():
     ap.add_argument("--nn-val-pairs", type=int, default=1024)
     ap.add_argument("--nn-epochs", type=int, default=3)
     ap.add_argument("--nn-r", type=int, default=2, help="network runs on the grid restricted r times")
-    ap.add_argument("--workload", choices=["warmstart", "batched", "large", "uneven"],
+    ap.add_argument("--workload", choices=["warmstart", "batched", "large", "uneven", "inference"],
                     default="warmstart",
                     help="warmstart: 2048x512 to ||div|| <= 1e-6, NN vs plain (default); "
                          "batched: 256 x 512x128 reference steps in shared launches; "
                          "large: 4096x1024 + LES to relative residual 1e-8, FP32 vs FP64; "
                          "uneven: 16 each of 256x64..2048x512 + LES to ||div|| <= 1e-6, "
-                         "NN vs plain")
+                         "NN vs plain; inference: the warm start alone (2048x512 fields, "
+                         "untrained network), for profiling its launches")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--per-size", type=int, default=16, help="uneven: instances per grid size")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -547,6 +583,10 @@ def main():
     if args.workload == "uneven":
         torch.cuda.set_device(0)
         run_uneven(args)
+        return
+    if args.workload == "inference":
+        torch.cuda.set_device(0)
+        run_inference(args)
         return
     if args.copy_calibration:
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
