@@ -1061,7 +1061,7 @@ static int tb_launch_pre_rc(const TbLevel& Lv, int nb, const float* pin, const f
                             float* Bc, const aj_mg_stats* gate, hipStream_t st) {
   if ((Lv.n | Lv.m) & 1) return fail(AJ_ERR_INVALID, "register-column smoother: odd level shape");
   using G = rc::PreGeo<W, R>;
-  const int XT = (int)cdiv(Lv.n, G::U), YT = (int)cdiv(Lv.m, G::UY);
+  const int XT = G::x_tiles(Lv.n), YT = (int)cdiv(Lv.m, G::UY);
   hipLaunchKernelGGL((k_tb_pre_rc<W, R>), dim3(8 * cdiv((long)XT * YT, 8), 1, nb), dim3(W * 64), 0,
                      st, Lv, XT, YT, pin, b, Y, Bc, gate);
   AJ_LAUNCH_CHECK("k_tb_pre_rc");
@@ -1074,7 +1074,7 @@ static int tb_launch_post_rc(const TbLevel& Lv, int nb, const float* Y, const fl
                              const aj_mg_stats* gate, hipStream_t st) {
   if ((Lv.n | Lv.m) & 1) return fail(AJ_ERR_INVALID, "register-column smoother: odd level shape");
   using G = rc::PostGeo<W, R>;
-  const int XT = (int)cdiv(Lv.n, G::U), YT = (int)cdiv(Lv.m, G::UY);
+  const int XT = G::x_tiles(Lv.n), YT = (int)cdiv(Lv.m, G::UY);
   hipLaunchKernelGGL((k_tb_post_rc<W, R>), dim3(8 * cdiv((long)XT * YT, 8), 1, nb), dim3(W * 64), 0,
                      st, Lv, XT, YT, Y, Ec, nc, mc, gamc, b, E, gate);
   AJ_LAUNCH_CHECK("k_tb_post_rc");

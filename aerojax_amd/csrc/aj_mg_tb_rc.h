@@ -49,6 +49,26 @@ struct Geo {
   static constexpr int U = ROWS - 2 * HX;    // kept rows
   static constexpr int UY = kCols - 2 * HY;  // kept columns
   static_assert(U >= 2, "tile shorter than its halo");
+  // x-tiles: the first keeps rows [0, F) and the last up to F rows (the
+  // inlet and outlet rows have no neighbour beyond them, so those tiles need
+  // no halo on that side), the others U rows; every kept range starts on an
+  // even row
+  static constexpr int F = ROWS - HX;
+  __host__ __device__ static int x_tiles(int n) {
+    return n <= ROWS ? 1 : 2 + (n - 2 * F > 0 ? (n - 2 * F + U - 1) / U : 0);
+  }
+  // origin gx0 and kept rows [k0, k1) of x-tile xt
+  __host__ __device__ static void x_tile(int xt, int n, int* gx0, int* k0, int* k1) {
+    if (xt == 0) {
+      *gx0 = 0;
+      *k0 = 0;
+      *k1 = n <= ROWS ? n : F;
+    } else {
+      *k0 = F + (xt - 1) * U;
+      *gx0 = *k0 - HX;
+      *k1 = *gx0 + ROWS >= n ? n : *k0 + U;
+    }
+  }
 };
 // pre-smoothing: 4 half-sweeps and the residual spoil 5 rows / columns from
 // an edge; post-smoothing: 4 half-sweeps spoil 4 rows, and 5 columns, since
@@ -178,8 +198,8 @@ __device__ __forceinline__ void weights(const aj::TbLevel& L, const LaneInfo& li
 template <int W, int R, bool GEN>
 __device__ __forceinline__ void pre_body(const aj::TbLevel& L, float (&p)[R][2],
                                          const float (&b)[R][2], const LaneInfo& li, int gx0,
-                                         int gy0, float* __restrict__ Y, float* __restrict__ Bc,
-                                         float (*xb)[W][2][kLanes]) {
+                                         int k0, int k1, int gy0, float* __restrict__ Y,
+                                         float* __restrict__ Bc, float (*xb)[W][2][kLanes]) {
   using G = PreGeo<W, R>;
   const int w = aj::wave_id(), lane = (int)(threadIdx.x & 63);
   const int gxw = gx0 + w * R;
@@ -227,7 +247,7 @@ __device__ __forceinline__ void pre_body(const aj::TbLevel& L, float (&p)[R][2],
 #pragma unroll
   for (int r = 0; r < R; r += 2) {
     const int gx = gxw + r;  // even
-    const bool rkeep = gx >= gx0 + 6 && gx < gx0 + 6 + G::U && gx < n;
+    const bool rkeep = gx >= k0 && gx < k1;  // k0 even: pairs kept together
     if (rkeep && lkeep) {
       *reinterpret_cast<float2*>(Y + (long)gx * m + li.gy) = make_float2(p[r][0], p[r][1]);
       *reinterpret_cast<float2*>(Y + (long)(gx + 1) * m + li.gy) =
@@ -244,7 +264,8 @@ __device__ __forceinline__ void pre_body(const aj::TbLevel& L, float (&p)[R][2],
 template <int W, int R, bool GEN>
 __device__ __forceinline__ void post_body(const aj::TbLevel& L, float (&p)[R][2],
                                           const float (&b)[R][2], const LaneInfo& li, int gx0,
-                                          float* __restrict__ E, float (*xb)[W][2][kLanes]) {
+                                          int k0, int k1, float* __restrict__ E,
+                                          float (*xb)[W][2][kLanes]) {
   using G = PostGeo<W, R>;
   const int w = aj::wave_id(), lane = (int)(threadIdx.x & 63);
   const int gxw = gx0 + w * R;
@@ -259,7 +280,7 @@ __device__ __forceinline__ void post_body(const aj::TbLevel& L, float (&p)[R][2]
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int gx = gxw + r;
-    const bool rkeep = gx >= gx0 + 4 && gx < gx0 + 4 + G::U && gx < n;
+    const bool rkeep = gx >= k0 && gx < k1;
     if (rkeep && lkeep)
       *reinterpret_cast<float2*>(E + (long)gx * m + li.gy) = make_float2(p[r][0], p[r][1]);
   }
@@ -291,7 +312,9 @@ __global__ __launch_bounds__(W * 64) void k_tb_pre_rc(TbLevel L, int XT, int YT,
   }
   const int gated = gate->converged;
   const int w = aj::wave_id(), lane = (int)(threadIdx.x & 63);
-  const int gx0 = xt * G::U - 6, gy0 = yt * G::UY - 6;
+  int gx0, k0, k1;
+  G::x_tile(xt, n, &gx0, &k0, &k1);
+  const int gy0 = yt * G::UY - 6;
   const rc::LaneInfo li = rc::lane_info(gy0, lane, m);
   const int gyl = li.in ? li.gy : 0;  // loads of lanes outside the grid read column 0
   float p[R][2], bb[R][2];
@@ -307,9 +330,9 @@ __global__ __launch_bounds__(W * 64) void k_tb_pre_rc(TbLevel L, int XT, int YT,
   }
   if (gated) return;
   if (gx0 >= 1 && gx0 + G::ROWS <= n - 1)
-    rc::pre_body<W, R, false>(L, p, bb, li, gx0, gy0, Y, Bc, xb);
+    rc::pre_body<W, R, false>(L, p, bb, li, gx0, k0, k1, gy0, Y, Bc, xb);
   else
-    rc::pre_body<W, R, true>(L, p, bb, li, gx0, gy0, Y, Bc, xb);
+    rc::pre_body<W, R, true>(L, p, bb, li, gx0, k0, k1, gy0, Y, Bc, xb);
 }
 
 // Post-smoothing kernel: Y + prolong(Ec) -> E; same grid.
@@ -337,7 +360,9 @@ __global__ __launch_bounds__(W * 64) void k_tb_post_rc(TbLevel L, int XT, int YT
   }
   const int gated = gate->converged;
   const int w = aj::wave_id(), lane = (int)(threadIdx.x & 63);
-  const int gx0 = xt * G::U - 4, gy0 = yt * G::UY - 6;
+  int gx0, k0, k1;
+  G::x_tile(xt, n, &gx0, &k0, &k1);
+  const int gy0 = yt * G::UY - 6;
   const int gxw = gx0 + w * R;  // even
   const rc::LaneInfo li = rc::lane_info(gy0, lane, m);
   const int gyl = li.in ? li.gy : 0;
@@ -389,7 +414,7 @@ __global__ __launch_bounds__(W * 64) void k_tb_post_rc(TbLevel L, int XT, int YT
     }
   }
   if (gx0 >= 1 && gx0 + G::ROWS <= n - 1)
-    rc::post_body<W, R, false>(L, p, bb, li, gx0, E, xb);
+    rc::post_body<W, R, false>(L, p, bb, li, gx0, k0, k1, E, xb);
   else
-    rc::post_body<W, R, true>(L, p, bb, li, gx0, E, xb);
+    rc::post_body<W, R, true>(L, p, bb, li, gx0, k0, k1, E, xb);
 }
