@@ -192,12 +192,13 @@ __global__ __launch_bounds__(W * 64) void kb_rk_rc(BatchWork w, BatchStepArgs<fl
     a.inv_dt = (float)(1.0 / s.dt);
     a.slip = 1;
     const int ty = it.sub % w.tiles_y[g], tx = it.sub / w.tiles_y[g];
+    int gx0, kx0, kx1;
+    G::x_tile(tx, a.nx, &gx0, &kx0, &kx1);
     if (a.ny <= 128) {
-      rk_rc_tile<W, R, false, true>(a, tx * G::U - G::HX, 0, 0, a.ny, xb);
+      rk_rc_tile<W, R, false, true>(a, gx0, kx0, kx1, 0, 0, a.ny, xb);
     } else {
       const int gy0 = ty * G::UY - G::HY;
-      rk_rc_tile<W, R, false, true>(a, tx * G::U - G::HX, gy0, gy0 + G::HY, gy0 + G::HY + G::UY,
-                                    xb);
+      rk_rc_tile<W, R, false, true>(a, gx0, kx0, kx1, gy0, gy0 + G::HY, gy0 + G::HY + G::UY, xb);
     }
     __syncthreads();  // the exchange buffers are reused by the next item
   }
@@ -836,7 +837,7 @@ int batch_rk_rc(aj_batch* b, const BatchStepArgs<float>& s, hipStream_t st) {
     for (int g = 0; g < w.ng; ++g) {
       const int ny = w.ny[g];
       w.tiles_y[g] = ny <= 128 ? 1 : (int)cdiv(ny, G::UY);
-      w.per_inst[g] = (int)cdiv(w.nx[g], G::U) * w.tiles_y[g];
+      w.per_inst[g] = G::x_tiles(w.nx[g]) * w.tiles_y[g];
       w.wstart[g] = tot;
       tot += w.per_inst[g] * b->gcount[g];
     }

@@ -64,6 +64,30 @@ int tune_pick(const std::string& key, int ncand,
               const std::function<int(int, hipStream_t)>& launch, hipStream_t st, int* pick);
 
 // ---- device helpers -----------------------------------------------------
+// x-tiles of `rows` rows with `hx` halo rows per internal side over a grid
+// of n rows: the first keeps rows [0, rows - hx) and the last up to
+// rows - hx rows (the inlet and outlet rows have no neighbour beyond them,
+// so those tiles need no halo on that side), the others rows - 2 hx; with
+// rows and hx even every kept range starts on an even row.
+__host__ __device__ inline int edge_x_tiles(int n, int rows, int hx) {
+  const int u = rows - 2 * hx, f = rows - hx;
+  return n <= rows ? 1 : 2 + (n - 2 * f > 0 ? (n - 2 * f + u - 1) / u : 0);
+}
+// origin gx0 and kept rows [k0, k1) of x-tile xt
+__host__ __device__ inline void edge_x_tile(int xt, int n, int rows, int hx, int* gx0, int* k0,
+                                            int* k1) {
+  const int u = rows - 2 * hx, f = rows - hx;
+  if (xt == 0) {
+    *gx0 = 0;
+    *k0 = 0;
+    *k1 = n <= rows ? n : f;
+  } else {
+    *k0 = f + (xt - 1) * u;
+    *gx0 = *k0 - hx;
+    *k1 = *gx0 + rows >= n ? n : *k0 + u;
+  }
+}
+
 __device__ __forceinline__ int clampi(int x, int lo, int hi) {
   return max(lo, min(x, hi));
 }

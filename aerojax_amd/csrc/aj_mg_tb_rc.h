@@ -49,25 +49,10 @@ struct Geo {
   static constexpr int U = ROWS - 2 * HX;    // kept rows
   static constexpr int UY = kCols - 2 * HY;  // kept columns
   static_assert(U >= 2, "tile shorter than its halo");
-  // x-tiles: the first keeps rows [0, F) and the last up to F rows (the
-  // inlet and outlet rows have no neighbour beyond them, so those tiles need
-  // no halo on that side), the others U rows; every kept range starts on an
-  // even row
-  static constexpr int F = ROWS - HX;
-  __host__ __device__ static int x_tiles(int n) {
-    return n <= ROWS ? 1 : 2 + (n - 2 * F > 0 ? (n - 2 * F + U - 1) / U : 0);
-  }
-  // origin gx0 and kept rows [k0, k1) of x-tile xt
+  // x-tiles with no halo on the inlet and outlet sides (edge_x_tiles)
+  __host__ __device__ static int x_tiles(int n) { return aj::edge_x_tiles(n, ROWS, HX); }
   __host__ __device__ static void x_tile(int xt, int n, int* gx0, int* k0, int* k1) {
-    if (xt == 0) {
-      *gx0 = 0;
-      *k0 = 0;
-      *k1 = n <= ROWS ? n : F;
-    } else {
-      *k0 = F + (xt - 1) * U;
-      *gx0 = *k0 - HX;
-      *k1 = *gx0 + ROWS >= n ? n : *k0 + U;
-    }
+    aj::edge_x_tile(xt, n, ROWS, HX, gx0, k0, k1);
   }
 };
 // pre-smoothing: 4 half-sweeps and the residual spoil 5 rows / columns from

@@ -36,9 +36,14 @@ struct RkRcGeo {
   static constexpr int HX = S + (DIV ? 1 : 0);  // halo rows per tile side
   static constexpr int HY = 4;                  // halo columns per strip side (strips)
   static constexpr int ROWS = W * R;
-  static constexpr int U = ROWS - 2 * HX;       // kept rows
+  static constexpr int U = ROWS - 2 * HX;       // kept rows (interior tiles)
   static constexpr int UY = 128 - 2 * HY;       // kept columns of a strip
   static_assert(U >= 2, "tile shorter than its halo");
+  // x-tiles with no halo on the inlet and outlet sides (edge_x_tiles)
+  __host__ __device__ static int x_tiles(int n) { return edge_x_tiles(n, ROWS, HX); }
+  __host__ __device__ static void x_tile(int xt, int n, int* gx0, int* k0, int* k1) {
+    edge_x_tile(xt, n, ROWS, HX, gx0, k0, k1);
+  }
 };
 
 // Per-launch scalars of the register-column predictor.
@@ -52,10 +57,12 @@ struct RkRcArgs {
 };
 
 // The whole tile: loads, stages, final mask, boundary values, rhs, stores.
-// gx0, gy0: origin; ky0, ky1: kept columns (even).  GEN: the tile reaches
-// rows 0, 1, nx - 2 or nx - 1 (per-row choices then apply).
+// gx0, gy0: origin; kx0, kx1: kept rows; ky0, ky1: kept columns (even).
+// GEN: the tile reaches rows 0, 1, nx - 2 or nx - 1 (per-row choices then
+// apply).
 template <int W, int R, bool FAST, bool DIV, bool GEN>
-__device__ __forceinline__ void rk_rc_body(const RkRcArgs& a, int gx0, int gy0, int ky0, int ky1,
+__device__ __forceinline__ void rk_rc_body(const RkRcArgs& a, int gx0, int kx0, int kx1, int gy0,
+                                           int ky0, int ky1,
                                            float4 (*xb)[W][2][64], const float (&u0)[R][2],
                                            const float (&v0)[R][2], const float (&iD)[R][2],
                                            const float (&nt)[R][2], const float (&mk)[R][2]) {
@@ -184,7 +191,7 @@ __device__ __forceinline__ void rk_rc_body(const RkRcArgs& a, int gx0, int gy0, 
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int gx = gxw + r;
-    const bool rkeep = gx >= gx0 + G::HX && gx < gx0 + G::ROWS - G::HX && gx < nx;
+    const bool rkeep = gx >= kx0 && gx < kx1;
     if (!(rkeep && lkeep)) continue;
     const long g = (long)gx * ny + gy;
     if (DIV) *reinterpret_cast<float2*>(a.rhs + g) = make_float2(rh[r][0], rh[r][1]);
@@ -215,10 +222,11 @@ __device__ __forceinline__ void rk_rc_body(const RkRcArgs& a, int gx0, int gy0, 
   }
 }
 
-// Loads of a tile and the body.  ky0, ky1: kept columns.
+// Loads of a tile and the body.  kx0, kx1: kept rows; ky0, ky1: kept
+// columns.
 template <int W, int R, bool FAST, bool DIV>
-__device__ __forceinline__ void rk_rc_tile(const RkRcArgs& a, int gx0, int gy0, int ky0, int ky1,
-                                           float4 (*xb)[W][2][64]) {
+__device__ __forceinline__ void rk_rc_tile(const RkRcArgs& a, int gx0, int kx0, int kx1, int gy0,
+                                           int ky0, int ky1, float4 (*xb)[W][2][64]) {
   const int wv = wave_id(), lane = (int)(threadIdx.x & 63);
   const int nx = a.nx, ny = a.ny;
   const int gxw = gx0 + wv * R;
@@ -254,9 +262,9 @@ __device__ __forceinline__ void rk_rc_tile(const RkRcArgs& a, int gx0, int gy0, 
   }
   using G = RkRcGeo<W, R, FAST, DIV>;
   if (gx0 >= 2 && gx0 + G::ROWS <= nx - 2)
-    rk_rc_body<W, R, FAST, DIV, false>(a, gx0, gy0, ky0, ky1, xb, u0, v0, iD, nt, mk);
+    rk_rc_body<W, R, FAST, DIV, false>(a, gx0, kx0, kx1, gy0, ky0, ky1, xb, u0, v0, iD, nt, mk);
   else
-    rk_rc_body<W, R, FAST, DIV, true>(a, gx0, gy0, ky0, ky1, xb, u0, v0, iD, nt, mk);
+    rk_rc_body<W, R, FAST, DIV, true>(a, gx0, kx0, kx1, gy0, ky0, ky1, xb, u0, v0, iD, nt, mk);
 }
 
 // Single grid: grid (8 * ceil(XT * YT / 8)); tiles numbered so that each XCD
@@ -270,7 +278,8 @@ __global__ __launch_bounds__(W * 64) void k_rk_rc(RkRcArgs a, int XT, int YT) {
   const int t = (k & 7) * per + (k >> 3);
   if (t >= T) return;
   const int xt = t / YT, yt = t - xt * YT;
-  const int gx0 = xt * G::U - G::HX;
+  int gx0, kx0, kx1;
+  G::x_tile(xt, a.nx, &gx0, &kx0, &kx1);
   int gy0, ky0, ky1;
   if (a.ny <= 128) {
     gy0 = 0;
@@ -281,7 +290,7 @@ __global__ __launch_bounds__(W * 64) void k_rk_rc(RkRcArgs a, int XT, int YT) {
     ky0 = gy0 + G::HY;
     ky1 = ky0 + G::UY;
   }
-  rk_rc_tile<W, R, FAST, DIV>(a, gx0, gy0, ky0, ky1, xb);
+  rk_rc_tile<W, R, FAST, DIV>(a, gx0, kx0, kx1, gy0, ky0, ky1, xb);
 }
 
 }  // namespace aj

@@ -64,7 +64,7 @@ static int launch_rk_rc(const RkArgs<float>& a, hipStream_t st) {
   r.sdf_scale = a.sdf_scale; r.inv_2dx = a.inv_2dx; r.inv_2dy = a.inv_2dy; r.inv_dt = a.inv_dt;
   r.slip = a.slip; r.rhs_masked = a.rhs_masked;
   using G = RkRcGeo<W, R, FAST, DIV>;
-  const int XT = (int)cdiv(a.nx, G::U), YT = a.ny <= 128 ? 1 : (int)cdiv(a.ny, G::UY);
+  const int XT = G::x_tiles(a.nx), YT = a.ny <= 128 ? 1 : (int)cdiv(a.ny, G::UY);
   hipLaunchKernelGGL((k_rk_rc<W, R, FAST, DIV>), dim3(8 * cdiv((long)XT * YT, 8)), dim3(W * 64),
                      0, st, r, XT, YT);
   AJ_LAUNCH_CHECK("k_rk_rc");
