@@ -282,17 +282,47 @@ __global__ __launch_bounds__(256) void kb_div(BatchWork w, BatchStepArgs<T> s) {
   }
 }
 
+// for_item_cells over the pairs (i, j), (i, j + 1) of even j (ny even): 2
+// pairs per thread, f(i, j) runs for the first cell of each pair
+template <typename F>
+__device__ __forceinline__ void for_item_pairs(int sub, int nx, int ny, F&& f) {
+  const int hy = ny >> 1;
+  const int q0 = sub * (kCellsPerItem / 2) + (int)threadIdx.x;
+  const int di = 256 / hy, dq = 256 - di * hy;
+  int i = q0 / hy, q = q0 - (q0 / hy) * hy;
+#pragma unroll
+  for (int k = 0; k < kCellsPerItem / 512; ++k) {
+    if (i < nx) f(i, 2 * q);
+    i += di;
+    q += dq;
+    if (q >= hy) {
+      q -= hy;
+      ++i;
+    }
+  }
+}
+
+// pairs: the six arrays are pair-aligned; an instance goes by pairs when its
+// row length is even and its offset keeps that alignment
 template <typename T>
-__global__ __launch_bounds__(256) void kb_project(BatchWork w, BatchStepArgs<T> s) {
+__global__ __launch_bounds__(256) void kb_project(BatchWork w, BatchStepArgs<T> s, int pairs) {
   for (XcdItems q(w.total); q.t < q.end; q.t += q.step) {
     const int item = q.base + q.t;
     const BatchItem it = batch_item(w, item);
     const int g = it.g, nx = w.nx[g], ny = w.ny[g];
-    for_item_cells(it.sub, nx, ny, [&](int i, int j) {
-      project_cell(s.us + it.off, s.vs + it.off, s.p + it.off, s.mask + it.off, s.uo + it.off,
-                   s.vo + it.off, nx, ny, i, j, T(s.dt), T(1.0 / (2.0 * w.dx[g])),
-                   T(1.0 / (2.0 * w.dy[g])), T(0.1 / w.dx[g]), T(s.U_inf));
-    });
+    const T dt = T(s.dt), i2x = T(1.0 / (2.0 * w.dx[g])), i2y = T(1.0 / (2.0 * w.dy[g]));
+    const T thr = T(0.1 / w.dx[g]), U = T(s.U_inf);
+    const T *us = s.us + it.off, *vs = s.vs + it.off, *p = s.p + it.off, *m = s.mask + it.off;
+    T *uo = s.uo + it.off, *vo = s.vo + it.off;
+    if (pairs && !((ny | (int)it.off) & 1)) {
+      for_item_pairs(it.sub, nx, ny, [&](int i, int j) {
+        project_pair(us, vs, p, m, uo, vo, nx, ny, i, j, dt, i2x, i2y, thr, U);
+      });
+    } else {
+      for_item_cells(it.sub, nx, ny, [&](int i, int j) {
+        project_cell(us, vs, p, m, uo, vo, nx, ny, i, j, dt, i2x, i2y, thr, U);
+      });
+    }
   }
 }
 
@@ -926,8 +956,12 @@ template <typename T>
 int project_launches(aj_batch* b, const BatchStepArgs<T>& s, hipStream_t st) {
   BatchWork wrow = make_work(b, 0, 1);
   ProfScope prof(AJ_PROF_PROJECT, (double)b->cells * sizeof(T) * 6, st);
+  int pairs = 1;
+  for (const void* a : {(const void*)s.us, (const void*)s.vs, (const void*)s.p,
+                        (const void*)s.mask, (const void*)s.uo, (const void*)s.vo})
+    if ((uintptr_t)a % (2 * sizeof(T))) pairs = 0;
   return batch_launch(b, "project", wrow.total, [&](dim3 g, hipStream_t t) -> int {
-    hipLaunchKernelGGL(kb_project<T>, g, dim3(256), 0, t, wrow, s);
+    hipLaunchKernelGGL(kb_project<T>, g, dim3(256), 0, t, wrow, s, pairs);
     AJ_LAUNCH_CHECK("kb_project");
     return AJ_OK;
   }, st);

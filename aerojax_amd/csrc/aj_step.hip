@@ -220,9 +220,30 @@ __global__ __launch_bounds__(256) void k_project(const T* __restrict__ us, const
   project_cell(us, vs, p, mask, uo, vo, nx, ny, i, j, dt, inv_2dx, inv_2dy, iface_thr, U_inf);
 }
 
+// k_project over pairs of cells (grid row_grid(nx, ny / 2), ny even)
+template <typename T>
+__global__ __launch_bounds__(256) void k_project_pair(
+    const T* __restrict__ us, const T* __restrict__ vs, const T* __restrict__ p,
+    const T* __restrict__ mask, T* __restrict__ uo, T* __restrict__ vo, int nx, int ny, T dt,
+    T inv_2dx, T inv_2dy, T iface_thr, T U_inf) {
+  int i, q;
+  row_block(&i, &q);
+  q += threadIdx.x;
+  if (2 * q >= ny) return;
+  project_pair(us, vs, p, mask, uo, vo, nx, ny, i, 2 * q, dt, inv_2dx, inv_2dy, iface_thr, U_inf);
+}
+
 // one block per 256-column segment of a row; blocks are renumbered per XCD
 // (row_block) so that the +-row stencil reads of a band of rows share an L2
 static dim3 row_grid(int nx, int ny) { return dim3(cdiv(ny, 256), nx); }
+
+// pairs need an even row length and pair-aligned arrays
+static bool pair_ok(int ny, size_t elem, const void* const* ptrs, int n) {
+  if (ny & 1) return false;
+  for (int k = 0; k < n; ++k)
+    if ((uintptr_t)ptrs[k] % (2 * elem)) return false;
+  return true;
+}
 
 static int check_grid(int nx, int ny) {
   if (nx < 4 || ny < 4) return fail(AJ_ERR_INVALID, "grid must be at least 4x4");
@@ -425,16 +446,22 @@ extern "C" int aj_project(int dtype, const void* us, const void* vs, const void*
   hipStream_t st = (hipStream_t)stream;
   // u*, v*, p, mask read; u, v written
   ProfScope prof(AJ_PROF_PROJECT, (double)nx * ny * (dtype == AJ_F64 ? 8 : 4) * 6, st);
+  const void* ptrs[6] = {us, vs, p, mask, uo, vo};
+  auto launch = [&](auto zero) {
+    using T = decltype(zero);
+    const T* a[4] = {(const T*)us, (const T*)vs, (const T*)p, (const T*)mask};
+    const T c[5] = {T(dt), T(1.0 / (2.0 * dx)), T(1.0 / (2.0 * dy)), T(0.1 / dx), T(U_inf)};
+    if (pair_ok(ny, sizeof(T), ptrs, 6))
+      hipLaunchKernelGGL(k_project_pair<T>, row_grid(nx, ny / 2), dim3(256), 0, st, a[0], a[1],
+                         a[2], a[3], (T*)uo, (T*)vo, nx, ny, c[0], c[1], c[2], c[3], c[4]);
+    else
+      hipLaunchKernelGGL(k_project<T>, row_grid(nx, ny), dim3(256), 0, st, a[0], a[1], a[2],
+                         a[3], (T*)uo, (T*)vo, nx, ny, c[0], c[1], c[2], c[3], c[4]);
+  };
   if (dtype == AJ_F32) {
-    hipLaunchKernelGGL(k_project<float>, row_grid(nx, ny), dim3(256), 0, st, (const float*)us,
-                       (const float*)vs, (const float*)p, (const float*)mask, (float*)uo,
-                       (float*)vo, nx, ny, float(dt), float(1.0 / (2.0 * dx)),
-                       float(1.0 / (2.0 * dy)), float(0.1 / dx), float(U_inf));
+    launch(0.f);
   } else if (dtype == AJ_F64) {
-    hipLaunchKernelGGL(k_project<double>, row_grid(nx, ny), dim3(256), 0, st, (const double*)us,
-                       (const double*)vs, (const double*)p, (const double*)mask, (double*)uo,
-                       (double*)vo, nx, ny, dt, 1.0 / (2.0 * dx), 1.0 / (2.0 * dy), 0.1 / dx,
-                       U_inf);
+    launch(0.0);
   } else {
     return fail(AJ_ERR_INVALID, "aj_project: bad dtype");
   }

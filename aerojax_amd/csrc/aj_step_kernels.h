@@ -398,6 +398,34 @@ __device__ __forceinline__ void div_rhs_cell(const T* __restrict__ us, const T* 
   rhs[g] = r;
 }
 
+// projected velocity of an interior cell from its stencil operands
+// (step.py:257-264, 305-349, 364-377): the pressure gradient, with its
+// normal component removed where the mask gradient marks an interface,
+// subtracted from u*, v*, then masked.  One function for the per-cell and
+// the per-pair kernels, with its multiply-adds spelled out, so that both give
+// the same bits.
+template <typename T>
+__device__ __forceinline__ void project_values(T pxp, T pxm, T pyp, T pym, T mxp, T mxm, T myp,
+                                               T mym, T usc, T vsc, T m0, T dt, T inv_2dx,
+                                               T inv_2dy, T iface_thr, T* un, T* vn) {
+#pragma clang fp contract(off)
+  const T gx0 = (pxp - pxm) * inv_2dx;
+  const T gy0 = (pyp - pym) * inv_2dy;
+  const T mx = (mxp - mxm) * inv_2dx;
+  const T my = (myp - mym) * inv_2dy;
+  const T gm = sqrt(fma(mx, mx, my * my));
+  T gx = gx0, gy = gy0;
+  if (gm > iface_thr) {
+    const T nxv = mx / (gm + T(1e-8));
+    const T nyv = my / (gm + T(1e-8));
+    const T dpn = fma(gx0, nxv, gy0 * nyv);
+    gx = fma(-dpn, nxv, gx0);
+    gy = fma(-dpn, nyv, gy0);
+  }
+  *un = masked(fma(-dt, gx, usc), m0);
+  *vn = masked(fma(-dt, gy, vsc), m0);
+}
+
 // projection at (i, j) (step.py:257-264, 305-349, 364-377, 394-397); the
 // boundary cells are written by their interior source
 template <typename T>
@@ -415,22 +443,10 @@ __device__ __forceinline__ void project_cell(const T* __restrict__ us, const T* 
     return;
   }
   if (!(i < nx - 1 && j > 0 && j < ny - 1)) return;  // written by a source
-  const T gx0 = (p[g + ny] - p[g - ny]) * inv_2dx;
-  const T gy0 = (p[g + 1] - p[g - 1]) * inv_2dy;
-  const T mx = (mask[g + ny] - mask[g - ny]) * inv_2dx;
-  const T my = (mask[g + 1] - mask[g - 1]) * inv_2dy;
-  const T gm = sqrt(mx * mx + my * my);
-  T gx = gx0, gy = gy0;
-  if (gm > iface_thr) {
-    const T nxv = mx / (gm + T(1e-8));
-    const T nyv = my / (gm + T(1e-8));
-    const T dpn = gx0 * nxv + gy0 * nyv;
-    gx = gx0 - dpn * nxv;
-    gy = gy0 - dpn * nyv;
-  }
-  const T m0 = mask[g];
-  const T un = masked(us[g] - dt * gx, m0);
-  const T vn = masked(vs[g] - dt * gy, m0);
+  T un, vn;
+  project_values(p[g + ny], p[g - ny], p[g + 1], p[g - 1], mask[g + ny], mask[g - ny],
+                 mask[g + 1], mask[g - 1], us[g], vs[g], mask[g], dt, inv_2dx, inv_2dy,
+                 iface_thr, &un, &vn);
   uo[g] = un;  // second masking is idempotent on the source cell
   vo[g] = vn;
   const bool out_row = (i == nx - 2);
@@ -447,6 +463,69 @@ __device__ __forceinline__ void project_cell(const T* __restrict__ us, const T* 
     uo[g + 1] = masked(un, mask[g + 1]);
     vo[g + 1] = T(0);
     if (out_row) { uo[g + ny + 1] = masked(un, mask[g + ny + 1]); vo[g + ny + 1] = T(0); }
+  }
+}
+
+template <typename T> struct Pair;
+template <> struct Pair<float> { using V = float2; };
+template <> struct Pair<double> { using V = double2; };
+
+// project_cell for the two cells (i, j) and (i, j + 1) of an even j on a grid
+// of even ny >= 4, with the six arrays aligned to a pair: the centre rows
+// move as one 2-vector per array, and the wall cells j = 0 and ny - 1 (each
+// in the pair of its source) and the outlet row (written with the pairs of
+// row nx - 2) come out as pairs too.  Same values as project_cell.
+template <typename T>
+__device__ __forceinline__ void project_pair(const T* __restrict__ us, const T* __restrict__ vs,
+                                             const T* __restrict__ p, const T* __restrict__ mask,
+                                             T* __restrict__ uo, T* __restrict__ vo, int nx,
+                                             int ny, int i, int j, T dt, T inv_2dx, T inv_2dy,
+                                             T iface_thr, T U_inf) {
+  using V = typename Pair<T>::V;
+  const long g = (long)i * ny + j;
+  const V mc = *reinterpret_cast<const V*>(mask + g);
+  if (i == 0) {
+    V u2, v2;
+    u2.x = masked(U_inf, mc.x);
+    u2.y = masked(U_inf, mc.y);
+    v2.x = v2.y = T(0);
+    *reinterpret_cast<V*>(uo + g) = u2;
+    *reinterpret_cast<V*>(vo + g) = v2;
+    return;
+  }
+  if (i >= nx - 1) return;  // outlet row: written by row nx - 2
+  // g - 1 and g + 2 stay inside the grid: rows i - 1 and i + 1 exist
+  const V pxp = *reinterpret_cast<const V*>(p + g + ny);
+  const V pxm = *reinterpret_cast<const V*>(p + g - ny);
+  const V pc = *reinterpret_cast<const V*>(p + g);
+  const V mxp = *reinterpret_cast<const V*>(mask + g + ny);
+  const V mxm = *reinterpret_cast<const V*>(mask + g - ny);
+  const V usc = *reinterpret_cast<const V*>(us + g);
+  const V vsc = *reinterpret_cast<const V*>(vs + g);
+  const T pym = p[g - 1], pyp = p[g + 2], mym = mask[g - 1], myp = mask[g + 2];
+  T ua, va, ub, vb;
+  project_values(pxp.x, pxm.x, pc.y, pym, mxp.x, mxm.x, mc.y, mym, usc.x, vsc.x, mc.x, dt,
+                 inv_2dx, inv_2dy, iface_thr, &ua, &va);
+  project_values(pxp.y, pxm.y, pyp, pc.x, mxp.y, mxm.y, myp, mc.x, usc.y, vsc.y, mc.y, dt,
+                 inv_2dx, inv_2dy, iface_thr, &ub, &vb);
+  // the wall cell of an edge pair takes its neighbour's u (v = 0)
+  const bool lo = (j == 0), hi = (j == ny - 2);
+  const T su = lo ? ub : ua, sv = lo ? T(0) : va;  // source of cell j
+  const T tu = hi ? ua : ub, tv = hi ? T(0) : vb;  // source of cell j + 1
+  V u2, v2;
+  u2.x = lo ? masked(su, mc.x) : su;
+  u2.y = hi ? masked(tu, mc.y) : tu;
+  v2.x = sv;
+  v2.y = tv;
+  *reinterpret_cast<V*>(uo + g) = u2;
+  *reinterpret_cast<V*>(vo + g) = v2;
+  if (i == nx - 2) {
+    u2.x = masked(su, mxp.x);
+    u2.y = masked(tu, mxp.y);
+    v2.x = masked(sv, mxp.x);
+    v2.y = masked(tv, mxp.y);
+    *reinterpret_cast<V*>(uo + g + ny) = u2;
+    *reinterpret_cast<V*>(vo + g + ny) = v2;
   }
 }
 

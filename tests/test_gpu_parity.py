@@ -219,6 +219,33 @@ def test_les_div_project_parity(gpu, shape, dtype):
         check(host(got[k]), ref[k], refd[k], dtype, f"project {shape[:2]} {k}")
 
 
+@pytest.mark.parametrize("shape", [(130, 70, 2.0, 1.0), (96, 8, 2.0, 0.5), (2048, 512, 4.0, 1.0)])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_project_pairs_equal_cells(gpu, shape, dtype):
+    """An even row length goes by pairs of cells; arrays that start one
+    element past a pair boundary take the per-cell kernel.  Same bits, walls
+    and outlet row included."""
+    import aerojax_amd as aj
+    c = case(*shape)
+    dx, dy, dt = c["dx"], c["dy"], 1e-3
+    p = 5.0 * np.sin(2 * c["X"]) * np.cos(3 * c["Y"])
+    args = [c["u"], c["v"], p, c["mask"]]
+
+    def shifted(a):
+        t = dev(a, dtype)
+        buf = torch.zeros(t.numel() + 1, dtype=t.dtype, device=t.device)
+        buf[1:].copy_(t.reshape(-1))
+        return buf[1:].view(t.shape)
+
+    pairs = aj.project(*[dev(a, dtype) for a in args], dt, dx, dy, 1.0)
+    cells = aj.project(*[shifted(a) for a in args], dt, dx, dy, 1.0)
+    ref = R.project(*args, dt, dx, dy, 1.0)
+    refd = R.project(*[a.astype(NP[dtype]) for a in args], dt, dx, dy, 1.0)
+    for k in range(2):
+        assert torch.equal(pairs[k], cells[k]), (shape[:2], k)
+        check(host(pairs[k]), ref[k], refd[k], dtype, f"project pairs {shape[:2]} {k}")
+
+
 # ------------------------------------------------------------------ multigrid
 
 @pytest.mark.parametrize("shape", [(512, 96), (2048, 512), (64, 32)])
