@@ -795,6 +795,10 @@ __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, doub
     if (hs) hs += z;
   }
   const int skipped = st->converged;  // this outer iteration was skipped
+  // the scalars thread 0 updates, read with the partial sums rather than one
+  // after another behind the reduction (only thread 0 writes them, after it)
+  const int outer0 = st->outer, cycles0 = st->cycles, parity0 = state[0];
+  const double tol0 = dtol[0];
   double acc = 0.0;
 #pragma unroll 8
   for (int k = threadIdx.x; k < np; k += 256) acc += partial[k];
@@ -803,17 +807,21 @@ __global__ void k_tb_norm_final(const double* __restrict__ partial, int np, doub
   if (threadIdx.x == 0) {
     const double nrm = sqrt(tot * dxdy);
     // norms[0] = initial residual, norms[k] = after outer iteration k
-    const int k = st->outer + (flip ? 1 : 0);
+    const int k = outer0 + (flip ? 1 : 0);
     if (k < AJ_MG_MAX_NORMS) st->norms[k] = nrm;
     if (flip) {
-      state[0] ^= 1;
+      state[0] = parity0 ^ 1;
       st->outer = k;
-      st->cycles += cycles_added;
+      st->cycles = cycles0 + cycles_added;
     }
     st->norm = nrm;
     if (!isfinite(nrm)) st->nonfinite = 1;
-    if (rel) dtol[0] *= nrm;
-    st->converged = (nrm <= dtol[0]) ? 1 : 0;
+    double tol = tol0;
+    if (rel) {
+      tol *= nrm;
+      dtol[0] = tol;
+    }
+    st->converged = (nrm <= tol) ? 1 : 0;
     tb_publish(st, hs);
   }
 }
